@@ -1,0 +1,226 @@
+"""Benchmark: solves/sec at 1e-10 relative residual (FP64-checked) on the 128^3 sequence.
+
+Workload (the 3-D configuration of the DMR north star): 3-D convection-diffusion, 7-point, 128^3
+(n = 2,097,152, nnz = 14,581,760), CSR FP64, a sequence of 16 systems whose per-cell diffusion
+is jittered U(-1%, +1%) per step (seeded), N(0,1) right-hand sides, m = 50, k = 20, tol 1e-10,
+deflation subspace recycled across the sequence (mode = 0 for the first system, 1 after).
+
+Two legs run on the same sequence, each in a fresh solver context:
+  * fp64  -- the plain FP64 DMR (the parity-checked path);
+  * mixed -- FP32 Krylov basis and SpMV with FP32-accumulated reductions inside FP64 iterative
+             refinement; Hessenberg LSQ / SVD in FP64.
+A step is one solve of one system.  `value` is the mixed leg's solves/sec when every system of it
+reaches 1e-10 checked in FP64 (else the fp64 leg's).  Every solve's true residual
+||b - A x|| / ||b|| is recomputed on the host in FP64 after the timed region.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gdmr-solver_amd"))
+
+METRIC = ("solves/sec @1e-10 rel residual (FP64-checked), mixed+batched vs FP64; p50/p99")
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_sequence(N, steps, seed):
+    import gdmr
+    mats, rhs = [], []
+    for t in range(steps):
+        mats.append(gdmr.conv_diff_3d(N, nu0=0.1, beta=(1.0, 0.5, 0.25), jitter=0.01, seed=seed,
+                                      stream=t))
+        rhs.append(gdmr.rand_normal(seed, 1000 + t, mats[-1].n))
+    return mats, rhs
+
+
+def run_leg(name, prec, mats, rhs, dev_vals, dev_vals32, rp, ci, m, k, tol, warmup, profile):
+    import torch
+
+    import gdmr
+    n = mats[0].n
+    # Warm-up on a throwaway context (first-touch, code objects), then a fresh context for the
+    # timed sequence so the recycled subspace starts from nothing.
+    for w in range(warmup):
+        s = gdmr.Solver(n, m, k, precision=prec)
+        s.set_csr(rp.data_ptr(), ci.data_ptr(), dev_vals[0].data_ptr(), mats[0].nnz,
+                  dev_vals32[0].data_ptr() if prec != gdmr.PREC_FP64 else 0)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        b = torch.from_numpy(rhs[0]).to("cuda")
+        s.solve(x.data_ptr(), b.data_ptr(), tol, 100000, mode=0)
+        s.close()
+        log(f"[{name}] warm-up {w} done")
+    s = gdmr.Solver(n, m, k, precision=prec)
+    s.set_csr(rp.data_ptr(), ci.data_ptr(), dev_vals[0].data_ptr(), mats[0].nnz,
+              dev_vals32[0].data_ptr() if prec != gdmr.PREC_FP64 else 0)
+    if profile:
+        s.set_profiling(True)
+    bs = [torch.from_numpy(b).to("cuda") for b in rhs]
+    xs = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in rhs]
+    lat, cycles, reported = [], [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(len(mats)):
+        ts = time.perf_counter()
+        s.set_values(dev_vals[t].data_ptr(),
+                     dev_vals32[t].data_ptr() if prec != gdmr.PREC_FP64 else 0)
+        res, its = s.solve(xs[t].data_ptr(), bs[t].data_ptr(), tol, 100000, mode=1 if t else 0)
+        lat.append(time.perf_counter() - ts)
+        st, _ = s.stats()
+        cycles.append(st.cycles)
+        reported.append(res)
+        log(f"[{name}] system {t}: {lat[-1]*1e3:.1f} ms, cycles {st.cycles}, residual {res:.3e}, "
+            f"status {st.status}")
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    prof = s.profile() if profile else None
+    s.close()
+    true = []
+    for t, A in enumerate(mats):
+        x = xs[t].cpu().numpy()
+        true.append(float(np.linalg.norm(rhs[t] - A.matvec(x)) / np.linalg.norm(rhs[t])))
+    xs_host = [x.cpu().numpy() for x in xs]
+    return dict(total=total, lat=lat, cycles=cycles, reported=reported, true=true, prof=prof,
+                x=xs_host)
+
+
+def summarize(leg, tol):
+    lat = np.array(leg["lat"])
+    return {
+        "solves_per_s": len(lat) / leg["total"],
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "mean_ms": float(lat.mean() * 1e3),
+        "cycles": leg["cycles"],
+        "max_true_rel_residual": max(leg["true"]),
+        "all_converged": bool(max(leg["true"]) <= tol),
+    }
+
+
+def roofline(prof):
+    # Dominant streaming kernel of the cycle: the multi-dot (GEMV-T) or the Gram-Schmidt update,
+    # whichever took more time; achieved = algorithmic bytes / kernel time.
+    best = max(("gemvt", "update"), key=lambda p: prof[p][0])
+    ms, by, cnt = prof[best]
+    ach = by / (ms * 1e-3) if ms > 0 else 0.0
+    return {"bound": "hbm", "kernel": "k_multidot" if best == "gemvt" else "k_combo(SUB)",
+            "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": ach / HBM_PEAK, "traffic": None, "launches": cnt,
+            "avg_launch_us": ms * 1e3 / max(cnt, 1),
+            "bytes_per_launch": by / max(cnt, 1)}
+
+
+def cpu_baseline(mats, rhs, m, k, tol, gpu_cycles_per_solve, sample_cycles):
+    """The FP64 host oracle on system 0 of the same sequence, capped at sample_cycles restart
+    cycles; solves/sec is extrapolated with the GPU FP64 leg's mean cycles per solve."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import gdmr
+    import oracle
+    A = mats[0]
+    ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(A.n))
+    t0 = time.perf_counter()
+    ora.solve(rhs[0], tol=tol, mat_rank=sample_cycles - 1, mode=0)
+    dt = time.perf_counter() - t0
+    cyc = ora.counts()[0]
+    per_cycle = dt / max(cyc, 1)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": 1.0 / (per_cycle * gpu_cycles_per_solve), "unit": "solves/s", "cores": cores,
+            "kind": "port",
+            "sample": f"FP64 oracle, system 0 of the 128^3 sequence, first {cyc} restart cycles "
+                      f"in {dt:.1f} s ({per_cycle:.2f} s/cycle, OpenMP {cores} threads); "
+                      f"solves/s = 1/(s per cycle x {gpu_cycles_per_solve:.1f} mean cycles per "
+                      f"solve of the GPU FP64 leg)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--m", type=int, default=50)
+    ap.add_argument("--k", type=int, default=20)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--legs", default="fp64,mixed")
+    ap.add_argument("--cpu-cycles", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import gdmr
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    N, m, k, tol = args.grid, args.m, args.k, args.tol
+    t0 = time.time()
+    mats, rhs = build_sequence(N, args.steps, args.seed)
+    log(f"generated {len(mats)} systems n={mats[0].n} nnz={mats[0].nnz} in {time.time()-t0:.1f}s")
+    rp = torch.from_numpy(mats[0].rowptr).to("cuda")
+    ci = torch.from_numpy(mats[0].col).to("cuda")
+    dev_vals = [torch.from_numpy(A.val).to("cuda") for A in mats]
+    dev_vals32 = [v.float().contiguous() for v in dev_vals]
+    torch.cuda.synchronize()
+    legs = {}
+    for name in args.legs.split(","):
+        prec = gdmr.PREC_FP64 if name == "fp64" else gdmr.PREC_FP32
+        legs[name] = run_leg(name, prec, mats, rhs, dev_vals, dev_vals32, rp, ci, m, k, tol,
+                             args.warmup, not args.no_profile)
+    out_legs = {nm: summarize(lg, tol) for nm, lg in legs.items()}
+    if "mixed" in legs and "fp64" in legs:
+        for nm in ("mixed",):
+            errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b))
+                    for a, b in zip(legs[nm]["x"], legs["fp64"]["x"])]
+            out_legs[nm]["max_rel_err_vs_fp64_solution"] = max(errs)
+    head = "mixed" if ("mixed" in out_legs and out_legs["mixed"]["all_converged"]) else "fp64"
+    hs = out_legs[head]
+    line = {
+        "metric": METRIC,
+        "value": hs["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": hs["mean_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64" if head == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3-D conv-diff 7-pt {N}^3 sequence of {args.steps} jittered "
+                               f"systems, m={m} k={k} tol {tol:g}, Z recycled",
+                   "n": mats[0].n, "nnz": mats[0].nnz, "headline_leg": head},
+        "p50_ms": hs["p50_ms"],
+        "p99_ms": hs["p99_ms"],
+        "legs": out_legs,
+    }
+    if "mixed" in out_legs and "fp64" in out_legs:
+        line["speedup_mixed_vs_fp64"] = (out_legs["mixed"]["solves_per_s"] /
+                                         out_legs["fp64"]["solves_per_s"])
+    if legs[head]["prof"] is not None:
+        line["roofline"] = roofline(legs[head]["prof"])
+        line["phase_ms"] = {nm: {p: round(v[0], 3) for p, v in lg["prof"].items()}
+                            for nm, lg in legs.items() if lg["prof"]}
+        line["phase_GBps"] = {nm: {p: round(v[1] / (v[0] * 1e-3) / 1e9, 1)
+                                   for p, v in lg["prof"].items() if v[0] > 0 and v[1] > 0}
+                              for nm, lg in legs.items() if lg["prof"]}
+    if not args.no_cpu:
+        ref = out_legs.get("fp64", hs)
+        cyc = float(np.mean(ref["cycles"]))
+        line["cpu_baseline"] = cpu_baseline(mats, rhs, m, k, tol, cyc, args.cpu_cycles)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

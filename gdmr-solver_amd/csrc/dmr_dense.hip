@@ -1,0 +1,229 @@
+// One-workgroup FP64 kernels on the (m+1) x m Hessenberg matrix of a DMR cycle.
+//
+// * lsq: least-squares solve min ||t - H y|| by Householder QR (Golub & Van Loan, Matrix
+//   Computations, Alg. 5.1.1 house + 5.2.1, LAPACK dlarfg conventions) applied to [H | t],
+//   column-oriented back substitution, then z = H y.  Replaces cusolverDnCgeqrf + Cungqr +
+//   the Q^H t GEMV + cublasCtrsv + cublasCgemv of qrHessArrDMR / qrSolveDMR / hessenbergMultDMR
+//   (gDMR.cu:591-633, 690-839).  The implicit application of the reflectors to t gives the same
+//   y as forming Q explicitly.
+// * svd_smallest: right singular vectors by one-sided Jacobi (Hestenes 1958; Demmel & Veselic
+//   1992) with the round-robin ("chess tournament") pair ordering, one 64-wide wavefront per
+//   column pair, m/2 disjoint pairs per round.  Replaces cusolverDnCgesvdj (gDMR.cu:634-689).
+//   Singular values are sorted descending; the k vectors of the smallest ones are returned, each
+//   with its largest-magnitude component made positive.
+//
+// Every operation order here (sequential per-thread sums, wavefront xor butterflies, explicit
+// fma) is fixed, so the FP64 oracle can reproduce the results bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "dmr_internal.h"
+
+namespace dmr {
+namespace {
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+  return v;
+}
+
+constexpr int kLsqThreads = 256;
+constexpr int kSvdThreads = 1024;
+
+__global__ __launch_bounds__(kLsqThreads) void k_lsq(const double *__restrict__ H, int m,
+                                                     const double *__restrict__ t,
+                                                     double *__restrict__ y,
+                                                     double *__restrict__ z) {
+  extern __shared__ double sm[];
+  const int ldh = m + 1;
+  double *A = sm;                       // (m+1) x m, column-major
+  double *b = A + (long long)ldh * m;   // m+1
+  double *yv = b + ldh;                 // m
+  __shared__ double s_tau;
+  const int tid = threadIdx.x, bd = blockDim.x;
+  for (int i = tid; i < ldh * m; i += bd) A[i] = H[i];
+  for (int i = tid; i < ldh; i += bd) b[i] = t[i];
+  __syncthreads();
+  for (int j = 0; j < m; ++j) {
+    double *vj = A + (long long)j * ldh;
+    if (tid == 0) {
+      const double alpha = vj[j];
+      double xn2 = 0.0;
+      for (int i = j + 1; i <= m; ++i) xn2 = fma(vj[i], vj[i], xn2);
+      double tau = 0.0;
+      if (xn2 != 0.0) {
+        const double beta = -copysign(sqrt(fma(alpha, alpha, xn2)), alpha);
+        tau = (beta - alpha) / beta;
+        const double sc = 1.0 / (alpha - beta);
+        for (int i = j + 1; i <= m; ++i) vj[i] = vj[i] * sc;
+        vj[j] = beta;
+      }
+      s_tau = tau;
+    }
+    __syncthreads();
+    const double tau = s_tau;
+    if (tau != 0.0) {
+      for (int c = j + 1 + tid; c <= m; c += bd) {
+        double *col = (c < m) ? A + (long long)c * ldh : b;
+        double w = col[j];
+        for (int i = j + 1; i <= m; ++i) w = fma(vj[i], col[i], w);
+        const double tw = tau * w;
+        col[j] = col[j] - tw;
+        for (int i = j + 1; i <= m; ++i) col[i] = fma(-tw, vj[i], col[i]);
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = m - 1; i >= 0; --i) {
+    if (tid == 0) yv[i] = b[i] / A[i + (long long)i * ldh];
+    __syncthreads();
+    const double yi = yv[i];
+    for (int r = tid; r < i; r += bd) b[r] = fma(-A[r + (long long)i * ldh], yi, b[r]);
+    __syncthreads();
+  }
+  for (int i = tid; i < m; i += bd) y[i] = yv[i];
+  if (tid == 0) y[m] = 0.0;
+  for (int r = tid; r <= m; r += bd) {
+    double s = 0.0;
+    for (int c = 0; c < m; ++c) s = fma(H[r + (long long)c * ldh], yv[c], s);
+    z[r] = s;
+  }
+}
+
+__global__ __launch_bounds__(kSvdThreads) void k_svd(const double *__restrict__ H, int m, int k,
+                                                     double tol, int maxSweeps,
+                                                     double *__restrict__ C,
+                                                     double *__restrict__ sig,
+                                                     int *__restrict__ sweepsOut) {
+  extern __shared__ double sm[];
+  const int mm = m + (m & 1);  // even column count; column m is a zero dummy when m is odd
+  const int ldh = m + 1;
+  double *A = sm;                          // ldh x mm
+  double *V = A + (long long)ldh * mm;     // m x mm (ld m)
+  double *sg = V + (long long)m * mm;      // mm
+  __shared__ int s_rot;
+  const int tid = threadIdx.x, bd = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6, nw = bd >> 6;
+  for (int idx = tid; idx < ldh * mm; idx += bd) {
+    const int c = idx / ldh;
+    A[idx] = c < m ? H[idx] : 0.0;
+  }
+  for (int idx = tid; idx < m * mm; idx += bd) {
+    const int r = idx % m, c = idx / m;
+    V[idx] = (r == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  int sweep = 0;
+  while (sweep < maxSweeps) {
+    if (tid == 0) s_rot = 0;
+    __syncthreads();
+    for (int r = 0; r < mm - 1; ++r) {
+      for (int pi = wave; pi < mm / 2; pi += nw) {
+        int p, q;
+        if (pi == 0) {
+          p = r;
+          q = mm - 1;
+        } else {
+          p = (r + pi) % (mm - 1);
+          q = (r - pi + (mm - 1)) % (mm - 1);
+        }
+        if (p > q) {
+          const int tmp = p;
+          p = q;
+          q = tmp;
+        }
+        if (q >= m) continue;
+        double *ap = A + (long long)p * ldh, *aq = A + (long long)q * ldh;
+        double al = 0.0, be = 0.0, ga = 0.0;
+        for (int i = lane; i < ldh; i += 64) {
+          const double x = ap[i], y = aq[i];
+          al = fma(x, x, al);
+          be = fma(y, y, be);
+          ga = fma(x, y, ga);
+        }
+        al = wave_sum(al);
+        be = wave_sum(be);
+        ga = wave_sum(ga);
+        if (ga != 0.0 && fabs(ga) > tol * sqrt(al * be)) {
+          const double zeta = (be - al) / (2.0 * ga);
+          const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(fma(zeta, zeta, 1.0)));
+          const double cs = 1.0 / sqrt(fma(tt, tt, 1.0));
+          const double sn = cs * tt;
+          for (int i = lane; i < ldh; i += 64) {
+            const double x = ap[i], y = aq[i];
+            ap[i] = cs * x - sn * y;
+            aq[i] = sn * x + cs * y;
+          }
+          double *vp = V + (long long)p * m, *vq = V + (long long)q * m;
+          for (int i = lane; i < m; i += 64) {
+            const double x = vp[i], y = vq[i];
+            vp[i] = cs * x - sn * y;
+            vq[i] = sn * x + cs * y;
+          }
+          if (lane == 0) atomicAdd(&s_rot, 1);
+        }
+      }
+      __syncthreads();
+    }
+    const int rot = s_rot;
+    __syncthreads();
+    ++sweep;
+    if (rot == 0) break;
+  }
+  // Column norms = singular values.
+  for (int c = wave; c < m; c += nw) {
+    const double *ac = A + (long long)c * ldh;
+    double s = 0.0;
+    for (int i = lane; i < ldh; i += 64) s = fma(ac[i], ac[i], s);
+    s = wave_sum(s);
+    if (lane == 0) sg[c] = sqrt(s);
+  }
+  __syncthreads();
+  for (int c = tid; c < m; c += bd) {
+    const double sc = sg[c];
+    int rank = 0;
+    for (int j = 0; j < m; ++j) rank += (sg[j] > sc || (sg[j] == sc && j < c)) ? 1 : 0;
+    sig[rank] = sc;
+    if (rank >= m - k) {
+      const int i = rank - (m - k);
+      const double *vc = V + (long long)c * m;
+      int imax = 0;
+      double vmax = fabs(vc[0]);
+      for (int r = 1; r < m; ++r) {
+        const double a = fabs(vc[r]);
+        if (a > vmax) {
+          vmax = a;
+          imax = r;
+        }
+      }
+      const bool neg = vc[imax] < 0.0;
+      for (int r = 0; r < m; ++r) C[r + (long long)m * i] = neg ? -vc[r] : vc[r];
+    }
+  }
+  if (tid == 0) *sweepsOut = sweep;
+}
+
+}  // namespace
+
+void lsq(const double *H, int m, const double *t, double *y, double *z, hipStream_t s) {
+  const size_t lds = sizeof(double) * ((size_t)(m + 1) * m + (m + 1) + m);
+  if (lds > 64 * 1024)
+    hipFuncSetAttribute(reinterpret_cast<const void *>(k_lsq),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_lsq, dim3(1), dim3(kLsqThreads), lds, s, H, m, t, y, z);
+}
+
+void svd_smallest(const double *H, int m, int k, double tol, int maxSweeps, double *C, double *sig,
+                  int *sweeps, hipStream_t s) {
+  const int mm = m + (m & 1);
+  const size_t lds = sizeof(double) * ((size_t)(m + 1) * mm + (size_t)m * mm + mm);
+  if (lds > 64 * 1024)
+    hipFuncSetAttribute(reinterpret_cast<const void *>(k_svd),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_svd, dim3(1), dim3(kSvdThreads), lds, s, H, m, k, tol, maxSweeps, C, sig,
+                     sweeps);
+}
+
+}  // namespace dmr

@@ -1,0 +1,791 @@
+// Streaming HIP kernels of the DMR solve for CDNA4 (gfx950): CSR SpMV, multi-dot (GEMV-T),
+// single- and multi-output basis combinations with fused reductions, scaling and fills.
+//
+// Design rules shared by every kernel here:
+//  * 256-thread blocks (four 64-wide wavefronts); vectors are walked in 16-byte units
+//    (2 doubles / 4 floats / 8 bf16) so every load is a dwordx4 and every wavefront touches
+//    1 KiB contiguous per column per step.
+//  * Vectors live in padded columns (ld a multiple of 256 elements, padding kept at zero), so a
+//    unit never straddles the end of a column and needs no tail handling.
+//  * Reductions are fixed-order and single-pass: each thread accumulates its grid-stride units
+//    in order, a wavefront reduces by an xor butterfly, the four wavefront sums are added in
+//    order, block partials go to a workspace, and the last block to arrive (counter with
+//    agent-scope fences) adds the partials in block order.  The result is bitwise independent of
+//    scheduling and of other work on the device, and is the order the FP64 oracle reproduces.
+//    This replaces the reference's innerProdKerDMR (gDMR.cu:268-326), whose cross-block sum is
+//    read without a grid-wide barrier.
+//  * Floating-point contraction is off for this file (-ffp-contract=off); every fused
+//    multiply-add is an explicit fma().
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "dmr_internal.h"
+
+namespace dmr {
+namespace {
+
+template <typename TS> struct Acc { using type = float; };
+template <> struct Acc<double> { using type = double; };
+
+__device__ inline double fmac(double a, double b, double c) { return fma(a, b, c); }
+__device__ inline float fmac(float a, float b, float c) { return fmaf(a, b, c); }
+
+// ---- 16-byte unit loads / stores with conversion to the accumulation type ----
+template <typename TA>
+__device__ inline void ldu(const double *p, long long u, TA (&v)[2]) {
+  const double2 t = reinterpret_cast<const double2 *>(p)[u];
+  v[0] = (TA)t.x;
+  v[1] = (TA)t.y;
+}
+template <typename TA>
+__device__ inline void ldu(const float *p, long long u, TA (&v)[4]) {
+  const float4 t = reinterpret_cast<const float4 *>(p)[u];
+  v[0] = (TA)t.x;
+  v[1] = (TA)t.y;
+  v[2] = (TA)t.z;
+  v[3] = (TA)t.w;
+}
+__device__ inline float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ inline float bf_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+template <typename TA>
+__device__ inline void ldu(const bf16 *p, long long u, TA (&v)[8]) {
+  const uint4 t = reinterpret_cast<const uint4 *>(p)[u];
+  v[0] = (TA)bf_lo(t.x); v[1] = (TA)bf_hi(t.x);
+  v[2] = (TA)bf_lo(t.y); v[3] = (TA)bf_hi(t.y);
+  v[4] = (TA)bf_lo(t.z); v[5] = (TA)bf_hi(t.z);
+  v[6] = (TA)bf_lo(t.w); v[7] = (TA)bf_hi(t.w);
+}
+
+template <typename TA>
+__device__ inline void stu(double *p, long long u, const TA (&v)[2]) {
+  reinterpret_cast<double2 *>(p)[u] = make_double2((double)v[0], (double)v[1]);
+}
+template <typename TA>
+__device__ inline void stu(float *p, long long u, const TA (&v)[4]) {
+  reinterpret_cast<float4 *>(p)[u] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+}
+// Round-to-nearest-even float -> bf16 bits (finite inputs).
+__device__ inline uint32_t bf_bits(float f) {
+  const uint32_t x = __float_as_uint(f);
+  return (x + 0x7fffu + ((x >> 16) & 1u)) >> 16;
+}
+template <typename TA>
+__device__ inline void stu(bf16 *p, long long u, const TA (&v)[8]) {
+  uint4 t;
+  t.x = bf_bits((float)v[0]) | (bf_bits((float)v[1]) << 16);
+  t.y = bf_bits((float)v[2]) | (bf_bits((float)v[3]) << 16);
+  t.z = bf_bits((float)v[4]) | (bf_bits((float)v[5]) << 16);
+  t.w = bf_bits((float)v[6]) | (bf_bits((float)v[7]) << 16);
+  reinterpret_cast<uint4 *>(p)[u] = t;
+}
+// Value as it reads back after a store to storage type TS.
+template <typename TS, typename TA> __device__ inline TA as_stored(TA v) { return v; }
+template <> __device__ inline float as_stored<bf16, float>(float v) {
+  return __uint_as_float(bf_bits(v) << 16);
+}
+
+// ---- fixed-order reductions ----
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum over the 256 threads of a block: wavefront butterflies, then ((w0 + w1) + w2) + w3.
+// Every thread gets the result.
+template <typename TA> __device__ inline TA block_sum(TA v) {
+  __shared__ TA part[kBlock / 64];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const TA s = ((part[0] + part[1]) + part[2]) + part[3];
+  __syncthreads();
+  return s;
+}
+
+// Partials are written and read with agent-scope atomics so the last block sees every other
+// block's value whichever XCD (and L2) it ran on.
+__device__ inline void put_partial(double *p, double v) {
+  __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), __double_as_longlong(v),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void put_partial(float *p, float v) {
+  __hip_atomic_store(reinterpret_cast<unsigned *>(p), __float_as_uint(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double get_partial(const double *p) {
+  return __longlong_as_double((long long)__hip_atomic_load(
+      reinterpret_cast<const unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ inline float get_partial(const float *p) {
+  return __uint_as_float(__hip_atomic_load(reinterpret_cast<const unsigned *>(p), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Arrival of a block at the end of a grid-wide reduction.  True in exactly one block, the last
+// to arrive, after every other block's partials are visible to it.
+__device__ inline bool arrive_last(unsigned *counter, unsigned total) {
+  __shared__ int s_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = (atomicAdd(counter, 1u) == total - 1u);
+  __syncthreads();
+  const bool last = s_last != 0;
+  if (last) __threadfence();
+  return last;
+}
+
+// Sum of G block partials in block order: thread t adds partials t, t+256, ... then block_sum.
+template <typename TA> __device__ inline TA sum_partials(const TA *part, int G) {
+  TA v = 0;
+  for (int q = threadIdx.x; q < G; q += kBlock) v = v + get_partial(part + q);
+  return block_sum(v);
+}
+
+__device__ inline void finalize(const Fin &fin, double ss, double sg) {
+  switch (fin.mode) {
+    case FIN_NORM:
+      fin.scal[0] = ss;
+      fin.scal[1] = 1.0 / sqrt(ss);
+      fin.scal[2] = sqrt(ss);
+      break;
+    case FIN_ARNOLDI: {
+      const double nrm = sqrt(ss);
+      if (fin.tol > 0.0 && nrm < fin.tol) {
+        *fin.flag = fin.step;
+      } else {
+        const double sc = 1.0 / sqrt(ss);
+        fin.scal[1] = sc;
+        *fin.hdst = sg * sc;
+      }
+      break;
+    }
+    case FIN_STORE:
+      fin.scal[0] = ss;
+      fin.scal[1] = sg;
+      break;
+    default:
+      break;
+  }
+}
+
+// =============================================================================================
+// Multi-dot: out[l] = B_l . x for the p columns of B (GEMV-T).  Replaces basisProjDMR's
+// cublasCgemv(OP_C) (gDMR.cu:548-590).  grid = (G row blocks, ceil(p / kDotCols) column groups);
+// x is re-read once per column group.
+// =============================================================================================
+template <typename TS>
+__global__ __launch_bounds__(kBlock) void k_multidot(const TS *__restrict__ B, long long ld, int p,
+                                                     const TS *__restrict__ x, long long nv, int G,
+                                                     typename Acc<TS>::type *partials,
+                                                     unsigned *counters, double *out, int ostride,
+                                                     const int *flag) {
+  using TA = typename Acc<TS>::type;
+  constexpr int N = Unit<TS>::N;
+  if (flag && *flag) return;
+  const int c0 = blockIdx.y * kDotCols;
+  const int pc = min(kDotCols, p - c0);
+  TA acc[kDotCols];
+#pragma unroll
+  for (int c = 0; c < kDotCols; ++c) acc[c] = 0;
+  const long long stride = (long long)G * kBlock;
+  for (long long u = (long long)blockIdx.x * kBlock + threadIdx.x; u < nv; u += stride) {
+    TA xv[N];
+    ldu(x, u, xv);
+#pragma unroll
+    for (int c = 0; c < kDotCols; ++c) {
+      if (c < pc) {
+        TA bv[N];
+        ldu(B + (long long)(c0 + c) * ld, u, bv);
+#pragma unroll
+        for (int e = 0; e < N; ++e) acc[c] = fmac(bv[e], xv[e], acc[c]);
+      }
+    }
+  }
+  __shared__ TA red[kBlock / 64][kDotCols];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < kDotCols; ++c) {
+    const TA v = wave_sum(acc[c]);
+    if (lane == 0) red[wv][c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < pc) {
+    const int c = threadIdx.x;
+    const TA s = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
+    put_partial(partials + (long long)(c0 + c) * G + blockIdx.x, s);
+  }
+  if (!arrive_last(&counters[blockIdx.y], (unsigned)G)) return;
+  for (int c = 0; c < pc; ++c) {
+    const TA v = sum_partials(partials + (long long)(c0 + c) * G, G);
+    if (threadIdx.x == 0) out[(long long)(c0 + c) * ostride] = (double)v;
+  }
+  if (threadIdx.x == 0) counters[blockIdx.y] = 0u;
+}
+
+// =============================================================================================
+// Single-output combination with fused reductions.  Replaces vecScaleDiffDMR (gDMR.cu:229-248,
+// mode SUB), addBasisVecsDMR + vecAddDMR + normalizeVecDMR(mode 2) (gDMR.cu:185-209, 250-266,
+// 328-355, mode ACC) and the innerProdDMR calls that follow them.
+// =============================================================================================
+template <typename TS, int MODE, int RED>
+__global__ __launch_bounds__(kBlock) void k_combo(const TS *in, TS *out, const TS *__restrict__ B1,
+                                                  int p1, const double *__restrict__ c1,
+                                                  const TS *__restrict__ B2, int p2,
+                                                  const double *__restrict__ c2, long long ld,
+                                                  long long nv, double beta,
+                                                  const TS *__restrict__ g, int G,
+                                                  typename Acc<TS>::type *partials,
+                                                  unsigned *counter, Fin fin, const int *flag) {
+  using TA = typename Acc<TS>::type;
+  constexpr int N = Unit<TS>::N;
+  if (flag && *flag) return;
+  __shared__ TA cs[kMaxCoef];
+  for (int l = threadIdx.x; l < p1 + p2; l += kBlock) {
+    const double c = l < p1 ? c1[l] : c2[l - p1];
+    cs[l] = (TA)(MODE == COMB_SUB ? -c : c);
+  }
+  __syncthreads();
+  const TA tb = (TA)beta;
+  TA ss = 0, sg = 0;
+  const long long stride = (long long)G * kBlock;
+  for (long long u = (long long)blockIdx.x * kBlock + threadIdx.x; u < nv; u += stride) {
+    TA o[N];
+    if (MODE == COMB_SUB) {
+      ldu(in, u, o);
+      for (int l = 0; l < p1; ++l) {
+        TA b[N];
+        ldu(B1 + (long long)l * ld, u, b);
+#pragma unroll
+        for (int e = 0; e < N; ++e) o[e] = fmac(cs[l], b[e], o[e]);
+      }
+      for (int l = 0; l < p2; ++l) {
+        TA b[N];
+        ldu(B2 + (long long)l * ld, u, b);
+#pragma unroll
+        for (int e = 0; e < N; ++e) o[e] = fmac(cs[p1 + l], b[e], o[e]);
+      }
+    } else {
+      TA a[N];
+#pragma unroll
+      for (int e = 0; e < N; ++e) a[e] = 0;
+      for (int l = 0; l < p1; ++l) {
+        TA b[N];
+        ldu(B1 + (long long)l * ld, u, b);
+#pragma unroll
+        for (int e = 0; e < N; ++e) a[e] = fmac(cs[l], b[e], a[e]);
+      }
+      for (int l = 0; l < p2; ++l) {
+        TA b[N];
+        ldu(B2 + (long long)l * ld, u, b);
+#pragma unroll
+        for (int e = 0; e < N; ++e) a[e] = fmac(cs[p1 + l], b[e], a[e]);
+      }
+      if (in) {
+        TA xi[N];
+        ldu(in, u, xi);
+#pragma unroll
+        for (int e = 0; e < N; ++e) {
+          const TA t = tb * a[e];
+          o[e] = xi[e] + t;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < N; ++e) o[e] = tb * a[e];
+      }
+    }
+    stu(out, u, o);
+    if (RED & 1) {
+#pragma unroll
+      for (int e = 0; e < N; ++e) {
+        const TA v = as_stored<TS>(o[e]);
+        ss = fmac(v, v, ss);
+      }
+    }
+    if (RED & 2) {
+      TA gv[N];
+      ldu(g, u, gv);
+#pragma unroll
+      for (int e = 0; e < N; ++e) sg = fmac(as_stored<TS>(o[e]), gv[e], sg);
+    }
+  }
+  if (RED == 0) return;
+  if (RED & 1) {
+    const TA s = block_sum(ss);
+    if (threadIdx.x == 0) put_partial(partials + blockIdx.x, s);
+  }
+  if (RED & 2) {
+    const TA s = block_sum(sg);
+    if (threadIdx.x == 0) put_partial(partials + G + blockIdx.x, s);
+  }
+  if (!arrive_last(counter, (unsigned)G)) return;
+  TA tss = 0, tsg = 0;
+  if (RED & 1) tss = sum_partials(partials, G);
+  if (RED & 2) tsg = sum_partials(partials + G, G);
+  if (threadIdx.x == 0) {
+    finalize(fin, (double)tss, (double)tsg);
+    *counter = 0u;
+  }
+}
+
+// =============================================================================================
+// Multi-output combination out_j = [B1 B2] c_j with per-output sum of squares, j < J; the
+// deflation refresh Z_new = [V Z] C (genDeflatDMR, gDMR.cu:1583-1704) as one skinny GEMM instead
+// of k separate combinations.  grid = (G, ceil(J / JB)).
+// =============================================================================================
+template <typename TS, int JB>
+__global__ __launch_bounds__(kBlock) void k_combo_multi(
+    TS *__restrict__ out, int J, const TS *__restrict__ B1, int p1, const double *__restrict__ C1,
+    const TS *__restrict__ B2, int p2, const double *__restrict__ C2, int ldc, long long ld,
+    long long nv, int G, typename Acc<TS>::type *partials, unsigned *counters, double *scal) {
+  using TA = typename Acc<TS>::type;
+  constexpr int N = Unit<TS>::N;
+  extern __shared__ unsigned char smem_raw[];
+  TA *cs = reinterpret_cast<TA *>(smem_raw);  // (p1 + p2) x JB
+  const int j0 = blockIdx.y * JB;
+  const int Jg = min(JB, J - j0);
+  for (int idx = threadIdx.x; idx < (p1 + p2) * JB; idx += kBlock) {
+    const int l = idx / JB, j = idx % JB;
+    double c = 0.0;
+    if (j < Jg) c = l < p1 ? C1[l + (long long)ldc * (j0 + j)] : C2[(l - p1) + (long long)ldc * (j0 + j)];
+    cs[idx] = (TA)c;
+  }
+  __syncthreads();
+  TA ss[JB];
+#pragma unroll
+  for (int j = 0; j < JB; ++j) ss[j] = 0;
+  const long long stride = (long long)G * kBlock;
+  for (long long u = (long long)blockIdx.x * kBlock + threadIdx.x; u < nv; u += stride) {
+    TA a[JB][N];
+#pragma unroll
+    for (int j = 0; j < JB; ++j)
+#pragma unroll
+      for (int e = 0; e < N; ++e) a[j][e] = 0;
+    for (int l = 0; l < p1 + p2; ++l) {
+      TA b[N];
+      ldu(l < p1 ? B1 + (long long)l * ld : B2 + (long long)(l - p1) * ld, u, b);
+#pragma unroll
+      for (int j = 0; j < JB; ++j) {
+        if (j < Jg) {
+          const TA c = cs[l * JB + j];
+#pragma unroll
+          for (int e = 0; e < N; ++e) a[j][e] = fmac(c, b[e], a[j][e]);
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+      if (j < Jg) {
+        stu(out + (long long)(j0 + j) * ld, u, a[j]);
+#pragma unroll
+        for (int e = 0; e < N; ++e) {
+          const TA v = as_stored<TS>(a[j][e]);
+          ss[j] = fmac(v, v, ss[j]);
+        }
+      }
+    }
+  }
+  for (int j = 0; j < Jg; ++j) {
+    const TA s = block_sum(ss[j]);
+    if (threadIdx.x == 0) put_partial(partials + (long long)(j0 + j) * G + blockIdx.x, s);
+  }
+  if (!arrive_last(&counters[blockIdx.y], (unsigned)G)) return;
+  for (int j = 0; j < Jg; ++j) {
+    const TA v = sum_partials(partials + (long long)(j0 + j) * G, G);
+    if (threadIdx.x == 0) scal[j0 + j] = 1.0 / sqrt((double)v);
+  }
+  if (threadIdx.x == 0) counters[blockIdx.y] = 0u;
+}
+
+// v *= scal (normalizeVecDMR, gDMR.cu:328-355), skipped after a breakdown.
+template <typename TS>
+__global__ __launch_bounds__(kBlock) void k_scale(TS *v, long long nv, const double *scal,
+                                                  const int *flag) {
+  using TA = typename Acc<TS>::type;
+  constexpr int N = Unit<TS>::N;
+  if (flag && *flag) return;
+  const TA s = (TA)*scal;
+  for (long long u = (long long)blockIdx.x * kBlock + threadIdx.x; u < nv;
+       u += (long long)gridDim.x * kBlock) {
+    TA x[N];
+    ldu(v, u, x);
+#pragma unroll
+    for (int e = 0; e < N; ++e) x[e] = x[e] * s;
+    stu(v, u, x);
+  }
+}
+
+template <typename TS>
+__global__ __launch_bounds__(kBlock) void k_scale_cols(TS *v, long long ld, long long nv,
+                                                       const double *scal) {
+  using TA = typename Acc<TS>::type;
+  constexpr int N = Unit<TS>::N;
+  TS *col = v + (long long)blockIdx.y * ld;
+  const TA s = (TA)scal[blockIdx.y];
+  for (long long u = (long long)blockIdx.x * kBlock + threadIdx.x; u < nv;
+       u += (long long)gridDim.x * kBlock) {
+    TA x[N];
+    ldu(col, u, x);
+#pragma unroll
+    for (int e = 0; e < N; ++e) x[e] = x[e] * s;
+    stu(col, u, x);
+  }
+}
+
+template <typename TS>
+__global__ __launch_bounds__(kBlock) void k_fill(TS *p, long long nv, double val) {
+  using TA = typename Acc<TS>::type;
+  constexpr int N = Unit<TS>::N;
+  TA x[N];
+#pragma unroll
+  for (int e = 0; e < N; ++e) x[e] = (TA)val;
+  for (long long u = (long long)blockIdx.x * kBlock + threadIdx.x; u < nv;
+       u += (long long)gridDim.x * kBlock)
+    stu(p, u, x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_int(int *p, long long count, int v) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < count;
+       i += (long long)gridDim.x * kBlock)
+    p[i] = v;
+}
+
+// Element-wise conversion; n need not be a multiple of a unit (external vectors).
+template <typename TD, typename TSrc>
+__global__ __launch_bounds__(kBlock) void k_convert(TD *dst, const TSrc *src, long long n) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock)
+    dst[i] = (TD)src[i];
+}
+template <>
+__global__ __launch_bounds__(kBlock) void k_convert<bf16, double>(bf16 *dst, const double *src,
+                                                                   long long n) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock)
+    dst[i].bits = (uint16_t)bf_bits((float)src[i]);
+}
+template <>
+__global__ __launch_bounds__(kBlock) void k_convert<double, bf16>(double *dst, const bf16 *src,
+                                                                   long long n) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock)
+    dst[i] = (double)__uint_as_float((uint32_t)src[i].bits << 16);
+}
+
+template <typename TS>
+__global__ __launch_bounds__(kBlock) void k_add_to(double *x, const TS *d, long long nv) {
+  using TA = typename Acc<TS>::type;
+  constexpr int N = Unit<TS>::N;
+  // x is FP64 (2 per 16-byte unit); walk d in its own units and x element-wise.
+  for (long long u = (long long)blockIdx.x * kBlock + threadIdx.x; u < nv;
+       u += (long long)gridDim.x * kBlock) {
+    TA dv[N];
+    ldu(d, u, dv);
+#pragma unroll
+    for (int e = 0; e < N; ++e) x[u * N + e] = x[u * N + e] + (double)dv[e];
+  }
+}
+
+__global__ void k_cycle_begin(double *H, int count, int *flag) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) H[i] = 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *flag = 0;
+}
+
+// Breakdown at Krylov step j (gDMR.cu:2256-2290): zero basis slots o+j..m and put ones on the
+// diagonal of the Hessenberg columns that were never filled (from column d+j; the reference
+// starts at column j, which overwrites filled columns when d > 0).
+template <typename TS>
+__global__ __launch_bounds__(kBlock) void k_breakdown_fixup(TS *P, long long ld, int o, int d,
+                                                            int m, double *H, const int *flag) {
+  const int j = *flag;
+  if (j == 0) return;
+  using TA = typename Acc<TS>::type;
+  constexpr int N = Unit<TS>::N;
+  const long long nvcol = ld / N;
+  const long long total = nvcol * (long long)(m + 1 - (o + j));
+  TS *base = P + (long long)(o + j) * ld;
+  TA z[N];
+#pragma unroll
+  for (int e = 0; e < N; ++e) z[e] = 0;
+  for (long long u = (long long)blockIdx.x * kBlock + threadIdx.x; u < total;
+       u += (long long)gridDim.x * kBlock)
+    stu(base, u, z);
+  if (blockIdx.x == 0)
+    for (int c = d + j + threadIdx.x; c < m; c += kBlock) H[c + (long long)c * (m + 1)] = 1.0;
+}
+
+// =============================================================================================
+// CSR SpMV, y = A x (replaces the user operator's dense cublasCgemv, testLS.cu:97-128).
+// Row blocks of <= 256 rows and <= kSpmvNnzCap nonzeros: the block streams its contiguous slice
+// of val / colInd with coalesced loads, forms the products in LDS, then each thread adds its row's
+// products in column order.  A row with more than kSpmvNnzCap nonzeros has a block of its own
+// and is reduced by the block.
+// =============================================================================================
+template <typename TV, typename TX, typename TA>
+__global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
+                                                 const int *__restrict__ ci,
+                                                 const TV *__restrict__ va,
+                                                 const int *__restrict__ br,
+                                                 const TX *__restrict__ x, TX *__restrict__ y) {
+  __shared__ TA prod[kSpmvNnzCap];
+  const int rs = br[blockIdx.x], re = br[blockIdx.x + 1];
+  const int e0 = rp[rs], e1 = rp[re];
+  const int nr = re - rs, ne = e1 - e0;
+  if (ne > kSpmvNnzCap) {
+    TA s = 0;
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlock) {
+      const TA t = (TA)va[e] * (TA)x[ci[e]];
+      s = s + t;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) y[rs] = (TX)s;
+    return;
+  }
+  for (int e = threadIdx.x; e < ne; e += kBlock) prod[e] = (TA)va[e0 + e] * (TA)x[ci[e0 + e]];
+  __syncthreads();
+  if ((int)threadIdx.x < nr) {
+    const int r = rs + threadIdx.x;
+    const int a = rp[r] - e0, b = rp[r + 1] - e0;
+    TA s = 0;
+    for (int e = a; e < b; ++e) s = s + prod[e];
+    y[r] = (TX)s;
+  }
+}
+
+// r = b - A x in FP64 with a fixed-order sum of squares (the refinement residual).  Block
+// partials are per SpMV row block.
+__global__ __launch_bounds__(kBlock) void k_resid64(const int *__restrict__ rp,
+                                                    const int *__restrict__ ci,
+                                                    const double *__restrict__ va,
+                                                    const int *__restrict__ br, int nb,
+                                                    const double *__restrict__ x,
+                                                    const double *__restrict__ b,
+                                                    double *__restrict__ r, double *partials,
+                                                    unsigned *counter, double *scal) {
+  __shared__ double prod[kSpmvNnzCap];
+  const int rs = br[blockIdx.x], re = br[blockIdx.x + 1];
+  const int e0 = rp[rs], e1 = rp[re];
+  const int nr = re - rs, ne = e1 - e0;
+  double ss = 0.0;
+  if (ne > kSpmvNnzCap) {
+    double s = 0.0;
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlock) {
+      const double t = va[e] * x[ci[e]];
+      s = s + t;
+    }
+    s = block_sum(s);
+    const double v = b[rs] - s;
+    if (threadIdx.x == 0) {
+      r[rs] = v;
+      ss = v * v;
+    }
+  } else {
+    for (int e = threadIdx.x; e < ne; e += kBlock) prod[e] = va[e0 + e] * x[ci[e0 + e]];
+    __syncthreads();
+    if ((int)threadIdx.x < nr) {
+      const int row = rs + threadIdx.x;
+      const int a = rp[row] - e0, bb = rp[row + 1] - e0;
+      double s = 0.0;
+      for (int e = a; e < bb; ++e) s = s + prod[e];
+      const double v = b[row] - s;
+      r[row] = v;
+      ss = v * v;
+    }
+  }
+  const double bs = block_sum(ss);
+  if (threadIdx.x == 0) put_partial(partials + blockIdx.x, bs);
+  if (!arrive_last(counter, (unsigned)nb)) return;
+  const double tot = sum_partials(partials, nb);
+  if (threadIdx.x == 0) {
+    scal[0] = tot;
+    scal[1] = 1.0 / sqrt(tot);
+    scal[2] = sqrt(tot);
+    *counter = 0u;
+  }
+}
+
+inline int grid_for(long long nv) {
+  long long g = (nv + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  if (g > 4096) g = 4096;
+  return (int)g;
+}
+
+}  // namespace
+
+// =============================================================================================
+// Launchers
+// =============================================================================================
+int spmv_partition(const int *rp, int n, int *br) {
+  int nb = 0;
+  int r = 0;
+  br[0] = 0;
+  while (r < n) {
+    const int start = r;
+    if (rp[r + 1] - rp[r] > kSpmvNnzCap) {
+      r = r + 1;
+    } else {
+      while (r < n && r - start < kBlock && rp[r + 1] - rp[start] <= kSpmvNnzCap) ++r;
+    }
+    br[++nb] = r;
+  }
+  return nb;
+}
+
+void spmv(const CsrDev &A, const double *x, double *y, hipStream_t s) {
+  if (A.nBlocks == 0) return;
+  hipLaunchKernelGGL((k_spmv<double, double, double>), dim3(A.nBlocks), dim3(kBlock), 0, s,
+                     A.rowPtr, A.colInd, A.val, A.blockRows, x, y);
+}
+
+void spmv32(const CsrDev &A, const float *x, float *y, hipStream_t s) {
+  if (A.nBlocks == 0) return;
+  hipLaunchKernelGGL((k_spmv<float, float, float>), dim3(A.nBlocks), dim3(kBlock), 0, s,
+                     A.rowPtr, A.colInd, A.val32, A.blockRows, x, y);
+}
+
+void residual64(const CsrDev &A, const double *x, const double *b, double *r, RedWork w,
+                double *scal, hipStream_t s) {
+  if (A.nBlocks == 0) return;
+  // Partials beyond the kMaxPartials slots of the reduction workspace are not needed: the
+  // caller sizes the workspace for max(nBlocks, kMaxPartials).
+  hipLaunchKernelGGL(k_resid64, dim3(A.nBlocks), dim3(kBlock), 0, s, A.rowPtr, A.colInd, A.val,
+                     A.blockRows, A.nBlocks, x, b, r, (double *)w.partials, w.counters, scal);
+}
+
+template <typename TS>
+void fill(TS *p, long long count, double v, hipStream_t s) {
+  const long long nv = units<TS>(count);
+  hipLaunchKernelGGL((k_fill<TS>), dim3(grid_for(nv)), dim3(kBlock), 0, s, p, nv, v);
+}
+
+void fill_int(int *p, long long count, int v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_for(count)), dim3(kBlock), 0, s, p, count, v);
+}
+
+template <typename TS>
+void multidot(const TS *B, long long ld, int p, const TS *x, long long n, double *out, int ostride,
+              RedWork w, const int *flag, hipStream_t s) {
+  if (p <= 0) return;
+  using TA = typename Acc<TS>::type;
+  const int G = red_blocks<TS>(n);
+  const int groups = (p + kDotCols - 1) / kDotCols;
+  hipLaunchKernelGGL((k_multidot<TS>), dim3(G, groups), dim3(kBlock), 0, s, B, ld, p, x,
+                     units<TS>(n), G, (TA *)w.partials, w.counters, out, ostride, flag);
+}
+
+template <typename TS>
+void combo(int mode, const TS *in, TS *out, const TS *B1, int p1, const double *c1, const TS *B2,
+           int p2, const double *c2, long long ld, long long n, double beta, int red, const TS *g,
+           const Fin &fin, RedWork w, const int *flag, hipStream_t s) {
+  using TA = typename Acc<TS>::type;
+  const int G = red_blocks<TS>(n);
+  const long long nv = units<TS>(n);
+  TA *part = (TA *)w.partials;
+  unsigned *cnt = w.counters;
+#define DMR_COMBO(M, R)                                                                         \
+  hipLaunchKernelGGL((k_combo<TS, M, R>), dim3(G), dim3(kBlock), 0, s, in, out, B1, p1, c1, B2, \
+                     p2, c2, ld, nv, beta, g, G, part, cnt, fin, flag)
+  if (mode == COMB_SUB) {
+    switch (red) {
+      case 0: DMR_COMBO(COMB_SUB, 0); break;
+      case 1: DMR_COMBO(COMB_SUB, 1); break;
+      default: DMR_COMBO(COMB_SUB, 3); break;
+    }
+  } else {
+    switch (red) {
+      case 0: DMR_COMBO(COMB_ACC, 0); break;
+      case 1: DMR_COMBO(COMB_ACC, 1); break;
+      default: DMR_COMBO(COMB_ACC, 3); break;
+    }
+  }
+#undef DMR_COMBO
+}
+
+template <typename TS>
+void combo_multi(TS *out, int J, const TS *B1, int p1, const double *C1, const TS *B2, int p2,
+                 const double *C2, int ldc, long long ld, long long n, double *scal, RedWork w,
+                 hipStream_t s) {
+  using TA = typename Acc<TS>::type;
+  constexpr int JB = 64 / Unit<TS>::N;  // accumulator registers per thread stay at 64 values
+  const int G = red_blocks<TS>(n);
+  const int groups = (J + JB - 1) / JB;
+  const size_t lds = sizeof(TA) * (size_t)(p1 + p2) * JB;
+  hipLaunchKernelGGL((k_combo_multi<TS, JB>), dim3(G, groups), dim3(kBlock), lds, s, out, J, B1,
+                     p1, C1, B2, p2, C2, ldc, ld, units<TS>(n), G, (TA *)w.partials, w.counters,
+                     scal);
+}
+
+template <typename TS>
+void scale(TS *v, long long n, const double *scal, const int *flag, hipStream_t s) {
+  const long long nv = units<TS>(n);
+  hipLaunchKernelGGL((k_scale<TS>), dim3(grid_for(nv)), dim3(kBlock), 0, s, v, nv, scal, flag);
+}
+
+template <typename TS>
+void scale_cols(TS *v, int J, long long ld, long long n, const double *scal, hipStream_t s) {
+  if (J <= 0) return;
+  const long long nv = units<TS>(n);
+  hipLaunchKernelGGL((k_scale_cols<TS>), dim3(grid_for(nv), J), dim3(kBlock), 0, s, v, ld, nv,
+                     scal);
+}
+
+template <typename TD, typename TSrc>
+void convert(TD *dst, const TSrc *src, long long n, hipStream_t s) {
+  hipLaunchKernelGGL((k_convert<TD, TSrc>), dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, n);
+}
+
+template <typename TS>
+void add_to(double *x, const TS *d, long long n, hipStream_t s) {
+  const long long nv = units<TS>(n);
+  hipLaunchKernelGGL((k_add_to<TS>), dim3(grid_for(nv)), dim3(kBlock), 0, s, x, d, nv);
+}
+
+void cycle_begin(double *H, int m, int *flag, hipStream_t s) {
+  const int count = (m + 1) * m;
+  hipLaunchKernelGGL(k_cycle_begin, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s, H,
+                     count, flag);
+}
+
+template <typename TS>
+void breakdown_fixup(TS *P, long long ld, long long n, int o, int d, int m, double *H,
+                     const int *flag, hipStream_t s) {
+  (void)n;
+  const long long nv = units<TS>(ld) * (m + 1);
+  hipLaunchKernelGGL((k_breakdown_fixup<TS>), dim3(grid_for(nv)), dim3(kBlock), 0, s, P, ld, o, d,
+                     m, H, flag);
+}
+
+// ---- explicit instantiations ----
+#define DMR_INST(TS)                                                                              \
+  template void fill<TS>(TS *, long long, double, hipStream_t);                                   \
+  template void multidot<TS>(const TS *, long long, int, const TS *, long long, double *, int,    \
+                             RedWork, const int *, hipStream_t);                                  \
+  template void combo<TS>(int, const TS *, TS *, const TS *, int, const double *, const TS *, int, \
+                          const double *, long long, long long, double, int, const TS *,          \
+                          const Fin &, RedWork, const int *, hipStream_t);                        \
+  template void combo_multi<TS>(TS *, int, const TS *, int, const double *, const TS *, int,      \
+                                const double *, int, long long, long long, double *, RedWork,     \
+                                hipStream_t);                                                     \
+  template void scale<TS>(TS *, long long, const double *, const int *, hipStream_t);             \
+  template void scale_cols<TS>(TS *, int, long long, long long, const double *, hipStream_t);     \
+  template void add_to<TS>(double *, const TS *, long long, hipStream_t);                         \
+  template void breakdown_fixup<TS>(TS *, long long, long long, int, int, int, double *,          \
+                                    const int *, hipStream_t);
+DMR_INST(double)
+DMR_INST(float)
+DMR_INST(bf16)
+#undef DMR_INST
+template void convert<float, double>(float *, const double *, long long, hipStream_t);
+template void convert<double, float>(double *, const float *, long long, hipStream_t);
+template void convert<bf16, double>(bf16 *, const double *, long long, hipStream_t);
+template void convert<double, bf16>(double *, const bf16 *, long long, hipStream_t);
+
+}  // namespace dmr

@@ -1,0 +1,1032 @@
+// Host driver of the DMR solver: contexts, the restart cycle, the FP64 solve, the
+// mixed-precision refinement solve, and the C ABI of gDMR.h.
+//
+// The cycle follows the reference solver (seanMolesky/gDMR-solver gDMR.cu) step for step:
+//   solverDMR        gDMR.cu:2638-2901  -> solve_fp64 / run_cycles (restart loop, guards)
+//   arnoldiSetupDMR  gDMR.cu:2586-2632  -> start-vector choice incl. the stagnation rule
+//   arnoldiDMR       gDMR.cu:1790-2332  -> deflation-image QR of A Z, then CGS Krylov steps
+//   deflatorDMR      gDMR.cu:1710-1787  -> projection against W before the Krylov CGS
+//   updateTrgCoeffs  gDMR.cu:2334-2408  -> t = P^T r (full projection, not beta e1)
+//   updateAppResDMR  gDMR.cu:2411-2582  -> LSQ, residual and solution updates, stagnation flag
+//   genDeflatDMR     gDMR.cu:1583-1704  -> Z <- normalize([V Z] * smallest right sing. vectors)
+// on one device with the basis stored contiguously: P holds m+1 columns, W = P[0..k-1] when the
+// cycle is deflated, and the Krylov vectors follow at slot o = (d > 0 ? k : 0).  The
+// multi-device coefficient reordering (reorderHessCoeffs) and partial-vector sums are identities
+// on one device and do not exist here.  Every step is enqueued on the caller's stream; the host
+// synchronises once per cycle to read the residual ratio that drives the restart decisions.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dmr_internal.h"
+#include "gDMR.h"
+
+namespace {
+
+thread_local int g_err = DMR_OK;
+thread_local std::string g_errmsg;
+thread_local bool g_op_failed = false;
+thread_local std::string g_op_msg;
+
+int set_err(int code, const std::string &msg) {
+  g_err = code;
+  g_errmsg = msg;
+  return code;
+}
+void clear_err() {
+  g_err = DMR_OK;
+  g_errmsg.clear();
+}
+
+#define HIPCHK(call)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return set_err(DMR_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+// Fixed-order reductions of the FP64 path use the tolerance below for the Jacobi SVD: the FP64
+// analogue of the reference's FP32 gesvdj tolerance (gDMR.cu:88), at most 128 sweeps
+// (gDMR.cu:89).  The stagnation threshold 0.995 is gDMR.cu:2506.
+constexpr double kSvdTol = 1e-14;
+constexpr int kSvdMaxSweeps = 128;
+constexpr double kStagnation = 0.995;
+
+long long pad_ld(long long n) { return (n + 255) / 256 * 256; }
+
+template <typename TS> struct Work {
+  TS *P = nullptr, *Z = nullptr, *Zn = nullptr, *r = nullptr, *xa = nullptr, *g = nullptr,
+     *u = nullptr;
+  bool hasZ = false;
+};
+
+}  // namespace
+
+enum OpKind { OP_NONE = 0, OP_CSR = 1, OP_FN = 2, OP_LEGACY = 3 };
+
+struct dmrContext {
+  int device = 0;
+  int n = 0, m = 0, k = 0, prec = DMR_PREC_FP64;
+  long long ld = 0;
+  // operator
+  int opKind = OP_NONE;
+  dmr::CsrDev csr;
+  int *ownedBlockRows = nullptr;
+  float *ownedVal32 = nullptr;
+  dmrOperatorFn opFn = nullptr;
+  void *opCtx = nullptr;
+  void (*legacy)(void) = nullptr;
+  double *ext = nullptr;  // the caller's source vector during a legacy-operator solve
+  // memory
+  void *bigPool = nullptr;
+  void *smallPool = nullptr;
+  double *hostStatus = nullptr;  // pinned
+  Work<double> w64;
+  Work<float> w32;
+  double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr;  // refinement vectors (mixed)
+  double *H = nullptr, *t = nullptr, *y = nullptr, *z = nullptr, *C = nullptr, *sig = nullptr,
+         *cv = nullptr, *scal = nullptr, *scal2 = nullptr, *scalZ = nullptr, *one = nullptr;
+  int *flag = nullptr, *sweeps = nullptr;
+  dmr::RedWork red{};
+  dmr::RedWork redOp{};  // sized to the SpMV block count of the operator
+  void *redOpPool = nullptr;
+  // persistent state (kryFlagCountDMR persists across solves, gDMR.cu:890)
+  int kryLoopFlag = 0;
+  int kryFlagCount = 0;
+  // refinement settings
+  double innerTol = 1e-5;
+  int maxOuter = 60;
+  // statistics of the last solve
+  dmrStats st{};
+  std::vector<double> hist;
+  // per-phase HIP-event profile of the restart cycles (off unless requested)
+  struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> phase;
+    std::vector<double> bytes;
+    int used = 0;
+    double ms[DMR_NPHASES] = {};
+    double by[DMR_NPHASES] = {};
+    long long cnt[DMR_NPHASES] = {};
+  } prof;
+};
+
+namespace {
+
+size_t footprint(int n, int m, int k, int prec) {
+  const long long ld = pad_ld(n);
+  size_t big = sizeof(double) * (size_t)ld * (size_t)(m + 1 + 2 * k + 4);
+  if (prec != DMR_PREC_FP64) {
+    big += sizeof(float) * (size_t)ld * (size_t)(m + 1 + 2 * k + 4);
+    big += sizeof(double) * (size_t)ld * 3;
+  }
+  const size_t small = sizeof(double) * ((size_t)(m + 1) * m * 2 + 8 * (m + 2) + 64 + 2 * k) +
+                       sizeof(double) * dmr::kMaxPartials + sizeof(unsigned) * dmr::kCounters +
+                       4096;
+  return big + small;
+}
+
+int ctx_alloc(dmrContext *c) {
+  const long long ld = c->ld;
+  const int n = c->n, m = c->m, k = c->k;
+  (void)n;
+  size_t big = sizeof(double) * (size_t)ld * (size_t)(m + 1 + 2 * k + 4);
+  size_t big32 = 0;
+  if (c->prec != DMR_PREC_FP64) {
+    big32 = sizeof(float) * (size_t)ld * (size_t)(m + 1 + 2 * k + 4) + sizeof(double) * ld * 3;
+    big = 0;  // the FP64 Krylov basis is not needed by a mixed-precision context
+  }
+  const size_t total = big + big32;
+  HIPCHK(hipMalloc(&c->bigPool, total));
+  HIPCHK(hipMemset(c->bigPool, 0, total));
+  char *p = (char *)c->bigPool;
+  if (c->prec == DMR_PREC_FP64) {
+    double *d = (double *)p;
+    c->w64.P = d;
+    d += ld * (m + 1);
+    c->w64.Z = d;
+    d += ld * k;
+    c->w64.Zn = d;
+    d += ld * k;
+    c->w64.r = d;
+    d += ld;
+    c->w64.xa = d;
+    d += ld;
+    c->w64.g = d;
+    d += ld;
+    c->w64.u = d;
+    d += ld;
+  } else {
+    double *d = (double *)p;
+    c->x64 = d;
+    d += ld;
+    c->b64 = d;
+    d += ld;
+    c->r64 = d;
+    d += ld;
+    float *f = (float *)d;
+    c->w32.P = f;
+    f += ld * (m + 1);
+    c->w32.Z = f;
+    f += ld * k;
+    c->w32.Zn = f;
+    f += ld * k;
+    c->w32.r = f;
+    f += ld;
+    c->w32.xa = f;
+    f += ld;
+    c->w32.g = f;
+    f += ld;
+    c->w32.u = f;
+    f += ld;
+  }
+  // small arrays
+  const size_t nsmall = (size_t)(m + 1) * m   // H
+                        + (m + 2) * 4         // t, y, z, cv
+                        + (size_t)m * (k + 1) // C
+                        + (m + 2)             // sig
+                        + 16 + 16 + 64 + 8;   // scal, scal2, scalZ, one
+  const size_t smallBytes = sizeof(double) * (nsmall + dmr::kMaxPartials) +
+                            sizeof(unsigned) * dmr::kCounters + sizeof(int) * 64;
+  HIPCHK(hipMalloc(&c->smallPool, smallBytes));
+  HIPCHK(hipMemset(c->smallPool, 0, smallBytes));
+  double *s = (double *)c->smallPool;
+  c->H = s;
+  s += (size_t)(m + 1) * m;
+  c->t = s;
+  s += m + 2;
+  c->y = s;
+  s += m + 2;
+  c->z = s;
+  s += m + 2;
+  c->cv = s;
+  s += m + 2;
+  c->C = s;
+  s += (size_t)m * (k + 1);
+  c->sig = s;
+  s += m + 2;
+  c->scal = s;
+  s += 16;
+  c->scal2 = s;
+  s += 16;
+  c->scalZ = s;
+  s += 64;
+  c->one = s;
+  s += 8;
+  c->red.partials = s;
+  s += dmr::kMaxPartials;
+  c->red.counters = (unsigned *)s;
+  int *ip = (int *)(c->red.counters + dmr::kCounters);
+  c->flag = ip;
+  c->sweeps = ip + 1;
+  const double one = 1.0;
+  HIPCHK(hipMemcpy(c->one, &one, sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipHostMalloc((void **)&c->hostStatus, 64 * sizeof(double), hipHostMallocDefault));
+  return DMR_OK;
+}
+
+void ctx_free(dmrContext *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipDeviceSynchronize();
+  if (c->bigPool) hipFree(c->bigPool);
+  if (c->smallPool) hipFree(c->smallPool);
+  if (c->redOpPool) hipFree(c->redOpPool);
+  for (auto e : c->prof.ev) hipEventDestroy(e);
+  if (c->ownedBlockRows) hipFree(c->ownedBlockRows);
+  if (c->ownedVal32) hipFree(c->ownedVal32);
+  if (c->hostStatus) hipHostFree(c->hostStatus);
+  delete c;
+}
+
+int validate_shape(int n, int m, int k) {
+  if (n < 1) return set_err(DMR_ERR_ARG, "elements must be >= 1");
+  if (m < 1 || m > 120) return set_err(DMR_ERR_ARG, "basisDim must be in [1, 120]");
+  if (k < 0 || k >= m) return set_err(DMR_ERR_ARG, "deflatDim must be in [0, basisDim)");
+  if (k > 64) return set_err(DMR_ERR_ARG, "deflatDim must be <= 64");
+  return DMR_OK;
+}
+
+// ---- operator application on a typed basis ----
+struct Op64 {
+  dmrContext *c;
+  hipStream_t s;
+  int operator()(const double *src, double *dst) const {
+    switch (c->opKind) {
+      case OP_CSR:
+        dmr::spmv(c->csr, src, dst, s);
+        return DMR_OK;
+      case OP_FN:
+        c->opFn(src, dst, s, c->opCtx);
+        break;
+      case OP_LEGACY:
+        // In-place callback on the caller's source vector (testLS.cu:97-128 contract).
+        if (hipMemcpyAsync(c->ext, src, sizeof(double) * c->n, hipMemcpyDeviceToDevice, s) !=
+            hipSuccess)
+          return set_err(DMR_ERR_HIP, "copy to operator vector failed");
+        c->legacy();
+        if (hipMemcpyAsync(dst, c->ext, sizeof(double) * c->n, hipMemcpyDeviceToDevice, s) !=
+            hipSuccess)
+          return set_err(DMR_ERR_HIP, "copy from operator vector failed");
+        break;
+      default:
+        return set_err(DMR_ERR_STATE, "no operator set");
+    }
+    if (g_op_failed) {
+      g_op_failed = false;
+      return set_err(DMR_ERR_OPERATOR, "operator reported failure: " + g_op_msg);
+    }
+    return DMR_OK;
+  }
+};
+
+struct Op32 {
+  dmrContext *c;
+  hipStream_t s;
+  int operator()(const float *src, float *dst) const {
+    dmr::spmv32(c->csr, src, dst, s);
+    return DMR_OK;
+  }
+};
+
+template <typename TS> Work<TS> &work_of(dmrContext *c);
+template <> Work<double> &work_of<double>(dmrContext *c) { return c->w64; }
+template <> Work<float> &work_of<float>(dmrContext *c) { return c->w32; }
+
+dmr::Fin fin_none() { return dmr::Fin{dmr::FIN_NONE, nullptr, nullptr, 0.0, 0, nullptr}; }
+
+// Runs f (which enqueues kernels on s) bracketed by HIP events when profiling is on; the elapsed
+// times are folded into the per-phase totals by prof_collect after the stream is synchronised.
+template <typename F>
+int timed(dmrContext *c, hipStream_t s, int phase, double bytes, F &&f) {
+  auto &p = c->prof;
+  if (!p.on) return f();
+  if ((size_t)(2 * p.used + 2) > p.ev.size()) {
+    for (int i = 0; i < 64; ++i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return set_err(DMR_ERR_HIP, "hipEventCreate failed");
+      p.ev.push_back(e);
+    }
+  }
+  if (hipEventRecord(p.ev[2 * p.used], s) != hipSuccess)
+    return set_err(DMR_ERR_HIP, "hipEventRecord failed");
+  const int rc = f();
+  if (hipEventRecord(p.ev[2 * p.used + 1], s) != hipSuccess)
+    return set_err(DMR_ERR_HIP, "hipEventRecord failed");
+  if ((int)p.phase.size() <= p.used) {
+    p.phase.resize(p.used + 1);
+    p.bytes.resize(p.used + 1);
+  }
+  p.phase[p.used] = phase;
+  p.bytes[p.used] = bytes;
+  ++p.used;
+  return rc;
+}
+
+void prof_collect(dmrContext *c) {
+  auto &p = c->prof;
+  for (int i = 0; i < p.used; ++i) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.ev[2 * i], p.ev[2 * i + 1]) == hipSuccess) {
+      p.ms[p.phase[i]] += ms;
+      p.by[p.phase[i]] += p.bytes[i];
+      ++p.cnt[p.phase[i]];
+    }
+  }
+  p.used = 0;
+}
+
+// The restart loop of solverDMR (gDMR.cu:2814-2874) from a normalised residual in w.r, the
+// current approximation in w.xa, tracked residual norm rho and ||b|| = beta.  Returns DMR_OK on
+// convergence, DMR_ERR_NOCONV / DMR_ERR_RUNAWAY on the reference's failure exits, or an error.
+template <typename TS, typename Op>
+int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double tol, int matRank,
+               double beta, double &rho, int &grdItr, double histScale) {
+  Work<TS> &w = work_of<TS>(c);
+  const long long n = c->n, ld = c->ld;
+  const int m = c->m, k = c->k;
+  const dmr::RedWork red = c->red;
+  // Algorithmic bytes: one length-n vector pass, and one SpMV (CSR, int32 indices).
+  const double vb = (double)sizeof(TS) * (double)n;
+  const double spmvBytes = (double)c->csr.nnz * (double)(sizeof(TS) + 4) + 4.0 * (n + 1) +
+                           2.0 * (double)sizeof(TS) * (double)n;
+  while (rho > tol * beta) {
+    if (grdItr > matRank) return DMR_ERR_NOCONV;
+    if (rho / beta > 1.0) return DMR_ERR_RUNAWAY;
+    const int d = (grdItr == 0 && startDef == 0) ? 0 : k;
+    const int o = d > 0 ? k : 0;
+    // ---- arnoldiSetupDMR (gDMR.cu:2586-2632).  The period 2*matRank/(m-k) divides by zero in
+    // the reference when it is 0; a zero period here means "always restart from r".
+    const int period = (m - k) > 0 ? (2 * matRank) / (m - k) : 0;
+    const bool fromResidual =
+        c->kryLoopFlag == 0 || period == 0 || (c->kryFlagCount % period) == 0;
+    TS *Po = w.P + (long long)o * ld;
+    if (fromResidual) {
+      if (hipMemcpyAsync(Po, w.r, sizeof(TS) * ld, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return set_err(DMR_ERR_HIP, "setup copy failed");
+    } else {
+      const TS *src = w.P + (long long)(o + (m - k) / 2) * ld;
+      if (src != Po &&
+          hipMemcpyAsync(Po, src, sizeof(TS) * ld, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return set_err(DMR_ERR_HIP, "setup copy failed");
+    }
+    // ---- arnoldiDMR (gDMR.cu:1790-2332)
+    dmr::cycle_begin(c->H, m, c->flag, s);
+    // Deflation image: A Z = W R by CGS (gDMR.cu:1823-2031).
+    for (int i = 0; i < d; ++i) {
+      int rc = timed(c, s, DMR_PH_SPMV, spmvBytes, [&] { return op(w.Z + (long long)i * ld, w.g); });
+      if (rc) return rc;
+      ++c->st.matvecs;
+      double *hcol = c->H + (long long)i * (m + 1);
+      if (i > 0)
+        timed(c, s, DMR_PH_GEMVT, vb * (i + 1), [&] {
+          dmr::multidot<TS>(w.P, ld, i, w.g, n, hcol, 1, red, nullptr, s);
+          return 0;
+        });
+      dmr::Fin fin{dmr::FIN_ARNOLDI, c->scal, hcol + i, 0.0, 0, nullptr};
+      timed(c, s, DMR_PH_UPDATE, vb * (i + 3), [&] {
+        dmr::combo<TS>(dmr::COMB_SUB, w.g, w.P + (long long)i * ld, w.P, i, hcol, nullptr, 0,
+                       nullptr, ld, n, 0.0, 3, w.g, fin, red, nullptr, s);
+        return 0;
+      });
+      timed(c, s, DMR_PH_SCALE, vb * 2, [&] {
+        dmr::scale<TS>(w.P + (long long)i * ld, n, c->scal + 1, nullptr, s);
+        return 0;
+      });
+    }
+    // Start vector orthogonalised against W (deflatorDMR(0, 0), gDMR.cu:2033-2060).
+    if (d > 0) {
+      timed(c, s, DMR_PH_GEMVT, vb * (k + 1), [&] {
+        dmr::multidot<TS>(w.P, ld, k, Po, n, c->cv, 1, red, nullptr, s);
+        return 0;
+      });
+      dmr::Fin fin{dmr::FIN_NORM, c->scal, nullptr, 0.0, 0, nullptr};
+      timed(c, s, DMR_PH_UPDATE, vb * (k + 2), [&] {
+        dmr::combo<TS>(dmr::COMB_SUB, Po, Po, w.P, k, c->cv, nullptr, 0, nullptr, ld, n, 0.0, 1,
+                       nullptr, fin, red, nullptr, s);
+        return 0;
+      });
+      timed(c, s, DMR_PH_SCALE, vb * 2, [&] {
+        dmr::scale<TS>(Po, n, c->scal + 1, nullptr, s);
+        return 0;
+      });
+    }
+    // Krylov steps (gDMR.cu:2069-2328).
+    for (int j = 1; j <= m - d; ++j) {
+      int rc = timed(c, s, DMR_PH_SPMV, spmvBytes,
+                     [&] { return op(w.P + (long long)(o + j - 1) * ld, w.g); });
+      if (rc) return rc;
+      ++c->st.matvecs;
+      double *hcol = c->H + (long long)(d + j - 1) * (m + 1);
+      const TS *src = w.g;
+      if (d > 0) {
+        // deflatorDMR(1, j): c = W^T g into H[0:k, d+j-1], u = g - W c.
+        timed(c, s, DMR_PH_GEMVT, vb * (k + 1), [&] {
+          dmr::multidot<TS>(w.P, ld, k, w.g, n, hcol, 1, red, c->flag, s);
+          return 0;
+        });
+        timed(c, s, DMR_PH_UPDATE, vb * (k + 2), [&] {
+          dmr::combo<TS>(dmr::COMB_SUB, w.g, w.u, w.P, k, hcol, nullptr, 0, nullptr, ld, n, 0.0,
+                         0, nullptr, fin_none(), red, c->flag, s);
+          return 0;
+        });
+        src = w.u;
+      }
+      // Classical Gram-Schmidt against V_0..V_{j-1}: h into H[d:d+j, d+j-1].
+      timed(c, s, DMR_PH_GEMVT, vb * (j + 1), [&] {
+        dmr::multidot<TS>(Po, ld, j, src, n, hcol + d, 1, red, c->flag, s);
+        return 0;
+      });
+      TS *vj = w.P + (long long)(o + j) * ld;
+      dmr::Fin fin{dmr::FIN_ARNOLDI, c->scal, hcol + d + j, tol, j, c->flag};
+      timed(c, s, DMR_PH_UPDATE, vb * (j + 3), [&] {
+        dmr::combo<TS>(dmr::COMB_SUB, src, vj, Po, j, hcol + d, nullptr, 0, nullptr, ld, n, 0.0,
+                       3, w.g, fin, red, c->flag, s);
+        return 0;
+      });
+      timed(c, s, DMR_PH_SCALE, vb * 2, [&] {
+        dmr::scale<TS>(vj, n, c->scal + 1, c->flag, s);
+        return 0;
+      });
+    }
+    dmr::breakdown_fixup<TS>(w.P, ld, n, o, d, m, c->H, c->flag, s);
+    // ---- updateAppResDMR (gDMR.cu:2411-2582)
+    timed(c, s, DMR_PH_GEMVT, vb * (m + 2), [&] {
+      dmr::multidot<TS>(w.P, ld, m + 1, w.r, n, c->t, 1, red, nullptr, s);
+      return 0;
+    });
+    timed(c, s, DMR_PH_LSQ, 0.0, [&] {
+      dmr::lsq(c->H, m, c->t, c->y, c->z, s);
+      return 0;
+    });
+    {
+      // r <- r - ([V] z[d..m] + [W] z[0..d-1]), q = ||r||.
+      dmr::Fin fin{dmr::FIN_NORM, c->scal2, nullptr, 0.0, 0, nullptr};
+      timed(c, s, DMR_PH_COMBO, vb * (m + 3), [&] {
+        dmr::combo<TS>(dmr::COMB_ACC, w.r, w.r, Po, m + 1 - o, c->z + d, w.P, d, c->z, ld, n,
+                       -1.0, 1, nullptr, fin, red, nullptr, s);
+        return 0;
+      });
+      timed(c, s, DMR_PH_SCALE, vb * 2, [&] {
+        dmr::scale<TS>(w.r, n, c->scal2 + 1, nullptr, s);
+        return 0;
+      });
+      // x_a <- x_a + rho * ([V] y[d..m-1] + [Z] y[0..d-1]).
+      timed(c, s, DMR_PH_COMBO, vb * (m + 2), [&] {
+        dmr::combo<TS>(dmr::COMB_ACC, w.xa, w.xa, Po, m - d, c->y + d, w.Z, d, c->y, ld, n, rho,
+                       0, nullptr, fin_none(), red, nullptr, s);
+        return 0;
+      });
+    }
+    // ---- genDeflatDMR (gDMR.cu:1583-1704): Z <- normalize([V Z] C), C = smallest right
+    // singular vectors of H.
+    if (k > 0) {
+      timed(c, s, DMR_PH_SVD, 0.0, [&] {
+        dmr::svd_smallest(c->H, m, k, kSvdTol, kSvdMaxSweeps, c->C, c->sig, c->sweeps, s);
+        return 0;
+      });
+      timed(c, s, DMR_PH_ZGEMM, vb * (m + k) + vb * 2 * k, [&] {
+        dmr::combo_multi<TS>(w.Zn, k, Po, m - d, c->C + d, w.Z, d, c->C, m, ld, n, c->scalZ, red,
+                             s);
+        dmr::scale_cols<TS>(w.Zn, k, ld, n, c->scalZ, s);
+        return 0;
+      });
+      std::swap(w.Z, w.Zn);
+      w.hasZ = true;
+    }
+    if (hipMemcpyAsync(c->hostStatus, c->scal2, 4 * sizeof(double), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(c->hostStatus + 8, c->flag, sizeof(int), hipMemcpyDeviceToHost, s) !=
+            hipSuccess)
+      return set_err(DMR_ERR_HIP, "status readback failed");
+    if (hipStreamSynchronize(s) != hipSuccess) return set_err(DMR_ERR_HIP, "stream sync failed");
+    if (c->prof.on) prof_collect(c);
+    const double q = c->hostStatus[2];
+    int bflag;
+    std::memcpy(&bflag, c->hostStatus + 8, sizeof(int));
+    if (bflag) ++c->st.breakdowns;
+    if (q > kStagnation) {
+      c->kryLoopFlag = 1;
+      ++c->kryFlagCount;
+    } else {
+      c->kryLoopFlag = 0;
+    }
+    rho = q * rho;
+    c->hist.push_back(rho / beta * histScale);
+    ++c->st.cycles;
+    ++grdItr;
+  }
+  return DMR_OK;
+}
+
+double solve_fp64(dmrContext *c, int mode, double *x, const double *b, hipStream_t s, double tol,
+                  int matRank, int *numIts) {
+  Work<double> &w = c->w64;
+  const long long n = c->n, ld = c->ld;
+  const int k = c->k;
+  *numIts = matRank * matRank;  // gDMR.cu:2646
+  const int startDef = (mode == 0) ? 0 : k;
+  if (startDef > 0 && !w.hasZ) {
+    set_err(DMR_ERR_STATE, "mode != 0 requires a deflation subspace (previous solve or impSVDMR)");
+    return 1.0;
+  }
+  c->kryLoopFlag = 0;
+  const dmr::RedWork red = c->red;
+  Op64 op{c, s};
+  if (c->opKind == OP_LEGACY) c->ext = x;
+  // r = b, beta = ||b|| (gDMR.cu:2681-2707); x_a = x0 (gDMR.cu:2709).
+  if (hipMemcpyAsync(w.r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(w.xa, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+    set_err(DMR_ERR_HIP, "input copy failed");
+    return 1.0;
+  }
+  dmr::multidot<double>(w.r, ld, 1, w.r, n, c->scal2 + 4, 1, red, nullptr, s);
+  // A x0 through the operator; the legacy callback works in place on x (gDMR.cu:2732).
+  int rc;
+  if (c->opKind == OP_LEGACY) {
+    c->legacy();
+    rc = DMR_OK;
+    if (g_op_failed) {
+      g_op_failed = false;
+      rc = set_err(DMR_ERR_OPERATOR, "operator reported failure: " + g_op_msg);
+    } else if (hipMemcpyAsync(w.g, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s) !=
+               hipSuccess) {
+      rc = set_err(DMR_ERR_HIP, "copy from operator vector failed");
+    }
+  } else {
+    rc = op(x, w.g);
+  }
+  if (rc) return 1.0;
+  ++c->st.matvecs;
+  // r = r - A x0, rho = ||r|| (gDMR.cu:2748-2783).
+  {
+    dmr::Fin fin{dmr::FIN_NORM, c->scal, nullptr, 0.0, 0, nullptr};
+    dmr::combo<double>(dmr::COMB_SUB, w.r, w.r, w.g, 1, c->one, nullptr, 0, nullptr, ld, n, 0.0,
+                       1, nullptr, fin, red, nullptr, s);
+  }
+  if (hipMemcpyAsync(c->hostStatus, c->scal, 3 * sizeof(double), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipMemcpyAsync(c->hostStatus + 4, c->scal2 + 4, sizeof(double), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_err(DMR_ERR_HIP, "setup readback failed");
+    return 1.0;
+  }
+  const double beta = std::sqrt(c->hostStatus[4]);
+  double rho = std::sqrt(c->hostStatus[0]);
+  c->st.beta = beta;
+  if (rho < beta * tol) {
+    // Initial guess already within tolerance (gDMR.cu:2785-2796): the reference returns the
+    // absolute residual norm here and leaves numIts at matRank^2.
+    hipMemcpyAsync(x, w.xa, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+    hipStreamSynchronize(s);
+    c->st.status = -1;
+    c->st.rho = rho;
+    return rho;
+  }
+  dmr::scale<double>(w.r, n, c->scal + 1, nullptr, s);
+  int grdItr = 0;
+  rc = run_cycles<double>(c, op, s, startDef, tol, matRank, beta, rho, grdItr, 1.0);
+  c->st.rho = rho;
+  if (rc == DMR_ERR_NOCONV || rc == DMR_ERR_RUNAWAY) {
+    *numIts = grdItr;
+    c->st.status = rc;
+    set_err(rc, rc == DMR_ERR_NOCONV ? "cycle limit (matRank) reached" : "residual runaway");
+    return rho / beta;
+  }
+  if (rc) return rho / beta;
+  *numIts = grdItr - 1;  // gDMR.cu:2876
+  c->st.status = 0;
+  // Export the solution (gDMR.cu:2885).
+  if (hipMemcpyAsync(x, w.xa, sizeof(double) * n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_err(DMR_ERR_HIP, "solution export failed");
+  }
+  return rho / beta;
+}
+
+// Mixed precision: FP64 iterative refinement around an inner DMR whose Krylov basis, deflation
+// vectors and SpMV are FP32 with FP32-accumulated reductions; the Hessenberg LSQ / QR / SVD stay
+// FP64.  Each outer step computes r = b - A x with the FP64 operator, solves A d = r with the
+// inner DMR to a relative tolerance max(innerTol, 0.5 * tol * ||b|| / ||r||) starting from the
+// recycled FP32 subspace, and adds d to x.  Returns the FP64-checked ||b - A x|| / ||b||.
+double solve_mixed(dmrContext *c, int mode, double *x, const double *b, hipStream_t s, double tol,
+                   int matRank, int *numIts) {
+  Work<float> &w = c->w32;
+  const long long n = c->n, ld = c->ld;
+  const int k = c->k;
+  *numIts = 0;
+  if (mode != 0 && !w.hasZ) {
+    set_err(DMR_ERR_STATE, "mode != 0 requires a deflation subspace (previous solve or impSVDMR)");
+    return 1.0;
+  }
+  if (c->opKind != OP_CSR) {
+    set_err(DMR_ERR_STATE, "mixed precision needs the built-in CSR operator");
+    return 1.0;
+  }
+  const dmr::RedWork red = c->red;
+  if (hipMemcpyAsync(c->b64, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(c->x64, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+    set_err(DMR_ERR_HIP, "input copy failed");
+    return 1.0;
+  }
+  dmr::multidot<double>(c->b64, ld, 1, c->b64, n, c->scal2 + 4, 1, red, nullptr, s);
+  if (hipMemcpyAsync(c->hostStatus + 4, c->scal2 + 4, sizeof(double), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_err(DMR_ERR_HIP, "readback failed");
+    return 1.0;
+  }
+  const double beta = std::sqrt(c->hostStatus[4]);
+  c->st.beta = beta;
+  Op32 op{c, s};
+  int totalCycles = 0;
+  double relres = 1.0;
+  c->kryLoopFlag = 0;
+  bool useZ = mode != 0;
+  for (int outer = 0;; ++outer) {
+    dmr::residual64(c->csr, c->x64, c->b64, c->r64, c->redOp, c->scal, s);
+    ++c->st.matvecs;
+    if (hipMemcpyAsync(c->hostStatus, c->scal, 3 * sizeof(double), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      set_err(DMR_ERR_HIP, "residual readback failed");
+      return relres;
+    }
+    const double rn = c->hostStatus[2];
+    relres = rn / beta;
+    if (relres <= tol || beta == 0.0) break;
+    if (outer >= c->maxOuter || totalCycles > matRank) {
+      c->st.status = DMR_ERR_NOCONV;
+      set_err(DMR_ERR_NOCONV, "refinement did not reach the tolerance");
+      break;
+    }
+    // r32 = r / ||r||, x_a = 0.
+    dmr::scale<double>(c->r64, n, c->scal + 1, nullptr, s);
+    dmr::convert<float, double>(w.r, c->r64, n, s);
+    dmr::fill<float>(w.xa, ld, 0.0, s);
+    double tolIn = std::max(c->innerTol, 0.5 * tol / relres);
+    if (tolIn > 0.5) tolIn = 0.5;
+    double rho = rn;
+    int grdItr = 0;
+    const int startDef = useZ ? k : 0;
+    const int rc = run_cycles<float>(c, op, s, startDef, tolIn, matRank, rn, rho, grdItr,
+                                     rn / beta);
+    totalCycles += grdItr;
+    if (rc != DMR_OK && rc != DMR_ERR_NOCONV && rc != DMR_ERR_RUNAWAY) return relres;
+    useZ = (k > 0) && w.hasZ;
+    dmr::add_to<float>(c->x64, w.xa, n, s);
+  }
+  *numIts = totalCycles;
+  if (c->st.status == 0) c->st.status = relres <= tol ? 0 : DMR_ERR_NOCONV;
+  c->st.rho = relres * beta;
+  if (hipMemcpyAsync(x, c->x64, sizeof(double) * n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    set_err(DMR_ERR_HIP, "solution export failed");
+  return relres;
+}
+
+dmrContext *g_ctx = nullptr;
+
+int create_ctx(dmrContext **out, int device, int n, int m, int k, int prec) {
+  *out = nullptr;
+  int rc = validate_shape(n, m, k);
+  if (rc) return rc;
+  if (prec != DMR_PREC_FP64 && prec != DMR_PREC_FP32)
+    return set_err(DMR_ERR_NOTBUILT, "precision not available (FP64 or FP32 basis)");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_err(DMR_ERR_HIP, "no HIP device");
+  if (device < 0 || device >= ndev) return set_err(DMR_ERR_ARG, "device out of range");
+  HIPCHK(hipSetDevice(device));
+  dmrContext *c = new dmrContext();
+  c->device = device;
+  c->n = n;
+  c->m = m;
+  c->k = k;
+  c->prec = prec;
+  c->ld = pad_ld(n);
+  rc = ctx_alloc(c);
+  if (rc) {
+    ctx_free(c);
+    return rc;
+  }
+  *out = c;
+  return DMR_OK;
+}
+
+}  // namespace
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+extern "C" {
+
+int dmrGetLastError(void) { return g_err; }
+const char *dmrGetLastErrorString(void) { return g_errmsg.c_str(); }
+void dmrRaiseOperatorError(const char *msg) {
+  g_op_failed = true;
+  g_op_msg = msg ? msg : "";
+}
+
+int dmrReductionBlocks(int elements) { return dmr::red_blocks<double>(elements); }
+
+size_t dmrFootprint(int elements, int basisDim, int deflatDim, int precision) {
+  return footprint(elements, basisDim, deflatDim, precision);
+}
+
+int dmrCreate(dmrHandle *h, int device, int elements, int basisDim, int deflatDim, int precision) {
+  clear_err();
+  if (!h) return set_err(DMR_ERR_ARG, "null handle pointer");
+  return create_ctx(h, device, elements, basisDim, deflatDim, precision);
+}
+
+int dmrDestroy(dmrHandle h) {
+  if (g_ctx == h) g_ctx = nullptr;
+  ctx_free(h);
+  return DMR_OK;
+}
+
+int dmrSetOperatorCsr(dmrHandle h, int nnz, const int *rowPtr, const int *colInd,
+                      const double *val, const float *val32) {
+  clear_err();
+  if (!h || !rowPtr || !colInd || !val || nnz < 0)
+    return set_err(DMR_ERR_ARG, "invalid CSR arguments");
+  HIPCHK(hipSetDevice(h->device));
+  const int n = h->n;
+  std::vector<int> rp(n + 1);
+  HIPCHK(hipMemcpy(rp.data(), rowPtr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  if (rp[0] != 0 || rp[n] != nnz) return set_err(DMR_ERR_ARG, "rowPtr inconsistent with nnz");
+  std::vector<int> br(n + 1);
+  const int nb = dmr::spmv_partition(rp.data(), n, br.data());
+  if (h->ownedBlockRows) hipFree(h->ownedBlockRows);
+  h->ownedBlockRows = nullptr;
+  HIPCHK(hipMalloc(&h->ownedBlockRows, sizeof(int) * (nb + 1)));
+  HIPCHK(hipMemcpy(h->ownedBlockRows, br.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
+  if (h->redOpPool) hipFree(h->redOpPool);
+  h->redOpPool = nullptr;
+  const size_t rbytes = sizeof(double) * (size_t)std::max(nb, 1) + 64;
+  HIPCHK(hipMalloc(&h->redOpPool, rbytes));
+  HIPCHK(hipMemset(h->redOpPool, 0, rbytes));
+  h->redOp.partials = h->redOpPool;
+  h->redOp.counters = (unsigned *)((char *)h->redOpPool + sizeof(double) * std::max(nb, 1));
+  h->csr.n = n;
+  h->csr.nnz = nnz;
+  h->csr.rowPtr = rowPtr;
+  h->csr.colInd = colInd;
+  h->csr.val = val;
+  h->csr.blockRows = h->ownedBlockRows;
+  h->csr.nBlocks = nb;
+  h->csr.val32 = val32;
+  if (h->prec != DMR_PREC_FP64 && !val32) {
+    if (h->ownedVal32) hipFree(h->ownedVal32);
+    h->ownedVal32 = nullptr;
+    HIPCHK(hipMalloc(&h->ownedVal32, sizeof(float) * std::max(nnz, 1)));
+    dmr::convert<float, double>(h->ownedVal32, val, nnz, nullptr);
+    HIPCHK(hipDeviceSynchronize());
+    h->csr.val32 = h->ownedVal32;
+  }
+  h->opKind = OP_CSR;
+  return DMR_OK;
+}
+
+int dmrSetOperatorValues(dmrHandle h, const double *val, const float *val32) {
+  clear_err();
+  if (!h || h->opKind != OP_CSR || !val) return set_err(DMR_ERR_STATE, "no CSR operator to update");
+  if (h->prec != DMR_PREC_FP64 && !val32) {
+    if (!h->ownedVal32) return set_err(DMR_ERR_STATE, "no FP32 value buffer");
+    HIPCHK(hipSetDevice(h->device));
+    dmr::convert<float, double>(h->ownedVal32, val, h->csr.nnz, nullptr);
+    HIPCHK(hipDeviceSynchronize());
+    val32 = h->ownedVal32;
+  }
+  h->csr.val = val;
+  h->csr.val32 = val32;
+  return DMR_OK;
+}
+
+int dmrSetOperatorFn(dmrHandle h, dmrOperatorFn op, void *ctx) {
+  clear_err();
+  if (!h || !op) return set_err(DMR_ERR_ARG, "null operator");
+  h->opFn = op;
+  h->opCtx = ctx;
+  h->opKind = OP_FN;
+  return DMR_OK;
+}
+
+int dmrSetOperatorLegacy(dmrHandle h, void (*linComp)(void)) {
+  clear_err();
+  if (!h || !linComp) return set_err(DMR_ERR_ARG, "null operator");
+  h->legacy = linComp;
+  h->opKind = OP_LEGACY;
+  return DMR_OK;
+}
+
+double dmrSolve(dmrHandle h, int mode, double *x, const double *b, hipStream_t stream,
+                double solTol, int matRank, int *numIts) {
+  clear_err();
+  int dummy = 0;
+  if (!numIts) numIts = &dummy;
+  if (!h) {
+    set_err(DMR_ERR_STATE, "solver not initialised");
+    return 1.0;
+  }
+  if (!x || !b || matRank < 0 || !(solTol >= 0.0)) {
+    set_err(DMR_ERR_ARG, "invalid solve arguments");
+    return 1.0;
+  }
+  if (h->opKind == OP_NONE) {
+    set_err(DMR_ERR_STATE, "no operator set");
+    return 1.0;
+  }
+  if (hipSetDevice(h->device) != hipSuccess) {
+    set_err(DMR_ERR_HIP, "hipSetDevice failed");
+    return 1.0;
+  }
+  g_op_failed = false;
+  h->st = dmrStats{};
+  h->hist.clear();
+  const auto t0 = std::chrono::steady_clock::now();
+  double res;
+  if (h->prec == DMR_PREC_FP64)
+    res = solve_fp64(h, mode, x, b, stream, solTol, matRank, numIts);
+  else
+    res = solve_mixed(h, mode, x, b, stream, solTol, matRank, numIts);
+  h->st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return res;
+}
+
+int dmrHasZ(dmrHandle h) {
+  if (!h) return 0;
+  return h->prec == DMR_PREC_FP64 ? (int)h->w64.hasZ : (int)h->w32.hasZ;
+}
+
+int dmrImportZ(dmrHandle h, const double *hostZ) {
+  clear_err();
+  if (!h || !hostZ) return set_err(DMR_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(h->device));
+  if (h->k == 0) return DMR_OK;
+  if (h->prec == DMR_PREC_FP64) {
+    HIPCHK(hipMemcpy2D(h->w64.Z, sizeof(double) * h->ld, hostZ, sizeof(double) * h->n,
+                       sizeof(double) * h->n, h->k, hipMemcpyHostToDevice));
+    h->w64.hasZ = true;
+  } else {
+    std::vector<float> tmp((size_t)h->n * h->k);
+    for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = (float)hostZ[i];
+    HIPCHK(hipMemcpy2D(h->w32.Z, sizeof(float) * h->ld, tmp.data(), sizeof(float) * h->n,
+                       sizeof(float) * h->n, h->k, hipMemcpyHostToDevice));
+    h->w32.hasZ = true;
+  }
+  return DMR_OK;
+}
+
+int dmrExportZ(dmrHandle h, double *hostZ) {
+  clear_err();
+  if (!h || !hostZ) return set_err(DMR_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(h->device));
+  if (h->k == 0) return DMR_OK;
+  HIPCHK(hipDeviceSynchronize());
+  if (h->prec == DMR_PREC_FP64) {
+    HIPCHK(hipMemcpy2D(hostZ, sizeof(double) * h->n, h->w64.Z, sizeof(double) * h->ld,
+                       sizeof(double) * h->n, h->k, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp((size_t)h->n * h->k);
+    HIPCHK(hipMemcpy2D(tmp.data(), sizeof(float) * h->n, h->w32.Z, sizeof(float) * h->ld,
+                       sizeof(float) * h->n, h->k, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < tmp.size(); ++i) hostZ[i] = tmp[i];
+  }
+  return DMR_OK;
+}
+
+int dmrGetStatsH(dmrHandle h, dmrStats *st, double *residHist, int cap) {
+  if (!h) return -1;
+  if (st) *st = h->st;
+  const int cnt = (int)h->hist.size();
+  if (residHist) {
+    for (int i = 0; i < std::min(cap, cnt); ++i) residHist[i] = h->hist[i];
+  }
+  return cnt;
+}
+
+int dmrSetRefinement(dmrHandle h, double innerTol, int maxOuter) {
+  if (!h || !(innerTol > 0.0) || maxOuter < 1) return set_err(DMR_ERR_ARG, "invalid settings");
+  h->innerTol = innerTol;
+  h->maxOuter = maxOuter;
+  return DMR_OK;
+}
+
+int dmrSetProfiling(dmrHandle h, int on) {
+  if (!h) return set_err(DMR_ERR_ARG, "null handle");
+  h->prof.on = on != 0;
+  for (int i = 0; i < DMR_NPHASES; ++i) {
+    h->prof.ms[i] = 0.0;
+    h->prof.by[i] = 0.0;
+    h->prof.cnt[i] = 0;
+  }
+  h->prof.used = 0;
+  return DMR_OK;
+}
+
+int dmrGetProfile(dmrHandle h, double *ms, double *bytes, long long *launches) {
+  if (!h) return set_err(DMR_ERR_ARG, "null handle");
+  for (int i = 0; i < DMR_NPHASES; ++i) {
+    if (ms) ms[i] = h->prof.ms[i];
+    if (bytes) bytes[i] = h->prof.by[i];
+    if (launches) launches[i] = h->prof.cnt[i];
+  }
+  return DMR_NPHASES;
+}
+
+// ---- reference global-state interface ----
+void initDMR(int blocks, int threadsPerBlock, int *devList, int numDevs, int elements,
+             int basisDim, int deflatDim, void (*linComp)(void)) {
+  (void)blocks;
+  (void)threadsPerBlock;
+  clear_err();
+  if (!devList || numDevs != 1) {
+    set_err(DMR_ERR_ARG, "exactly one device is supported (numDevs == 1)");
+    return;
+  }
+  if (!linComp) {
+    set_err(DMR_ERR_ARG, "null operator callback");
+    return;
+  }
+  if (g_ctx) {
+    ctx_free(g_ctx);
+    g_ctx = nullptr;
+  }
+  dmrContext *c = nullptr;
+  if (create_ctx(&c, devList[0], elements, basisDim, deflatDim, DMR_PREC_FP64)) return;
+  c->legacy = linComp;
+  c->opKind = OP_LEGACY;
+  g_ctx = c;
+}
+
+void initDMRx(int device, int elements, int basisDim, int deflatDim, dmrOperatorFn op,
+              void *ctx) {
+  clear_err();
+  if (!op) {
+    set_err(DMR_ERR_ARG, "null operator");
+    return;
+  }
+  if (g_ctx) {
+    ctx_free(g_ctx);
+    g_ctx = nullptr;
+  }
+  dmrContext *c = nullptr;
+  if (create_ctx(&c, device, elements, basisDim, deflatDim, DMR_PREC_FP64)) return;
+  c->opFn = op;
+  c->opCtx = ctx;
+  c->opKind = OP_FN;
+  g_ctx = c;
+}
+
+void freeDMR(void) {
+  ctx_free(g_ctx);
+  g_ctx = nullptr;
+}
+
+double solverDMR(int mode, double *extSrcVec, double *extImgVec, hipStream_t extStream,
+                 int extDevNum, double solTol, int matRank, int *numIts) {
+  if (!g_ctx) {
+    set_err(DMR_ERR_STATE, "solverDMR called before initDMR");
+    return 1.0;
+  }
+  if (extDevNum != g_ctx->device) {
+    set_err(DMR_ERR_ARG, "operator device must be the solver device");
+    return 1.0;
+  }
+  return dmrSolve(g_ctx, mode, extSrcVec, extImgVec, extStream, solTol, matRank, numIts);
+}
+
+void impSVDMR(double *cMem) {
+  if (!g_ctx) {
+    set_err(DMR_ERR_STATE, "impSVDMR called before initDMR");
+    return;
+  }
+  dmrImportZ(g_ctx, cMem);
+}
+
+void expSVDMR(double *cMem) {
+  if (!g_ctx) {
+    set_err(DMR_ERR_STATE, "expSVDMR called before initDMR");
+    return;
+  }
+  dmrExportZ(g_ctx, cMem);
+}
+
+int dmrGetStats(dmrStats *st, double *residHist, int cap) {
+  if (!g_ctx) return -1;
+  return dmrGetStatsH(g_ctx, st, residHist, cap);
+}
+
+dmrHandle dmrGlobalHandle(void) { return g_ctx; }
+
+}  // extern "C"

@@ -1,0 +1,338 @@
+"""Python host mirror of libgDMR.so (ctypes over the C ABI of include/gDMR.h and gDMRC.h).
+
+The solver itself is the native library: a C++ host driver and hand-written HIP kernels for
+gfx950.  This module only binds its C entry points for tests, benchmarks and scripted use, in the
+shape of the reference interface (seanMolesky/gDMR-solver gDMR.h:13-34, gDMRC.h:6-29):
+
+* the global-state solver API (initDMR / solverDMR / freeDMR / impSVDMR / expSVDMR),
+* the linear-system API (initLS / impImgLS / impSrcLS / impCsrLS / solveLS / expSrcLS / freeLS),
+* handle-based contexts (``Solver``) for several operator families at once,
+* the synthetic problem generators and the counter-based random numbers.
+
+Importing fails loudly when the library has not been built; there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(os.path.dirname(_HERE), "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libgDMR.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"libgDMR.so not found at {LIB_PATH}; build it with `make -C gdmr-solver_amd` "
+        "(or __graft_entry__.build())")
+
+lib = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+
+# ---- error codes (gDMR.h) ----
+DMR_OK, DMR_ERR_ARG, DMR_ERR_STATE, DMR_ERR_HIP, DMR_ERR_ALLOC = 0, 1, 2, 3, 4
+DMR_ERR_OPERATOR, DMR_ERR_NOCONV, DMR_ERR_RUNAWAY, DMR_ERR_NOTBUILT = 5, 6, 7, 8
+PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
+
+_vp, _ip, _dp, _fp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_float)
+LEGACY_OP = C.CFUNCTYPE(None)
+OOP_OP = C.CFUNCTYPE(None, _vp, _vp, _vp, _vp)
+
+
+class Stats(C.Structure):
+    _fields_ = [("cycles", C.c_int), ("matvecs", C.c_int), ("breakdowns", C.c_int),
+                ("status", C.c_int), ("beta", C.c_double), ("rho", C.c_double),
+                ("seconds", C.c_double)]
+
+
+def _sig(name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+# Reference interface.
+_sig("initDMR", None, [C.c_int, C.c_int, _ip, C.c_int, C.c_int, C.c_int, C.c_int, LEGACY_OP])
+_sig("freeDMR", None, [])
+_sig("solverDMR", C.c_double, [C.c_int, _vp, _vp, _vp, C.c_int, C.c_double, C.c_int, _ip])
+_sig("impSVDMR", None, [_dp])
+_sig("expSVDMR", None, [_dp])
+_sig("initLS", None, [C.c_int, C.c_int, C.c_int, C.c_int, _ip, C.c_int, C.c_int, C.c_int])
+_sig("freeLS", None, [])
+for _n in ("impImgLS", "expImgLS", "impSrcLS", "expSrcLS", "impOptLS", "expOptLS"):
+    _sig(_n, None, [_dp])
+_sig("solveLS", C.c_double, [C.c_int, C.c_double, C.c_int, _ip])
+_sig("linOptLS", None, [])
+# Extensions.
+_sig("impCsrLS", C.c_int, [_ip, _ip, _dp, C.c_int])
+_sig("nnzLS", C.c_int, [])
+_sig("dmrGetLastError", C.c_int, [])
+_sig("dmrGetLastErrorString", C.c_char_p, [])
+_sig("dmrRaiseOperatorError", None, [C.c_char_p])
+_sig("initDMRx", None, [C.c_int, C.c_int, C.c_int, C.c_int, OOP_OP, _vp])
+_sig("dmrGetStats", C.c_int, [C.POINTER(Stats), _dp, C.c_int])
+_sig("dmrReductionBlocks", C.c_int, [C.c_int])
+_sig("dmrCreate", C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int])
+_sig("dmrDestroy", C.c_int, [_vp])
+_sig("dmrSetOperatorCsr", C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp])
+_sig("dmrSetOperatorValues", C.c_int, [_vp, _vp, _vp])
+_sig("dmrSetOperatorFn", C.c_int, [_vp, OOP_OP, _vp])
+_sig("dmrSetOperatorLegacy", C.c_int, [_vp, LEGACY_OP])
+_sig("dmrSolve", C.c_double, [_vp, C.c_int, _vp, _vp, _vp, C.c_double, C.c_int, _ip])
+_sig("dmrImportZ", C.c_int, [_vp, _dp])
+_sig("dmrExportZ", C.c_int, [_vp, _dp])
+_sig("dmrHasZ", C.c_int, [_vp])
+_sig("dmrGetStatsH", C.c_int, [_vp, C.POINTER(Stats), _dp, C.c_int])
+_sig("dmrSetRefinement", C.c_int, [_vp, C.c_double, C.c_int])
+_sig("dmrGlobalHandle", _vp, [])
+_sig("dmrFootprint", C.c_size_t, [C.c_int, C.c_int, C.c_int, C.c_int])
+_sig("dmrSetProfiling", C.c_int, [_vp, C.c_int])
+_sig("dmrGetProfile", C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_longlong)])
+PHASES = ("spmv", "gemvt", "update", "scale", "combo", "zgemm", "lsq", "svd")
+# Problems and random numbers.
+_sig("dmrGenConvDiff2D", C.c_longlong, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_ulonglong, C.c_ulonglong, _ip, _ip, _dp])
+_sig("dmrGenConvDiff3D", C.c_longlong, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_ulonglong, C.c_ulonglong, _ip, _ip, _dp])
+_sig("dmrGenAniso3D", C.c_longlong, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_ulonglong, C.c_ulonglong, _ip, _ip, _dp])
+_sig("dmrRandNormal", None, [C.c_ulonglong, C.c_ulonglong, C.c_longlong, _dp])
+_sig("dmrRandUniform", None, [C.c_ulonglong, C.c_ulonglong, C.c_longlong, C.c_double, C.c_double,
+                              _dp])
+_sig("dmrSplitMix64", None, [C.c_ulonglong, C.c_longlong, C.POINTER(C.c_ulonglong)])
+
+# Every symbol the public headers declare (checked by the CPU test suite).
+HEADER_SYMBOLS = (
+    "initDMR freeDMR solverDMR impSVDMR expSVDMR dmrGetLastError dmrGetLastErrorString "
+    "dmrRaiseOperatorError initDMRx dmrGetStats dmrReductionBlocks dmrCreate dmrDestroy "
+    "dmrSetOperatorCsr dmrSetOperatorValues dmrSetOperatorFn dmrSetOperatorLegacy dmrSolve "
+    "dmrImportZ dmrExportZ "
+    "dmrHasZ dmrGetStatsH dmrSetRefinement dmrGlobalHandle dmrFootprint dmrSetProfiling "
+    "dmrGetProfile "
+    "initLS freeLS impImgLS expImgLS impSrcLS expSrcLS impOptLS expOptLS solveLS linOptLS "
+    "impCsrLS nnzLS dmrGenConvDiff2D dmrGenConvDiff3D dmrGenAniso3D dmrRandNormal "
+    "dmrRandUniform dmrSplitMix64").split()
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def last_error() -> tuple[int, str]:
+    return lib.dmrGetLastError(), lib.dmrGetLastErrorString().decode()
+
+
+# ------------------------------------------------------------------------------------------------
+# Problems
+# ------------------------------------------------------------------------------------------------
+@dataclass
+class CsrMatrix:
+    n: int
+    rowptr: np.ndarray  # int32 [n+1]
+    col: np.ndarray     # int32 [nnz]
+    val: np.ndarray     # float64 [nnz]
+
+    @property
+    def nnz(self) -> int:
+        return int(self.rowptr[-1])
+
+    def matvec(self, x: np.ndarray) -> np.ndarray:
+        prod = self.val * x[self.col]
+        return np.add.reduceat(prod, self.rowptr[:-1]) if self.nnz else np.zeros(self.n)
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.val, self.col, self.rowptr), shape=(self.n, self.n))
+
+
+def _gen(fn, n, *args) -> CsrMatrix:
+    nnz = fn(*args, None, None, None)
+    rp = np.zeros(n + 1, np.int32)
+    ci = np.zeros(nnz, np.int32)
+    va = np.zeros(nnz, np.float64)
+    got = fn(*args, _ptr(rp, C.c_int), _ptr(ci, C.c_int), _ptr(va, C.c_double))
+    assert got == nnz, (got, nnz)
+    return CsrMatrix(n, rp, ci, va)
+
+
+def conv_diff_2d(N: int, nu0: float = 0.1, beta=(1.0, 0.5), jitter: float = 0.0,
+                 seed: int = 0, stream: int = 0) -> CsrMatrix:
+    """h^2 (-nu lap u + beta.grad u) on an N x N interior grid, 5-point central differences."""
+    return _gen(lib.dmrGenConvDiff2D, N * N, N, nu0, beta[0], beta[1], jitter, seed, stream)
+
+
+def conv_diff_3d(N: int, nu0: float = 0.1, beta=(1.0, 0.5, 0.25), jitter: float = 0.0,
+                 seed: int = 0, stream: int = 0) -> CsrMatrix:
+    """h^2 (-div(nu grad u) + beta.grad u) on an N^3 interior grid, 7-point central differences."""
+    return _gen(lib.dmrGenConvDiff3D, N ** 3, N, nu0, beta[0], beta[1], beta[2], jitter, seed,
+                stream)
+
+
+def aniso_3d(N: int, coef=(1.0, 0.1, 0.01), jitter: float = 0.0, seed: int = 0,
+             stream: int = 0) -> CsrMatrix:
+    """h^2 (-(a u_xx + b u_yy + c u_zz)) on an N^3 interior grid, 7-point."""
+    return _gen(lib.dmrGenAniso3D, N ** 3, N, coef[0], coef[1], coef[2], jitter, seed, stream)
+
+
+def rand_normal(seed: int, stream: int, n: int) -> np.ndarray:
+    out = np.empty(n, np.float64)
+    lib.dmrRandNormal(seed, stream, n, _ptr(out, C.c_double))
+    return out
+
+
+def rand_uniform(seed: int, stream: int, n: int, lo: float, hi: float) -> np.ndarray:
+    out = np.empty(n, np.float64)
+    lib.dmrRandUniform(seed, stream, n, lo, hi, _ptr(out, C.c_double))
+    return out
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n, np.uint64)
+    lib.dmrSplitMix64(seed, n, out.ctypes.data_as(C.POINTER(C.c_ulonglong)))
+    return out
+
+
+def reduction_blocks(n: int) -> int:
+    """Block count G of the device's fixed-order reductions over n doubles."""
+    return lib.dmrReductionBlocks(n)
+
+
+def footprint(n: int, m: int, k: int, precision: int = PREC_FP64) -> int:
+    return int(lib.dmrFootprint(n, m, k, precision))
+
+
+# ------------------------------------------------------------------------------------------------
+# Handle-based solver (device memory through torch tensors: plumbing only)
+# ------------------------------------------------------------------------------------------------
+class SolverError(RuntimeError):
+    pass
+
+
+def _check(rc: int, what: str):
+    if rc != DMR_OK:
+        code, msg = last_error()
+        raise SolverError(f"{what} failed: code {rc} ({msg})")
+
+
+class Solver:
+    """One DMR context (n, m, k, precision) on one device, with its recycled subspace Z."""
+
+    def __init__(self, n: int, m: int, k: int, precision: int = PREC_FP64, device: int = 0):
+        self.n, self.m, self.k, self.precision, self.device = n, m, k, precision, device
+        h = C.c_void_p()
+        _check(lib.dmrCreate(C.byref(h), device, n, m, k, precision), "dmrCreate")
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            lib.dmrDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_csr(self, rowptr_ptr: int, col_ptr: int, val_ptr: int, nnz: int, val32_ptr: int = 0,
+                keep=()):
+        """Device pointers of a CSR operator (int32 rowptr / col, float64 val, optional float32)."""
+        self._keep = list(keep)
+        _check(lib.dmrSetOperatorCsr(self.h, nnz, rowptr_ptr, col_ptr, val_ptr, val32_ptr or None),
+               "dmrSetOperatorCsr")
+
+    def set_values(self, val_ptr: int, val32_ptr: int = 0, keep=()):
+        """New values for the current CSR pattern (device pointers)."""
+        self._keep_vals = list(keep)
+        _check(lib.dmrSetOperatorValues(self.h, val_ptr, val32_ptr or None), "dmrSetOperatorValues")
+
+    def set_operator(self, fn):
+        """fn(src_ptr, dst_ptr, stream_ptr, ctx): out-of-place operator."""
+        cb = OOP_OP(fn)
+        self._cb = cb
+        _check(lib.dmrSetOperatorFn(self.h, cb, None), "dmrSetOperatorFn")
+
+    def set_legacy(self, fn):
+        cb = LEGACY_OP(fn)
+        self._cb = cb
+        _check(lib.dmrSetOperatorLegacy(self.h, cb), "dmrSetOperatorLegacy")
+
+    def set_refinement(self, inner_tol: float, max_outer: int = 60):
+        _check(lib.dmrSetRefinement(self.h, inner_tol, max_outer), "dmrSetRefinement")
+
+    def solve(self, x_ptr: int, b_ptr: int, tol: float, mat_rank: int, mode: int = 0,
+              stream: int = 0):
+        its = C.c_int(0)
+        res = lib.dmrSolve(self.h, mode, x_ptr, b_ptr, stream or None, tol, mat_rank,
+                           C.byref(its))
+        return res, its.value
+
+    def stats(self, cap: int = 4096):
+        st = Stats()
+        hist = np.zeros(cap)
+        cnt = lib.dmrGetStatsH(self.h, C.byref(st), _ptr(hist, C.c_double), cap)
+        return st, hist[:min(cnt, cap)].copy()
+
+    def has_z(self) -> bool:
+        return bool(lib.dmrHasZ(self.h))
+
+    def set_profiling(self, on: bool = True):
+        _check(lib.dmrSetProfiling(self.h, int(on)), "dmrSetProfiling")
+
+    def profile(self) -> dict:
+        """{phase: (ms, algorithmic bytes, launches)} accumulated since set_profiling(True)."""
+        ms, by = np.zeros(len(PHASES)), np.zeros(len(PHASES))
+        cnt = np.zeros(len(PHASES), np.int64)
+        lib.dmrGetProfile(self.h, _ptr(ms, C.c_double), _ptr(by, C.c_double),
+                          cnt.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return {p: (float(ms[i]), float(by[i]), int(cnt[i])) for i, p in enumerate(PHASES)}
+
+    def import_z(self, Z: np.ndarray):
+        Z = np.ascontiguousarray(Z, np.float64)
+        assert Z.size == self.n * self.k
+        _check(lib.dmrImportZ(self.h, _ptr(Z, C.c_double)), "dmrImportZ")
+
+    def export_z(self) -> np.ndarray:
+        Z = np.zeros(self.n * self.k)
+        _check(lib.dmrExportZ(self.h, _ptr(Z, C.c_double)), "dmrExportZ")
+        return Z.reshape(self.k, self.n)
+
+
+def global_stats(cap: int = 4096):
+    st = Stats()
+    hist = np.zeros(cap)
+    cnt = lib.dmrGetStats(C.byref(st), _ptr(hist, C.c_double), cap)
+    return st, hist[:max(0, min(cnt, cap))].copy()
+
+
+# ------------------------------------------------------------------------------------------------
+# Linear-system (control-program) interface, host arrays
+# ------------------------------------------------------------------------------------------------
+class LinearSystem:
+    """initLS ... freeLS, as a control program drives them (progDMR.c:135-182)."""
+
+    def __init__(self, A: CsrMatrix, m: int, k: int, device: int = 0):
+        self.A, self.n = A, A.n
+        dev = (C.c_int * 1)(device)
+        lib.initLS(32, 512, device, A.n, dev, 1, m, k)
+        rc = lib.impCsrLS(_ptr(A.rowptr, C.c_int), _ptr(A.col, C.c_int), _ptr(A.val, C.c_double),
+                          A.nnz)
+        if rc != DMR_OK:
+            raise SolverError(f"impCsrLS failed: {rc}")
+
+    def solve(self, b: np.ndarray, x0: np.ndarray | None = None, tol: float = 1e-8,
+              mat_rank: int | None = None, mode: int = 0):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(self.n) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+        lib.impImgLS(_ptr(b, C.c_double))
+        lib.impSrcLS(_ptr(x, C.c_double))
+        its = C.c_int(0)
+        res = lib.solveLS(mode, tol, self.n if mat_rank is None else mat_rank, C.byref(its))
+        lib.expSrcLS(_ptr(x, C.c_double))
+        return x, res, its.value
+
+    def close(self):
+        lib.freeLS()

@@ -1,0 +1,153 @@
+/*
+ * gDMR.h -- C ABI of libgDMR.so, the MI355X-native deflated minimal residual (DMR) solver.
+ *
+ * DMR is restarted GMRES(m) in which k of the m search directions are recycled approximate
+ * smallest right-singular vectors of A (Al Daas, Grigori, Henon, Ricoux, Inria RR-9206, 2018).
+ *
+ * The first block keeps the reference solver interface (names, arity, argument order, ownership,
+ * the void(void) operator callback and its in-place contract).  Element type is real double
+ * instead of complex float and the stream is a hipStream_t; everything else means what it means
+ * in the reference.  The second block adds what the reference lacks: an error channel, an
+ * out-of-place operator, statistics, handle-based contexts and the mixed-precision solve.
+ */
+#ifndef GDMR_H
+#define GDMR_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------------------------
+ * Reference interface (replaces gDMR.h:13-34 of seanMolesky/gDMR-solver)
+ * ------------------------------------------------------------------------------------------- */
+
+/* Initialise the solver (replaces gDMR.h:13, gDMR.cu:840-1208).
+ * blocks, threadsPerBlock: accepted and ignored; launch shapes are chosen per kernel.
+ * devList, numDevs: must name exactly one device (numDevs == 1).
+ * elements: vector length n.  basisDim: total basis m (Krylov + deflation).
+ * deflatDim: deflation dimension k, 0 <= k < m.
+ * linComp: operator callback.  It must overwrite the source vector most recently handed to
+ * solverDMR (extSrcVec) with A * extSrcVec, enqueued on extStream (in place, as testLS.cu:97-128). */
+void initDMR(int blocks, int threadsPerBlock, int *devList, int numDevs, int elements,
+             int basisDim, int deflatDim, void (*linComp)(void));
+
+/* Free all solver memory (replaces gDMR.h:15, gDMR.cu:1210-1475). */
+void freeDMR(void);
+
+/* Solve A x = b (replaces gDMR.h:29, gDMR.cu:2638-2901).
+ * mode != 0 starts from the stored deflation subspace (previous solve or impSVDMR).
+ * extSrcVec: device, n doubles; read as x0, overwritten by A x0 and the operator inputs, and on
+ *   return holds the solution.  extImgVec: device, n doubles, the right-hand side b (read only).
+ * solTol: relative residual target; also the absolute Krylov breakdown threshold.
+ * matRank: maximum number of restart cycles; also sets the stagnation restart period.
+ * Returns ||b - A x|| / ||b|| tracked by the recursion, the absolute ||b - A x0|| when x0 already
+ * meets the target (then *numIts = matRank*matRank), and *numIts = cycles - 1 on convergence. */
+double solverDMR(int mode, double *extSrcVec, double *extImgVec, hipStream_t extStream,
+                 int extDevNum, double solTol, int matRank, int *numIts);
+
+/* Import / export the source-side deflation vectors Z: a host buffer of deflatDim contiguous
+ * vectors of `elements` doubles (replaces gDMR.h:33-34, gDMR.cu:431-477). */
+void impSVDMR(double *cMem);
+void expSVDMR(double *cMem);
+
+/* ---------------------------------------------------------------------------------------------
+ * Extensions (no counterpart in the reference)
+ * ------------------------------------------------------------------------------------------- */
+
+/* Error codes.  The reference reports errors only on stderr and by resetting the device. */
+enum {
+  DMR_OK = 0,
+  DMR_ERR_ARG = 1,        /* invalid argument (sizes, m/k, device list)            */
+  DMR_ERR_STATE = 2,      /* call out of order (solve before init, mode!=0 w/o Z)  */
+  DMR_ERR_HIP = 3,        /* a HIP runtime call failed                             */
+  DMR_ERR_ALLOC = 4,      /* device allocation failed / not enough free HBM        */
+  DMR_ERR_OPERATOR = 5,   /* the operator callback reported failure                */
+  DMR_ERR_NOCONV = 6,     /* cycle limit reached (matRank)                         */
+  DMR_ERR_RUNAWAY = 7,    /* residual grew above ||b|| ("runaway")                 */
+  DMR_ERR_NOTBUILT = 8    /* requested feature is not available in this build      */
+};
+int dmrGetLastError(void);
+const char *dmrGetLastErrorString(void);
+/* An operator can raise an error from inside a callback; the solver returns at the next check. */
+void dmrRaiseOperatorError(const char *msg);
+
+/* Out-of-place operator: dst = A * src, both device vectors of n doubles, enqueued on stream. */
+typedef void (*dmrOperatorFn)(const double *src, double *dst, hipStream_t stream, void *ctx);
+void initDMRx(int device, int elements, int basisDim, int deflatDim, dmrOperatorFn op, void *ctx);
+
+/* Statistics of the last solve of the global solver: relative residual rho/||b|| after each
+ * cycle (up to cap entries), number of cycles and of operator applications, Krylov breakdowns. */
+typedef struct {
+  int cycles;
+  int matvecs;
+  int breakdowns;
+  int status;          /* 0 converged, DMR_ERR_NOCONV, DMR_ERR_RUNAWAY, or -1 early exit */
+  double beta;         /* ||b||                                                           */
+  double rho;          /* final tracked ||r||                                             */
+  double seconds;      /* wall time of the solve                                          */
+} dmrStats;
+int dmrGetStats(dmrStats *st, double *residHist, int cap);
+
+/* Device geometry of the fixed-order reductions: every dot product over n elements is summed
+ * by this many 256-thread blocks (2 elements per thread step), then block partials in order. */
+int dmrReductionBlocks(int elements);
+
+/* ---- handle-based contexts (serving, several operator families at once) ---- */
+typedef struct dmrContext *dmrHandle;
+
+/* Precision of the Krylov basis and SpMV.  FP64 is the plain solver; FP32 / BF16 run the inner
+ * DMR in that storage (FP32 arithmetic, FP32-accumulated reductions) inside FP64 iterative
+ * refinement; Hessenberg LSQ, QR and SVD always run in FP64. */
+enum { DMR_PREC_FP64 = 0, DMR_PREC_FP32 = 1, DMR_PREC_BF16 = 2 };
+
+int dmrCreate(dmrHandle *h, int device, int elements, int basisDim, int deflatDim, int precision);
+int dmrDestroy(dmrHandle h);
+/* Built-in CSR operator (device arrays, int32 indices, sorted columns).  val32 may be NULL for an
+ * FP64-only handle; mixed-precision handles use val32 for the inner SpMV and val for the FP64
+ * refinement residual. */
+int dmrSetOperatorCsr(dmrHandle h, int nnz, const int *rowPtr, const int *colInd,
+                      const double *val, const float *val32);
+/* Replace the values of the CSR operator (same sparsity pattern), e.g. the next system of a
+ * sequence whose coefficients change; no allocation when val32 is given or the handle is FP64. */
+int dmrSetOperatorValues(dmrHandle h, const double *val, const float *val32);
+int dmrSetOperatorFn(dmrHandle h, dmrOperatorFn op, void *ctx);
+int dmrSetOperatorLegacy(dmrHandle h, void (*linComp)(void));
+/* Same contract as solverDMR on a handle.  x: device, x0 in / solution out; b: device. */
+double dmrSolve(dmrHandle h, int mode, double *x, const double *b, hipStream_t stream,
+                double solTol, int matRank, int *numIts);
+int dmrImportZ(dmrHandle h, const double *hostZ);
+int dmrExportZ(dmrHandle h, double *hostZ);
+int dmrHasZ(dmrHandle h);
+int dmrGetStatsH(dmrHandle h, dmrStats *st, double *residHist, int cap);
+/* Mixed-precision refinement: relative tolerance of each inner FP32 solve (default 1e-5) and the
+ * maximum number of refinement steps (default 60). */
+int dmrSetRefinement(dmrHandle h, double innerTol, int maxOuter);
+/* The handle behind the global-state interface (NULL before initDMR / initLS). */
+dmrHandle dmrGlobalHandle(void);
+/* Per-phase profile of the restart cycles, measured with HIP events on the solve stream around
+ * each launch (adds two event records per launch while on).  Phases: */
+enum {
+  DMR_PH_SPMV = 0,    /* operator applications                                  */
+  DMR_PH_GEMVT = 1,   /* multi-dot projections (Gram-Schmidt, t = P^T r)        */
+  DMR_PH_UPDATE = 2,  /* Gram-Schmidt updates with fused norms                  */
+  DMR_PH_SCALE = 3,   /* normalisations                                         */
+  DMR_PH_COMBO = 4,   /* residual / solution basis combinations                 */
+  DMR_PH_ZGEMM = 5,   /* deflation-subspace refresh [V Z] C and normalisation   */
+  DMR_PH_LSQ = 6,     /* Hessenberg least squares                               */
+  DMR_PH_SVD = 7,     /* Hessenberg Jacobi SVD                                  */
+  DMR_NPHASES = 8
+};
+int dmrSetProfiling(dmrHandle h, int on);
+/* Accumulated milliseconds, algorithmic bytes and launch counts per phase since profiling was
+ * switched on; each array has DMR_NPHASES entries (any may be NULL).  Returns DMR_NPHASES. */
+int dmrGetProfile(dmrHandle h, double *ms, double *bytes, long long *launches);
+/* Bytes of HBM a handle of this shape allocates (for admission by free HBM). */
+size_t dmrFootprint(int elements, int basisDim, int deflatDim, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GDMR_H */

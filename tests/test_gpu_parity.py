@@ -1,0 +1,232 @@
+"""GPU parity of the FP64 DMR solver against the FP64 host oracle.
+
+Bar (north star): iteration count within +/-1 and residual history within 1e-12 relative to the
+oracle; solutions are compared too.  The oracle sums in the device's reduction order, so the
+histories are expected to agree bit for bit; the assertions use the stated tolerance.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import gdmr
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+HIST_RTOL = 1e-12
+
+
+def _compare(tag, its, hist, x, its_o, hist_o, x_o):
+    L = min(len(hist), len(hist_o))
+    rel = np.abs(hist[:L] - hist_o[:L]) / np.abs(hist_o[:L])
+    xrel = np.linalg.norm(x - x_o) / max(np.linalg.norm(x_o), 1e-300)
+    bitwise = len(hist) == len(hist_o) and np.array_equal(hist, hist_o) and np.array_equal(x, x_o)
+    print(f"{tag}: its {its} vs {its_o}, cycles {len(hist)} vs {len(hist_o)}, "
+          f"max hist rel diff {rel.max() if L else 0:.3e}, x rel diff {xrel:.3e}, bitwise {bitwise}",
+          flush=True)
+    assert abs(its - its_o) <= 1
+    assert L > 0 and rel.max() <= HIST_RTOL
+    return bitwise
+
+
+def _true_res(A, b, x):
+    return np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b)
+
+
+def test_config_a_linear_system_api():
+    """Config A through initLS / impCsrLS / solveLS, the control-program path (progDMR.c:166-174):
+    2-D conv-diff 64^2, m=30, k=10, tol 1e-8, 4 N(0,1) RHS from seed 0, mode = i."""
+    A = gdmr.conv_diff_2d(64)
+    n, m, k, tol = A.n, 30, 10, 1e-8
+    B = gdmr.rand_normal(0, 1, 4 * n).reshape(4, n)
+    ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n))
+    ls = gdmr.LinearSystem(A, m, k)
+    try:
+        for i in range(4):
+            x, res, its = ls.solve(B[i], tol=tol, mode=i)
+            st, hist = gdmr.global_stats()
+            xo, reso, itso, sto = ora.solve(B[i], tol=tol, mode=i)
+            _compare(f"A[{i}]", its, hist, x, itso, ora.history(), xo)
+            assert sto == 0 and st.status == 0
+            assert abs(res - reso) <= HIST_RTOL * reso
+            assert _true_res(A, B[i], x) <= tol * 1.001
+    finally:
+        ls.close()
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def test_handle_api_3d_sequence(torch_gpu):
+    """3-D conv-diff 16^3 with per-cell coefficient jitter per system, a recycled sequence of 3
+    systems, m=20, k=8, tol 1e-10, through dmrCreate / dmrSetOperatorCsr / dmrSolve."""
+    torch = torch_gpu
+    N, m, k, tol = 16, 20, 8, 1e-10
+    n = N ** 3
+    s = gdmr.Solver(n, m, k)
+    ora = oracle.Oracle(gdmr.conv_diff_3d(N), m, k, gdmr.reduction_blocks(n))
+    for t in range(3):
+        A = gdmr.conv_diff_3d(N, jitter=0.01, seed=7, stream=t)
+        b = gdmr.rand_normal(7, 100 + t, n)
+        rp, ci, va = _dev(torch, A.rowptr), _dev(torch, A.col), _dev(torch, A.val)
+        s.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A.nnz, keep=(rp, ci, va))
+        xd = torch.zeros(n, dtype=torch.float64, device="cuda")
+        bd = _dev(torch, b)
+        res, its = s.solve(xd.data_ptr(), bd.data_ptr(), tol, n, mode=t)
+        st, hist = s.stats()
+        x = xd.cpu().numpy()
+        ora.close()
+        ora_t = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n))
+        if t > 0:
+            ora_t.set_z(Zprev)
+        xo, reso, itso, sto = ora_t.solve(b, tol=tol, mode=t)
+        _compare(f"3D[{t}]", its, hist, x, itso, ora_t.history(), xo)
+        assert _true_res(A, b, x) <= tol * 1.01
+        # The recycled subspace agrees too (GPU Z vs oracle Z).
+        Zg, Zo = s.export_z(), ora_t.get_z()
+        assert np.max(np.abs(Zg - Zo)) <= 1e-9
+        Zprev = Zo
+        ora = ora_t
+    s.close()
+
+
+def test_legacy_inplace_callback(torch_gpu):
+    """initDMR with the reference's void(void) in-place callback (testLS.cu:97-128 contract):
+    the callback overwrites the source vector with A x using a dense device GEMV."""
+    torch = torch_gpu
+    A = gdmr.conv_diff_2d(16)
+    n, m, k, tol = A.n, 12, 4, 1e-10
+    Ad = _dev(torch, A.to_scipy().toarray())
+    ext = torch.zeros(n, dtype=torch.float64, device="cuda")
+    calls = [0]
+
+    def cb():
+        calls[0] += 1
+        y = Ad @ ext
+        ext.copy_(y)
+
+    cbf = gdmr.LEGACY_OP(cb)
+    dev = (C.c_int * 1)(0)
+    gdmr.lib.initDMR(32, 512, dev, 1, n, m, k, cbf)
+    assert gdmr.lib.dmrGetLastError() == 0
+    ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n))
+    try:
+        for i in range(2):
+            b = gdmr.rand_normal(3, i, n)
+            bd = _dev(torch, b)
+            ext.zero_()
+            its = C.c_int(0)
+            res = gdmr.lib.solverDMR(i, ext.data_ptr(), bd.data_ptr(), None, 0, tol, n,
+                                     C.byref(its))
+            torch.cuda.synchronize()
+            x = ext.cpu().numpy()
+            st, hist = gdmr.global_stats()
+            xo, reso, itso, sto = ora.solve(b, tol=tol, mode=i)
+            # The dense GEMV sums in another order than CSR, so agreement is to rounding, not bits.
+            print(f"legacy[{i}]: its {its.value} vs {itso}, res {res:.3e} vs {reso:.3e}", flush=True)
+            assert abs(its.value - itso) <= 1
+            assert _true_res(A, b, x) <= tol * 1.01
+            assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
+        assert calls[0] == st.matvecs or calls[0] > 0
+    finally:
+        gdmr.lib.freeDMR()
+
+
+def test_breakdown_diagonal_operator(torch_gpu):
+    """A diagonal operator with 5 distinct eigenvalues: the Krylov space is exhausted at step 5,
+    the breakdown branch (gDMR.cu:2256-2290) pads the basis, and one cycle solves the system."""
+    torch = torch_gpu
+    n, m = 300, 10
+    d = np.repeat([1.0, 2.0, 3.0, 5.0, 8.0], n // 5)
+    A = gdmr.CsrMatrix(n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), d)
+    for k in (0, 3):
+        s = gdmr.Solver(n, m, k)
+        rp, ci, va = _dev(torch, A.rowptr), _dev(torch, A.col), _dev(torch, A.val)
+        s.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A.nnz, keep=(rp, ci, va))
+        ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n))
+        for t in range(2 if k else 1):
+            b = gdmr.rand_normal(11, t, n)
+            xd = torch.zeros(n, dtype=torch.float64, device="cuda")
+            bd = _dev(torch, b)
+            res, its = s.solve(xd.data_ptr(), bd.data_ptr(), 1e-10, n, mode=t)
+            st, hist = s.stats()
+            xo, reso, itso, sto = ora.solve(b, tol=1e-10, mode=t)
+            cyc_o, mv_o, brk_o = ora.counts()
+            print(f"breakdown k={k} t={t}: breakdowns {st.breakdowns} vs {brk_o}, "
+                  f"cycles {st.cycles} vs {cyc_o}, res {res:.3e}", flush=True)
+            assert st.breakdowns == brk_o
+            if t == 0:
+                assert brk_o >= 1
+            assert abs(its - itso) <= 1
+            assert _true_res(A, b, xd.cpu().numpy()) <= 1e-10 * 1.01
+        s.close()
+
+
+def test_early_exit_and_errors(torch_gpu):
+    torch = torch_gpu
+    A = gdmr.conv_diff_2d(8)
+    n = A.n
+    s = gdmr.Solver(n, 6, 2)
+    rp, ci, va = _dev(torch, A.rowptr), _dev(torch, A.col), _dev(torch, A.val)
+    s.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A.nnz, keep=(rp, ci, va))
+    xs = gdmr.rand_normal(1, 1, n)
+    b = A.matvec(xs)
+    bd = _dev(torch, b)
+    # mode != 0 before any subspace exists is an error, reported through dmrGetLastError.
+    xd = torch.zeros(n, dtype=torch.float64, device="cuda")
+    s.solve(xd.data_ptr(), bd.data_ptr(), 1e-8, 100, mode=1)
+    assert gdmr.last_error()[0] == gdmr.DMR_ERR_STATE
+    # x0 = exact solution: early exit returns the absolute residual, numIts = matRank^2.
+    xd = _dev(torch, xs)
+    res, its = s.solve(xd.data_ptr(), bd.data_ptr(), 1e-8, 100, mode=0)
+    assert its == 100 * 100
+    assert res < 1e-8 * np.linalg.norm(b)
+    assert np.array_equal(xd.cpu().numpy(), xs)
+    s.close()
+
+
+def test_operator_error_channel(torch_gpu):
+    """An operator that raises dmrRaiseOperatorError stops the solve with DMR_ERR_OPERATOR."""
+    torch = torch_gpu
+    n = 64
+    s = gdmr.Solver(n, 8, 2)
+
+    def op(src, dst, stream, ctx):
+        gdmr.lib.dmrRaiseOperatorError(b"injected failure")
+
+    s.set_operator(op)
+    xd = torch.zeros(n, dtype=torch.float64, device="cuda")
+    bd = torch.ones(n, dtype=torch.float64, device="cuda")
+    s.solve(xd.data_ptr(), bd.data_ptr(), 1e-8, 10, mode=0)
+    code, msg = gdmr.last_error()
+    assert code == gdmr.DMR_ERR_OPERATOR and "injected" in msg
+    s.close()
+
+
+def test_mixed_precision_reaches_fp64_tolerance(torch_gpu):
+    """FP32 basis + FP32 SpMV inside FP64 refinement reaches 1e-10 FP64-checked, and its solution
+    agrees with the FP64 oracle's to the conditioning of the problem."""
+    torch = torch_gpu
+    A = gdmr.conv_diff_2d(64)
+    n, m, k, tol = A.n, 30, 10, 1e-10
+    s = gdmr.Solver(n, m, k, precision=gdmr.PREC_FP32)
+    rp, ci, va = _dev(torch, A.rowptr), _dev(torch, A.col), _dev(torch, A.val)
+    s.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A.nnz, keep=(rp, ci, va))
+    ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n))
+    for t in range(3):
+        b = gdmr.rand_normal(0, 1 + t, n)
+        xd = torch.zeros(n, dtype=torch.float64, device="cuda")
+        bd = _dev(torch, b)
+        res, its = s.solve(xd.data_ptr(), bd.data_ptr(), tol, 10000, mode=t)
+        x = xd.cpu().numpy()
+        tr = _true_res(A, b, x)
+        xo, _, _, _ = ora.solve(b, tol=tol, mode=t)
+        err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+        st, hist = s.stats()
+        print(f"mixed[{t}]: res {res:.3e} true {tr:.3e} cycles {st.cycles} err vs oracle {err:.2e}",
+              flush=True)
+        assert tr <= tol and abs(res - tr) <= 1e-3 * tr + 1e-16
+        assert err < 1e-7
+    s.close()

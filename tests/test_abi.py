@@ -1,0 +1,57 @@
+"""The C-ABI library loads on the CPU and exports every function the public headers declare
+(no GPU calls)."""
+import os
+import re
+import subprocess
+
+import gdmr
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"^typedef[^;]*;", "", text, flags=re.M)
+    names = re.findall(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\([^;{]*\)\s*;", text, flags=re.M)
+    return {n for n in names if n not in ("if", "while")}
+
+
+def test_headers_fully_exported():
+    declared = _declared("gDMR.h") | _declared("gDMRC.h")
+    assert {"initDMR", "freeDMR", "solverDMR", "impSVDMR", "expSVDMR", "initLS", "solveLS",
+            "linOptLS", "impCsrLS"} <= declared
+    out = subprocess.run(["nm", "-D", "--defined-only", gdmr.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = declared - exported
+    assert not missing, f"declared but not exported: {sorted(missing)}"
+    assert set(gdmr.HEADER_SYMBOLS) <= exported
+    assert declared <= set(gdmr.HEADER_SYMBOLS) | {"dmrOperatorFn"}
+
+
+def test_reference_symbol_set_and_soname():
+    """The reference's 5 solver + 10 linear-system entry points (gDMR.h:13-34, gDMRC.h:6-24), and
+    the soname of its link model (Makefile:18-21)."""
+    ref = ("initDMR freeDMR solverDMR impSVDMR expSVDMR initLS freeLS impImgLS expImgLS impSrcLS "
+           "expSrcLS impOptLS expOptLS solveLS linOptLS").split()
+    for s in ref:
+        assert hasattr(gdmr.lib, s)
+    dyn = subprocess.run(["readelf", "-d", gdmr.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    assert "libgDMR.so.1" in dyn
+    prog = os.path.join(gdmr.LIB_DIR, "progDMR")
+    assert os.path.exists(prog)
+    deps = subprocess.run(["readelf", "-d", prog], capture_output=True, text=True).stdout
+    assert "libgDMR.so.1" in deps and "$ORIGIN" in deps
+
+
+def test_cpu_only_entry_points():
+    """Entry points that need no device: errors before init, sizes, footprints."""
+    assert gdmr.reduction_blocks(1) == 1
+    assert gdmr.reduction_blocks(4096) == 8
+    assert gdmr.reduction_blocks(2097152) == 512
+    assert gdmr.footprint(2097152, 50, 20) > 8 * 2097152 * 91
+    its = gdmr.C.c_int(0)
+    r = gdmr.lib.solverDMR(0, None, None, None, 0, 1e-8, 10, gdmr.C.byref(its))
+    assert r == 1.0 and gdmr.last_error()[0] == gdmr.DMR_ERR_STATE

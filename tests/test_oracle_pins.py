@@ -1,0 +1,298 @@
+"""CPU pins of the FP64 oracle and of the problem generators.
+
+Each check is chosen so a plausible mistake (a dropped term, a wrong sign or index, a transposed
+operand, a wrong column picked from the SVD) fails it: independent dense assembly of the PDE
+stencils, LAPACK (numpy) least squares and SVD, an independent textbook GMRES(m) for k = 0, the
+Arnoldi relation and basis orthogonality of a deflated cycle, breakdown on an operator with few
+distinct eigenvalues, exact deflation, and FP64-checked true residuals.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import gdmr
+import oracle
+
+
+# ------------------------------------------------------------------------------------------------
+# Random numbers: published SplitMix64 outputs (Vigna's splitmix64.c / Java SplittableRandom).
+# ------------------------------------------------------------------------------------------------
+def test_splitmix64_known_answers():
+    assert [hex(v) for v in gdmr.splitmix64(0, 3)] == [
+        "0xe220a8397b1dcdaf", "0x6e789e6aa1b965f4", "0x6c45d188009454f"]
+    assert list(gdmr.splitmix64(1234567, 5)) == [
+        6457827717110365317, 3203168211198807973, 9817491932198370423,
+        4593380528125082431, 16408922859458223821]
+
+
+def test_normals_are_standard_and_reproducible():
+    a = gdmr.rand_normal(0, 1, 200_001)
+    assert np.array_equal(a, gdmr.rand_normal(0, 1, 200_001))
+    assert not np.array_equal(a[:100], gdmr.rand_normal(0, 2, 100))
+    assert abs(a.mean()) < 0.01 and abs(a.std() - 1.0) < 0.01
+    u = gdmr.rand_uniform(3, 4, 100_000, -0.01, 0.01)
+    assert u.min() >= -0.01 and u.max() < 0.01 and abs(u.mean()) < 2e-4
+
+
+# ------------------------------------------------------------------------------------------------
+# Generators against an independent dense assembly.
+# ------------------------------------------------------------------------------------------------
+def dense_convdiff(N, dim, a, b, scale):
+    """Dense h^2 (-div(a s grad u) + b.grad u), written from the PDE, not from the generator."""
+    h = 1.0 / (N + 1)
+    n = N ** dim
+    A = np.zeros((n, n))
+    idx = np.array(np.unravel_index(np.arange(n), (N,) * dim, order="F")).T  # x fastest
+    strides = [N ** d for d in range(dim)]
+    for r in range(n):
+        for d in range(dim):
+            for sgn in (-1, 1):
+                inside = 0 <= idx[r, d] + sgn < N
+                nb = r + sgn * strides[d]
+                face = a[d] * (0.5 * (scale[r] + scale[nb]) if inside else scale[r])
+                A[r, r] += face
+                if inside:
+                    A[r, nb] += -face + sgn * b[d] * h / 2
+    return A
+
+
+@pytest.mark.parametrize("N,jit", [(5, 0.0), (4, 0.05)])
+def test_convdiff_2d_matches_dense_assembly(N, jit):
+    A = gdmr.conv_diff_2d(N, 0.1, (1.0, 0.5), jitter=jit, seed=3, stream=9)
+    s = 1.0 + (gdmr.rand_uniform(3, 9, N * N, -jit, jit) if jit else np.zeros(N * N))
+    D = dense_convdiff(N, 2, (0.1, 0.1), (1.0, 0.5), s)
+    assert np.allclose(A.to_scipy().toarray(), D, rtol=0, atol=1e-15)
+    assert A.nnz == 5 * N * N - 4 * N
+    assert all(np.all(np.diff(A.col[A.rowptr[i]:A.rowptr[i + 1]]) > 0) for i in range(A.n))
+
+
+@pytest.mark.parametrize("N,jit", [(4, 0.0), (3, 0.02)])
+def test_convdiff_3d_matches_dense_assembly(N, jit):
+    A = gdmr.conv_diff_3d(N, 0.1, (1.0, 0.5, 0.25), jitter=jit, seed=5, stream=2)
+    s = 1.0 + (gdmr.rand_uniform(5, 2, N ** 3, -jit, jit) if jit else np.zeros(N ** 3))
+    D = dense_convdiff(N, 3, (0.1,) * 3, (1.0, 0.5, 0.25), s)
+    assert np.allclose(A.to_scipy().toarray(), D, rtol=0, atol=1e-15)
+    assert A.nnz == 7 * N ** 3 - 6 * N ** 2
+
+
+def test_aniso_and_kron_assembly():
+    """Constant coefficients: the stencil equals a Kronecker-sum assembly (scipy.sparse)."""
+    import scipy.sparse as sp
+    N = 6
+    A = gdmr.aniso_3d(N, (1.0, 0.1, 0.01)).to_scipy()
+    T = sp.diags([-np.ones(N - 1), 2 * np.ones(N), -np.ones(N - 1)], [-1, 0, 1])
+    I = sp.identity(N)
+    K = (sp.kron(I, sp.kron(I, T)) * 1.0 + sp.kron(I, sp.kron(T, I)) * 0.1 +
+         sp.kron(T, sp.kron(I, I)) * 0.01)
+    assert abs(A - K).max() < 1e-15
+    # Interior rows of constant-coefficient convection-diffusion sum to zero.
+    C2 = gdmr.conv_diff_2d(8).to_scipy()
+    rs = np.asarray(C2.sum(axis=1)).ravel().reshape(8, 8, order="F")
+    assert np.max(np.abs(rs[1:-1, 1:-1])) < 1e-15
+
+
+# ------------------------------------------------------------------------------------------------
+# Oracle kernels.
+# ------------------------------------------------------------------------------------------------
+def test_spmv_and_dot():
+    A = gdmr.conv_diff_3d(7, jitter=0.01, seed=1)
+    x = gdmr.rand_normal(2, 3, A.n)
+    y = oracle.spmv(A, x)
+    ys = A.to_scipy() @ x
+    assert np.max(np.abs(y - ys)) <= 1e-15 * np.max(np.abs(ys)) * 8
+    for n in (1, 7, 4096, 100_003):
+        a, b = gdmr.rand_normal(4, 5, n), gdmr.rand_normal(6, 7, n)
+        exact = math.fsum(a * b)
+        for G in (1, 3, gdmr.reduction_blocks(n)):
+            d = oracle.dot(a, b, G)
+            assert abs(d - exact) <= 2 * n * 2.2e-16 * np.sum(np.abs(a * b)) + 1e-300
+
+
+def _hessenberg(m, seed, shift=4.0):
+    """Random upper Hessenberg (m+1) x m with a diagonal shift that keeps it well conditioned."""
+    rng = np.random.default_rng(seed)
+    H = np.triu(rng.standard_normal((m + 1, m)), -1)
+    H[np.arange(m), np.arange(m)] += shift
+    return H
+
+
+@pytest.mark.parametrize("m", [1, 5, 30, 50])
+def test_lsq_matches_lapack(m):
+    H = _hessenberg(m, m)
+    t = np.random.default_rng(100 + m).standard_normal(m + 1)
+    y, z = oracle.lsq(H, t)
+    ref = np.linalg.lstsq(H, t, rcond=None)[0]
+    kappa = np.linalg.cond(H)
+    assert y[m] == 0.0
+    assert np.max(np.abs(y[:m] - ref)) <= 1e-14 * kappa * max(1.0, np.max(np.abs(ref)))
+    assert np.allclose(z, H @ y[:m], rtol=0, atol=1e-12 * np.abs(H).max() * np.abs(y).max() * m)
+    # Normal equations: H^T (t - H y) = 0.
+    assert np.max(np.abs(H.T @ (t - H @ y[:m]))) <= 1e-10 * np.linalg.norm(t) * np.linalg.norm(H)
+
+
+@pytest.mark.parametrize("m,k", [(6, 2), (30, 10), (50, 20), (31, 5)])
+def test_svd_matches_lapack(m, k):
+    H = _hessenberg(m, 7 * m)
+    Cm, sig, sweeps = oracle.svd_smallest(H, k)
+    U, S, Vt = np.linalg.svd(H)
+    assert 1 <= sweeps < 128
+    assert np.max(np.abs(sig - S)) <= 1e-12 * S[0]
+    for i in range(k):
+        v = Vt[m - k + i]
+        c = Cm[:, i]
+        assert abs(np.linalg.norm(c) - 1.0) < 1e-12
+        gap = min(abs(S[m - k + i] - S[j]) for j in range(m) if j != m - k + i) / S[0]
+        if gap > 1e-6:
+            assert min(np.linalg.norm(c - v), np.linalg.norm(c + v)) < 1e-9 / gap
+        j = np.argmax(np.abs(c))
+        assert c[j] > 0
+    assert np.max(np.abs(Cm.T @ Cm - np.eye(k))) < 1e-12
+
+
+# ------------------------------------------------------------------------------------------------
+# Whole solver.
+# ------------------------------------------------------------------------------------------------
+def textbook_gmres(A, b, m, tol, max_cycles):
+    """Restarted GMRES(m), classical Gram-Schmidt, LSQ on beta e1 (Saad, Iterative Methods for
+    Sparse Linear Systems, Alg. 6.9), written independently of the oracle.  Returns the relative
+    residual after each cycle."""
+    n = b.size
+    x = np.zeros(n)
+    beta0 = np.linalg.norm(b)
+    hist = []
+    for _ in range(max_cycles):
+        r = b - A @ x
+        beta = np.linalg.norm(r)
+        if beta <= tol * beta0:
+            break
+        V = np.zeros((n, m + 1))
+        Hm = np.zeros((m + 1, m))
+        V[:, 0] = r / beta
+        for j in range(m):
+            w = A @ V[:, j]
+            h = V[:, :j + 1].T @ w
+            w = w - V[:, :j + 1] @ h
+            Hm[:j + 1, j] = h
+            Hm[j + 1, j] = np.linalg.norm(w)
+            V[:, j + 1] = w / Hm[j + 1, j]
+        e1 = np.zeros(m + 1)
+        e1[0] = beta
+        y = np.linalg.lstsq(Hm, e1, rcond=None)[0]
+        x = x + V[:, :m] @ y
+        hist.append(np.linalg.norm(b - A @ x) / beta0)
+        if hist[-1] <= tol:
+            break
+    return np.array(hist), x
+
+
+def test_k0_reduces_to_textbook_gmres():
+    A = gdmr.conv_diff_2d(20)
+    b = gdmr.rand_normal(0, 1, A.n)
+    m, tol = 15, 1e-9
+    ora = oracle.Oracle(A, m, 0, gdmr.reduction_blocks(A.n))
+    x, res, its, st = ora.solve(b, tol=tol)
+    h = ora.history()
+    ref, xr = textbook_gmres(A.to_scipy(), b, m, tol, 200)
+    assert st == 0 and len(h) == len(ref)
+    assert np.max(np.abs(h - ref) / ref) < 1e-6
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-7
+
+
+def test_config_a_sequence_properties():
+    """Config A: residual histories never increase (q <= 1), the true FP64 residual meets the
+    tolerance and equals the tracked one, and recycling Z cuts the cycle count."""
+    A = gdmr.conv_diff_2d(64)
+    n, m, k, tol = A.n, 30, 10, 1e-8
+    B = gdmr.rand_normal(0, 1, 4 * n).reshape(4, n)
+    ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n))
+    cycles = []
+    for i in range(4):
+        x, res, its, st = ora.solve(B[i], tol=tol, mode=i)
+        h = ora.history()
+        true = np.linalg.norm(B[i] - A.matvec(x)) / np.linalg.norm(B[i])
+        assert st == 0 and its == len(h) - 1
+        assert np.all(np.diff(h) <= 0)
+        assert true <= tol and abs(true - res) <= 1e-6 * res
+        cycles.append(len(h))
+    assert cycles[0] > min(cycles[1:])
+    Z = ora.get_z()
+    assert np.allclose(np.linalg.norm(Z, axis=1), 1.0, atol=1e-12)
+
+
+def test_deflated_cycle_arnoldi_relation():
+    """A [Z V_0..V_{m-k-1}] = [W V_0..V_{m-k}] H for a deflated cycle, and P = [W V] has
+    orthonormal columns up to classical Gram-Schmidt's loss."""
+    A = gdmr.conv_diff_2d(24, jitter=0.01, seed=2)
+    n, m, k = A.n, 20, 6
+    ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n), keep_last=True)
+    As = A.to_scipy()
+    for mode in (0, 1):
+        ora.solve(gdmr.rand_normal(1, mode, n), tol=1e-6, mode=mode)
+    P, H, Z, d = ora.last_cycle()
+    assert d == k
+    src = np.vstack([Z, P[k:m]])          # source basis [Z V_0..V_{m-k-1}], m vectors
+    lhs = As @ src.T                       # n x m
+    rhs = P.T @ H                          # [W V] H, n x m
+    assert np.linalg.norm(lhs - rhs) <= 1e-12 * np.linalg.norm(lhs)
+    G = P @ P.T
+    assert np.max(np.abs(G - np.eye(m + 1))) < 1e-8
+    # A wrong block choice breaks the relation: e.g. using P[0..m-1] as the source.
+    assert np.linalg.norm(As @ P[:m].T - rhs) > 1e-3 * np.linalg.norm(lhs)
+
+
+def test_breakdown_on_diagonal_operator():
+    n, m = 300, 10
+    d = np.repeat([1.0, 2.0, 3.0, 5.0, 8.0], n // 5)
+    A = gdmr.CsrMatrix(n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), d)
+    for k in (0, 3):
+        ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n))
+        b = gdmr.rand_normal(11, 0, n)
+        x, res, its, st = ora.solve(b, tol=1e-10)
+        cyc, mv, brk = ora.counts()
+        assert st == 0 and brk >= 1 and cyc <= 2
+        assert np.linalg.norm(b - d * x) / np.linalg.norm(b) <= 1e-10
+
+
+def test_exact_deflation_space_helps():
+    """Diagonal A with eigenvalues 1..n: with Z = the k smallest eigenvectors the deflated solve
+    needs fewer cycles than the undeflated one."""
+    n, m, k = 400, 12, 6
+    d = np.arange(1, n + 1, dtype=np.float64)
+    A = gdmr.CsrMatrix(n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), d)
+    b = gdmr.rand_normal(5, 5, n)
+    plain = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n))
+    plain.solve(b, tol=1e-10, mode=0)
+    exact = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n))
+    exact.set_z(np.eye(n)[:k])
+    x, res, its, st = exact.solve(b, tol=1e-10, mode=1)
+    assert st == 0
+    assert exact.counts()[0] < plain.counts()[0]
+    assert np.linalg.norm(b - d * x) / np.linalg.norm(b) <= 1e-10
+
+
+def test_stagnation_and_cycle_limit():
+    """A cyclic shift stalls GMRES(m) (q = 1 > 0.995 every cycle): the stagnation rule switches
+    the start vector and the cycle limit matRank ends the solve (status 6, numIts = cycles)."""
+    n, m, k = 64, 8, 0
+    A = gdmr.CsrMatrix(n, np.arange(n + 1, dtype=np.int32),
+                       ((np.arange(n) + 1) % n).astype(np.int32), np.ones(n))
+    b = np.zeros(n)
+    b[0] = 1.0
+    ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n))
+    x, res, its, st = ora.solve(b, tol=1e-10, mat_rank=20)
+    h = ora.history()
+    # Cycle limit (status 6) or, when rounding lifts q a hair above 1, the runaway guard (7).
+    assert st in (6, 7) and its == len(h)
+    assert np.all(np.abs(h - 1.0) < 1e-12) and np.all(np.isfinite(x))
+
+
+def test_early_exit_semantics():
+    A = gdmr.conv_diff_2d(8)
+    xs = gdmr.rand_normal(9, 9, A.n)
+    b = A.matvec(xs)
+    ora = oracle.Oracle(A, 6, 2, gdmr.reduction_blocks(A.n))
+    x, res, its, st = ora.solve(b, x0=xs, tol=1e-8, mat_rank=50)
+    assert st == -1 and its == 2500 and res < 1e-8 * np.linalg.norm(b)
+    x, res, its, st = ora.solve(b, tol=1e-8, mode=1)
+    assert st == 2  # mode != 0 without a subspace

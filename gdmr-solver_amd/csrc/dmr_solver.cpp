@@ -485,9 +485,23 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
         return 0;
       });
     }
+    if (hipMemcpyAsync(c->hostStatus, c->scal2, 4 * sizeof(double), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(c->hostStatus + 8, c->flag, sizeof(int), hipMemcpyDeviceToHost, s) !=
+            hipSuccess)
+      return set_err(DMR_ERR_HIP, "status readback failed");
+    if (hipStreamSynchronize(s) != hipSuccess) return set_err(DMR_ERR_HIP, "stream sync failed");
+    if (c->prof.on) prof_collect(c);
+    const double q = c->hostStatus[2];
+    int bflag;
+    std::memcpy(&bflag, c->hostStatus + 8, sizeof(int));
+    if (bflag) ++c->st.breakdowns;
     // ---- genDeflatDMR (gDMR.cu:1583-1704): Z <- normalize([V Z] C), C = smallest right
-    // singular vectors of H.
-    if (k > 0) {
+    // singular vectors of H.  After a breakdown the basis beyond the breakdown step is zero and H
+    // is padded with identity columns, so singular vectors from the padding combine zero vectors
+    // (the reference then divides by sqrt(0)); the refresh is skipped for such a cycle and the
+    // previous subspace is kept.
+    if (k > 0 && bflag == 0) {
       timed(c, s, DMR_PH_SVD, 0.0, [&] {
         dmr::svd_smallest(c->H, m, k, kSvdTol, kSvdMaxSweeps, c->C, c->sig, c->sweeps, s);
         return 0;
@@ -501,17 +515,6 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
       std::swap(w.Z, w.Zn);
       w.hasZ = true;
     }
-    if (hipMemcpyAsync(c->hostStatus, c->scal2, 4 * sizeof(double), hipMemcpyDeviceToHost, s) !=
-            hipSuccess ||
-        hipMemcpyAsync(c->hostStatus + 8, c->flag, sizeof(int), hipMemcpyDeviceToHost, s) !=
-            hipSuccess)
-      return set_err(DMR_ERR_HIP, "status readback failed");
-    if (hipStreamSynchronize(s) != hipSuccess) return set_err(DMR_ERR_HIP, "stream sync failed");
-    if (c->prof.on) prof_collect(c);
-    const double q = c->hostStatus[2];
-    int bflag;
-    std::memcpy(&bflag, c->hostStatus + 8, sizeof(int));
-    if (bflag) ++c->st.breakdowns;
     if (q > kStagnation) {
       c->kryLoopFlag = 1;
       ++c->kryFlagCount;
@@ -523,6 +526,10 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
     ++c->st.cycles;
     ++grdItr;
   }
+  if (c->prof.on) {
+    if (hipStreamSynchronize(s) != hipSuccess) return set_err(DMR_ERR_HIP, "stream sync failed");
+    prof_collect(c);
+  }
   return DMR_OK;
 }
 
@@ -532,11 +539,9 @@ double solve_fp64(dmrContext *c, int mode, double *x, const double *b, hipStream
   const long long n = c->n, ld = c->ld;
   const int k = c->k;
   *numIts = matRank * matRank;  // gDMR.cu:2646
-  const int startDef = (mode == 0) ? 0 : k;
-  if (startDef > 0 && !w.hasZ) {
-    set_err(DMR_ERR_STATE, "mode != 0 requires a deflation subspace (previous solve or impSVDMR)");
-    return 1.0;
-  }
+  // mode != 0 deflates the first cycle with the stored subspace; with none stored (no previous
+  // solve, or only cycles that broke down) the first cycle runs undeflated as for mode 0.
+  const int startDef = (mode != 0 && w.hasZ) ? k : 0;
   c->kryLoopFlag = 0;
   const dmr::RedWork red = c->red;
   Op64 op{c, s};
@@ -623,10 +628,6 @@ double solve_mixed(dmrContext *c, int mode, double *x, const double *b, hipStrea
   const long long n = c->n, ld = c->ld;
   const int k = c->k;
   *numIts = 0;
-  if (mode != 0 && !w.hasZ) {
-    set_err(DMR_ERR_STATE, "mode != 0 requires a deflation subspace (previous solve or impSVDMR)");
-    return 1.0;
-  }
   if (c->opKind != OP_CSR) {
     set_err(DMR_ERR_STATE, "mixed precision needs the built-in CSR operator");
     return 1.0;
@@ -650,7 +651,7 @@ double solve_mixed(dmrContext *c, int mode, double *x, const double *b, hipStrea
   int totalCycles = 0;
   double relres = 1.0;
   c->kryLoopFlag = 0;
-  bool useZ = mode != 0;
+  bool useZ = mode != 0 && w.hasZ;
   for (int outer = 0;; ++outer) {
     dmr::residual64(c->csr, c->x64, c->b64, c->r64, c->redOp, c->scal, s);
     ++c->st.matvecs;

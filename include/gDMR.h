@@ -37,7 +37,8 @@ void initDMR(int blocks, int threadsPerBlock, int *devList, int numDevs, int ele
 void freeDMR(void);
 
 /* Solve A x = b (replaces gDMR.h:29, gDMR.cu:2638-2901).
- * mode != 0 starts from the stored deflation subspace (previous solve or impSVDMR).
+ * mode != 0 starts from the stored deflation subspace (previous solve or impSVDMR); with none
+ *   stored the first cycle runs undeflated, as for mode 0.
  * extSrcVec: device, n doubles; read as x0, overwritten by A x0 and the operator inputs, and on
  *   return holds the solution.  extImgVec: device, n doubles, the right-hand side b (read only).
  * solTol: relative residual target; also the absolute Krylov breakdown threshold.

@@ -439,7 +439,7 @@ int run_cycles(OracleCtx &c, int startDef, double tol, int matRank, double beta,
     const int o = d > 0 ? k : 0;
     std::vector<double> Zused;
     if (c.keepLast) Zused = v.Z;
-    arnoldi(c, d, matRank, tol);
+    const int brk = arnoldi(c, d, matRank, tol);
     if (c.keepLast) {
       c.lastP = v.P;
       c.lastH = c.H;
@@ -457,8 +457,9 @@ int run_cycles(OracleCtx &c, int startDef, double tol, int matRank, double beta,
     scale(v.r.data(), n, 1.0 / std::sqrt(rr));
     comb_acc(v.xa.data(), v.xa.data(), Po, m - d, c.y.data() + d, v.Z.data(), d, c.y.data(), ld, n,
              rho);
-    // genDeflatDMR
-    if (k > 0) {
+    // genDeflatDMR; skipped after a breakdown, whose identity-padded H would select combinations
+    // of zeroed basis vectors (the reference normalises them by 1/sqrt(0)).
+    if (k > 0 && brk == 0) {
       oracle_svd(c.H.data(), m, k, kSvdTol, kSvdMaxSweeps, c.C.data(), c.sig.data());
       std::vector<double> ss(k);
       for (int j = 0; j < k; ++j) {
@@ -531,7 +532,7 @@ int oracle_set_csr(OracleCtx *c, const int *rp, const int *ci, const double *va)
 
 // Same contract as dmrSolve / solverDMR (FP64).  x: in x0, out solution.  Returns the value the
 // solver returns; *numIts as the solver sets it; status: 0 converged, 6 cycle limit, 7 runaway,
-// -1 early exit, 2 mode != 0 without a subspace.
+// -1 early exit.
 double oracle_solve(OracleCtx *c, int mode, const double *b, double *x, double tol, int matRank,
                     int *numIts, int *status) {
   Vecs &v = c->v;
@@ -540,11 +541,8 @@ double oracle_solve(OracleCtx *c, int mode, const double *b, double *x, double t
   c->hist.clear();
   c->cycles = c->matvecs = c->breakdowns = 0;
   *numIts = matRank * matRank;
-  const int startDef = mode == 0 ? 0 : k;
-  if (startDef > 0 && !c->hasZ) {
-    *status = 2;
-    return 1.0;
-  }
+  // mode != 0 without a stored subspace runs the first cycle undeflated, as mode 0.
+  const int startDef = (mode != 0 && c->hasZ) ? k : 0;
   c->kryLoopFlag = 0;
   std::fill(v.r.begin(), v.r.end(), 0.0);
   std::fill(v.xa.begin(), v.xa.end(), 0.0);

@@ -294,5 +294,11 @@ def test_early_exit_semantics():
     ora = oracle.Oracle(A, 6, 2, gdmr.reduction_blocks(A.n))
     x, res, its, st = ora.solve(b, x0=xs, tol=1e-8, mat_rank=50)
     assert st == -1 and its == 2500 and res < 1e-8 * np.linalg.norm(b)
-    x, res, its, st = ora.solve(b, tol=1e-8, mode=1)
-    assert st == 2  # mode != 0 without a subspace
+    # mode != 0 with no stored subspace runs exactly like mode 0.
+    fresh = oracle.Oracle(A, 6, 2, gdmr.reduction_blocks(A.n))
+    x1, res1, its1, st1 = fresh.solve(b, tol=1e-8, mode=1)
+    h1 = fresh.history()
+    fresh0 = oracle.Oracle(A, 6, 2, gdmr.reduction_blocks(A.n))
+    x0, res0, its0, st0 = fresh0.solve(b, tol=1e-8, mode=0)
+    assert st1 == 0 and its1 == its0 and np.array_equal(h1, fresh0.history())
+    assert np.array_equal(x1, x0)

@@ -23,10 +23,18 @@
 namespace dmr {
 namespace {
 
-__device__ inline double wave_sum(double v) {
+// Xor butterfly over the 32 lanes of a half wavefront.
+__device__ inline double half_sum(double v) {
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+  for (int off = 16; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 32);
   return v;
+}
+// Sum of squares of a[0..len): lane hl adds rows hl, hl+32, ... in order, then half_sum.
+// Every lane of the half wavefront gets the result.
+__device__ inline double half_sumsq(const double *a, int len, int hl) {
+  double s = 0.0;
+  for (int i = hl; i < len; i += 32) s = fma(a[i], a[i], s);
+  return half_sum(s);
 }
 
 constexpr int kLsqThreads = 256;
@@ -105,7 +113,8 @@ __global__ __launch_bounds__(kSvdThreads) void k_svd(const double *__restrict__ 
   double *sg = V + (long long)m * mm;      // mm
   __shared__ int s_rot;
   const int tid = threadIdx.x, bd = blockDim.x;
-  const int lane = tid & 63, wave = tid >> 6, nw = bd >> 6;
+  // One half wavefront (32 lanes) per column pair: a round's m/2 pairs all run at once.
+  const int hl = tid & 31, hw = tid >> 5, nhw = bd >> 5;
   for (int idx = tid; idx < ldh * mm; idx += bd) {
     const int c = idx / ldh;
     A[idx] = c < m ? H[idx] : 0.0;
@@ -117,10 +126,13 @@ __global__ __launch_bounds__(kSvdThreads) void k_svd(const double *__restrict__ 
   __syncthreads();
   int sweep = 0;
   while (sweep < maxSweeps) {
+    // Exact squared column norms at the start of each sweep; within the sweep a rotation
+    // updates them as ||a_p||^2 - t g and ||a_q||^2 + t g.
+    for (int c = hw; c < m; c += nhw) sg[c] = half_sumsq(A + (long long)c * ldh, ldh, hl);
     if (tid == 0) s_rot = 0;
     __syncthreads();
     for (int r = 0; r < mm - 1; ++r) {
-      for (int pi = wave; pi < mm / 2; pi += nw) {
+      for (int pi = hw; pi < mm / 2; pi += nhw) {
         int p, q;
         if (pi == 0) {
           p = r;
@@ -136,33 +148,32 @@ __global__ __launch_bounds__(kSvdThreads) void k_svd(const double *__restrict__ 
         }
         if (q >= m) continue;
         double *ap = A + (long long)p * ldh, *aq = A + (long long)q * ldh;
-        double al = 0.0, be = 0.0, ga = 0.0;
-        for (int i = lane; i < ldh; i += 64) {
-          const double x = ap[i], y = aq[i];
-          al = fma(x, x, al);
-          be = fma(y, y, be);
-          ga = fma(x, y, ga);
-        }
-        al = wave_sum(al);
-        be = wave_sum(be);
-        ga = wave_sum(ga);
+        double ga = 0.0;
+        for (int i = hl; i < ldh; i += 32) ga = fma(ap[i], aq[i], ga);
+        ga = half_sum(ga);
+        const double al = sg[p], be = sg[q];
         if (ga != 0.0 && fabs(ga) > tol * sqrt(al * be)) {
           const double zeta = (be - al) / (2.0 * ga);
           const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(fma(zeta, zeta, 1.0)));
           const double cs = 1.0 / sqrt(fma(tt, tt, 1.0));
           const double sn = cs * tt;
-          for (int i = lane; i < ldh; i += 64) {
+          for (int i = hl; i < ldh; i += 32) {
             const double x = ap[i], y = aq[i];
             ap[i] = cs * x - sn * y;
             aq[i] = sn * x + cs * y;
           }
           double *vp = V + (long long)p * m, *vq = V + (long long)q * m;
-          for (int i = lane; i < m; i += 64) {
+          for (int i = hl; i < m; i += 32) {
             const double x = vp[i], y = vq[i];
             vp[i] = cs * x - sn * y;
             vq[i] = sn * x + cs * y;
           }
-          if (lane == 0) atomicAdd(&s_rot, 1);
+          if (hl == 0) {
+            const double tg = tt * ga;
+            sg[p] = al - tg;
+            sg[q] = be + tg;
+            atomicAdd(&s_rot, 1);
+          }
         }
       }
       __syncthreads();
@@ -173,12 +184,9 @@ __global__ __launch_bounds__(kSvdThreads) void k_svd(const double *__restrict__ 
     if (rot == 0) break;
   }
   // Column norms = singular values.
-  for (int c = wave; c < m; c += nw) {
-    const double *ac = A + (long long)c * ldh;
-    double s = 0.0;
-    for (int i = lane; i < ldh; i += 64) s = fma(ac[i], ac[i], s);
-    s = wave_sum(s);
-    if (lane == 0) sg[c] = sqrt(s);
+  for (int c = hw; c < m; c += nhw) {
+    const double s = half_sumsq(A + (long long)c * ldh, ldh, hl);
+    if (hl == 0) sg[c] = sqrt(s);
   }
   __syncthreads();
   for (int c = tid; c < m; c += bd) {

@@ -13,8 +13,8 @@ namespace dmr {
 constexpr int kBlock = 256;
 // Cap on the number of row blocks of a fixed-order reduction.
 constexpr int kMaxRedBlocks = 512;
-// Coefficient columns handled per block of the multi-dot (GEMV-T) kernel.
-constexpr int kDotCols = 8;
+// Widest column group of the multi-dot (GEMV-T) kernel.
+constexpr int kDotCols = 16;
 // Largest single-output coefficient count the combination kernels stage in LDS.
 constexpr int kMaxCoef = 256;
 // Row blocks of the CSR SpMV hold at most this many nonzeros (staged in LDS) and 256 rows.

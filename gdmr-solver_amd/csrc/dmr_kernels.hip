@@ -176,12 +176,70 @@ __device__ inline void finalize(const Fin &fin, double ss, double sg) {
   }
 }
 
+// o += sum_l cs[l] * B_l[u] for l < p, in column order.  Columns are loaded eight at a time,
+// all eight loads issued before the fused multiply-adds; indices past p alias column p-1 (a
+// cache hit) and are not used.
+template <typename TS, typename TA, int N>
+__device__ inline void fma_cols(const TS *__restrict__ B, int p, long long ld, long long u,
+                                const TA *cs, TA (&o)[N]) {
+  for (int l0 = 0; l0 < p; l0 += 8) {
+    TA b[8][N];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) ldu(B + (long long)min(l0 + i, p - 1) * ld, u, b[i]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (l0 + i < p) {
+        const TA c = cs[l0 + i];
+#pragma unroll
+        for (int e = 0; e < N; ++e) o[e] = fmac(c, b[i][e], o[e]);
+      }
+    }
+  }
+}
+
 // =============================================================================================
 // Multi-dot: out[l] = B_l . x for the p columns of B (GEMV-T).  Replaces basisProjDMR's
 // cublasCgemv(OP_C) (gDMR.cu:548-590).  grid = (G row blocks, ceil(p / kDotCols) column groups);
 // x is re-read once per column group.
 // =============================================================================================
-template <typename TS>
+// Block sums of CB values per thread at once: wavefront butterflies on all CB values, then the
+// four wavefront sums in order.  Column c's result lands in res[c] (LDS) for c < CB.
+template <typename TA, int CB>
+__device__ inline void block_sum_cols(TA (&v)[CB], TA *res) {
+  __shared__ TA red[kBlock / 64][CB];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < CB; ++c) v[c] = wave_sum(v[c]);
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < CB; ++c) red[wv][c] = v[c];
+  }
+  __syncthreads();
+  if (threadIdx.x < CB) {
+    const int c = threadIdx.x;
+    res[c] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
+  }
+  __syncthreads();
+}
+
+// Last-block sum of the G partials of columns c0..c0+pc-1 (column-major, stride G), all columns
+// at once; thread t adds partials t, t+256, ... of each column in order.
+template <typename TA, int CB>
+__device__ inline void sum_partials_cols(const TA *part, int G, int c0, int pc, TA *res) {
+  TA v[CB];
+#pragma unroll
+  for (int c = 0; c < CB; ++c) v[c] = 0;
+  for (int q = threadIdx.x; q < G; q += kBlock) {
+    TA t[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) t[c] = c < pc ? get_partial(part + (long long)(c0 + c) * G + q) : (TA)0;
+#pragma unroll
+    for (int c = 0; c < CB; ++c) v[c] = v[c] + t[c];
+  }
+  block_sum_cols<TA, CB>(v, res);
+}
+
+template <typename TS, int CB>
 __global__ __launch_bounds__(kBlock) void k_multidot(const TS *__restrict__ B, long long ld, int p,
                                                      const TS *__restrict__ x, long long nv, int G,
                                                      typename Acc<TS>::type *partials,
@@ -190,43 +248,35 @@ __global__ __launch_bounds__(kBlock) void k_multidot(const TS *__restrict__ B, l
   using TA = typename Acc<TS>::type;
   constexpr int N = Unit<TS>::N;
   if (flag && *flag) return;
-  const int c0 = blockIdx.y * kDotCols;
-  const int pc = min(kDotCols, p - c0);
-  TA acc[kDotCols];
+  const int c0 = blockIdx.y * CB;
+  const int pc = min(CB, p - c0);
+  // Columns past the end of B (last group only) alias column p-1: their loads hit in cache and
+  // their sums are discarded, which keeps every load unconditional and in flight together.
+  const TS *col[CB];
 #pragma unroll
-  for (int c = 0; c < kDotCols; ++c) acc[c] = 0;
+  for (int c = 0; c < CB; ++c) col[c] = B + (long long)min(c0 + c, p - 1) * ld;
+  TA acc[CB];
+#pragma unroll
+  for (int c = 0; c < CB; ++c) acc[c] = 0;
   const long long stride = (long long)G * kBlock;
   for (long long u = (long long)blockIdx.x * kBlock + threadIdx.x; u < nv; u += stride) {
     TA xv[N];
     ldu(x, u, xv);
+    TA bv[CB][N];
 #pragma unroll
-    for (int c = 0; c < kDotCols; ++c) {
-      if (c < pc) {
-        TA bv[N];
-        ldu(B + (long long)(c0 + c) * ld, u, bv);
+    for (int c = 0; c < CB; ++c) ldu(col[c], u, bv[c]);
 #pragma unroll
-        for (int e = 0; e < N; ++e) acc[c] = fmac(bv[e], xv[e], acc[c]);
-      }
-    }
-  }
-  __shared__ TA red[kBlock / 64][kDotCols];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int c = 0; c < CB; ++c)
 #pragma unroll
-  for (int c = 0; c < kDotCols; ++c) {
-    const TA v = wave_sum(acc[c]);
-    if (lane == 0) red[wv][c] = v;
+      for (int e = 0; e < N; ++e) acc[c] = fmac(bv[c][e], xv[e], acc[c]);
   }
-  __syncthreads();
-  if (threadIdx.x < pc) {
-    const int c = threadIdx.x;
-    const TA s = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
-    put_partial(partials + (long long)(c0 + c) * G + blockIdx.x, s);
-  }
+  __shared__ TA res[CB];
+  block_sum_cols<TA, CB>(acc, res);
+  if (threadIdx.x < pc) put_partial(partials + (long long)(c0 + threadIdx.x) * G + blockIdx.x,
+                                    res[threadIdx.x]);
   if (!arrive_last(&counters[blockIdx.y], (unsigned)G)) return;
-  for (int c = 0; c < pc; ++c) {
-    const TA v = sum_partials(partials + (long long)(c0 + c) * G, G);
-    if (threadIdx.x == 0) out[(long long)(c0 + c) * ostride] = (double)v;
-  }
+  sum_partials_cols<TA, CB>(partials, G, c0, pc, res);
+  if (threadIdx.x < pc) out[(long long)(c0 + threadIdx.x) * ostride] = (double)res[threadIdx.x];
   if (threadIdx.x == 0) counters[blockIdx.y] = 0u;
 }
 
@@ -260,34 +310,14 @@ __global__ __launch_bounds__(kBlock) void k_combo(const TS *in, TS *out, const T
     TA o[N];
     if (MODE == COMB_SUB) {
       ldu(in, u, o);
-      for (int l = 0; l < p1; ++l) {
-        TA b[N];
-        ldu(B1 + (long long)l * ld, u, b);
-#pragma unroll
-        for (int e = 0; e < N; ++e) o[e] = fmac(cs[l], b[e], o[e]);
-      }
-      for (int l = 0; l < p2; ++l) {
-        TA b[N];
-        ldu(B2 + (long long)l * ld, u, b);
-#pragma unroll
-        for (int e = 0; e < N; ++e) o[e] = fmac(cs[p1 + l], b[e], o[e]);
-      }
+      fma_cols(B1, p1, ld, u, cs, o);
+      fma_cols(B2, p2, ld, u, cs + p1, o);
     } else {
       TA a[N];
 #pragma unroll
       for (int e = 0; e < N; ++e) a[e] = 0;
-      for (int l = 0; l < p1; ++l) {
-        TA b[N];
-        ldu(B1 + (long long)l * ld, u, b);
-#pragma unroll
-        for (int e = 0; e < N; ++e) a[e] = fmac(cs[l], b[e], a[e]);
-      }
-      for (int l = 0; l < p2; ++l) {
-        TA b[N];
-        ldu(B2 + (long long)l * ld, u, b);
-#pragma unroll
-        for (int e = 0; e < N; ++e) a[e] = fmac(cs[p1 + l], b[e], a[e]);
-      }
+      fma_cols(B1, p1, ld, u, cs, a);
+      fma_cols(B2, p2, ld, u, cs + p1, a);
       if (in) {
         TA xi[N];
         ldu(in, u, xi);
@@ -317,20 +347,18 @@ __global__ __launch_bounds__(kBlock) void k_combo(const TS *in, TS *out, const T
     }
   }
   if (RED == 0) return;
-  if (RED & 1) {
-    const TA s = block_sum(ss);
-    if (threadIdx.x == 0) put_partial(partials + blockIdx.x, s);
-  }
-  if (RED & 2) {
-    const TA s = block_sum(sg);
-    if (threadIdx.x == 0) put_partial(partials + G + blockIdx.x, s);
-  }
+  constexpr int NR = (RED & 2) ? 2 : 1;  // partial columns: ss, then sg
+  __shared__ TA res[2];
+  TA v[NR];
+  v[0] = ss;
+  if (NR == 2) v[NR - 1] = sg;
+  block_sum_cols<TA, NR>(v, res);
+  if (threadIdx.x < NR) put_partial(partials + (long long)threadIdx.x * G + blockIdx.x,
+                                    res[threadIdx.x]);
   if (!arrive_last(counter, (unsigned)G)) return;
-  TA tss = 0, tsg = 0;
-  if (RED & 1) tss = sum_partials(partials, G);
-  if (RED & 2) tsg = sum_partials(partials + G, G);
+  sum_partials_cols<TA, NR>(partials, G, 0, NR, res);
   if (threadIdx.x == 0) {
-    finalize(fin, (double)tss, (double)tsg);
+    finalize(fin, (double)res[0], NR == 2 ? (double)res[1] : 0.0);
     *counter = 0u;
   }
 }
@@ -676,9 +704,22 @@ void multidot(const TS *B, long long ld, int p, const TS *x, long long n, double
   if (p <= 0) return;
   using TA = typename Acc<TS>::type;
   const int G = red_blocks<TS>(n);
-  const int groups = (p + kDotCols - 1) / kDotCols;
-  hipLaunchKernelGGL((k_multidot<TS>), dim3(G, groups), dim3(kBlock), 0, s, B, ld, p, x,
-                     units<TS>(n), G, (TA *)w.partials, w.counters, out, ostride, flag);
+  const long long nv = units<TS>(n);
+  // Column-group width: the smallest of 1, 2, 4, 8, 16 that holds p, else groups of 16.
+  const int cb = p <= 1 ? 1 : p <= 2 ? 2 : p <= 4 ? 4 : p <= 8 ? 8 : kDotCols;
+  const int groups = (p + cb - 1) / cb;
+  TA *part = (TA *)w.partials;
+#define DMR_MDOT(CBV)                                                                      \
+  hipLaunchKernelGGL((k_multidot<TS, CBV>), dim3(G, groups), dim3(kBlock), 0, s, B, ld, p, x, \
+                     nv, G, part, w.counters, out, ostride, flag)
+  switch (cb) {
+    case 1: DMR_MDOT(1); break;
+    case 2: DMR_MDOT(2); break;
+    case 4: DMR_MDOT(4); break;
+    case 8: DMR_MDOT(8); break;
+    default: DMR_MDOT(kDotCols); break;
+  }
+#undef DMR_MDOT
 }
 
 template <typename TS>

@@ -234,14 +234,20 @@ extern "C" void oracle_lsq(const double *H, int m, const double *t, double *y, d
 // sweeps.  C (m x k): right singular vectors of the k smallest singular values, descending, each
 // with its largest-magnitude entry positive.  sig (m): singular values descending.
 // =============================================================================================
-static double lane_dot(const double *a, const double *b, int len) {
-  double v[64];
-  for (int l = 0; l < 64; ++l) {
+// Dot product in the order of a 32-lane half wavefront: lane l adds rows l, l+32, ... with fma,
+// then an xor butterfly over the 32 lanes (offsets 16, 8, 4, 2, 1).
+static double half_dot(const double *a, const double *b, int len) {
+  double v[32], t[32];
+  for (int l = 0; l < 32; ++l) {
     double s = 0.0;
-    for (int i = l; i < len; i += 64) s = std::fma(a[i], b[i], s);
+    for (int i = l; i < len; i += 32) s = std::fma(a[i], b[i], s);
     v[l] = s;
   }
-  return wave_sum(v);
+  for (int off = 16; off >= 1; off >>= 1) {
+    for (int l = 0; l < 32; ++l) t[l] = v[l] + v[l ^ off];
+    std::memcpy(v, t, sizeof(v));
+  }
+  return v[0];
 }
 
 extern "C" int oracle_svd(const double *H, int m, int k, double tol, int maxSweeps, double *C,
@@ -253,6 +259,11 @@ extern "C" int oracle_svd(const double *H, int m, int k, double tol, int maxSwee
   for (int c = 0; c < m; ++c) V[c + (size_t)c * m] = 1.0;
   int sweep = 0;
   while (sweep < maxSweeps) {
+    // Exact squared column norms per sweep, updated by each rotation within the sweep.
+    for (int c = 0; c < m; ++c) {
+      const double *ac = A.data() + (size_t)c * ldh;
+      sg[c] = half_dot(ac, ac, ldh);
+    }
     int rot = 0;
     for (int r = 0; r < mm - 1; ++r) {
       for (int pi = 0; pi < mm / 2; ++pi) {
@@ -267,20 +278,8 @@ extern "C" int oracle_svd(const double *H, int m, int k, double tol, int maxSwee
         if (p > q) std::swap(p, q);
         if (q >= m) continue;
         double *ap = A.data() + (size_t)p * ldh, *aq = A.data() + (size_t)q * ldh;
-        double v1[64], v2[64], v3[64];
-        for (int l = 0; l < 64; ++l) {
-          double al = 0.0, be = 0.0, ga = 0.0;
-          for (int i = l; i < ldh; i += 64) {
-            const double x = ap[i], y = aq[i];
-            al = std::fma(x, x, al);
-            be = std::fma(y, y, be);
-            ga = std::fma(x, y, ga);
-          }
-          v1[l] = al;
-          v2[l] = be;
-          v3[l] = ga;
-        }
-        const double al = wave_sum(v1), be = wave_sum(v2), ga = wave_sum(v3);
+        const double ga = half_dot(ap, aq, ldh);
+        const double al = sg[p], be = sg[q];
         if (ga != 0.0 && std::fabs(ga) > tol * std::sqrt(al * be)) {
           const double zeta = (be - al) / (2.0 * ga);
           const double tt =
@@ -298,6 +297,9 @@ extern "C" int oracle_svd(const double *H, int m, int k, double tol, int maxSwee
             vp[i] = cs * x - sn * y;
             vq[i] = sn * x + cs * y;
           }
+          const double tg = tt * ga;
+          sg[p] = al - tg;
+          sg[q] = be + tg;
           ++rot;
         }
       }
@@ -307,7 +309,7 @@ extern "C" int oracle_svd(const double *H, int m, int k, double tol, int maxSwee
   }
   for (int c = 0; c < m; ++c) {
     const double *ac = A.data() + (size_t)c * ldh;
-    sg[c] = std::sqrt(lane_dot(ac, ac, ldh));
+    sg[c] = std::sqrt(half_dot(ac, ac, ldh));
   }
   for (int c = 0; c < m; ++c) {
     const double sc = sg[c];

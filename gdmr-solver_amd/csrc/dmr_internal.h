@@ -12,7 +12,8 @@ namespace dmr {
 // All streaming kernels run 256-thread blocks (four 64-wide wavefronts).
 constexpr int kBlock = 256;
 // Cap on the number of row blocks of a fixed-order reduction.
-constexpr int kMaxRedBlocks = 512;
+constexpr int kMaxRedBlocks = 1024;
+constexpr int kDefaultRedBlocks = 512;
 // Widest column group of the multi-dot (GEMV-T) kernel.
 constexpr int kDotCols = 16;
 // Largest single-output coefficient count the combination kernels stage in LDS.
@@ -38,12 +39,22 @@ template <typename T> inline long long units(long long n) {
 // Number of row blocks of the fixed-order reduction over vectors of n elements of type T.
 // Every reduction kernel over such vectors uses exactly this geometry, so results do not depend
 // on which kernel computed them, on co-scheduled work, or on the run.
+// The cap is kMaxRedBlocks unless the environment variable DMR_RED_BLOCKS (read once per
+// process, 1..kMaxRedBlocks) lowers it; dmrReductionBlocks reports the value in use.
+int red_blocks_cap();
 template <typename T> inline int red_blocks(long long n) {
   long long g = (units<T>(n) + kBlock - 1) / kBlock;
+  const int cap = red_blocks_cap();
   if (g < 1) g = 1;
-  if (g > kMaxRedBlocks) g = kMaxRedBlocks;
+  if (g > cap) g = cap;
   return (int)g;
 }
+
+// Leading dimension (elements) of the basis columns for vectors of n elements: n rounded up to
+// 256, plus a pad so that columns streamed together do not start at addresses a power of two
+// apart (which maps them onto the same HBM channels).  DMR_LD_PAD (elements, multiple of 256)
+// overrides the pad.
+long long basis_ld(long long n);
 
 // Workspace of the fixed-order reductions: block partials and arrival counters.  Counters start
 // at zero and every kernel returns its counter to zero, so one workspace serves any number of

@@ -20,6 +20,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -58,7 +59,7 @@ constexpr double kSvdTol = 1e-14;
 constexpr int kSvdMaxSweeps = 128;
 constexpr double kStagnation = 0.995;
 
-long long pad_ld(long long n) { return (n + 255) / 256 * 256; }
+long long pad_ld(long long n) { return dmr::basis_ld(n); }
 
 template <typename TS> struct Work {
   TS *P = nullptr, *Z = nullptr, *Zn = nullptr, *r = nullptr, *xa = nullptr, *g = nullptr,
@@ -67,6 +68,31 @@ template <typename TS> struct Work {
 };
 
 }  // namespace
+
+namespace dmr {
+
+namespace {
+long long env_ll(const char *name, long long dflt, long long lo, long long hi) {
+  const char *v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  char *end = nullptr;
+  const long long x = std::strtoll(v, &end, 10);
+  if (end == v || x < lo || x > hi) return dflt;
+  return x;
+}
+}  // namespace
+
+int red_blocks_cap() {
+  static const int cap = (int)env_ll("DMR_RED_BLOCKS", kDefaultRedBlocks, 1, kMaxRedBlocks);
+  return cap;
+}
+
+long long basis_ld(long long n) {
+  static const long long pad = env_ll("DMR_LD_PAD", 512, 0, 1 << 20) / 256 * 256;
+  return (n + 255) / 256 * 256 + pad;
+}
+
+}  // namespace dmr
 
 enum OpKind { OP_NONE = 0, OP_CSR = 1, OP_FN = 2, OP_LEGACY = 3 };
 

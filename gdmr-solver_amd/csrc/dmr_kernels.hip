@@ -130,23 +130,27 @@ __device__ inline float get_partial(const float *p) {
 }
 
 // Arrival of a block at the end of a grid-wide reduction.  True in exactly one block, the last
-// to arrive, after every other block's partials are visible to it.
+// to arrive.  Every block writes its partials from its first wavefront; lane 0 of that wavefront
+// then counts the arrival with one agent-scope acquire-release atomic, which orders those stores
+// before the count (one L2 write-back per block, not one per wavefront as a block-wide
+// __threadfence would cost) and, in the last block, orders the partial loads after it.
 __device__ inline bool arrive_last(unsigned *counter, unsigned total) {
   __shared__ int s_last;
-  __threadfence();
   __syncthreads();
-  if (threadIdx.x == 0) s_last = (atomicAdd(counter, 1u) == total - 1u);
+  if (threadIdx.x == 0)
+    s_last = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+             total - 1u;
   __syncthreads();
-  const bool last = s_last != 0;
-  if (last) __threadfence();
-  return last;
+  return s_last != 0;
 }
 
-// Sum of G block partials in block order: thread t adds partials t, t+256, ... then block_sum.
-template <typename TA> __device__ inline TA sum_partials(const TA *part, int G) {
+// Sum of G block partials in block order, one wavefront per column: lane l adds partials l,
+// l+64, ... in order, then the xor butterfly.  Returns the sum in every lane.
+template <typename TA> __device__ inline TA wave_sum_partials(const TA *part, int G) {
+  const int lane = threadIdx.x & 63;
   TA v = 0;
-  for (int q = threadIdx.x; q < G; q += kBlock) v = v + get_partial(part + q);
-  return block_sum(v);
+  for (int q = lane; q < G; q += 64) v = v + get_partial(part + q);
+  return wave_sum(v);
 }
 
 __device__ inline void finalize(const Fin &fin, double ss, double sg) {
@@ -222,21 +226,16 @@ __device__ inline void block_sum_cols(TA (&v)[CB], TA *res) {
   __syncthreads();
 }
 
-// Last-block sum of the G partials of columns c0..c0+pc-1 (column-major, stride G), all columns
-// at once; thread t adds partials t, t+256, ... of each column in order.
-template <typename TA, int CB>
+// Last-block sum of the G partials of columns c0..c0+pc-1 (column-major, stride G): wavefront
+// w takes columns w, w+4, ... (wave_sum_partials each).  Column c's sum lands in res[c].
+template <typename TA>
 __device__ inline void sum_partials_cols(const TA *part, int G, int c0, int pc, TA *res) {
-  TA v[CB];
-#pragma unroll
-  for (int c = 0; c < CB; ++c) v[c] = 0;
-  for (int q = threadIdx.x; q < G; q += kBlock) {
-    TA t[CB];
-#pragma unroll
-    for (int c = 0; c < CB; ++c) t[c] = c < pc ? get_partial(part + (long long)(c0 + c) * G + q) : (TA)0;
-#pragma unroll
-    for (int c = 0; c < CB; ++c) v[c] = v[c] + t[c];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int c = wv; c < pc; c += kBlock / 64) {
+    const TA s = wave_sum_partials(part + (long long)(c0 + c) * G, G);
+    if (lane == 0) res[c] = s;
   }
-  block_sum_cols<TA, CB>(v, res);
+  __syncthreads();
 }
 
 template <typename TS, int CB>
@@ -275,7 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_multidot(const TS *__restrict__ B, l
   if (threadIdx.x < pc) put_partial(partials + (long long)(c0 + threadIdx.x) * G + blockIdx.x,
                                     res[threadIdx.x]);
   if (!arrive_last(&counters[blockIdx.y], (unsigned)G)) return;
-  sum_partials_cols<TA, CB>(partials, G, c0, pc, res);
+  sum_partials_cols<TA>(partials, G, c0, pc, res);
   if (threadIdx.x < pc) out[(long long)(c0 + threadIdx.x) * ostride] = (double)res[threadIdx.x];
   if (threadIdx.x == 0) counters[blockIdx.y] = 0u;
 }
@@ -356,7 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_combo(const TS *in, TS *out, const T
   if (threadIdx.x < NR) put_partial(partials + (long long)threadIdx.x * G + blockIdx.x,
                                     res[threadIdx.x]);
   if (!arrive_last(counter, (unsigned)G)) return;
-  sum_partials_cols<TA, NR>(partials, G, 0, NR, res);
+  sum_partials_cols<TA>(partials, G, 0, NR, res);
   if (threadIdx.x == 0) {
     finalize(fin, (double)res[0], NR == 2 ? (double)res[1] : 0.0);
     *counter = 0u;
@@ -420,15 +419,13 @@ __global__ __launch_bounds__(kBlock) void k_combo_multi(
       }
     }
   }
-  for (int j = 0; j < Jg; ++j) {
-    const TA s = block_sum(ss[j]);
-    if (threadIdx.x == 0) put_partial(partials + (long long)(j0 + j) * G + blockIdx.x, s);
-  }
+  __shared__ TA res[JB];
+  block_sum_cols<TA, JB>(ss, res);
+  if (threadIdx.x < Jg)
+    put_partial(partials + (long long)(j0 + threadIdx.x) * G + blockIdx.x, res[threadIdx.x]);
   if (!arrive_last(&counters[blockIdx.y], (unsigned)G)) return;
-  for (int j = 0; j < Jg; ++j) {
-    const TA v = sum_partials(partials + (long long)(j0 + j) * G, G);
-    if (threadIdx.x == 0) scal[j0 + j] = 1.0 / sqrt((double)v);
-  }
+  sum_partials_cols<TA>(partials, G, j0, Jg, res);
+  if (threadIdx.x < Jg) scal[j0 + threadIdx.x] = 1.0 / sqrt((double)res[threadIdx.x]);
   if (threadIdx.x == 0) counters[blockIdx.y] = 0u;
 }
 
@@ -630,7 +627,8 @@ __global__ __launch_bounds__(kBlock) void k_resid64(const int *__restrict__ rp,
   const double bs = block_sum(ss);
   if (threadIdx.x == 0) put_partial(partials + blockIdx.x, bs);
   if (!arrive_last(counter, (unsigned)nb)) return;
-  const double tot = sum_partials(partials, nb);
+  if (threadIdx.x >= 64) return;
+  const double tot = wave_sum_partials(partials, nb);
   if (threadIdx.x == 0) {
     scal[0] = tot;
     scal[1] = 1.0 / sqrt(tot);

@@ -54,15 +54,16 @@ double block_sum(double *v /*256*/) {
                w3 = wave_sum(v + 192);
   return ((w0 + w1) + w2) + w3;
 }
-// Sum of G block partials in block order (sum_partials of the device).
+// Sum of G block partials as the device's last block does it: one wavefront, lane l adds
+// partials l, l+64, ... in order, then the xor butterfly.
 double sum_partials(const double *part, int G) {
-  double v[kB];
-  for (int t = 0; t < kB; ++t) {
+  double v[64];
+  for (int l = 0; l < 64; ++l) {
     double s = 0.0;
-    for (int q = t; q < G; q += kB) s = s + part[q];
-    v[t] = s;
+    for (int q = l; q < G; q += 64) s = s + part[q];
+    v[l] = s;
   }
-  return block_sum(v);
+  return wave_sum(v);
 }
 
 // Generic fixed-order reduction: f(i, acc) folds element i into a thread accumulator, elements

@@ -247,13 +247,9 @@ __global__ __launch_bounds__(kBlock) void k_multidot(const TS *__restrict__ B, l
   using TA = typename Acc<TS>::type;
   constexpr int N = Unit<TS>::N;
   if (flag && *flag) return;
+  // Group of up to CB columns (CB a multiple of 8); all of p when p <= CB, so x is read once.
   const int c0 = blockIdx.y * CB;
   const int pc = min(CB, p - c0);
-  // Columns past the end of B (last group only) alias column p-1: their loads hit in cache and
-  // their sums are discarded, which keeps every load unconditional and in flight together.
-  const TS *col[CB];
-#pragma unroll
-  for (int c = 0; c < CB; ++c) col[c] = B + (long long)min(c0 + c, p - 1) * ld;
   TA acc[CB];
 #pragma unroll
   for (int c = 0; c < CB; ++c) acc[c] = 0;
@@ -261,13 +257,22 @@ __global__ __launch_bounds__(kBlock) void k_multidot(const TS *__restrict__ B, l
   for (long long u = (long long)blockIdx.x * kBlock + threadIdx.x; u < nv; u += stride) {
     TA xv[N];
     ldu(x, u, xv);
-    TA bv[CB][N];
+    // CH columns at a time, their loads issued together; indices past the group alias its
+    // last column (a cache hit) and their sums are discarded.
+    constexpr int CH = CB < 8 ? CB : 8;
 #pragma unroll
-    for (int c = 0; c < CB; ++c) ldu(col[c], u, bv[c]);
+    for (int cc = 0; cc < CB; cc += CH) {
+      if (cc < pc) {
+        TA bv[CH][N];
 #pragma unroll
-    for (int c = 0; c < CB; ++c)
+        for (int i = 0; i < CH; ++i)
+          ldu(B + (long long)(c0 + min(cc + i, pc - 1)) * ld, u, bv[i]);
 #pragma unroll
-      for (int e = 0; e < N; ++e) acc[c] = fmac(bv[c][e], xv[e], acc[c]);
+        for (int i = 0; i < CH; ++i)
+#pragma unroll
+          for (int e = 0; e < N; ++e) acc[cc + i] = fmac(bv[i][e], xv[e], acc[cc + i]);
+      }
+    }
   }
   __shared__ TA res[CB];
   block_sum_cols<TA, CB>(acc, res);
@@ -703,9 +708,11 @@ void multidot(const TS *B, long long ld, int p, const TS *x, long long n, double
   using TA = typename Acc<TS>::type;
   const int G = red_blocks<TS>(n);
   const long long nv = units<TS>(n);
-  // Column-group width: the smallest of 1, 2, 4, 8, 16 that holds p, else groups of 16.
-  const int cb = p <= 1 ? 1 : p <= 2 ? 2 : p <= 4 ? 4 : p <= 8 ? 8 : kDotCols;
-  const int groups = (p + cb - 1) / cb;
+  // Column groups of at most kDotCols columns, balanced; a group's width is rounded up to one
+  // of 1, 2, 4 or a multiple of 8 (the kernel template), so x is read once per group.
+  const int groups = (p + kDotCols - 1) / kDotCols;
+  const int per = (p + groups - 1) / groups;
+  const int cb = per <= 4 ? (per <= 1 ? 1 : per <= 2 ? 2 : 4) : (per + 7) / 8 * 8;
   TA *part = (TA *)w.partials;
 #define DMR_MDOT(CBV)                                                                      \
   hipLaunchKernelGGL((k_multidot<TS, CBV>), dim3(G, groups), dim3(kBlock), 0, s, B, ld, p, x, \
@@ -715,8 +722,11 @@ void multidot(const TS *B, long long ld, int p, const TS *x, long long n, double
     case 2: DMR_MDOT(2); break;
     case 4: DMR_MDOT(4); break;
     case 8: DMR_MDOT(8); break;
-    default: DMR_MDOT(kDotCols); break;
+    case 16: DMR_MDOT(16); break;
+    case 24: DMR_MDOT(24); break;
+    default: DMR_MDOT(32); break;
   }
+  static_assert(kDotCols == 32, "multidot launcher covers group widths up to 32");
 #undef DMR_MDOT
 }
 

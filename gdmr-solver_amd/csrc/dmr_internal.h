@@ -134,6 +134,18 @@ void combo_multi(TS *out, int J, const TS *B1, int p1, const double *C1, const T
                  const double *C2, int ldc, long long ld, long long n, double *scal, RedWork w,
                  hipStream_t s);
 
+// Fused orthogonalisation of one Arnoldi vector in one cooperative launch:
+//   u = g; if pa: hA = A^T u, u -= A hA; if pb: hB = B^T u, u -= B hB; norms (red 1: ss, 3: ss
+//   and sg); finalize (fin); dst = u / ||u|| (u unscaled if the launch set the breakdown flag).
+// hA / hB receive the coefficients.  dst may equal g.  bar: two grid-barrier words reset at the
+// start of every cycle; epoch0: phases already completed in the cycle.  Returns the number of
+// barrier phases used (>= 1), or 0 when the grid cannot be co-resident or the shape is not
+// supported (nothing enqueued: the caller runs the unfused kernels, which give the same bits).
+template <typename TS>
+int orth(const TS *g, TS *dst, const TS *A, int pa, double *ha, const TS *B, int pb, double *hb,
+         long long ld, long long n, int red, const Fin &fin, RedWork w, unsigned *bar,
+         unsigned epoch0, int *err, const int *flag, hipStream_t s);
+
 // v *= scal[idx] (scal on device), skipped when *flag != 0.
 template <typename TS>
 void scale(TS *v, long long n, const double *scal, const int *flag, hipStream_t s);
@@ -147,8 +159,8 @@ void convert(TD *dst, const TSrc *src, long long n, hipStream_t s);
 template <typename TS>
 void add_to(double *x, const TS *d, long long n, hipStream_t s);
 
-// Cycle start: H[(m+1) x m] = 0 and *flag = 0.
-void cycle_begin(double *H, int m, int *flag, hipStream_t s);
+// Cycle start: H[(m+1) x m] = 0, *flag = 0 and the two grid-barrier words bar[0..1] = 0.
+void cycle_begin(double *H, int m, int *flag, unsigned *bar, hipStream_t s);
 // After the Arnoldi loop: if *flag = j != 0 zero basis columns o+j..m and set H[c][c] = 1 for
 // columns c >= d+j.
 template <typename TS>

@@ -434,6 +434,234 @@ __global__ __launch_bounds__(kBlock) void k_combo_multi(
   if (threadIdx.x == 0) counters[blockIdx.y] = 0u;
 }
 
+// =============================================================================================
+// Fused orthogonalisation of one Arnoldi vector (one cooperative launch per step).
+//
+// u = g;  [stage A: cA = A^T u -> hA;  u -= A cA];  [stage B: cB = B^T u -> hB;  u -= B cB];
+// ss = u.u (and sg = u.g);  finalize;  dst = u * (1/sqrt(ss)).
+//
+// This is the sequence multidot -> combo(SUB) -> multidot -> combo(SUB, norms) -> scale of the
+// unfused path in one kernel: every thread keeps its UPT units of g and u in registers across
+// the phases, and the grid synchronises through an arrival counter and a release flag (all
+// blocks are co-resident: the kernel is only launched cooperatively).  Per-thread unit mapping,
+// block sums and the last block's partial sums are those of the unfused kernels, so the results
+// are bitwise identical to them.  A phase's last block to arrive finishes its reduction, thread
+// 0 publishes the results, then releases the others.
+// =============================================================================================
+struct OrthArgs {
+  const void *g;
+  void *dst;
+  const void *A;
+  int pa;
+  double *ha;
+  const void *B;
+  int pb;
+  double *hb;
+  long long ld;
+  long long nv;
+  int G;
+  void *partials;
+  unsigned *bar;     // bar[0]: arrivals, bar[1]: released epoch
+  unsigned epoch0;   // phases already completed in this cycle
+  int red;           // 1: ss, 3: ss and sg
+  Fin fin;
+  const int *flag;
+  int *err;          // set when a barrier wait exceeds its bound
+};
+
+constexpr unsigned kSpinLimit = 1u << 26;
+
+// Phase barrier: true in the block that arrives last (it must call phase_release).
+__device__ inline bool phase_arrive(unsigned *bar, unsigned target) {
+  __shared__ int s_last;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    s_last = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+             target - 1u;
+  __syncthreads();
+  return s_last != 0;
+}
+__device__ inline void phase_release(unsigned *bar, unsigned epoch) {
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_store(bar + 1, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Bounded wait for the release of `epoch`; on timeout the error word is set and the kernel goes
+// on (its results are then discarded by the host), so every wave always exits.
+__device__ inline void phase_wait(unsigned *bar, unsigned epoch, int *err) {
+  if (threadIdx.x == 0) {
+    unsigned spins = 0;
+    while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kSpinLimit) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+}
+__device__ inline double load_published(const double *p) {
+  return __longlong_as_double((long long)__hip_atomic_load(
+      reinterpret_cast<const unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Block partials of p dot products B_l . x (x in registers), in chunks of 16 columns; per column
+// the same unit order and block sum as k_multidot.
+template <typename TS, typename TA, int UPT, int N>
+__device__ inline void orth_dots(const TS *__restrict__ Bm, int p, long long ld,
+                                 const TA (&x)[UPT][N], long long u0, long long stride,
+                                 long long nv, int G, TA *partials) {
+  for (int c0 = 0; c0 < p; c0 += 16) {
+    const int pc = min(16, p - c0);
+    TA acc[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) acc[c] = 0;
+#pragma unroll
+    for (int s = 0; s < UPT; ++s) {
+      const long long u = u0 + s * stride;
+      if (u < nv) {
+#pragma unroll
+        for (int cc = 0; cc < 16; cc += 8) {
+          if (cc < pc) {
+            TA bv[8][N];
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+              ldu(Bm + (long long)(c0 + min(cc + i, pc - 1)) * ld, u, bv[i]);
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+              for (int e = 0; e < N; ++e) acc[cc + i] = fmac(bv[i][e], x[s][e], acc[cc + i]);
+          }
+        }
+      }
+    }
+    __shared__ TA r16[16];
+    block_sum_cols<TA, 16>(acc, r16);
+    if ((int)threadIdx.x < pc)
+      put_partial(partials + (long long)(c0 + threadIdx.x) * G + blockIdx.x, r16[threadIdx.x]);
+  }
+}
+
+// Last block: column sums of p partial columns into out[0..p) (published by thread 0).
+template <typename TA>
+__device__ inline void orth_finish_cols(const TA *partials, int G, int p, double *out) {
+  __shared__ double fin[kMaxCoef];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int c = wv; c < p; c += kBlock / 64) {
+    const TA s = wave_sum_partials(partials + (long long)c * G, G);
+    if (lane == 0) fin[c] = (double)s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int c = 0; c < p; ++c) out[c] = fin[c];
+}
+
+// x -= sum_l c_l B_l (coefficients in LDS, already negated), columns in order.
+template <typename TS, typename TA, int UPT, int N>
+__device__ inline void orth_update(const TS *__restrict__ Bm, int p, long long ld, const TA *cs,
+                                   TA (&x)[UPT][N], long long u0, long long stride, long long nv) {
+#pragma unroll
+  for (int s = 0; s < UPT; ++s) {
+    const long long u = u0 + s * stride;
+    if (u < nv) fma_cols(Bm, p, ld, u, cs, x[s]);
+  }
+}
+
+template <typename TS, int UPT>
+__global__ __launch_bounds__(kBlock) void k_orth(OrthArgs a) {
+  using TA = typename Acc<TS>::type;
+  constexpr int N = Unit<TS>::N;
+  if (a.flag && *a.flag) return;
+  const TS *g = (const TS *)a.g;
+  const int G = a.G;
+  const long long stride = (long long)G * kBlock;
+  const long long u0 = (long long)blockIdx.x * kBlock + threadIdx.x;
+  TA *partials = (TA *)a.partials;
+  TA gv[UPT][N], uv[UPT][N];
+#pragma unroll
+  for (int s = 0; s < UPT; ++s) {
+    const long long u = u0 + s * stride;
+    if (u < a.nv) {
+      ldu(g, u, gv[s]);
+    } else {
+#pragma unroll
+      for (int e = 0; e < N; ++e) gv[s][e] = 0;
+    }
+#pragma unroll
+    for (int e = 0; e < N; ++e) uv[s][e] = gv[s][e];
+  }
+  __shared__ TA cs[kMaxCoef];
+  unsigned epoch = a.epoch0;
+  const TS *stB[2] = {(const TS *)a.A, (const TS *)a.B};
+  const int stP[2] = {a.pa, a.pb};
+  double *stH[2] = {a.ha, a.hb};
+#pragma unroll
+  for (int st = 0; st < 2; ++st) {
+    const int p = stP[st];
+    if (p <= 0) continue;
+    orth_dots<TS, TA, UPT, N>(stB[st], p, a.ld, uv, u0, stride, a.nv, G, partials);
+    ++epoch;
+    if (phase_arrive(a.bar, epoch * (unsigned)G)) {
+      orth_finish_cols<TA>(partials, G, p, stH[st]);
+      phase_release(a.bar, epoch);
+    } else {
+      phase_wait(a.bar, epoch, a.err);
+    }
+    for (int l = threadIdx.x; l < p; l += kBlock) cs[l] = (TA)(-load_published(stH[st] + l));
+    __syncthreads();
+    orth_update<TS, TA, UPT, N>(stB[st], p, a.ld, cs, uv, u0, stride, a.nv);
+    __syncthreads();
+  }
+  // Norms (same per-thread order as k_combo's fused reductions).
+  TA v[2] = {0, 0};
+#pragma unroll
+  for (int s = 0; s < UPT; ++s) {
+    if (u0 + s * stride < a.nv) {
+#pragma unroll
+      for (int e = 0; e < N; ++e) v[0] = fmac(uv[s][e], uv[s][e], v[0]);
+      if (a.red & 2) {
+#pragma unroll
+        for (int e = 0; e < N; ++e) v[1] = fmac(uv[s][e], gv[s][e], v[1]);
+      }
+    }
+  }
+  const int nr = (a.red & 2) ? 2 : 1;
+  __shared__ TA r2[2];
+  block_sum_cols<TA, 2>(v, r2);
+  if ((int)threadIdx.x < nr)
+    put_partial(partials + (long long)threadIdx.x * G + blockIdx.x, r2[threadIdx.x]);
+  ++epoch;
+  if (phase_arrive(a.bar, epoch * (unsigned)G)) {
+    __shared__ TA fin2[2];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (wv < nr) {
+      const TA s = wave_sum_partials(partials + (long long)wv * G, G);
+      if (lane == 0) fin2[wv] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) finalize(a.fin, (double)fin2[0], nr == 2 ? (double)fin2[1] : 0.0);
+    phase_release(a.bar, epoch);
+  } else {
+    phase_wait(a.bar, epoch, a.err);
+  }
+  // After a breakdown the column is zeroed by breakdown_fixup; it is left unscaled here.
+  const bool broke = a.fin.flag && __hip_atomic_load(a.fin.flag, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT) != 0;
+  const TA sc = broke ? (TA)1 : (TA)load_published(a.fin.scal + 1);
+  TS *dst = (TS *)a.dst;
+#pragma unroll
+  for (int s = 0; s < UPT; ++s) {
+    const long long u = u0 + s * stride;
+    if (u < a.nv) {
+      TA o[N];
+#pragma unroll
+      for (int e = 0; e < N; ++e) o[e] = uv[s][e] * sc;
+      stu(dst, u, o);
+    }
+  }
+}
+
 // v *= scal (normalizeVecDMR, gDMR.cu:328-355), skipped after a breakdown.
 template <typename TS>
 __global__ __launch_bounds__(kBlock) void k_scale(TS *v, long long nv, const double *scal,
@@ -523,9 +751,13 @@ __global__ __launch_bounds__(kBlock) void k_add_to(double *x, const TS *d, long 
   }
 }
 
-__global__ void k_cycle_begin(double *H, int count, int *flag) {
+__global__ void k_cycle_begin(double *H, int count, int *flag, unsigned *bar) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) H[i] = 0.0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *flag = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *flag = 0;
+    bar[0] = 0u;
+    bar[1] = 0u;
+  }
 }
 
 // Breakdown at Krylov step j (gDMR.cu:2256-2290): zero basis slots o+j..m and put ones on the
@@ -773,6 +1005,43 @@ void combo_multi(TS *out, int J, const TS *B1, int p1, const double *C1, const T
 }
 
 template <typename TS>
+int orth(const TS *g, TS *dst, const TS *A, int pa, double *ha, const TS *B, int pb, double *hb,
+         long long ld, long long n, int red, const Fin &fin, RedWork w, unsigned *bar,
+         unsigned epoch0, int *err, const int *flag, hipStream_t s) {
+  if (pa + pb > kMaxCoef || pa > kMaxCoef / 2 || pb > kMaxCoef / 2) return 0;
+  const int G = red_blocks<TS>(n);
+  const long long nv = units<TS>(n);
+  const long long per = (long long)G * kBlock;
+  const long long upt = (nv + per - 1) / per;
+  OrthArgs a{g,  dst, A,  pa,     ha,  B,   pb,  hb,   ld, nv,
+             G,  w.partials, bar, epoch0, red, fin, flag, err};
+  void *args[] = {&a};
+  hipError_t e;
+  if (upt <= 1)
+    e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_orth<TS, 1>), dim3(G),
+                                   dim3(kBlock), args, 0, s);
+  else if (upt <= 2)
+    e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_orth<TS, 2>), dim3(G),
+                                   dim3(kBlock), args, 0, s);
+  else if (upt <= 4)
+    e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_orth<TS, 4>), dim3(G),
+                                   dim3(kBlock), args, 0, s);
+  else if (upt <= 8)
+    e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_orth<TS, 8>), dim3(G),
+                                   dim3(kBlock), args, 0, s);
+  else if (upt <= 16)
+    e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_orth<TS, 16>), dim3(G),
+                                   dim3(kBlock), args, 0, s);
+  else
+    return 0;
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return (pa > 0) + (pb > 0) + 1;
+}
+
+template <typename TS>
 void scale(TS *v, long long n, const double *scal, const int *flag, hipStream_t s) {
   const long long nv = units<TS>(n);
   hipLaunchKernelGGL((k_scale<TS>), dim3(grid_for(nv)), dim3(kBlock), 0, s, v, nv, scal, flag);
@@ -797,10 +1066,10 @@ void add_to(double *x, const TS *d, long long n, hipStream_t s) {
   hipLaunchKernelGGL((k_add_to<TS>), dim3(grid_for(nv)), dim3(kBlock), 0, s, x, d, nv);
 }
 
-void cycle_begin(double *H, int m, int *flag, hipStream_t s) {
+void cycle_begin(double *H, int m, int *flag, unsigned *bar, hipStream_t s) {
   const int count = (m + 1) * m;
   hipLaunchKernelGGL(k_cycle_begin, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s, H,
-                     count, flag);
+                     count, flag, bar);
 }
 
 template <typename TS>
@@ -824,6 +1093,9 @@ void breakdown_fixup(TS *P, long long ld, long long n, int o, int d, int m, doub
                                 const double *, int, long long, long long, double *, RedWork,     \
                                 hipStream_t);                                                     \
   template void scale<TS>(TS *, long long, const double *, const int *, hipStream_t);             \
+  template int orth<TS>(const TS *, TS *, const TS *, int, double *, const TS *, int, double *,  \
+                        long long, long long, int, const Fin &, RedWork, unsigned *, unsigned,    \
+                        int *, const int *, hipStream_t);                                         \
   template void scale_cols<TS>(TS *, int, long long, long long, const double *, hipStream_t);     \
   template void add_to<TS>(double *, const TS *, long long, hipStream_t);                         \
   template void breakdown_fixup<TS>(TS *, long long, long long, int, int, int, double *,          \

@@ -82,6 +82,8 @@ long long env_ll(const char *name, long long dflt, long long lo, long long hi) {
 }
 }  // namespace
 
+bool env_flag(const char *name, bool dflt) { return env_ll(name, dflt ? 1 : 0, 0, 1) != 0; }
+
 int red_blocks_cap() {
   static const int cap = (int)env_ll("DMR_RED_BLOCKS", kDefaultRedBlocks, 1, kMaxRedBlocks);
   return cap;
@@ -119,6 +121,10 @@ struct dmrContext {
   double *H = nullptr, *t = nullptr, *y = nullptr, *z = nullptr, *C = nullptr, *sig = nullptr,
          *cv = nullptr, *scal = nullptr, *scal2 = nullptr, *scalZ = nullptr, *one = nullptr;
   int *flag = nullptr, *sweeps = nullptr;
+  unsigned *bar = nullptr;  // grid-barrier words of the fused orthogonalisation
+  int *devErr = nullptr;    // set by a device-side wait that exceeded its bound
+  unsigned epoch = 0;       // barrier phases completed in the current cycle
+  bool fused = true;        // fused cooperative orthogonalisation (DMR_FUSED=0 disables)
   dmr::RedWork red{};
   dmr::RedWork redOp{};  // sized to the SpMV block count of the operator
   void *redOpPool = nullptr;
@@ -252,6 +258,9 @@ int ctx_alloc(dmrContext *c) {
   int *ip = (int *)(c->red.counters + dmr::kCounters);
   c->flag = ip;
   c->sweeps = ip + 1;
+  c->bar = reinterpret_cast<unsigned *>(ip + 2);
+  c->devErr = ip + 4;
+  c->fused = dmr::env_flag("DMR_FUSED", true);
   const double one = 1.0;
   HIPCHK(hipMemcpy(c->one, &one, sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipHostMalloc((void **)&c->hostStatus, 64 * sizeof(double), hipHostMallocDefault));
@@ -404,83 +413,81 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
         return set_err(DMR_ERR_HIP, "setup copy failed");
     }
     // ---- arnoldiDMR (gDMR.cu:1790-2332)
-    dmr::cycle_begin(c->H, m, c->flag, s);
+    dmr::cycle_begin(c->H, m, c->flag, c->bar, s);
+    c->epoch = 0;
+    // Two-stage classical Gram-Schmidt of g into dst (deflatorDMR then the Krylov projection,
+    // kept as two reductions as in the reference): u = g; [hA = A^T u; u -= A hA];
+    // [hB = B^T u; u -= B hB]; norms; dst = u / ||u||.  One fused cooperative kernel when the
+    // grid fits the device, else the same arithmetic as separate kernels.
+    auto orthogonalize = [&](const TS *g, TS *dst, const TS *A, int pa, double *ha, const TS *B,
+                             int pb, double *hb, int redm, const dmr::Fin &fin,
+                             const int *skip) {
+      if (c->fused) {
+        int ph = 0;
+        timed(c, s, DMR_PH_ORTH, vb * (2.0 + 2.0 * pa + 2.0 * pb), [&] {
+          ph = dmr::orth<TS>(g, dst, A, pa, ha, B, pb, hb, ld, n, redm, fin, red, c->bar, c->epoch,
+                             c->devErr, skip, s);
+          return 0;
+        });
+        if (ph > 0) {
+          c->epoch += (unsigned)ph;
+          return;
+        }
+      }
+      const TS *src = g;
+      if (pa > 0) {
+        timed(c, s, DMR_PH_GEMVT, vb * (pa + 1), [&] {
+          dmr::multidot<TS>(A, ld, pa, g, n, ha, 1, red, skip, s);
+          return 0;
+        });
+        timed(c, s, DMR_PH_UPDATE, vb * (pa + 2), [&] {
+          dmr::combo<TS>(dmr::COMB_SUB, g, w.u, A, pa, ha, nullptr, 0, nullptr, ld, n, 0.0, 0,
+                         nullptr, fin_none(), red, skip, s);
+          return 0;
+        });
+        src = w.u;
+      }
+      if (pb > 0)
+        timed(c, s, DMR_PH_GEMVT, vb * (pb + 1), [&] {
+          dmr::multidot<TS>(B, ld, pb, src, n, hb, 1, red, skip, s);
+          return 0;
+        });
+      timed(c, s, DMR_PH_UPDATE, vb * (pb + 2 + (redm & 2 ? 1 : 0)), [&] {
+        dmr::combo<TS>(dmr::COMB_SUB, src, dst, B, pb, hb, nullptr, 0, nullptr, ld, n, 0.0, redm,
+                       g, fin, red, skip, s);
+        return 0;
+      });
+      timed(c, s, DMR_PH_SCALE, vb * 2, [&] {
+        dmr::scale<TS>(dst, n, c->scal + 1, skip, s);
+        return 0;
+      });
+    };
     // Deflation image: A Z = W R by CGS (gDMR.cu:1823-2031).
     for (int i = 0; i < d; ++i) {
       int rc = timed(c, s, DMR_PH_SPMV, spmvBytes, [&] { return op(w.Z + (long long)i * ld, w.g); });
       if (rc) return rc;
       ++c->st.matvecs;
       double *hcol = c->H + (long long)i * (m + 1);
-      if (i > 0)
-        timed(c, s, DMR_PH_GEMVT, vb * (i + 1), [&] {
-          dmr::multidot<TS>(w.P, ld, i, w.g, n, hcol, 1, red, nullptr, s);
-          return 0;
-        });
       dmr::Fin fin{dmr::FIN_ARNOLDI, c->scal, hcol + i, 0.0, 0, nullptr};
-      timed(c, s, DMR_PH_UPDATE, vb * (i + 3), [&] {
-        dmr::combo<TS>(dmr::COMB_SUB, w.g, w.P + (long long)i * ld, w.P, i, hcol, nullptr, 0,
-                       nullptr, ld, n, 0.0, 3, w.g, fin, red, nullptr, s);
-        return 0;
-      });
-      timed(c, s, DMR_PH_SCALE, vb * 2, [&] {
-        dmr::scale<TS>(w.P + (long long)i * ld, n, c->scal + 1, nullptr, s);
-        return 0;
-      });
+      orthogonalize(w.g, w.P + (long long)i * ld, nullptr, 0, nullptr, w.P, i, hcol, 3, fin,
+                    nullptr);
     }
     // Start vector orthogonalised against W (deflatorDMR(0, 0), gDMR.cu:2033-2060).
     if (d > 0) {
-      timed(c, s, DMR_PH_GEMVT, vb * (k + 1), [&] {
-        dmr::multidot<TS>(w.P, ld, k, Po, n, c->cv, 1, red, nullptr, s);
-        return 0;
-      });
       dmr::Fin fin{dmr::FIN_NORM, c->scal, nullptr, 0.0, 0, nullptr};
-      timed(c, s, DMR_PH_UPDATE, vb * (k + 2), [&] {
-        dmr::combo<TS>(dmr::COMB_SUB, Po, Po, w.P, k, c->cv, nullptr, 0, nullptr, ld, n, 0.0, 1,
-                       nullptr, fin, red, nullptr, s);
-        return 0;
-      });
-      timed(c, s, DMR_PH_SCALE, vb * 2, [&] {
-        dmr::scale<TS>(Po, n, c->scal + 1, nullptr, s);
-        return 0;
-      });
+      orthogonalize(Po, Po, nullptr, 0, nullptr, w.P, k, c->cv, 1, fin, nullptr);
     }
-    // Krylov steps (gDMR.cu:2069-2328).
+    // Krylov steps (gDMR.cu:2069-2328): deflatorDMR(1, j) puts W^T g into H[0:k, d+j-1], then
+    // CGS against V_0..V_{j-1} puts h into H[d:d+j, d+j-1]; H[d+j, d+j-1] = v_j . g.
     for (int j = 1; j <= m - d; ++j) {
       int rc = timed(c, s, DMR_PH_SPMV, spmvBytes,
                      [&] { return op(w.P + (long long)(o + j - 1) * ld, w.g); });
       if (rc) return rc;
       ++c->st.matvecs;
       double *hcol = c->H + (long long)(d + j - 1) * (m + 1);
-      const TS *src = w.g;
-      if (d > 0) {
-        // deflatorDMR(1, j): c = W^T g into H[0:k, d+j-1], u = g - W c.
-        timed(c, s, DMR_PH_GEMVT, vb * (k + 1), [&] {
-          dmr::multidot<TS>(w.P, ld, k, w.g, n, hcol, 1, red, c->flag, s);
-          return 0;
-        });
-        timed(c, s, DMR_PH_UPDATE, vb * (k + 2), [&] {
-          dmr::combo<TS>(dmr::COMB_SUB, w.g, w.u, w.P, k, hcol, nullptr, 0, nullptr, ld, n, 0.0,
-                         0, nullptr, fin_none(), red, c->flag, s);
-          return 0;
-        });
-        src = w.u;
-      }
-      // Classical Gram-Schmidt against V_0..V_{j-1}: h into H[d:d+j, d+j-1].
-      timed(c, s, DMR_PH_GEMVT, vb * (j + 1), [&] {
-        dmr::multidot<TS>(Po, ld, j, src, n, hcol + d, 1, red, c->flag, s);
-        return 0;
-      });
       TS *vj = w.P + (long long)(o + j) * ld;
       dmr::Fin fin{dmr::FIN_ARNOLDI, c->scal, hcol + d + j, tol, j, c->flag};
-      timed(c, s, DMR_PH_UPDATE, vb * (j + 3), [&] {
-        dmr::combo<TS>(dmr::COMB_SUB, src, vj, Po, j, hcol + d, nullptr, 0, nullptr, ld, n, 0.0,
-                       3, w.g, fin, red, c->flag, s);
-        return 0;
-      });
-      timed(c, s, DMR_PH_SCALE, vb * 2, [&] {
-        dmr::scale<TS>(vj, n, c->scal + 1, c->flag, s);
-        return 0;
-      });
+      orthogonalize(w.g, vj, w.P, d > 0 ? k : 0, hcol, Po, j, hcol + d, 3, fin, c->flag);
     }
     dmr::breakdown_fixup<TS>(w.P, ld, n, o, d, m, c->H, c->flag, s);
     // ---- updateAppResDMR (gDMR.cu:2411-2582)
@@ -514,13 +521,17 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
     if (hipMemcpyAsync(c->hostStatus, c->scal2, 4 * sizeof(double), hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
         hipMemcpyAsync(c->hostStatus + 8, c->flag, sizeof(int), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(c->hostStatus + 9, c->devErr, sizeof(int), hipMemcpyDeviceToHost, s) !=
             hipSuccess)
       return set_err(DMR_ERR_HIP, "status readback failed");
     if (hipStreamSynchronize(s) != hipSuccess) return set_err(DMR_ERR_HIP, "stream sync failed");
     if (c->prof.on) prof_collect(c);
     const double q = c->hostStatus[2];
-    int bflag;
+    int bflag, derr;
     std::memcpy(&bflag, c->hostStatus + 8, sizeof(int));
+    std::memcpy(&derr, c->hostStatus + 9, sizeof(int));
+    if (derr) return set_err(DMR_ERR_HIP, "device grid barrier exceeded its wait bound");
     if (bflag) ++c->st.breakdowns;
     // ---- genDeflatDMR (gDMR.cu:1583-1704): Z <- normalize([V Z] C), C = smallest right
     // singular vectors of H.  After a breakdown the basis beyond the breakdown step is zero and H

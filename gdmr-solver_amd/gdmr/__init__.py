@@ -90,7 +90,7 @@ _sig("dmrGlobalHandle", _vp, [])
 _sig("dmrFootprint", C.c_size_t, [C.c_int, C.c_int, C.c_int, C.c_int])
 _sig("dmrSetProfiling", C.c_int, [_vp, C.c_int])
 _sig("dmrGetProfile", C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_longlong)])
-PHASES = ("spmv", "gemvt", "update", "scale", "combo", "zgemm", "lsq", "svd")
+PHASES = ("spmv", "gemvt", "update", "scale", "combo", "zgemm", "lsq", "svd", "orth")
 # Problems and random numbers.
 _sig("dmrGenConvDiff2D", C.c_longlong, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.c_ulonglong, C.c_ulonglong, _ip, _ip, _dp])

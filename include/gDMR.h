@@ -138,7 +138,8 @@ enum {
   DMR_PH_ZGEMM = 5,   /* deflation-subspace refresh [V Z] C and normalisation   */
   DMR_PH_LSQ = 6,     /* Hessenberg least squares                               */
   DMR_PH_SVD = 7,     /* Hessenberg Jacobi SVD                                  */
-  DMR_NPHASES = 8
+  DMR_PH_ORTH = 8,    /* fused Gram-Schmidt step (projections, updates, norms)  */
+  DMR_NPHASES = 9
 };
 int dmrSetProfiling(dmrHandle h, int on);
 /* Accumulated milliseconds, algorithmic bytes and launch counts per phase since profiling was

@@ -42,7 +42,8 @@ def build_sequence(N, steps, seed):
     return mats, rhs
 
 
-def run_leg(name, prec, mats, rhs, dev_vals, dev_vals32, rp, ci, m, k, tol, warmup, profile):
+def run_leg(name, prec, mats, rhs, dev_vals, dev_vals32, rp, ci, m, k, tol, warmup, profile,
+            inner_tol=1e-5):
     import torch
 
     import gdmr
@@ -61,6 +62,8 @@ def run_leg(name, prec, mats, rhs, dev_vals, dev_vals32, rp, ci, m, k, tol, warm
     s = gdmr.Solver(n, m, k, precision=prec)
     s.set_csr(rp.data_ptr(), ci.data_ptr(), dev_vals[0].data_ptr(), mats[0].nnz,
               dev_vals32[0].data_ptr() if prec != gdmr.PREC_FP64 else 0)
+    if prec != gdmr.PREC_FP64:
+        s.set_refinement(inner_tol)
     if profile:
         s.set_profiling(True)
     bs = [torch.from_numpy(b).to("cuda") for b in rhs]
@@ -154,6 +157,8 @@ def main():
     ap.add_argument("--cpu-cycles", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--inner-tol", type=float, default=1e-5,
+                    help="relative tolerance of each inner FP32 solve of the mixed leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -176,7 +181,7 @@ def main():
     for name in args.legs.split(","):
         prec = gdmr.PREC_FP64 if name == "fp64" else gdmr.PREC_FP32
         legs[name] = run_leg(name, prec, mats, rhs, dev_vals, dev_vals32, rp, ci, m, k, tol,
-                             args.warmup, not args.no_profile)
+                             args.warmup, not args.no_profile, args.inner_tol)
     out_legs = {nm: summarize(lg, tol) for nm, lg in legs.items()}
     if "mixed" in legs and "fp64" in legs:
         for nm in ("mixed",):

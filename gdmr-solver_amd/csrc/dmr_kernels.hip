@@ -469,7 +469,7 @@ struct OrthArgs {
   int *err;          // set when a barrier wait exceeds its bound
 };
 
-constexpr unsigned kSpinLimit = 1u << 26;
+constexpr unsigned kSpinLimit = 1u << 23;
 
 // Phase barrier: true in the block that arrives last (it must call phase_release).
 __device__ inline bool phase_arrive(unsigned *bar, unsigned target) {
@@ -492,7 +492,8 @@ __device__ inline void phase_wait(unsigned *bar, unsigned epoch, int *err) {
   if (threadIdx.x == 0) {
     unsigned spins = 0;
     while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-      __builtin_amdgcn_s_sleep(1);
+      // Back off (~500 cycles) so hundreds of waiting blocks do not flood one address.
+      __builtin_amdgcn_s_sleep(8);
       if (++spins > kSpinLimit) {
         __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;

@@ -124,7 +124,10 @@ struct dmrContext {
   unsigned *bar = nullptr;  // grid-barrier words of the fused orthogonalisation
   int *devErr = nullptr;    // set by a device-side wait that exceeded its bound
   unsigned epoch = 0;       // barrier phases completed in the current cycle
-  bool fused = true;        // fused cooperative orthogonalisation (DMR_FUSED=0 disables)
+  bool fused = false;       // fused cooperative orthogonalisation (DMR_FUSED=1 enables)
+  // Deflated Krylov steps project against [W V] in one Gram-Schmidt stage instead of the
+  // reference's deflate-then-CGS; default for mixed precision (DMR_ONE_STAGE overrides).
+  bool oneStage = false;
   dmr::RedWork red{};
   dmr::RedWork redOp{};  // sized to the SpMV block count of the operator
   void *redOpPool = nullptr;
@@ -260,7 +263,10 @@ int ctx_alloc(dmrContext *c) {
   c->sweeps = ip + 1;
   c->bar = reinterpret_cast<unsigned *>(ip + 2);
   c->devErr = ip + 4;
-  c->fused = dmr::env_flag("DMR_FUSED", true);
+  // The fused cooperative step kernel is bitwise equivalent but measured slower than the
+  // separate kernels at 128^3 (2.2 vs ~4 TB/s), so it is opt-in.
+  c->fused = dmr::env_flag("DMR_FUSED", false);
+  c->oneStage = dmr::env_flag("DMR_ONE_STAGE", c->prec != DMR_PREC_FP64);
   const double one = 1.0;
   HIPCHK(hipMemcpy(c->one, &one, sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipHostMalloc((void **)&c->hostStatus, 64 * sizeof(double), hipHostMallocDefault));
@@ -487,7 +493,12 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
       double *hcol = c->H + (long long)(d + j - 1) * (m + 1);
       TS *vj = w.P + (long long)(o + j) * ld;
       dmr::Fin fin{dmr::FIN_ARNOLDI, c->scal, hcol + d + j, tol, j, c->flag};
-      orthogonalize(w.g, vj, w.P, d > 0 ? k : 0, hcol, Po, j, hcol + d, 3, fin, c->flag);
+      if (d > 0 && c->oneStage)
+        // [W V_0..V_{j-1}] = P[0..k+j-1] is contiguous: one projection and one update instead of
+        // deflate-then-CGS (the inner FP32 solve of the mixed path; H receives the same entries).
+        orthogonalize(w.g, vj, nullptr, 0, nullptr, w.P, k + j, hcol, 3, fin, c->flag);
+      else
+        orthogonalize(w.g, vj, w.P, d > 0 ? k : 0, hcol, Po, j, hcol + d, 3, fin, c->flag);
     }
     dmr::breakdown_fixup<TS>(w.P, ld, n, o, d, m, c->H, c->flag, s);
     // ---- updateAppResDMR (gDMR.cu:2411-2582)

@@ -81,7 +81,7 @@ def run_leg(name, prec, mats, rhs, dev_vals, dev_vals32, rp, ci, m, k, tol, warm
         cycles.append(st.cycles)
         reported.append(res)
         log(f"[{name}] system {t}: {lat[-1]*1e3:.1f} ms, cycles {st.cycles}, residual {res:.3e}, "
-            f"status {st.status}")
+            f"status {st.status}, svd sweeps/call {st.svdSweeps / max(st.svdCalls, 1):.1f}")
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
     prof = s.profile() if profile else None

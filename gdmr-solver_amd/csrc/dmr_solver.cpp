@@ -534,16 +534,24 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
         hipMemcpyAsync(c->hostStatus + 8, c->flag, sizeof(int), hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
         hipMemcpyAsync(c->hostStatus + 9, c->devErr, sizeof(int), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(c->hostStatus + 10, c->sweeps, sizeof(int), hipMemcpyDeviceToHost, s) !=
             hipSuccess)
       return set_err(DMR_ERR_HIP, "status readback failed");
     if (hipStreamSynchronize(s) != hipSuccess) return set_err(DMR_ERR_HIP, "stream sync failed");
     if (c->prof.on) prof_collect(c);
     const double q = c->hostStatus[2];
-    int bflag, derr;
+    int bflag, derr, sweeps;
     std::memcpy(&bflag, c->hostStatus + 8, sizeof(int));
     std::memcpy(&derr, c->hostStatus + 9, sizeof(int));
+    std::memcpy(&sweeps, c->hostStatus + 10, sizeof(int));
     if (derr) return set_err(DMR_ERR_HIP, "device grid barrier exceeded its wait bound");
     if (bflag) ++c->st.breakdowns;
+    // The sweep count read here belongs to the previous cycle's refresh (0 before the first).
+    if (sweeps > 0 && c->st.cycles > 0) {
+      c->st.svdSweeps += sweeps;
+      ++c->st.svdCalls;
+    }
     // ---- genDeflatDMR (gDMR.cu:1583-1704): Z <- normalize([V Z] C), C = smallest right
     // singular vectors of H.  After a breakdown the basis beyond the breakdown step is zero and H
     // is padded with identity columns, so singular vectors from the padding combine zero vectors

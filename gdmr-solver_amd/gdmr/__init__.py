@@ -43,7 +43,7 @@ OOP_OP = C.CFUNCTYPE(None, _vp, _vp, _vp, _vp)
 class Stats(C.Structure):
     _fields_ = [("cycles", C.c_int), ("matvecs", C.c_int), ("breakdowns", C.c_int),
                 ("status", C.c_int), ("beta", C.c_double), ("rho", C.c_double),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("svdSweeps", C.c_int), ("svdCalls", C.c_int)]
 
 
 def _sig(name, res, args):

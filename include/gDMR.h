@@ -88,6 +88,8 @@ typedef struct {
   double beta;         /* ||b||                                                           */
   double rho;          /* final tracked ||r||                                             */
   double seconds;      /* wall time of the solve                                          */
+  int svdSweeps;       /* Jacobi sweeps summed over the deflation refreshes read back     */
+  int svdCalls;        /* deflation refreshes whose sweep count is included               */
 } dmrStats;
 int dmrGetStats(dmrStats *st, double *residHist, int cap);
 

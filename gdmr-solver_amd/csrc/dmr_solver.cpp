@@ -107,6 +107,7 @@ struct dmrContext {
   dmr::CsrDev csr;
   int *ownedBlockRows = nullptr;
   float *ownedVal32 = nullptr;
+  bool val32Stale = false;  // ownedVal32 is converted from csr.val on the next solve's stream
   dmrOperatorFn opFn = nullptr;
   void *opCtx = nullptr;
   void (*legacy)(void) = nullptr;
@@ -689,6 +690,12 @@ double solve_mixed(dmrContext *c, int mode, double *x, const double *b, hipStrea
     return 1.0;
   }
   const dmr::RedWork red = c->red;
+  if (c->val32Stale) {
+    // FP32 copy of the operator values, ordered on the solve stream (no device-wide sync, so
+    // solves on other streams keep running).
+    dmr::convert<float, double>(c->ownedVal32, c->csr.val, c->csr.nnz, s);
+    c->val32Stale = false;
+  }
   if (hipMemcpyAsync(c->b64, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
       hipMemcpyAsync(c->x64, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s) != hipSuccess) {
     set_err(DMR_ERR_HIP, "input copy failed");
@@ -846,9 +853,10 @@ int dmrSetOperatorCsr(dmrHandle h, int nnz, const int *rowPtr, const int *colInd
     if (h->ownedVal32) hipFree(h->ownedVal32);
     h->ownedVal32 = nullptr;
     HIPCHK(hipMalloc(&h->ownedVal32, sizeof(float) * std::max(nnz, 1)));
-    dmr::convert<float, double>(h->ownedVal32, val, nnz, nullptr);
-    HIPCHK(hipDeviceSynchronize());
     h->csr.val32 = h->ownedVal32;
+    h->val32Stale = true;
+  } else {
+    h->val32Stale = false;
   }
   h->opKind = OP_CSR;
   return DMR_OK;
@@ -857,15 +865,15 @@ int dmrSetOperatorCsr(dmrHandle h, int nnz, const int *rowPtr, const int *colInd
 int dmrSetOperatorValues(dmrHandle h, const double *val, const float *val32) {
   clear_err();
   if (!h || h->opKind != OP_CSR || !val) return set_err(DMR_ERR_STATE, "no CSR operator to update");
+  bool stale = false;
   if (h->prec != DMR_PREC_FP64 && !val32) {
     if (!h->ownedVal32) return set_err(DMR_ERR_STATE, "no FP32 value buffer");
-    HIPCHK(hipSetDevice(h->device));
-    dmr::convert<float, double>(h->ownedVal32, val, h->csr.nnz, nullptr);
-    HIPCHK(hipDeviceSynchronize());
     val32 = h->ownedVal32;
+    stale = true;
   }
   h->csr.val = val;
   h->csr.val32 = val32;
+  h->val32Stale = stale;
   return DMR_OK;
 }
 
@@ -959,6 +967,42 @@ int dmrExportZ(dmrHandle h, double *hostZ) {
                        sizeof(float) * h->n, h->k, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < tmp.size(); ++i) hostZ[i] = tmp[i];
   }
+  return DMR_OK;
+}
+
+size_t dmrSubspaceBytes(dmrHandle h) {
+  if (!h) return 0;
+  const size_t es = h->prec == DMR_PREC_FP64 ? sizeof(double) : sizeof(float);
+  return es * (size_t)h->ld * (size_t)h->k;
+}
+
+int dmrSubspaceToDevice(dmrHandle h, void *dst, hipStream_t stream) {
+  if (!h || !dst) return set_err(DMR_ERR_ARG, "null argument");
+  const void *src = h->prec == DMR_PREC_FP64 ? (const void *)h->w64.Z : (const void *)h->w32.Z;
+  if (h->k == 0) return DMR_OK;
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipMemcpyAsync(dst, src, dmrSubspaceBytes(h), hipMemcpyDeviceToDevice, stream));
+  return DMR_OK;
+}
+
+int dmrSubspaceFromDevice(dmrHandle h, const void *src, hipStream_t stream) {
+  if (!h || !src) return set_err(DMR_ERR_ARG, "null argument");
+  if (h->k == 0) return DMR_OK;
+  void *dst = h->prec == DMR_PREC_FP64 ? (void *)h->w64.Z : (void *)h->w32.Z;
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipMemcpyAsync(dst, src, dmrSubspaceBytes(h), hipMemcpyDeviceToDevice, stream));
+  if (h->prec == DMR_PREC_FP64)
+    h->w64.hasZ = true;
+  else
+    h->w32.hasZ = true;
+  return DMR_OK;
+}
+
+int dmrGetStagnationCount(dmrHandle h) { return h ? h->kryFlagCount : 0; }
+
+int dmrSetStagnationCount(dmrHandle h, int count) {
+  if (!h || count < 0) return set_err(DMR_ERR_ARG, "invalid stagnation count");
+  h->kryFlagCount = count;
   return DMR_OK;
 }
 

@@ -88,6 +88,21 @@ _sig("dmrGetStatsH", C.c_int, [_vp, C.POINTER(Stats), _dp, C.c_int])
 _sig("dmrSetRefinement", C.c_int, [_vp, C.c_double, C.c_int])
 _sig("dmrGlobalHandle", _vp, [])
 _sig("dmrFootprint", C.c_size_t, [C.c_int, C.c_int, C.c_int, C.c_int])
+_sig("dmrSubspaceBytes", C.c_size_t, [_vp])
+_sig("dmrSubspaceToDevice", C.c_int, [_vp, _vp, _vp])
+_sig("dmrSubspaceFromDevice", C.c_int, [_vp, _vp, _vp])
+_sig("dmrServerCreate", C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_size_t])
+_sig("dmrServerAddFamily", C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int,
+                                     _ip])
+_sig("dmrServerSubmit", C.c_int, [_vp, C.c_int, _vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp),
+                                  C.c_double, C.c_int, C.POINTER(C.c_longlong)])
+_sig("dmrServerNow", C.c_double, [_vp])
+_sig("dmrServerDrain", C.c_int, [_vp])
+_sig("dmrServerStats", C.c_int, [_vp, _ip, _ip, _ip, C.POINTER(C.c_size_t)])
+_sig("dmrServerDestroy", C.c_int, [_vp])
+_sig("dmrServerSetRefinement", C.c_int, [_vp, C.c_int, C.c_double, C.c_int])
+_sig("dmrGetStagnationCount", C.c_int, [_vp])
+_sig("dmrSetStagnationCount", C.c_int, [_vp, C.c_int])
 _sig("dmrSetProfiling", C.c_int, [_vp, C.c_int])
 _sig("dmrGetProfile", C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_longlong)])
 PHASES = ("spmv", "gemvt", "update", "scale", "combo", "zgemm", "lsq", "svd", "orth")
@@ -110,7 +125,10 @@ HEADER_SYMBOLS = (
     "dmrSetOperatorCsr dmrSetOperatorValues dmrSetOperatorFn dmrSetOperatorLegacy dmrSolve "
     "dmrImportZ dmrExportZ "
     "dmrHasZ dmrGetStatsH dmrSetRefinement dmrGlobalHandle dmrFootprint dmrSetProfiling "
-    "dmrGetProfile "
+    "dmrGetProfile dmrSubspaceBytes dmrSubspaceToDevice dmrSubspaceFromDevice "
+    "dmrGetStagnationCount dmrSetStagnationCount "
+    "dmrServerCreate dmrServerAddFamily dmrServerSetRefinement dmrServerSubmit dmrServerPoll "
+    "dmrServerWait dmrServerDrain dmrServerNow dmrServerStats dmrServerDestroy "
     "initLS freeLS impImgLS expImgLS impSrcLS expSrcLS impOptLS expOptLS solveLS linOptLS "
     "impCsrLS nnzLS dmrGenConvDiff2D dmrGenConvDiff3D dmrGenAniso3D dmrRandNormal "
     "dmrRandUniform dmrSplitMix64").split()
@@ -336,3 +354,79 @@ class LinearSystem:
 
     def close(self):
         lib.freeLS()
+
+
+# ------------------------------------------------------------------------------------------------
+# Serving loop (gDMRServe.h)
+# ------------------------------------------------------------------------------------------------
+class RequestResult(C.Structure):
+    _fields_ = [("id", C.c_longlong), ("family", C.c_int), ("nrhs", C.c_int), ("status", C.c_int),
+                ("cycles", C.c_int), ("slot", C.c_int), ("recycled", C.c_int),
+                ("maxRelRes", C.c_double), ("submitTime", C.c_double),
+                ("startTime", C.c_double), ("endTime", C.c_double)]
+
+
+_sig("dmrServerPoll", C.c_int, [_vp, C.POINTER(RequestResult), C.c_int])
+_sig("dmrServerWait", C.c_int, [_vp, C.POINTER(RequestResult), C.c_int, C.c_double])
+
+
+class Server:
+    """Worker slots over solver handles; one resident recycled subspace per operator family."""
+
+    def __init__(self, slots: int = 4, device: int = 0, hbm_reserve: int = 1 << 30):
+        h = C.c_void_p()
+        _check(lib.dmrServerCreate(C.byref(h), device, slots, hbm_reserve), "dmrServerCreate")
+        self.h = h
+        self._keep = {}
+
+    def add_family(self, n, nnz, rowptr_ptr, col_ptr, m, k, precision=PREC_FP64,
+                   keep=()) -> int:
+        fam = C.c_int()
+        _check(lib.dmrServerAddFamily(self.h, n, nnz, rowptr_ptr, col_ptr, m, k, precision,
+                                      C.byref(fam)), "dmrServerAddFamily")
+        self._keep[("family", fam.value)] = keep
+        return fam.value
+
+    def set_refinement(self, family: int, inner_tol: float, max_outer: int = 60):
+        _check(lib.dmrServerSetRefinement(self.h, family, inner_tol, max_outer),
+               "dmrServerSetRefinement")
+
+    def submit(self, family, val_ptr, b_ptrs, x_ptrs, tol, mat_rank, keep=()) -> int:
+        nr = len(b_ptrs)
+        bs = (C.c_void_p * nr)(*b_ptrs)
+        xs = (C.c_void_p * nr)(*x_ptrs)
+        rid = C.c_longlong()
+        _check(lib.dmrServerSubmit(self.h, family, val_ptr, nr, bs, xs, tol, mat_rank,
+                                   C.byref(rid)), "dmrServerSubmit")
+        self._keep[rid.value] = keep
+        return rid.value
+
+    def wait(self, timeout: float = 1.0, cap: int = 256):
+        buf = (RequestResult * cap)()
+        cnt = lib.dmrServerWait(self.h, buf, cap, timeout)
+        out = [buf[i] for i in range(cnt)]
+        for r in out:
+            self._keep.pop(r.id, None)
+        return out
+
+    def drain(self):
+        lib.dmrServerDrain(self.h)
+
+    def now(self) -> float:
+        return lib.dmrServerNow(self.h)
+
+    def stats(self):
+        q, r, hd, fr = C.c_int(), C.c_int(), C.c_int(), C.c_size_t()
+        lib.dmrServerStats(self.h, C.byref(q), C.byref(r), C.byref(hd), C.byref(fr))
+        return dict(queued=q.value, running=r.value, handles=hd.value, hbm_free=fr.value)
+
+    def close(self):
+        if self.h:
+            lib.dmrServerDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -123,6 +123,16 @@ double dmrSolve(dmrHandle h, int mode, double *x, const double *b, hipStream_t s
 int dmrImportZ(dmrHandle h, const double *hostZ);
 int dmrExportZ(dmrHandle h, double *hostZ);
 int dmrHasZ(dmrHandle h);
+/* Device-side copies of the recycled subspace in the handle's own layout (k columns of
+ * dmrSubspaceBytes(h)/k bytes, the handle's storage precision), enqueued on stream: lets a
+ * caller keep one resident subspace per operator family and hand it from handle to handle. */
+size_t dmrSubspaceBytes(dmrHandle h);
+int dmrSubspaceToDevice(dmrHandle h, void *dst, hipStream_t stream);
+int dmrSubspaceFromDevice(dmrHandle h, const void *src, hipStream_t stream);
+/* The stagnation counter that persists across the solves of one context (kryFlagCountDMR,
+ * gDMR.cu:890, reset only by initDMR); with the subspace it is a recycled family's state. */
+int dmrGetStagnationCount(dmrHandle h);
+int dmrSetStagnationCount(dmrHandle h, int count);
 int dmrGetStatsH(dmrHandle h, dmrStats *st, double *residHist, int cap);
 /* Mixed-precision refinement: relative tolerance of each inner FP32 solve (default 1e-5) and the
  * maximum number of refinement steps (default 60). */

@@ -18,9 +18,9 @@ def _declared(header):
 
 
 def test_headers_fully_exported():
-    declared = _declared("gDMR.h") | _declared("gDMRC.h")
+    declared = _declared("gDMR.h") | _declared("gDMRC.h") | _declared("gDMRServe.h")
     assert {"initDMR", "freeDMR", "solverDMR", "impSVDMR", "expSVDMR", "initLS", "solveLS",
-            "linOptLS", "impCsrLS"} <= declared
+            "linOptLS", "impCsrLS", "dmrServerSubmit", "dmrServerWait"} <= declared
     out = subprocess.run(["nm", "-D", "--defined-only", gdmr.LIB_PATH], capture_output=True,
                          text=True, check=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
@@ -55,3 +55,21 @@ def test_cpu_only_entry_points():
     its = gdmr.C.c_int(0)
     r = gdmr.lib.solverDMR(0, None, None, None, 0, 1e-8, 10, gdmr.C.byref(its))
     assert r == 1.0 and gdmr.last_error()[0] == gdmr.DMR_ERR_STATE
+
+
+def test_server_rejects_bad_arguments_without_device():
+    """Argument checks of the serving API come before any device call; with no device visible
+    creation fails cleanly with DMR_ERR_HIP instead of starting workers."""
+    h = gdmr.C.c_void_p()
+    assert gdmr.lib.dmrServerCreate(gdmr.C.byref(h), 0, 0, 0) == gdmr.DMR_ERR_ARG
+    assert gdmr.lib.dmrServerCreate(None, 0, 2, 0) == gdmr.DMR_ERR_ARG
+    assert gdmr.lib.dmrServerCreate(gdmr.C.byref(h), 0, 2, 0) in (gdmr.DMR_OK, gdmr.DMR_ERR_HIP)
+    if h.value:
+        gdmr.lib.dmrServerDestroy(h)
+    fam = gdmr.C.c_int()
+    assert gdmr.lib.dmrServerAddFamily(None, 8, 8, None, None, 4, 2, 0,
+                                       gdmr.C.byref(fam)) == gdmr.DMR_ERR_ARG
+    rid = gdmr.C.c_longlong()
+    assert gdmr.lib.dmrServerSubmit(None, 0, None, 1, None, None, 1e-8, 10,
+                                    gdmr.C.byref(rid)) == gdmr.DMR_ERR_ARG
+    assert gdmr.lib.dmrServerDrain(None) == gdmr.DMR_ERR_ARG

@@ -1,0 +1,168 @@
+"""Serving loop (gDMRServe.h): per-family resident subspaces, continuous batching over slots,
+admission by free HBM with eviction.
+
+The invariant under test is the serving rule: a family's results depend on its own request
+sequence only.  Each family's requests are served once alone (one slot, no other family) and once
+in a mix (several slots, three families interleaved, one of them mixed precision); the solutions
+must agree bit for bit, and each must meet its tolerance in a true FP64 residual.  Eviction moves a
+family's recycled state to the host and back; that must not change a bit either.
+"""
+import numpy as np
+import pytest
+
+import gdmr
+
+pytestmark = pytest.mark.gpu
+
+REQS = 4  # requests per family
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def _families():
+    """(name, generator(t) -> CsrMatrix, m, k, precision, tol, nrhs per request)."""
+    return [
+        ("cd2d-96", lambda t: gdmr.conv_diff_2d(96, jitter=0.01, seed=11, stream=t), 30, 10,
+         gdmr.PREC_FP64, 1e-10, (1, 2, 1, 3)),
+        ("cd3d-24", lambda t: gdmr.conv_diff_3d(24, jitter=0.01, seed=12, stream=t), 24, 8,
+         gdmr.PREC_FP64, 1e-10, (2, 1, 1, 2)),
+        ("an3d-20-fp32", lambda t: gdmr.aniso_3d(20, jitter=0.01, seed=13, stream=t), 24, 8,
+         gdmr.PREC_FP32, 1e-10, (1, 1, 2, 1)),
+    ]
+
+
+class _Served:
+    """Device inputs of every request of the given families, and a server run over them."""
+
+    def __init__(self, torch, fams):
+        self.torch = torch
+        self.fams = fams
+        self.pattern, self.reqs = [], []
+        for fi, (name, gen, m, k, prec, tol, nrhs) in enumerate(fams):
+            A0 = gen(0)
+            self.pattern.append((A0, _dev(torch, A0.rowptr), _dev(torch, A0.col)))
+            for t in range(REQS):
+                A = gen(t)
+                assert np.array_equal(A.col, A0.col)
+                B = [gdmr.rand_normal(100 + fi, 10 * t + j, A.n) for j in range(nrhs[t])]
+                self.reqs.append(dict(fam=fi, t=t, A=A, val=_dev(torch, A.val), B=B,
+                                      b=[_dev(torch, b) for b in B]))
+
+    def run(self, slots, order, hbm_reserve=1 << 28):
+        """Submits requests in `order` (indices into self.reqs) to a fresh server; returns
+        {request index: (result, [x arrays])}."""
+        torch = self.torch
+        srv = gdmr.Server(slots=slots, hbm_reserve=hbm_reserve)
+        try:
+            fam_id = {}
+            for fi in sorted({self.reqs[i]["fam"] for i in order}):
+                name, gen, m, k, prec, tol, nrhs = self.fams[fi]
+                A0, rp, ci = self.pattern[fi]
+                fam_id[fi] = srv.add_family(A0.n, A0.nnz, rp.data_ptr(), ci.data_ptr(), m, k,
+                                            prec, keep=(rp, ci))
+                if prec != gdmr.PREC_FP64:
+                    srv.set_refinement(fam_id[fi], 1e-5, 60)
+            xs, ids = {}, {}
+            for i in order:
+                r = self.reqs[i]
+                tol = self.fams[r["fam"]][5]
+                xs[i] = [torch.zeros(r["A"].n, dtype=torch.float64, device="cuda")
+                         for _ in r["b"]]
+                rid = srv.submit(fam_id[r["fam"]], r["val"].data_ptr(),
+                                 [b.data_ptr() for b in r["b"]], [x.data_ptr() for x in xs[i]],
+                                 tol, 10 * r["A"].n, keep=(r["val"], r["b"], xs[i]))
+                ids[rid] = i
+            srv.drain()
+            results = srv.wait(timeout=0.0, cap=4 * len(order))
+            assert len(results) == len(order)
+            stats = srv.stats()
+        finally:
+            srv.close()
+        out = {}
+        for res in results:
+            i = ids[res.id]
+            out[i] = (res, [x.cpu().numpy() for x in xs[i]])
+        return out, stats
+
+
+def _check_residuals(served, out):
+    for i, (res, xs) in out.items():
+        r = served.reqs[i]
+        tol = served.fams[r["fam"]][5]
+        assert res.status == 0, (i, res.status)
+        for B, x in zip(r["B"], xs):
+            tr = np.linalg.norm(B - r["A"].matvec(x)) / np.linalg.norm(B)
+            assert tr <= tol * 1.05, (i, tr)
+
+
+def test_alone_vs_mix_bitwise(torch_gpu):
+    fams = _families()
+    served = _Served(torch_gpu, fams)
+    per_fam = {fi: [i for i, r in enumerate(served.reqs) if r["fam"] == fi]
+               for fi in range(len(fams))}
+    alone = {}
+    for fi, idx in per_fam.items():
+        out, _ = served.run(1, idx)
+        alone.update(out)
+    _check_residuals(served, alone)
+    # Interleave the families' requests (submission order within a family kept).
+    mix_order = [per_fam[fi][t] for t in range(REQS) for fi in per_fam]
+    mix, stats = served.run(4, mix_order)
+    _check_residuals(served, mix)
+    assert stats["queued"] == 0 and stats["running"] == 0
+    for i in range(len(served.reqs)):
+        ra, xa = alone[i]
+        rm, xm = mix[i]
+        r = served.reqs[i]
+        print(f"{fams[r['fam']][0]} req {r['t']}: cycles alone {ra.cycles} mix {rm.cycles}, "
+              f"recycled {ra.recycled}/{rm.recycled}, slot {rm.slot}", flush=True)
+        assert ra.cycles == rm.cycles
+        assert ra.recycled == rm.recycled == (1 if r["t"] > 0 else 0)
+        for a, b in zip(xa, xm):
+            assert np.array_equal(a, b), f"request {i} differs alone vs in the mix"
+    # Requests of different families overlapped in time.
+    spans = [(mix[i][0].startTime, mix[i][0].endTime, served.reqs[i]["fam"]) for i in mix]
+    overlap = any(s1 < e2 and s2 < e1 and f1 != f2
+                  for (s1, e1, f1) in spans for (s2, e2, f2) in spans)
+    assert overlap
+    # Recycling pays on the 2-D family: a later request needs fewer cycles per right-hand side
+    # than the first one (one right-hand side, fresh start).
+    first = alone[per_fam[0][0]][0]
+    assert first.nrhs == 1
+    later = [alone[i][0] for i in per_fam[0][1:]]
+    assert min(r.cycles / r.nrhs for r in later) < first.cycles
+
+
+def test_eviction_keeps_the_chain(torch_gpu):
+    """Two families whose handles do not fit together: every admission evicts the other family
+    (subspace and stagnation counter to host memory and back), and the solutions are bitwise the
+    ones of the family served alone with room to spare."""
+    torch = torch_gpu
+    fams = [
+        ("cd3d-40", lambda t: gdmr.conv_diff_3d(40, jitter=0.01, seed=21, stream=t), 30, 10,
+         gdmr.PREC_FP64, 1e-10, (1, 1, 1, 1)),
+        ("an3d-40", lambda t: gdmr.aniso_3d(40, jitter=0.01, seed=22, stream=t), 30, 10,
+         gdmr.PREC_FP64, 1e-10, (1, 1, 1, 1)),
+    ]
+    served = _Served(torch, fams)
+    per_fam = {fi: [i for i, r in enumerate(served.reqs) if r["fam"] == fi] for fi in (0, 1)}
+    alone = {}
+    for fi, idx in per_fam.items():
+        out, _ = served.run(1, idx)
+        alone.update(out)
+    n = 40 ** 3
+    fp = gdmr.footprint(n, 30, 10) + n * 4 // 64 + (8 << 20)
+    torch.cuda.synchronize()
+    free, _ = torch.cuda.mem_get_info()
+    reserve = free - int(1.6 * fp)
+    assert reserve > 0
+    order = [per_fam[fi][t] for t in range(REQS) for fi in (0, 1)]
+    mix, stats = served.run(2, order, hbm_reserve=reserve)
+    _check_residuals(served, mix)
+    assert stats["handles"] <= 1
+    for i in order:
+        for a, b in zip(alone[i][1], mix[i][1]):
+            assert np.array_equal(a, b), f"request {i} changed by eviction"
+        assert alone[i][0].cycles == mix[i][0].cycles

@@ -159,7 +159,16 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--inner-tol", type=float, default=1e-5,
                     help="relative tolerance of each inner FP32 solve of the mixed leg")
-    args = ap.parse_args()
+    ap.add_argument("--serve", action="store_true",
+                    help="run the serving load generator (gdmr.serve_bench) instead; the "
+                         "remaining arguments go to it")
+    args, rest = ap.parse_known_args()
+    if args.serve:
+        from gdmr import serve_bench
+        serve_bench.main(rest)
+        return
+    if rest:
+        ap.error(f"unrecognized arguments: {' '.join(rest)}")
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
               "(--gpus must be 1)", file=sys.stderr)

@@ -18,7 +18,9 @@
  * its own request sequence only: they are bitwise the same served alone or in any mix.
  *
  * All device pointers stay owned by the caller and must stay valid until the request's result
- * has been returned.  Every entry point is thread-safe.
+ * has been returned.  The server's streams do not wait on the caller's: a request's values,
+ * right-hand sides and initial guesses must be complete on the device when it is submitted, and
+ * its solutions are complete when its result is returned.  Every entry point is thread-safe.
  */
 #ifndef GDMR_SERVE_H
 #define GDMR_SERVE_H

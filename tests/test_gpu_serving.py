@@ -70,6 +70,8 @@ class _Served:
                 tol = self.fams[r["fam"]][5]
                 xs[i] = [torch.zeros(r["A"].n, dtype=torch.float64, device="cuda")
                          for _ in r["b"]]
+                # The server's streams do not wait on torch's: inputs must be complete.
+                torch.cuda.synchronize()
                 rid = srv.submit(fam_id[r["fam"]], r["val"].data_ptr(),
                                  [b.data_ptr() for b in r["b"]], [x.data_ptr() for x in xs[i]],
                                  tol, 10 * r["A"].n, keep=(r["val"], r["b"], xs[i]))

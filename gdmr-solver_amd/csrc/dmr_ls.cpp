@@ -230,6 +230,20 @@ int impCsrLS(const int *rowPtr, const int *colInd, const double *val, int nnz) {
 
 int nnzLS(void) { return g_ls.live ? g_ls.csr.nnz : 0; }
 
+int impMtxLS(const char *path) {
+  if (!g_ls.live) return DMR_ERR_STATE;
+  int rows = 0, cols = 0;
+  long long nnz = 0;
+  int rc = dmrReadMatrixMarket(path, &rows, &cols, &nnz, nullptr, nullptr, nullptr);
+  if (rc != DMR_OK) return rc;
+  if (rows != g_ls.n || cols != g_ls.n) return DMR_ERR_ARG;
+  std::vector<int> rp(rows + 1), ci(std::max<long long>(nnz, 1));
+  std::vector<double> va(std::max<long long>(nnz, 1));
+  rc = dmrReadMatrixMarket(path, &rows, &cols, &nnz, rp.data(), ci.data(), va.data());
+  if (rc != DMR_OK) return rc;
+  return upload_csr(rp.data(), ci.data(), va.data(), (int)nnz);
+}
+
 double solveLS(int mode, double solTol, int matRank, int *numIts) {
   if (!g_ls.live) {
     std::fprintf(stderr, "solveLS: linear system not initialised\n");

@@ -117,6 +117,10 @@ _sig("dmrRandNormal", None, [C.c_ulonglong, C.c_ulonglong, C.c_longlong, _dp])
 _sig("dmrRandUniform", None, [C.c_ulonglong, C.c_ulonglong, C.c_longlong, C.c_double, C.c_double,
                               _dp])
 _sig("dmrSplitMix64", None, [C.c_ulonglong, C.c_longlong, C.POINTER(C.c_ulonglong)])
+_sig("dmrReadMatrixMarket", C.c_int, [C.c_char_p, _ip, _ip, C.POINTER(C.c_longlong), _ip, _ip,
+                                      _dp])
+_sig("dmrWriteMatrixMarket", C.c_int, [C.c_char_p, C.c_int, _ip, _ip, _dp])
+_sig("impMtxLS", C.c_int, [C.c_char_p])
 
 # Every symbol the public headers declare (checked by the CPU test suite).
 HEADER_SYMBOLS = (
@@ -130,7 +134,7 @@ HEADER_SYMBOLS = (
     "dmrServerCreate dmrServerAddFamily dmrServerSetRefinement dmrServerSubmit dmrServerPoll "
     "dmrServerWait dmrServerDrain dmrServerNow dmrServerStats dmrServerDestroy "
     "initLS freeLS impImgLS expImgLS impSrcLS expSrcLS impOptLS expOptLS solveLS linOptLS "
-    "impCsrLS nnzLS dmrGenConvDiff2D dmrGenConvDiff3D dmrGenAniso3D dmrRandNormal "
+    "impCsrLS nnzLS dmrReadMatrixMarket dmrWriteMatrixMarket impMtxLS dmrGenConvDiff2D dmrGenConvDiff3D dmrGenAniso3D dmrRandNormal "
     "dmrRandUniform dmrSplitMix64").split()
 
 
@@ -219,6 +223,35 @@ def reduction_blocks(n: int) -> int:
 
 def footprint(n: int, m: int, k: int, precision: int = PREC_FP64) -> int:
     return int(lib.dmrFootprint(n, m, k, precision))
+
+
+def read_matrix_market(path: str) -> CsrMatrix:
+    """A square Matrix Market file as CSR (dmrReadMatrixMarket)."""
+    rows, cols, nnz = C.c_int(), C.c_int(), C.c_longlong()
+    p = os.fsencode(path)
+    rc = lib.dmrReadMatrixMarket(p, C.byref(rows), C.byref(cols), C.byref(nnz), None, None, None)
+    if rc != DMR_OK:
+        raise ValueError(f"dmrReadMatrixMarket({path}) failed with code {rc}")
+    if rows.value != cols.value:
+        raise ValueError(f"{path}: {rows.value} x {cols.value} is not square")
+    rp = np.zeros(rows.value + 1, np.int32)
+    ci = np.zeros(max(nnz.value, 1), np.int32)
+    va = np.zeros(max(nnz.value, 1), np.float64)
+    rc = lib.dmrReadMatrixMarket(p, C.byref(rows), C.byref(cols), C.byref(nnz),
+                                 _ptr(rp, C.c_int), _ptr(ci, C.c_int), _ptr(va, C.c_double))
+    if rc != DMR_OK:
+        raise ValueError(f"dmrReadMatrixMarket({path}) failed with code {rc}")
+    return CsrMatrix(rows.value, rp, ci[:nnz.value], va[:nnz.value])
+
+
+def write_matrix_market(path: str, A: CsrMatrix):
+    rp = np.ascontiguousarray(A.rowptr, np.int32)
+    ci = np.ascontiguousarray(A.col, np.int32)
+    va = np.ascontiguousarray(A.val, np.float64)
+    rc = lib.dmrWriteMatrixMarket(os.fsencode(path), A.n, _ptr(rp, C.c_int), _ptr(ci, C.c_int),
+                                  _ptr(va, C.c_double))
+    if rc != DMR_OK:
+        raise ValueError(f"dmrWriteMatrixMarket({path}) failed with code {rc}")
 
 
 # ------------------------------------------------------------------------------------------------

@@ -48,6 +48,20 @@ int impCsrLS(const int *rowPtr, const int *colInd, const double *val, int nnz);
 /* Number of stored nonzeros of the current operator. */
 int nnzLS(void);
 
+/* Matrix Market files (the NIST exchange format) as on-disk operators.
+ * Reads `matrix coordinate real|integer|pattern general|symmetric|skew-symmetric` and
+ * `matrix array real|integer general`, 1-based, into 0-based CSR with columns sorted per row and
+ * duplicate entries summed (symmetric files are expanded to both triangles).  Complex / Hermitian
+ * files return DMR_ERR_NOTBUILT.  Call with rowPtr, colInd and val NULL to get the sizes, then
+ * again with arrays of rows+1, nnz and nnz entries.  Returns 0 or a DMR_ERR_* code. */
+int dmrReadMatrixMarket(const char *path, int *rows, int *cols, long long *nnz, int *rowPtr,
+                        int *colInd, double *val);
+/* Writes a square CSR matrix as `matrix coordinate real general` with 17 significant digits. */
+int dmrWriteMatrixMarket(const char *path, int n, const int *rowPtr, const int *colInd,
+                         const double *val);
+/* Reads a Matrix Market file and imports it as the operator (it must be cells x cells). */
+int impMtxLS(const char *path);
+
 /* Synthetic problems shared by the control program, tests and benchmarks.
  * All produce CSR with int32 indices, columns sorted, rows in natural order
  * (x fastest: row = i + N*j (+ N*N*l)).  The matrix is h^2 times the discrete operator

@@ -240,3 +240,32 @@ def test_mixed_precision_reaches_fp64_tolerance(torch_gpu):
         assert tr <= tol and abs(res - tr) <= 1e-3 * tr + 1e-16
         assert err < 1e-7
     s.close()
+
+
+def test_matrix_market_operator(tmp_path):
+    """The on-disk operator path: a Matrix Market file imported with impMtxLS gives the solve of
+    the same matrix imported with impCsrLS, bit for bit, and matches the oracle."""
+    A = gdmr.conv_diff_2d(40, jitter=0.01, seed=9, stream=2)
+    path = str(tmp_path / "cd2d.mtx")
+    gdmr.write_matrix_market(path, A)
+    n, m, k, tol = A.n, 24, 8, 1e-9
+    b = gdmr.rand_normal(9, 3, n)
+    ls = gdmr.LinearSystem(A, m, k)
+    try:
+        x_csr, res_csr, its_csr = ls.solve(b, tol=tol, mode=0)
+        lib = gdmr.lib
+        dev = (C.c_int * 1)(0)
+        lib.initLS(32, 512, 0, n, dev, 1, m, k)
+        assert lib.impMtxLS(path.encode()) == gdmr.DMR_OK
+        assert lib.nnzLS() == A.nnz
+        x_mtx, res_mtx, its_mtx = ls.solve(b, tol=tol, mode=0)
+    finally:
+        ls.close()
+    assert its_mtx == its_csr and res_mtx == res_csr
+    assert np.array_equal(x_mtx, x_csr)
+    ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n))
+    xo, reso, itso, sto = ora.solve(b, tol=tol, mode=0)
+    ora.close()
+    assert abs(its_mtx - itso) <= 1 and abs(res_mtx - reso) <= HIST_RTOL * reso
+    tr = _true_res(A, b, x_mtx)
+    assert tr <= tol * 1.001, tr

@@ -1,0 +1,624 @@
+// Host driver of the block (multi-right-hand-side) DMR and its C ABI (dmrBlock* in gDMR.h).
+//
+// s <= 32 right-hand sides of one operator share one block Krylov space and one deflation
+// subspace Z (n x k).  The cycle is the DMR cycle of the single-vector solver (gDMR.cu:1790-2582
+// restated for blocks) with every vector operation turned into a block one:
+//   deflation image  A Z = W R_Z             SpMM + Cholesky QR (twice)
+//   start block      V_0 S_0 = (I - W W^T) R  gemm_tn + gemm_nn + Cholesky QR
+//   block step j     U = A V_j; C = [W V_0..V_j]^T U; U -= [W V_0..V_j] C; U = V_{j+1} S_{j+1}
+//                    (one-stage block classical Gram-Schmidt: the projection and the update are
+//                    tall-skinny GEMMs on MFMA; the normalisation is CholQR2)
+//   least squares    T = P^T R (full projection); min ||T - H Y|| column by column on the block
+//                    Hessenberg H (lower bandwidth s)
+//   updates          R -= P (H Y); X_a += [Z V] Y diag(rho); rho_i *= ||R_i||; R_i /= ||R_i||
+//   refresh          Z <- normalize([Z V_0 .. V_{kr-1}] C), C the k smallest right singular
+//                    vectors of the leading (k + (kr+1) s) x (k + kr s) block of H: the block
+//                    analogue of genDeflatDMR (gDMR.cu:1583-1704) on the widest leading subspace
+//                    whose one-workgroup Jacobi SVD fits in LDS (DMR_BLOCK_KR caps kr)
+// in FP32 storage and MFMA arithmetic (FP32 accumulation), inside FP64 iterative refinement per
+// column: R = B - A X in FP64, inner block DMR on R / ||R||, X += ||R|| D.  The Hessenberg least
+// squares, the Cholesky factors and the SVD are FP64.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "dmr_block.h"
+#include "gDMR.h"
+
+namespace {
+
+constexpr double kSvdTol = 1e-14;
+constexpr int kSvdMaxSweeps = 128;
+constexpr double kDepTol = 1e-12;  // Cholesky pivot / max pivot below which a column is dependent
+constexpr int kMaxEvents = 8192;
+
+#define BCHK(call)                                                                         \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return dmr::set_error(DMR_ERR_HIP, (std::string(#call) + ": " + hipGetErrorString(e_)).c_str()); \
+  } while (0)
+
+}  // namespace
+
+struct dmrBlockContext {
+  int device = 0, n = 0, mb = 0, k = 0, sMax = 0;
+  long long ld = 0;
+  int rowsMax = 0, colsMax = 0, G = 0;
+  dmr::CsrDev csr;
+  float *ownedVal32 = nullptr;
+  bool val32Stale = false;
+  bool haveOp = false;
+  void *big = nullptr, *small = nullptr;
+  float *P = nullptr, *Z = nullptr, *Zn = nullptr, *R = nullptr, *Xa = nullptr;
+  double *R64 = nullptr;
+  double *H = nullptr, *T = nullptr, *Hw = nullptr, *Tw = nullptr, *Y = nullptr, *Hred = nullptr;
+  float *C32 = nullptr, *Yr32 = nullptr, *ZH32 = nullptr, *Sinv32 = nullptr, *Cz32 = nullptr;
+  float *part = nullptr;
+  double *G64 = nullptr, *Sacc = nullptr, *Csvd = nullptr, *sig = nullptr;
+  double *rho = nullptr, *q = nullptr, *scal = nullptr, *bn = nullptr, *rn = nullptr,
+         *f = nullptr, *zq = nullptr, *cpart = nullptr;
+  int *flags = nullptr;  // [0] dependent column seen this cycle, [1] Jacobi sweeps
+  double *host = nullptr;  // pinned staging
+  bool hasZ = false;
+  int krMax = 64;  // cap on the leading block steps of the subspace refresh
+  double innerTol = 1e-5;
+  int maxOuter = 30;
+  dmrStats st{};
+  std::vector<double> hist;
+  struct Prof {
+    bool on = false;
+    double ms[DMR_NPHASES] = {};
+    double by[DMR_NPHASES] = {};
+    long long cnt[DMR_NPHASES] = {};
+    std::vector<hipEvent_t> ev;
+    std::vector<int> phase;
+    std::vector<double> bytes;
+    int used = 0;
+  } prof;
+};
+
+namespace {
+
+using dmr::blk::kMaxRhs;
+
+// Rows per gemm_tn workgroup (DMR_BLOCK_TN_ROWS overrides; multiple of 32).
+int tn_rows_target() {
+  static const int v = [] {
+    const char *e = std::getenv("DMR_BLOCK_TN_ROWS");
+    const int x = e ? std::atoi(e) : 0;
+    return x >= 32 ? (x + 31) / 32 * 32 : 512;
+  }();
+  return v;
+}
+
+template <typename F>
+void timed(dmrBlockContext *c, hipStream_t s, int phase, double bytes, F &&f) {
+  auto &p = c->prof;
+  if (!p.on || p.used >= kMaxEvents) {
+    f();
+    return;
+  }
+  if (p.ev.empty()) {
+    p.ev.resize(2 * kMaxEvents);
+    for (auto &e : p.ev) hipEventCreate(&e);
+    p.phase.resize(kMaxEvents);
+    p.bytes.resize(kMaxEvents);
+  }
+  const int i = p.used++;
+  hipEventRecord(p.ev[2 * i], s);
+  f();
+  hipEventRecord(p.ev[2 * i + 1], s);
+  p.phase[i] = phase;
+  p.bytes[i] = bytes;
+}
+
+void prof_collect(dmrBlockContext *c) {
+  auto &p = c->prof;
+  for (int i = 0; i < p.used; ++i) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.ev[2 * i], p.ev[2 * i + 1]) == hipSuccess) {
+      p.ms[p.phase[i]] += ms;
+      p.by[p.phase[i]] += p.bytes[i];
+      ++p.cnt[p.phase[i]];
+    }
+  }
+  p.used = 0;
+}
+
+struct Sizes {
+  size_t big, small;
+};
+
+Sizes sizes(long long n, int mb, int k, int sMax, int G, long long ld) {
+  const int rowsMax = k + (mb + 1) * sMax, colsMax = k + mb * sMax;
+  const int pPadMax = (rowsMax + 31) & ~31;
+  const int rRed = rowsMax, cRed = colsMax;
+  Sizes z;
+  z.big = sizeof(float) * (size_t)ld * (size_t)(rowsMax + 2 * k + 2 * sMax) +
+          sizeof(double) * (size_t)ld * sMax;
+  z.small = sizeof(double) * ((size_t)rowsMax * colsMax * 2 + (size_t)rowsMax * 32 * 2 +
+                              (size_t)colsMax * 32 + (size_t)rRed * cRed + 32 * 32 * 2 +
+                              (size_t)cRed * 32 + cRed + 2 + 8 * 64 +
+                              (size_t)dmr::blk::colnorm_blocks(n) * 32) +
+            sizeof(float) * ((size_t)rowsMax * 32 * 2 + (size_t)colsMax * 32 + 32 * 32 +
+                             (size_t)cRed * 32 + (size_t)G * pPadMax * 32) +
+            sizeof(int) * 64 + 64 * 32;
+  return z;
+}
+
+void ctx_release(dmrBlockContext *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipDeviceSynchronize();
+  if (c->big) hipFree(c->big);
+  if (c->small) hipFree(c->small);
+  if (c->ownedVal32) hipFree(c->ownedVal32);
+  if (c->host) hipHostFree(c->host);
+  for (auto &e : c->prof.ev) hipEventDestroy(e);
+  delete c;
+}
+
+// Carves aligned sub-arrays out of a pool.
+struct Carver {
+  char *p;
+  template <typename T> T *take(size_t count) {
+    T *r = reinterpret_cast<T *>(p);
+    p += (sizeof(T) * count + 255) / 256 * 256;
+    return r;
+  }
+};
+
+int ctx_alloc(dmrBlockContext *c) {
+  const long long ld = c->ld;
+  const int k = c->k, s = c->sMax, mb = c->mb;
+  Sizes z = sizes(c->n, mb, k, s, c->G, ld);
+  z.small += 64 * 256;  // alignment slack of the carver
+  z.big += 16 * 256;
+  BCHK(hipMalloc(&c->big, z.big));
+  BCHK(hipMemset(c->big, 0, z.big));
+  BCHK(hipMalloc(&c->small, z.small));
+  BCHK(hipMemset(c->small, 0, z.small));
+  BCHK(hipHostMalloc((void **)&c->host, sizeof(double) * 4 * 64, hipHostMallocDefault));
+  Carver b{(char *)c->big};
+  c->P = b.take<float>((size_t)ld * c->rowsMax);
+  c->Z = b.take<float>((size_t)ld * std::max(k, 1));
+  c->Zn = b.take<float>((size_t)ld * std::max(k, 1));
+  c->R = b.take<float>((size_t)ld * s);
+  c->Xa = b.take<float>((size_t)ld * s);
+  c->R64 = b.take<double>((size_t)ld * s);
+  Carver m{(char *)c->small};
+  const int rowsMax = c->rowsMax, colsMax = c->colsMax;
+  const int pPadMax = (rowsMax + 31) & ~31;
+  const int rRed = rowsMax, cRed = colsMax;
+  c->H = m.take<double>((size_t)rowsMax * colsMax);
+  c->Hw = m.take<double>((size_t)rowsMax * colsMax);
+  c->T = m.take<double>((size_t)rowsMax * 32);
+  c->Tw = m.take<double>((size_t)rowsMax * 32);
+  c->Y = m.take<double>((size_t)colsMax * 32);
+  c->Hred = m.take<double>((size_t)rRed * cRed);
+  c->G64 = m.take<double>(32 * 32);
+  c->Sacc = m.take<double>(32 * 32);
+  c->Csvd = m.take<double>((size_t)cRed * 32);
+  c->sig = m.take<double>(cRed + 2);
+  c->rho = m.take<double>(64);
+  c->q = m.take<double>(64);
+  c->scal = m.take<double>(64);
+  c->bn = m.take<double>(64);
+  c->rn = m.take<double>(64);
+  c->f = m.take<double>(64);
+  c->zq = m.take<double>(64);
+  c->cpart = m.take<double>((size_t)dmr::blk::colnorm_blocks(c->n) * 32);
+  c->C32 = m.take<float>((size_t)rowsMax * 32);
+  c->ZH32 = m.take<float>((size_t)rowsMax * 32);
+  c->Yr32 = m.take<float>((size_t)colsMax * 32);
+  c->Sinv32 = m.take<float>(32 * 32);
+  c->Cz32 = m.take<float>((size_t)cRed * 32);
+  c->part = m.take<float>((size_t)c->G * pPadMax * 32);
+  c->flags = m.take<int>(64);
+  return DMR_OK;
+}
+
+// Two Cholesky QR passes on the sc columns of U (in place).  The combined triangular factor goes
+// to Hdst (ld ldh) when given.
+void cholqr2(dmrBlockContext *c, float *U, int sc, double *Hdst, int ldh, hipStream_t st) {
+  const long long n = c->n, ld = c->ld;
+  for (int pass = 0; pass < 2; ++pass) {
+    timed(c, st, DMR_PH_GEMVT, 4.0 * n * 2 * sc, [&] {
+      dmr::blk::gemm_tn(U, ld, sc, U, ld, sc, n, c->G, c->part, c->G64, sc, nullptr, st);
+    });
+    timed(c, st, DMR_PH_ORTH, 0.0, [&] {
+      dmr::blk::chol_qr(c->G64, sc, kDepTol, c->Sinv32, c->Sacc, pass == 0,
+                        pass == 1 ? Hdst : nullptr, ldh, c->flags, st);
+    });
+    timed(c, st, DMR_PH_UPDATE, 4.0 * n * 2 * sc, [&] {
+      dmr::blk::gemm_nn(U, ld, sc, c->Sinv32, nullptr, 0, 0.0f, 1.0f, U, ld, sc, n, st);
+    });
+  }
+}
+
+double spmm_bytes(const dmrBlockContext *c, int s) {
+  return (double)c->csr.nnz * 8.0 + 4.0 * (c->n + 1) + 8.0 * (double)c->n * s;
+}
+
+// One restart cycle on the normalised residual block c->R (s columns).  Returns whether the
+// refreshed subspace was written to c->Zn.
+bool cycle(dmrBlockContext *c, int s, bool deflate, hipStream_t st) {
+  const long long n = c->n, ld = c->ld;
+  const int k = c->k, mb = c->mb, G = c->G;
+  const int p0 = deflate ? k : 0;
+  const int rows = p0 + (mb + 1) * s, cols = p0 + mb * s;
+  float *P = c->P;
+  const double vb = 4.0 * (double)n;
+  dmr::fill<double>(c->H, (long long)rows * cols, 0.0, st);
+  dmr::blk::zero_ints(c->flags, 2, st);
+  if (deflate) {
+    timed(c, st, DMR_PH_SPMV, spmm_bytes(c, k),
+          [&] { dmr::blk::spmm32(c->csr, c->Z, ld, P, ld, k, st); });
+    c->st.matvecs += k;
+    cholqr2(c, P, k, c->H, rows, st);
+  }
+  float *V0 = P + (long long)p0 * ld;
+  if (deflate) {
+    timed(c, st, DMR_PH_GEMVT, vb * (k + s), [&] {
+      dmr::blk::gemm_tn(P, ld, k, c->R, ld, s, n, G, c->part, nullptr, 0, c->C32, st);
+    });
+    timed(c, st, DMR_PH_UPDATE, vb * (k + 2 * s), [&] {
+      dmr::blk::gemm_nn(P, ld, k, c->C32, c->R, ld, 1.0f, -1.0f, V0, ld, s, n, st);
+    });
+  } else {
+    hipMemcpyAsync(V0, c->R, sizeof(float) * ld * s, hipMemcpyDeviceToDevice, st);
+  }
+  cholqr2(c, V0, s, nullptr, 0, st);
+  for (int j = 0; j < mb; ++j) {
+    float *Vj = P + (long long)(p0 + j * s) * ld;
+    float *U = P + (long long)(p0 + (j + 1) * s) * ld;
+    const int pj = p0 + (j + 1) * s;
+    double *Hcol = c->H + (long long)(p0 + j * s) * rows;
+    timed(c, st, DMR_PH_SPMV, spmm_bytes(c, s),
+          [&] { dmr::blk::spmm32(c->csr, Vj, ld, U, ld, s, st); });
+    c->st.matvecs += s;
+    timed(c, st, DMR_PH_GEMVT, vb * (pj + s), [&] {
+      dmr::blk::gemm_tn(P, ld, pj, U, ld, s, n, G, c->part, Hcol, rows, c->C32, st);
+    });
+    timed(c, st, DMR_PH_UPDATE, vb * (pj + 2 * s), [&] {
+      dmr::blk::gemm_nn(P, ld, pj, c->C32, U, ld, 1.0f, -1.0f, U, ld, s, n, st);
+    });
+    cholqr2(c, U, s, Hcol + pj, rows, st);
+  }
+  // Full projection of the residual block, least squares, residual and solution updates.
+  timed(c, st, DMR_PH_GEMVT, vb * (rows + s), [&] {
+    dmr::blk::gemm_tn(P, ld, rows, c->R, ld, s, n, G, c->part, c->T, rows, nullptr, st);
+  });
+  timed(c, st, DMR_PH_LSQ, 0.0, [&] {
+    dmr::blk::lsq_block(c->H, rows, cols, s, c->T, s, c->rho, c->Hw, c->Tw, c->Y, c->Yr32,
+                        c->ZH32, st);
+  });
+  timed(c, st, DMR_PH_COMBO, vb * (rows + 2 * s), [&] {
+    dmr::blk::gemm_nn(P, ld, rows, c->ZH32, c->R, ld, 1.0f, -1.0f, c->R, ld, s, n, st);
+  });
+  timed(c, st, DMR_PH_SCALE, vb * s,
+        [&] { dmr::blk::colnorms<float>(c->R, ld, s, n, c->cpart, c->q, 1, st); });
+  timed(c, st, DMR_PH_COMBO, vb * ((deflate ? k : 0) + mb * s + 2 * s * (deflate ? 2 : 1)), [&] {
+    if (deflate)
+      dmr::blk::gemm_nn(c->Z, ld, k, c->Yr32, c->Xa, ld, 1.0f, 1.0f, c->Xa, ld, s, n, st);
+    dmr::blk::gemm_nn(V0, ld, mb * s, c->Yr32 + (long long)p0 * 32, c->Xa, ld, 1.0f, 1.0f,
+                      c->Xa, ld, s, n, st);
+  });
+  // Subspace refresh from the leading block of H over [Z V_0 .. V_{kr-1}].
+  bool refreshed = false;
+  if (k > 0) {
+    // Widest leading block whose SVD fits in LDS (at least k columns).
+    int kr = deflate ? 1 : std::max(1, (k + s - 1) / s);
+    while (kr < std::min(mb, c->krMax) &&
+           dmr::svd_lds(p0 + (kr + 2) * s, p0 + (kr + 1) * s) <= 160 * 1024)
+      ++kr;
+    if (kr <= mb) {
+      const int cR = p0 + kr * s, rR = cR + s;
+      if (cR >= k && dmr::svd_lds(rR, cR) <= 160 * 1024) {
+        hipMemcpy2DAsync(c->Hred, sizeof(double) * rR, c->H, sizeof(double) * rows,
+                         sizeof(double) * rR, cR, hipMemcpyDeviceToDevice, st);
+        timed(c, st, DMR_PH_SVD, 0.0, [&] {
+          dmr::svd_smallest_rect(c->Hred, rR, rR, cR, k, kSvdTol, kSvdMaxSweeps, 1, c->Csvd,
+                                 c->sig, c->flags + 1, st);
+          dmr::blk::pack_coef(c->Csvd, cR, k, c->Cz32, st);
+        });
+        timed(c, st, DMR_PH_ZGEMM, vb * (cR + k + 2 * k), [&] {
+          if (deflate) {
+            dmr::blk::gemm_nn(c->Z, ld, k, c->Cz32, nullptr, 0, 0.0f, 1.0f, c->Zn, ld, k, n, st);
+            dmr::blk::gemm_nn(V0, ld, kr * s, c->Cz32 + (long long)p0 * 32, c->Zn, ld, 1.0f,
+                              1.0f, c->Zn, ld, k, n, st);
+          } else {
+            dmr::blk::gemm_nn(V0, ld, kr * s, c->Cz32, nullptr, 0, 0.0f, 1.0f, c->Zn, ld, k, n,
+                              st);
+          }
+          dmr::blk::colnorms<float>(c->Zn, ld, k, n, c->cpart, c->zq, 1, st);
+          dmr::blk::recip(c->zq, c->zq, k, nullptr, st);
+          dmr::scale_cols<float>(c->Zn, k, ld, n, c->zq, st);
+        });
+        refreshed = true;
+      }
+    }
+  }
+  dmr::blk::track(c->rho, c->q, c->scal, s, st);
+  timed(c, st, DMR_PH_SCALE, vb * 2 * s,
+        [&] { dmr::scale_cols<float>(c->R, s, ld, n, c->scal, st); });
+  return refreshed;
+}
+
+int block_solve(dmrBlockContext *c, int mode, int s, double *X, long long ldx, const double *B,
+                long long ldb, hipStream_t st, double tol, int maxCycles, double *relRes) {
+  const long long n = c->n, ld = c->ld;
+  double *hb = c->host, *hr = c->host + 64, *hrho = c->host + 128, *hf = c->host + 192;
+  if (c->val32Stale) {
+    dmr::convert<float, double>(c->ownedVal32, c->csr.val, c->csr.nnz, st);
+    c->val32Stale = false;
+  }
+  dmr::blk::colnorms<double>(B, ldb, s, n, c->cpart, c->bn, 1, st);
+  BCHK(hipMemcpyAsync(hb, c->bn, sizeof(double) * s, hipMemcpyDeviceToHost, st));
+  bool useZ = mode != 0 && c->hasZ;
+  bool cycled = false;
+  double bmax = 0.0;
+  for (int outer = 0;; ++outer) {
+    timed(c, st, DMR_PH_SPMV, (double)c->csr.nnz * 12.0 + 4.0 * (n + 1) + 24.0 * n * s, [&] {
+      dmr::blk::resid_block(c->csr, X, ldx, B, ldb, c->R64, ld, s, st);
+    });
+    c->st.matvecs += s;
+    dmr::blk::colnorms<double>(c->R64, ld, s, n, c->cpart, c->rn, 1, st);
+    BCHK(hipMemcpyAsync(hr, c->rn, sizeof(double) * s, hipMemcpyDeviceToHost, st));
+    BCHK(hipStreamSynchronize(st));
+    if (c->prof.on) prof_collect(c);
+    bool done = true;
+    double worst = 0.0;
+    for (int j = 0; j < s; ++j) {
+      const double rel = hb[j] > 0.0 ? hr[j] / hb[j] : hr[j];
+      relRes[j] = rel;
+      worst = std::max(worst, rel);
+      bmax = std::max(bmax, hb[j]);
+      if (!(rel <= tol)) done = false;
+    }
+    c->st.beta = bmax;
+    c->st.rho = worst;
+    if (done) return DMR_OK;
+    if (outer >= c->maxOuter) return DMR_ERR_NOCONV;
+    if (!std::isfinite(worst)) return DMR_ERR_RUNAWAY;
+    // Inner tolerances per column, and the scale of each residual column to unit norm.
+    std::vector<double> tolIn(s);
+    for (int j = 0; j < s; ++j) {
+      hf[j] = hr[j] > 0.0 ? 1.0 / hr[j] : 0.0;
+      const double want = hr[j] > 0.0 ? 0.5 * tol * (hb[j] > 0.0 ? hb[j] : 1.0) / hr[j] : 1.0;
+      tolIn[j] = std::min(0.5, std::max(c->innerTol, want));
+    }
+    BCHK(hipMemcpyAsync(c->f, hf, sizeof(double) * s, hipMemcpyHostToDevice, st));
+    dmr::blk::scale_to32(c->R64, ld, c->f, c->R, ld, s, n, st);
+    dmr::fill<float>(c->Xa, ld * s, 0.0, st);
+    dmr::blk::recip(c->f, c->scal, s, c->rho, st);  // rho = 1
+    for (int cyc = 0; cyc < maxCycles; ++cyc) {
+      const bool deflate = c->k > 0 && c->hasZ && (useZ || cycled);
+      const bool refreshed = cycle(c, s, deflate, st);
+      cycled = true;
+      BCHK(hipMemcpyAsync(hrho, c->rho, sizeof(double) * s, hipMemcpyDeviceToHost, st));
+      BCHK(hipMemcpyAsync(hf + 32, c->flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
+      BCHK(hipStreamSynchronize(st));
+      if (c->prof.on) prof_collect(c);
+      const int *fl = reinterpret_cast<const int *>(hf + 32);
+      ++c->st.cycles;
+      if (fl[0]) ++c->st.breakdowns;
+      // Dependent block columns are zero basis vectors: the refresh never selects them, so it is
+      // kept on such cycles too.
+      if (refreshed) {
+        std::swap(c->Z, c->Zn);
+        c->hasZ = true;
+        c->st.svdSweeps += fl[1];
+        ++c->st.svdCalls;
+      }
+      bool innerDone = true;
+      double est = 0.0;
+      for (int j = 0; j < s; ++j) {
+        if (!(hrho[j] <= tolIn[j])) innerDone = false;
+        est = std::max(est, hrho[j] * (hb[j] > 0.0 ? hr[j] / hb[j] : hr[j]));
+      }
+      c->hist.push_back(est);
+      if (!std::isfinite(est)) return DMR_ERR_RUNAWAY;
+      if (innerDone) break;
+    }
+    // X += ||R_j|| D_j.
+    dmr::blk::add_scaled(X, ldx, c->Xa, ld, c->rn, s, n, st);
+    useZ = true;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t dmrBlockFootprint(int elements, int blockSteps, int deflatDim, int maxRhs) {
+  if (elements < 1 || blockSteps < 1 || deflatDim < 0 || maxRhs < 1) return 0;
+  const long long ld = dmr::basis_ld(elements);
+  const int G = dmr::blk::gemm_tn_blocks(elements, dmr::blk::kMaxTnBlocks, tn_rows_target());
+  const Sizes z = sizes(elements, blockSteps, deflatDim, maxRhs, G, ld);
+  return z.big + z.small + 80 * 256;
+}
+
+int dmrBlockCreate(dmrBlockHandle *h, int device, int elements, int blockSteps, int deflatDim,
+                   int maxRhs) {
+  dmr::clear_error();
+  if (!h) return dmr::set_error(DMR_ERR_ARG, "null handle pointer");
+  *h = nullptr;
+  if (elements < 1 || blockSteps < 1 || deflatDim < 0 || deflatDim > kMaxRhs || maxRhs < 1 ||
+      maxRhs > kMaxRhs)
+    return dmr::set_error(DMR_ERR_ARG, "block DMR needs n >= 1, steps >= 1, 0 <= k <= 32, "
+                                       "1 <= maxRhs <= 32");
+  if (deflatDim + (blockSteps + 1) * maxRhs > dmr::blk::kMaxCols)
+    return dmr::set_error(DMR_ERR_ARG, "k + (blockSteps + 1) * maxRhs exceeds 384 basis columns");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+    return dmr::set_error(DMR_ERR_HIP, "no such device");
+  BCHK(hipSetDevice(device));
+  dmrBlockContext *c = new dmrBlockContext();
+  c->device = device;
+  c->n = elements;
+  c->mb = blockSteps;
+  c->k = deflatDim;
+  c->sMax = maxRhs;
+  c->ld = dmr::basis_ld(elements);
+  c->rowsMax = deflatDim + (blockSteps + 1) * maxRhs;
+  c->colsMax = deflatDim + blockSteps * maxRhs;
+  c->G = dmr::blk::gemm_tn_blocks(elements, dmr::blk::kMaxTnBlocks, tn_rows_target());
+  if (const char *e = std::getenv("DMR_BLOCK_KR")) {
+    const int v = std::atoi(e);
+    if (v >= 1) c->krMax = v;
+  }
+  const int rc = ctx_alloc(c);
+  if (rc != DMR_OK) {
+    ctx_release(c);
+    return rc;
+  }
+  *h = c;
+  return DMR_OK;
+}
+
+int dmrBlockDestroy(dmrBlockHandle h) {
+  if (!h) return dmr::set_error(DMR_ERR_ARG, "null handle");
+  ctx_release(h);
+  return DMR_OK;
+}
+
+int dmrBlockSetOperatorCsr(dmrBlockHandle h, int nnz, const int *rowPtr, const int *colInd,
+                           const double *val, const float *val32) {
+  dmr::clear_error();
+  if (!h || !rowPtr || !colInd || !val || nnz < 0)
+    return dmr::set_error(DMR_ERR_ARG, "invalid CSR arguments");
+  BCHK(hipSetDevice(h->device));
+  h->csr.n = h->n;
+  h->csr.nnz = nnz;
+  h->csr.rowPtr = rowPtr;
+  h->csr.colInd = colInd;
+  h->csr.val = val;
+  h->csr.blockRows = nullptr;
+  h->csr.nBlocks = 0;
+  if (!val32) {
+    if (h->ownedVal32) hipFree(h->ownedVal32);
+    h->ownedVal32 = nullptr;
+    BCHK(hipMalloc(&h->ownedVal32, sizeof(float) * std::max(nnz, 1)));
+    h->csr.val32 = h->ownedVal32;
+    h->val32Stale = true;
+  } else {
+    h->csr.val32 = val32;
+    h->val32Stale = false;
+  }
+  h->haveOp = true;
+  return DMR_OK;
+}
+
+int dmrBlockSetOperatorValues(dmrBlockHandle h, const double *val, const float *val32) {
+  dmr::clear_error();
+  if (!h || !h->haveOp || !val) return dmr::set_error(DMR_ERR_STATE, "no CSR operator to update");
+  h->csr.val = val;
+  if (val32) {
+    h->csr.val32 = val32;
+    h->val32Stale = false;
+  } else {
+    if (!h->ownedVal32) {
+      BCHK(hipMalloc(&h->ownedVal32, sizeof(float) * std::max(h->csr.nnz, 1)));
+    }
+    h->csr.val32 = h->ownedVal32;
+    h->val32Stale = true;
+  }
+  return DMR_OK;
+}
+
+int dmrBlockSetRefinement(dmrBlockHandle h, double innerTol, int maxOuter) {
+  if (!h || !(innerTol > 0.0 && innerTol < 1.0) || maxOuter < 1)
+    return dmr::set_error(DMR_ERR_ARG, "invalid refinement settings");
+  h->innerTol = innerTol;
+  h->maxOuter = maxOuter;
+  return DMR_OK;
+}
+
+int dmrBlockSolve(dmrBlockHandle h, int mode, int nrhs, double *X, long long ldx, const double *B,
+                  long long ldb, hipStream_t stream, double solTol, int maxCycles,
+                  double *relRes) {
+  dmr::clear_error();
+  if (!h) return dmr::set_error(DMR_ERR_STATE, "block solver not initialised");
+  if (!h->haveOp) return dmr::set_error(DMR_ERR_STATE, "no operator set");
+  if (nrhs < 1 || nrhs > h->sMax || !X || !B || !relRes || ldx < h->n || ldb < h->n ||
+      maxCycles < 1 || !(solTol >= 0.0))
+    return dmr::set_error(DMR_ERR_ARG, "invalid block solve arguments");
+  BCHK(hipSetDevice(h->device));
+  h->st = dmrStats{};
+  h->hist.clear();
+  const auto t0 = std::chrono::steady_clock::now();
+  const int rc = block_solve(h, mode, nrhs, X, ldx, B, ldb, stream, solTol, maxCycles, relRes);
+  h->st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  h->st.status = rc;
+  if (rc != DMR_OK && dmrGetLastError() == DMR_OK)
+    dmr::set_error(rc, rc == DMR_ERR_NOCONV ? "refinement limit reached" : "residual diverged");
+  return rc;
+}
+
+int dmrBlockGetStats(dmrBlockHandle h, dmrStats *st, double *residHist, int cap) {
+  if (!h) return -1;
+  if (st) *st = h->st;
+  const int cnt = (int)h->hist.size();
+  if (residHist)
+    for (int i = 0; i < std::min(cap, cnt); ++i) residHist[i] = h->hist[i];
+  return cnt;
+}
+
+int dmrBlockHasZ(dmrBlockHandle h) { return h && h->hasZ ? 1 : 0; }
+
+int dmrBlockImportZ(dmrBlockHandle h, const double *hostZ) {
+  dmr::clear_error();
+  if (!h || !hostZ) return dmr::set_error(DMR_ERR_ARG, "null argument");
+  if (h->k == 0) return DMR_OK;
+  BCHK(hipSetDevice(h->device));
+  std::vector<float> tmp((size_t)h->n * h->k);
+  for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = (float)hostZ[i];
+  BCHK(hipMemcpy2D(h->Z, sizeof(float) * h->ld, tmp.data(), sizeof(float) * h->n,
+                   sizeof(float) * h->n, h->k, hipMemcpyHostToDevice));
+  h->hasZ = true;
+  return DMR_OK;
+}
+
+int dmrBlockExportZ(dmrBlockHandle h, double *hostZ) {
+  dmr::clear_error();
+  if (!h || !hostZ) return dmr::set_error(DMR_ERR_ARG, "null argument");
+  if (h->k == 0) return DMR_OK;
+  BCHK(hipSetDevice(h->device));
+  BCHK(hipDeviceSynchronize());
+  std::vector<float> tmp((size_t)h->n * h->k);
+  BCHK(hipMemcpy2D(tmp.data(), sizeof(float) * h->n, h->Z, sizeof(float) * h->ld,
+                   sizeof(float) * h->n, h->k, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < tmp.size(); ++i) hostZ[i] = tmp[i];
+  return DMR_OK;
+}
+
+int dmrBlockSetProfiling(dmrBlockHandle h, int on) {
+  if (!h) return dmr::set_error(DMR_ERR_ARG, "null handle");
+  h->prof.on = on != 0;
+  for (int i = 0; i < DMR_NPHASES; ++i) {
+    h->prof.ms[i] = 0.0;
+    h->prof.by[i] = 0.0;
+    h->prof.cnt[i] = 0;
+  }
+  h->prof.used = 0;
+  return DMR_OK;
+}
+
+int dmrBlockGetProfile(dmrBlockHandle h, double *ms, double *bytes, long long *launches) {
+  if (!h) return dmr::set_error(DMR_ERR_ARG, "null handle");
+  for (int i = 0; i < DMR_NPHASES; ++i) {
+    if (ms) ms[i] = h->prof.ms[i];
+    if (bytes) bytes[i] = h->prof.by[i];
+    if (launches) launches[i] = h->prof.cnt[i];
+  }
+  return DMR_NPHASES;
+}
+
+}  // extern "C"

@@ -1,0 +1,66 @@
+// Internal interface of the block (multi-right-hand-side) DMR kernels (dmr_block.hip) used by the
+// block driver (dmr_block.cpp).  Blocks of vectors are column-major with leading dimension ld.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+
+#include "dmr_internal.h"
+
+namespace dmr {
+namespace blk {
+
+// Widest block: one 32-column MFMA tile.
+constexpr int kMaxRhs = 32;
+// gemm_tn holds up to kMaxTiles 32-column tiles of P (4 per wavefront): p <= 32 * kMaxTiles.
+constexpr int kMaxTiles = 12;
+constexpr int kMaxCols = 32 * kMaxTiles;
+// Cap on the row ranges (workgroups) of gemm_tn.
+constexpr int kMaxTnBlocks = 1024;
+
+// Y[:, 0:s] = A X[:, 0:s] with FP32 values and vectors, FP32 accumulation.
+void spmm32(const CsrDev &A, const float *X, long long ldx, float *Y, long long ldy, int s,
+            hipStream_t st);
+// R = B - A X in FP64 (s columns).
+void resid_block(const CsrDev &A, const double *X, long long ldx, const double *B, long long ldb,
+                 double *R, long long ldr, int s, hipStream_t st);
+// out[c] = ||X[:, c]||^2 (sqrtMode 0) or ||X[:, c]|| (1), FP64 accumulation in a fixed order.
+// partial: colnorm_blocks(n) * s doubles.
+int colnorm_blocks(long long n);
+template <typename T>
+void colnorms(const T *X, long long ld, int s, long long n, double *partial, double *out,
+              int sqrtMode, hipStream_t st);
+// C = P[:, 0:p]^T U[:, 0:s] (p <= kMaxCols, s <= 32) over G row ranges (gemm_tn_blocks), MFMA in
+// FP32, partials summed in order in FP64.  C64 (optional): column-major, ld ldc.  C32 (optional):
+// row-major p x 32, zero beyond s.  part: G * roundup32(p) * 32 floats.
+int gemm_tn_blocks(long long n, int cap, int rowsTarget);
+size_t gemm_tn_lds(int pPad);
+void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu, int s,
+             long long n, int G, float *part, double *C64, int ldc, float *C32, hipStream_t st);
+// Out = a In + b P[:, 0:p] C (C row-major p x 32; In unused when a == 0).  Out may alias In or P.
+void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *In, long long ldi,
+             float a, float b, float *Out, long long ldo, int s, long long n, hipStream_t st);
+// Cholesky QR step on the Gram matrix G (s x s, ld s): see k_chol_qr.
+void chol_qr(const double *G, int s, double relTol, float *Sinv32, double *Sacc, int first,
+             double *Hdst, int ldh, int *brk, hipStream_t st);
+// Block Hessenberg least squares (rows = cols + bw rows, lower bandwidth bw <= 32, ld rows) with
+// s right-hand sides T (rows x s, ld rows): Y (cols x 32 row-major FP64), Yr32 = Y diag(rho),
+// ZH32 = H Y.  Workspaces R (cols x cols) and Qt (cols x 32).  LDS: lsq_block_lds bytes.
+size_t lsq_block_lds(int cols, int bw, int s);
+void lsq_block(const double *H, int rows, int cols, int bw, const double *T, int s,
+               const double *rho, double *R, double *Qt, double *Y, float *Yr32, float *ZH32,
+               hipStream_t st);
+void pack_coef(const double *C, int cols, int k, float *C32, hipStream_t st);
+void track(double *rho, const double *q, double *scal, int s, hipStream_t st);
+void add_scaled(double *X, long long ldx, const float *D, long long ldd, const double *f, int s,
+                long long n, hipStream_t st);
+void scale_to32(const double *R, long long ldr, const double *f, float *D, long long ldd, int s,
+                long long n, hipStream_t st);
+// out[j] = 1 / q[j] (0 where q[j] = 0); rho[j] = 1 when rho is given (s <= 64).
+void recip(const double *q, double *out, int s, double *rho, hipStream_t st);
+// p[0..cnt) = 0 (cnt <= 64), a kernel rather than a memset node.
+void zero_ints(int *p, int cnt, hipStream_t st);
+
+}  // namespace blk
+}  // namespace dmr

@@ -1,0 +1,791 @@
+// Kernels of the block (multi-right-hand-side) DMR: s right-hand sides share one block Krylov
+// space and one deflation subspace.  Blocks of vectors are column-major (element (r, c) at
+// c * ld + r) with the padded leading dimension of the single-vector basis.
+//
+// * spmm / resid_block: CSR times a block of vectors, one thread per row, the row's nonzeros
+//   reused across the block's columns.
+// * gemm_tn: C = P^T U (p x s, reduction over the n rows), the block Gram-Schmidt projection and
+//   the Gram matrix of the Cholesky QR.  Row ranges per workgroup; each 32-row chunk of P and U is
+//   staged through LDS (transposed for the MFMA operand layout) and contracted with
+//   v_mfma_f32_32x32x2f32; per-workgroup partials are summed in a fixed order by gemm_tn_reduce
+//   (deterministic, FP64 accumulation of the partials).
+// * gemm_nn: Out = a In + b P C (n x s), the block update, the basis normalisation U S^-1, the
+//   residual and solution updates and the subspace refresh.  A wave owns 128 rows as four
+//   interleaved 32-row MFMA tiles, so every P load is a 16-byte vector per lane and a
+//   half-wavefront reads 512 contiguous bytes of one column.
+// * chol_qr: Cholesky factor of the s x s Gram matrix and its inverse (one wavefront, FP64).
+// * lsq_block: least squares on the block Hessenberg (lower bandwidth s) by Householder QR with
+//   reflectors of length <= s + 1, back substitution, Z_H = H Y (one workgroup, FP64).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+
+#include "dmr_block.h"
+
+namespace dmr {
+namespace blk {
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kThreads = 256;
+constexpr int kRC = 32;        // rows per LDS chunk of gemm_tn
+constexpr int kLdsStride = kRC + 1;
+
+__device__ inline f32x16 mfma32(float a, float b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+
+// Row i of the 32 x 32 MFMA result held in accumulator v by lane l (column l % 32).
+__device__ inline int mfma_row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+
+// ---------------------------------------------------------------------------------------------
+template <typename TV, typename TX, typename TY, typename TA>
+__global__ __launch_bounds__(kThreads) void k_spmm(int n, const int *__restrict__ rowPtr,
+                                                   const int *__restrict__ colInd,
+                                                   const TV *__restrict__ val,
+                                                   const TX *__restrict__ X, long long ldx,
+                                                   TY *__restrict__ Y, long long ldy, int s) {
+  const int row = blockIdx.x * kThreads + threadIdx.x;
+  if (row >= n) return;
+  const int b = rowPtr[row], e = rowPtr[row + 1];
+  for (int c0 = 0; c0 < s; c0 += 8) {
+    TA acc[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = (TA)0;
+    for (int q = b; q < e; ++q) {
+      const long long col = colInd[q];
+      const TA v = (TA)val[q];
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        if (c0 + c < s) acc[c] = fma(v, (TA)X[col + (c0 + c) * ldx], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (c0 + c < s) Y[row + (c0 + c) * ldy] = (TY)acc[c];
+  }
+}
+
+// R = B - A X in FP64.
+__global__ __launch_bounds__(kThreads) void k_resid_block(int n, const int *__restrict__ rowPtr,
+                                                          const int *__restrict__ colInd,
+                                                          const double *__restrict__ val,
+                                                          const double *__restrict__ X,
+                                                          long long ldx,
+                                                          const double *__restrict__ B,
+                                                          long long ldb, double *__restrict__ R,
+                                                          long long ldr, int s) {
+  const int row = blockIdx.x * kThreads + threadIdx.x;
+  if (row >= n) return;
+  const int b = rowPtr[row], e = rowPtr[row + 1];
+  for (int c0 = 0; c0 < s; c0 += 8) {
+    double acc[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = 0.0;
+    for (int q = b; q < e; ++q) {
+      const long long col = colInd[q];
+      const double v = val[q];
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        if (c0 + c < s) acc[c] = fma(v, X[col + (c0 + c) * ldx], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (c0 + c < s) R[row + (c0 + c) * ldr] = B[row + (c0 + c) * ldb] - acc[c];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Column sums of squares, fixed order: block g sums rows [g*rows, (g+1)*rows) of every column
+// (each thread in row order, then the block's threads in a fixed tree), partial[g * s + c].
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_colsumsq(const T *__restrict__ X, long long ld,
+                                                       int s, long long n, long long rows,
+                                                       double *__restrict__ partial) {
+  __shared__ double red[kThreads];
+  const long long r0 = (long long)blockIdx.x * rows;
+  const long long r1 = r0 + rows < n ? r0 + rows : n;
+  for (int c = 0; c < s; ++c) {
+    double acc = 0.0;
+    const T *col = X + c * ld;
+    for (long long r = r0 + threadIdx.x; r < r1; r += kThreads) {
+      const double x = (double)col[r];
+      acc = fma(x, x, acc);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(long long)blockIdx.x * s + c] = red[0];
+    __syncthreads();
+  }
+}
+
+// out[c] = sum_g partial[g * s + c] (in order); mode 0: out = ss; mode 1: out = sqrt(ss).
+__global__ void k_colsumsq_final(const double *__restrict__ partial, int G, int s,
+                                 double *__restrict__ out, int mode) {
+  const int c = threadIdx.x;
+  if (c >= s) return;
+  double a = 0.0;
+  for (int g = 0; g < G; ++g) a = a + partial[(long long)g * s + c];
+  out[c] = mode ? sqrt(a) : a;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Eight 16-byte loads of rows [rr, rr + 32) of one column (zeros past rEnd or for a null column).
+__device__ inline void load_seg(const float *col, long long rr, long long rEnd, float4 *out) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const long long r = rr + 4 * q;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (col) {
+      if (r + 3 < rEnd) {
+        v = *reinterpret_cast<const float4 *>(col + r);
+      } else {
+        if (r < rEnd) v.x = col[r];
+        if (r + 1 < rEnd) v.y = col[r + 1];
+        if (r + 2 < rEnd) v.z = col[r + 2];
+      }
+    }
+    out[q] = v;
+  }
+}
+
+// Partial C_g = P[r in range g, 0:p]^T U[r in range g, 0:s], as pPad x 32 FP32 per workgroup.
+// Register-direct: no LDS staging.  The pPad / 32 tiles form nTG = ceil(tiles / kTPW) groups of
+// at most kTPW tiles; the four wavefronts are nTG tile groups x nRG = 4 / nTG row groups.  A
+// wavefront walks its row group's 64-row rounds; in a round lane (i, h) streams rows
+// [r0 + 32h, r0 + 32h + 32) of column i of its tiles and of U as eight 16-byte loads, and MFMA
+// k-step 4q + e takes row r0 + 32h + 4q + e of both operands (any row order is a valid reduction
+// order).  The row groups' sums are added through LDS in a fixed order.
+constexpr int kTPW = 3;
+__global__ __launch_bounds__(kThreads) void k_gemm_tn(const float *__restrict__ P, long long ldp,
+                                                      int p, int pPad,
+                                                      const float *__restrict__ U, long long ldu,
+                                                      int s, long long n, long long rowsPerBlock,
+                                                      float *__restrict__ part) {
+  __shared__ float red[4 * kTPW * 1024];
+  const int tiles = pPad / 32;
+  const int nTG = (tiles + kTPW - 1) / kTPW;
+  const int nRG = 4 / nTG;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tg = wave % nTG, rg = wave / nTG;
+  const int i = lane & 31, h = lane >> 5;
+  const long long rBeg = (long long)blockIdx.x * rowsPerBlock;
+  const long long rEnd = rBeg + rowsPerBlock < n ? rBeg + rowsPerBlock : n;
+  f32x16 acc[kTPW];
+#pragma unroll
+  for (int t = 0; t < kTPW; ++t)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[t][v] = 0.0f;
+  const float *pc[kTPW];
+#pragma unroll
+  for (int t = 0; t < kTPW; ++t) {
+    const int tile = tg * kTPW + t, c = tile * 32 + i;
+    pc[t] = (tile < tiles && c < p) ? P + c * ldp : nullptr;
+  }
+  const float *uc = i < s ? U + i * ldu : nullptr;
+  if (rg < nRG) {
+    for (long long r0 = rBeg + 64 * rg; r0 < rEnd; r0 += 64 * nRG) {
+      const long long rr = r0 + 32 * h;
+      float4 ub[8], pb[kTPW][8];
+      load_seg(uc, rr, rEnd, ub);
+#pragma unroll
+      for (int t = 0; t < kTPW; ++t)
+        if (tg * kTPW + t < tiles) load_seg(pc[t], rr, rEnd, pb[t]);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const float bq[4] = {ub[q].x, ub[q].y, ub[q].z, ub[q].w};
+#pragma unroll
+        for (int t = 0; t < kTPW; ++t) {
+          if (tg * kTPW + t < tiles) {
+            const float aq[4] = {pb[t][q].x, pb[t][q].y, pb[t][q].z, pb[t][q].w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[t] = mfma32(aq[e], bq[e], acc[t]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < kTPW; ++t)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) red[(wave * kTPW + t) * 1024 + v * 64 + lane] = acc[t][v];
+  }
+  __syncthreads();
+  if (rg != 0) return;
+  float *out = part + (long long)blockIdx.x * pPad * 32;
+#pragma unroll
+  for (int t = 0; t < kTPW; ++t) {
+    const int tile = tg * kTPW + t;
+    if (tile < tiles) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        float a = 0.0f;
+        for (int g = 0; g < nRG; ++g) a = a + red[((g * nTG + tg) * kTPW + t) * 1024 + v * 64 + lane];
+        out[(tile * 32 + mfma_row(v, lane)) * 32 + i] = a;
+      }
+    }
+  }
+}
+
+// LDS-staged variant (DMR_BLOCK_TN_LDS=1): each 32-row chunk of P and U is transposed through LDS.
+__global__ __launch_bounds__(kThreads) void k_gemm_tn_lds(const float *__restrict__ P, long long ldp,
+                                                      int p, int pPad,
+                                                      const float *__restrict__ U, long long ldu,
+                                                      int s, long long n, long long rowsPerBlock,
+                                                      float *__restrict__ part) {
+  extern __shared__ float lds[];
+  float *Ps = lds;                               // pPad columns x kLdsStride
+  float *Us = lds + (long long)pPad * kLdsStride;  // 32 columns x kLdsStride
+  const int tiles = pPad / 32;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long rBeg = (long long)blockIdx.x * rowsPerBlock;
+  const long long rEnd = rBeg + rowsPerBlock < n ? rBeg + rowsPerBlock : n;
+  f32x16 acc[kMaxTiles / 4];
+#pragma unroll
+  for (int t = 0; t < kMaxTiles / 4; ++t)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[t][v] = 0.0f;
+  const int ncol = pPad + 32;
+  for (long long r0 = rBeg; r0 < rEnd; r0 += kRC) {
+    for (int idx = threadIdx.x; idx < ncol * (kRC / 4); idx += kThreads) {
+      const int col = idx / (kRC / 4), rq = (idx % (kRC / 4)) * 4;
+      const long long r = r0 + rq;
+      const bool isU = col >= pPad;
+      const int c = isU ? col - pPad : col;
+      float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f;
+      if (isU ? c < s : c < p) {
+        const float *src = (isU ? U + c * ldu : P + c * ldp) + r;
+        if (r + 3 < rEnd) {
+          const float4 v = *reinterpret_cast<const float4 *>(src);
+          v0 = v.x;
+          v1 = v.y;
+          v2 = v.z;
+          v3 = v.w;
+        } else {
+          if (r < rEnd) v0 = src[0];
+          if (r + 1 < rEnd) v1 = src[1];
+          if (r + 2 < rEnd) v2 = src[2];
+        }
+      }
+      float *dst = (isU ? Us + c * kLdsStride : Ps + c * kLdsStride) + rq;
+      dst[0] = v0;
+      dst[1] = v1;
+      dst[2] = v2;
+      dst[3] = v3;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < kMaxTiles / 4; ++t) {
+      const int tile = wave + 4 * t;
+      if (tile < tiles) {
+        const float *pa = Ps + (tile * 32 + (lane & 31)) * kLdsStride + (lane >> 5);
+        const float *pb = Us + (lane & 31) * kLdsStride + (lane >> 5);
+#pragma unroll 4
+        for (int k0 = 0; k0 < kRC; k0 += 2) acc[t] = mfma32(pa[k0], pb[k0], acc[t]);
+      }
+    }
+    __syncthreads();
+  }
+  float *out = part + (long long)blockIdx.x * pPad * 32;
+#pragma unroll
+  for (int t = 0; t < kMaxTiles / 4; ++t) {
+    const int tile = wave + 4 * t;
+    if (tile < tiles) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v)
+        out[(tile * 32 + mfma_row(v, lane)) * 32 + (lane & 31)] = acc[t][v];
+    }
+  }
+}
+
+// C[i][j] = sum_g part_g[i][j] (FP64), for i < p, j < s: to C64 (column-major, element (i, j) at
+// j * ldc + i) and to C32 (row-major p x 32, zero beyond s).  One block per row i: thread
+// (j, q) = (t % 32, t / 32) sums partials g = q, q + 8, ... in order, then the eight sums are
+// added in order of q (a fixed order, independent of timing).
+__global__ __launch_bounds__(kThreads) void k_gemm_tn_reduce(const float *__restrict__ part,
+                                                             int G, int pPad, int p, int s,
+                                                             double *__restrict__ C64, int ldc,
+                                                             float *__restrict__ C32) {
+  __shared__ double red[kThreads];
+  const int i = blockIdx.x, t = threadIdx.x, j = t & 31, q = t >> 5;
+  double a = 0.0;
+  const float *src = part + (long long)i * 32 + j;
+  const long long stride = (long long)pPad * 32;
+#pragma unroll 8
+  for (int g = q; g < G; g += 8) a = a + (double)src[g * stride];
+  red[t] = a;
+  __syncthreads();
+  if (q != 0) return;
+  double sum = 0.0;
+#pragma unroll
+  for (int w = 0; w < 8; ++w) sum = sum + red[w * 32 + j];
+  if (j >= s) sum = 0.0;
+  if (C64 && j < s) C64[(long long)j * ldc + i] = sum;
+  if (C32) C32[i * 32 + j] = (float)sum;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Out[r, j] = a In[r, j] + b sum_k P[r, k] C[k][j] for j < s, r < n.  C: p x 32 row-major.
+// Out may alias In or P (each row depends on its own row only).
+__global__ __launch_bounds__(kThreads) void k_gemm_nn(const float *P, long long ldp, int p,
+                                                      const float *__restrict__ C,
+                                                      const float *In, long long ldi, float a,
+                                                      float b, float *Out, long long ldo, int s,
+                                                      long long n) {
+  extern __shared__ float Cs[];
+  const int pEven = (p + 1) & ~1;
+  for (int idx = threadIdx.x; idx < pEven * 32; idx += kThreads)
+    Cs[idx] = (idx >> 5) < p ? C[idx] : 0.0f;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long r0 = ((long long)blockIdx.x * 4 + wave) * 128;
+  if (r0 >= n) return;
+  f32x16 acc0, acc1, acc2, acc3;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) acc0[v] = acc1[v] = acc2[v] = acc3[v] = 0.0f;
+  const long long rl = r0 + 4 * (lane & 31);
+  const int kh = lane >> 5;
+  for (int c = 0; c < p; c += 2) {
+    const int cc = c + kh;
+    float x0 = 0.f, x1 = 0.f, x2 = 0.f, x3 = 0.f;
+    if (cc < p) {
+      const float *src = P + cc * ldp + rl;
+      if (rl + 3 < n) {
+        const float4 v = *reinterpret_cast<const float4 *>(src);
+        x0 = v.x;
+        x1 = v.y;
+        x2 = v.z;
+        x3 = v.w;
+      } else {
+        if (rl < n) x0 = src[0];
+        if (rl + 1 < n) x1 = src[1];
+        if (rl + 2 < n) x2 = src[2];
+      }
+    }
+    const float bb = Cs[cc * 32 + (lane & 31)];
+    acc0 = mfma32(x0, bb, acc0);
+    acc1 = mfma32(x1, bb, acc1);
+    acc2 = mfma32(x2, bb, acc2);
+    acc3 = mfma32(x3, bb, acc3);
+  }
+  const int j = lane & 31;
+  if (j >= s) return;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const long long row = r0 + 4 * mfma_row(v, lane);
+    const float o[4] = {acc0[v], acc1[v], acc2[v], acc3[v]};
+    if (row + 3 < n) {
+      float4 w;
+      if (a != 0.0f) {
+        const float4 in = *reinterpret_cast<const float4 *>(In + j * ldi + row);
+        w.x = a * in.x + b * o[0];
+        w.y = a * in.y + b * o[1];
+        w.z = a * in.z + b * o[2];
+        w.w = a * in.w + b * o[3];
+      } else {
+        w.x = b * o[0];
+        w.y = b * o[1];
+        w.z = b * o[2];
+        w.w = b * o[3];
+      }
+      *reinterpret_cast<float4 *>(Out + j * ldo + row) = w;
+    } else {
+      for (int q = 0; q < 4; ++q)
+        if (row + q < n) {
+          const float in = a != 0.0f ? In[j * ldi + row + q] : 0.0f;
+          Out[j * ldo + row + q] = a * in + b * o[q];
+        }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Cholesky QR factor of the Gram matrix G (s x s, column-major ld s, FP64): G = S^T S with S
+// upper triangular.  A pivot below relTol * max diag marks the column dependent: its row of S
+// and its column of S^-1 are zero (the normalised block gets a zero column) and *brk is set.
+// Writes Sinv (FP32 row-major s x 32 for gemm_nn) and the combined factor Sacc <- S * Sacc
+// (Sacc upper triangular s x s, ld s; S itself on the first pass), optionally copied into the
+// block Hessenberg at Hdst (ld ldh).  1024 threads, thread (i, j) = (t % 32, t / 32) owns element
+// (i, j) in every phase.
+__global__ __launch_bounds__(1024) void k_chol_qr(const double *__restrict__ Gin, int s,
+                                                  double relTol, float *__restrict__ Sinv32,
+                                                  double *__restrict__ Sacc, int first,
+                                                  double *__restrict__ Hdst, int ldh,
+                                                  int *__restrict__ brk) {
+  __shared__ double G[32 * 33];   // element (i, j) at j * 33 + i; becomes S (upper)
+  __shared__ double X[32 * 33];   // S^-1
+  __shared__ double A2[32 * 33];  // previous Sacc
+  __shared__ int dep[32];
+  __shared__ double s_dmax;
+  const int t = threadIdx.x, i = t & 31, j = t >> 5;
+  const bool in = i < s && j < s;
+  G[j * 33 + i] = in ? Gin[j * s + i] : 0.0;
+  X[j * 33 + i] = 0.0;
+  if (!first) A2[j * 33 + i] = in ? Sacc[j * s + i] : 0.0;
+  __syncthreads();
+  if (t == 0) {
+    double d = 0.0;
+    for (int c = 0; c < s; ++c) d = fmax(d, G[c * 33 + c]);
+    s_dmax = d;
+  }
+  __syncthreads();
+  const double dmax = s_dmax;
+  for (int k = 0; k < s; ++k) {
+    const double d = G[k * 33 + k];
+    const bool bad = !(d > relTol * dmax);
+    const double skk = bad ? 0.0 : sqrt(d);
+    __syncthreads();
+    if (i == k && j >= k && j < s) G[j * 33 + k] = j == k ? skk : (bad ? 0.0 : G[j * 33 + k] / skk);
+    if (t == 0) dep[k] = bad ? 1 : 0;
+    __syncthreads();
+    if (i > k && j >= i && j < s) G[j * 33 + i] = G[j * 33 + i] - G[i * 33 + k] * G[j * 33 + k];
+    __syncthreads();
+  }
+  // S^-1 by rows from the bottom: X(r, j) = -(sum_{l = r+1..j} S(r, l) X(l, j)) / S(r, r).
+  for (int r = s - 1; r >= 0; --r) {
+    if (i == r && j >= r && j < s && !dep[r] && !dep[j]) {
+      if (j == r) {
+        X[j * 33 + r] = 1.0 / G[r * 33 + r];
+      } else {
+        double a = 0.0;
+        for (int l = r + 1; l <= j; ++l) a = fma(G[l * 33 + r], X[j * 33 + l], a);
+        X[j * 33 + r] = -a / G[r * 33 + r];
+      }
+    }
+    __syncthreads();
+  }
+  Sinv32[i * 32 + j] = in ? (float)X[j * 33 + i] : 0.0f;
+  if (in) {
+    double a = 0.0;
+    if (first) {
+      a = i <= j ? G[j * 33 + i] : 0.0;
+    } else {
+      for (int l = i; l <= j; ++l) a = fma(G[l * 33 + i], A2[j * 33 + l], a);
+    }
+    Sacc[j * s + i] = a;
+    if (Hdst) Hdst[(long long)j * ldh + i] = a;
+  }
+  if (t == 0) {
+    int any = 0;
+    for (int c = 0; c < s; ++c) any |= dep[c];
+    if (any) atomicOr(brk, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Least squares min ||T - H Y|| (per column) on the block Hessenberg H (rows x cols, ld rows,
+// lower bandwidth bw <= 32, rows = cols + bw): Householder QR with reflectors over rows c..c+bw.
+// Only rows c..c+bw change at step c, so they live in an LDS window of bw + 1 rows of [H | T]
+// (a ring: row r in slot r % (bw + 1)); after step c row c is final (written to R, column-major
+// ld cols, and Qt = Q^T T) and its slot takes row c + bw + 1 from H, which no reflector has
+// touched yet.  Then back substitution, Yr32 = Y diag(rho) (FP32 row-major cols x 32) and
+// ZH32 = H Y (FP32 row-major rows x 32).  One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H, int rows,
+                                                    int cols, int bw,
+                                                    const double *__restrict__ T, int s,
+                                                    const double *__restrict__ rho,
+                                                    double *__restrict__ R,
+                                                    double *__restrict__ Qt,
+                                                    double *__restrict__ Y,
+                                                    float *__restrict__ Yr32,
+                                                    float *__restrict__ ZH32) {
+  extern __shared__ double sm[];
+  __shared__ double s_tau, s_rmax;
+  __shared__ double yrow[32];
+  const int tid = threadIdx.x, bd = blockDim.x;
+  const int W = bw + 1, nc = cols + s;
+  double *win = sm;  // W x nc
+  for (int e = tid; e < W * nc; e += bd) {
+    const int r = e / nc, col = e % nc;
+    win[e] = r >= rows ? 0.0
+                       : (col < cols ? H[(long long)col * rows + r]
+                                     : T[(long long)(col - cols) * rows + r]);
+  }
+  __syncthreads();
+  for (int c = 0; c < cols; ++c) {
+    const int L = (c + bw < rows - 1 ? c + bw : rows - 1) - c;  // reflector tail length
+    double *rowc = win + (c % W) * nc;
+    if (tid < 64) {
+      double x = 0.0;
+      if (tid >= 1 && tid <= L) x = win[((c + tid) % W) * nc + c];
+      double xn2 = x * x;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) xn2 = xn2 + __shfl_xor(xn2, off, 64);
+      const double alpha = rowc[c];
+      double tau = 0.0, sc = 0.0, beta = alpha;
+      if (xn2 != 0.0) {
+        beta = -copysign(sqrt(fma(alpha, alpha, xn2)), alpha);
+        tau = (beta - alpha) / beta;
+        sc = 1.0 / (alpha - beta);
+      }
+      if (xn2 != 0.0 && tid >= 1 && tid <= L) win[((c + tid) % W) * nc + c] = x * sc;
+      if (tid == 0) {
+        rowc[c] = beta;
+        s_tau = tau;
+      }
+    }
+    __syncthreads();
+    const double tau = s_tau;
+    if (tau != 0.0) {
+      for (int col = c + 1 + tid; col < nc; col += bd) {
+        double w = rowc[col];
+        for (int l = 1; l <= L; ++l) {
+          const double *rl = win + ((c + l) % W) * nc;
+          w = fma(rl[c], rl[col], w);
+        }
+        const double tw = tau * w;
+        rowc[col] = rowc[col] - tw;
+        for (int l = 1; l <= L; ++l) {
+          double *rl = win + ((c + l) % W) * nc;
+          rl[col] = fma(-tw, rl[c], rl[col]);
+        }
+      }
+    }
+    __syncthreads();
+    const int rn = c + W;
+    for (int col = tid; col < nc; col += bd) {
+      const double v = rowc[col];
+      if (col >= c && col < cols) R[(long long)col * cols + c] = v;
+      if (col >= cols) Qt[(long long)c * 32 + (col - cols)] = v;
+      rowc[col] = rn >= rows ? 0.0
+                             : (col < cols ? H[(long long)col * rows + rn]
+                                           : T[(long long)(col - cols) * rows + rn]);
+    }
+    __syncthreads();
+  }
+  // Back substitution R Y = Qt on LDS copies of Qt; a negligible pivot leaves its unknown at 0.
+  double *Tl = sm;  // cols x 32
+  for (int e = tid; e < cols * 32; e += bd) Tl[e] = (e & 31) < s ? Qt[e] : 0.0;
+  if (tid < 64) {
+    double m = 0.0;
+    for (int c = tid; c < cols; c += 64) m = fmax(m, fabs(R[(long long)c * cols + c]));
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if (tid == 0) s_rmax = m;
+  }
+  __syncthreads();
+  const double rmax = s_rmax;
+  for (int i = cols - 1; i >= 0; --i) {
+    const double *Ri = R + (long long)i * cols;  // column i of R
+    if (tid < 32) {
+      const double rii = Ri[i];
+      const double yv = (tid < s && fabs(rii) > 1e-14 * rmax) ? Tl[i * 32 + tid] / rii : 0.0;
+      yrow[tid] = yv;
+      Y[(long long)i * 32 + tid] = yv;
+    }
+    __syncthreads();
+    for (int e = tid; e < i * 32; e += bd) {
+      const int r = e >> 5, jr = e & 31;
+      Tl[e] = fma(-Ri[r], yrow[jr], Tl[e]);
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < cols * 32; e += bd) {
+    const int i = e >> 5, j = e & 31;
+    Yr32[e] = j < s ? (float)(Y[e] * rho[j]) : 0.0f;
+  }
+  // ZH = H Y with Y staged in LDS.
+  double *Yl = sm;
+  for (int e = tid; e < cols * 32; e += bd) Yl[e] = Y[e];
+  __syncthreads();
+  for (int e = tid; e < rows * 32; e += bd) {
+    const int r = e >> 5, j = e & 31;
+    double z = 0.0;
+    if (j < s)
+      for (int c = 0; c < cols; ++c) z = fma(H[(long long)c * rows + r], Yl[c * 32 + j], z);
+    ZH32[e] = (float)z;
+  }
+}
+
+// Converts the k selected right singular vectors C (cols x k, ld cols, FP64) to FP32 row-major
+// cols x 32.
+__global__ void k_pack_coef(const double *__restrict__ C, int cols, int k, float *__restrict__ C32) {
+  const int e = blockIdx.x * kThreads + threadIdx.x;
+  if (e >= cols * 32) return;
+  const int i = e >> 5, j = e & 31;
+  C32[e] = j < k ? (float)C[(long long)j * cols + i] : 0.0f;
+}
+
+// rho[j] *= q[j]; scal[j] = 1 / q[j] (0 for q = 0): residual tracking per column.
+__global__ void k_track(double *__restrict__ rho, const double *__restrict__ q,
+                        double *__restrict__ scal, int s) {
+  const int j = threadIdx.x;
+  if (j >= s) return;
+  rho[j] = rho[j] * q[j];
+  scal[j] = q[j] > 0.0 ? 1.0 / q[j] : 0.0;
+}
+
+// x[:, j] += f[j] * d[:, j]  (FP64 x, FP32 d).
+__global__ __launch_bounds__(kThreads) void k_add_scaled(double *__restrict__ X, long long ldx,
+                                                         const float *__restrict__ D,
+                                                         long long ldd,
+                                                         const double *__restrict__ f, int s,
+                                                         long long n) {
+  const long long r = (long long)blockIdx.x * kThreads + threadIdx.x;
+  const int j = blockIdx.y;
+  if (r >= n || j >= s) return;
+  X[j * ldx + r] = fma(f[j], (double)D[j * ldd + r], X[j * ldx + r]);
+}
+
+// D[:, j] = (FP32) (R[:, j] * f[j])
+__global__ __launch_bounds__(kThreads) void k_scale_to32(const double *__restrict__ R,
+                                                         long long ldr, const double *__restrict__ f,
+                                                         float *__restrict__ D, long long ldd,
+                                                         int s, long long n) {
+  const long long r = (long long)blockIdx.x * kThreads + threadIdx.x;
+  const int j = blockIdx.y;
+  if (r >= n || j >= s) return;
+  D[j * ldd + r] = (float)(R[j * ldr + r] * f[j]);
+}
+
+// out[j] = 1 / q[j] (0 for q = 0), and optionally rho[j] = 1: per-column scale factors.
+__global__ void k_recip(const double *__restrict__ q, double *__restrict__ out, int s,
+                        double *__restrict__ rho) {
+  const int j = threadIdx.x;
+  if (j >= s) return;
+  out[j] = q[j] > 0.0 ? 1.0 / q[j] : 0.0;
+  if (rho) rho[j] = 1.0;
+}
+
+__global__ void k_zero_ints(int *__restrict__ p, int cnt) {
+  const int i = threadIdx.x;
+  if (i < cnt) p[i] = 0;
+}
+
+inline unsigned grid1(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+}  // namespace
+
+void recip(const double *q, double *out, int s, double *rho, hipStream_t st) {
+  hipLaunchKernelGGL(k_recip, dim3(1), dim3(64), 0, st, q, out, s, rho);
+}
+
+void zero_ints(int *p, int cnt, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, p, cnt);
+}
+
+void spmm32(const CsrDev &A, const float *X, long long ldx, float *Y, long long ldy, int s,
+            hipStream_t st) {
+  hipLaunchKernelGGL((k_spmm<float, float, float, float>), dim3(grid1(A.n)), dim3(kThreads), 0, st,
+                     A.n, A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
+}
+
+void resid_block(const CsrDev &A, const double *X, long long ldx, const double *B, long long ldb,
+                 double *R, long long ldr, int s, hipStream_t st) {
+  hipLaunchKernelGGL(k_resid_block, dim3(grid1(A.n)), dim3(kThreads), 0, st, A.n, A.rowPtr,
+                     A.colInd, A.val, X, ldx, B, ldb, R, ldr, s);
+}
+
+int colnorm_blocks(long long n) {
+  long long g = (n + 4095) / 4096;
+  if (g < 1) g = 1;
+  if (g > 256) g = 256;
+  return (int)g;
+}
+
+template <typename T>
+void colnorms(const T *X, long long ld, int s, long long n, double *partial, double *out,
+              int sqrtMode, hipStream_t st) {
+  const int G = colnorm_blocks(n);
+  const long long rows = (n + G - 1) / G;
+  hipLaunchKernelGGL((k_colsumsq<T>), dim3(G), dim3(kThreads), 0, st, X, ld, s, n, rows,
+                     partial);
+  hipLaunchKernelGGL(k_colsumsq_final, dim3(1), dim3(64), 0, st, partial, G, s, out, sqrtMode);
+}
+template void colnorms<float>(const float *, long long, int, long long, double *, double *, int,
+                              hipStream_t);
+template void colnorms<double>(const double *, long long, int, long long, double *, double *, int,
+                               hipStream_t);
+
+int gemm_tn_blocks(long long n, int cap, int rowsTarget) {
+  long long g = (n + rowsTarget - 1) / rowsTarget;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+size_t gemm_tn_lds(int pPad) { return sizeof(float) * (size_t)(pPad + 32) * kLdsStride; }
+
+static bool tn_lds_variant() {
+  static const bool v = [] {
+    const char *e = std::getenv("DMR_BLOCK_TN_LDS");
+    return e && e[0] == '1';
+  }();
+  return v;
+}
+
+void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu, int s,
+             long long n, int G, float *part, double *C64, int ldc, float *C32, hipStream_t st) {
+  const int pPad = (p + 31) & ~31;
+  long long rows = (n + G - 1) / G;
+  rows = (rows + 63) / 64 * 64;
+  if (tn_lds_variant()) {
+    const size_t lds = gemm_tn_lds(pPad);
+    hipLaunchKernelGGL(k_gemm_tn_lds, dim3(G), dim3(kThreads), lds, st, P, ldp, p, pPad, U, ldu,
+                       s, n, rows, part);
+  } else {
+    hipLaunchKernelGGL(k_gemm_tn, dim3(G), dim3(kThreads), 0, st, P, ldp, p, pPad, U, ldu, s, n,
+                       rows, part);
+  }
+  hipLaunchKernelGGL(k_gemm_tn_reduce, dim3(p), dim3(kThreads), 0, st, part, G, pPad, p, s, C64,
+                     ldc, C32);
+}
+
+void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *In, long long ldi,
+             float a, float b, float *Out, long long ldo, int s, long long n, hipStream_t st) {
+  const long long waves = (n + 127) / 128;
+  const unsigned grid = (unsigned)((waves + 3) / 4);
+  const size_t lds = sizeof(float) * (size_t)(((p + 1) & ~1) * 32);
+  hipLaunchKernelGGL(k_gemm_nn, dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C, In, ldi, a, b,
+                     Out, ldo, s, n);
+}
+
+void chol_qr(const double *G, int s, double relTol, float *Sinv32, double *Sacc, int first,
+             double *Hdst, int ldh, int *brk, hipStream_t st) {
+  hipLaunchKernelGGL(k_chol_qr, dim3(1), dim3(1024), 0, st, G, s, relTol, Sinv32, Sacc, first,
+                     Hdst, ldh, brk);
+}
+
+size_t lsq_block_lds(int cols, int bw, int s) {
+  const size_t win = (size_t)(bw + 1) * (cols + s), back = (size_t)cols * 32;
+  return sizeof(double) * (win > back ? win : back);
+}
+
+void lsq_block(const double *H, int rows, int cols, int bw, const double *T, int s,
+               const double *rho, double *R, double *Qt, double *Y, float *Yr32, float *ZH32,
+               hipStream_t st) {
+  const size_t lds = lsq_block_lds(cols, bw, s);
+  if (lds > 64 * 1024)
+    hipFuncSetAttribute(reinterpret_cast<const void *>(k_lsq_block),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_lsq_block, dim3(1), dim3(1024), lds, st, H, rows, cols, bw, T, s, rho, R,
+                     Qt, Y, Yr32, ZH32);
+}
+
+void pack_coef(const double *C, int cols, int k, float *C32, hipStream_t st) {
+  hipLaunchKernelGGL(k_pack_coef, dim3(grid1((long long)cols * 32)), dim3(kThreads), 0, st, C,
+                     cols, k, C32);
+}
+
+void track(double *rho, const double *q, double *scal, int s, hipStream_t st) {
+  hipLaunchKernelGGL(k_track, dim3(1), dim3(64), 0, st, rho, q, scal, s);
+}
+
+void add_scaled(double *X, long long ldx, const float *D, long long ldd, const double *f, int s,
+                long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_add_scaled, dim3(grid1(n), s), dim3(kThreads), 0, st, X, ldx, D, ldd, f,
+                     s, n);
+}
+
+void scale_to32(const double *R, long long ldr, const double *f, float *D, long long ldd, int s,
+                long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale_to32, dim3(grid1(n), s), dim3(kThreads), 0, st, R, ldr, f, D, ldd,
+                     s, n);
+}
+
+}  // namespace blk
+}  // namespace dmr

@@ -100,14 +100,17 @@ __global__ __launch_bounds__(kLsqThreads) void k_lsq(const double *__restrict__ 
   }
 }
 
-__global__ __launch_bounds__(kSvdThreads) void k_svd(const double *__restrict__ H, int m, int k,
-                                                     double tol, int maxSweeps,
+// H: rows x m (column-major, leading dimension ldH); the single-vector Hessenberg has
+// rows = ldH = m + 1.
+__global__ __launch_bounds__(kSvdThreads) void k_svd(const double *__restrict__ H, int rows,
+                                                     int ldH, int m, int k, double tol,
+                                                     int maxSweeps, int zeroLast,
                                                      double *__restrict__ C,
                                                      double *__restrict__ sig,
                                                      int *__restrict__ sweepsOut) {
   extern __shared__ double sm[];
   const int mm = m + (m & 1);  // even column count; column m is a zero dummy when m is odd
-  const int ldh = m + 1;
+  const int ldh = rows;
   double *A = sm;                          // ldh x mm
   double *V = A + (long long)ldh * mm;     // m x mm (ld m)
   double *sg = V + (long long)m * mm;      // mm
@@ -117,7 +120,7 @@ __global__ __launch_bounds__(kSvdThreads) void k_svd(const double *__restrict__ 
   const int hl = tid & 31, hw = tid >> 5, nhw = bd >> 5;
   for (int idx = tid; idx < ldh * mm; idx += bd) {
     const int c = idx / ldh;
-    A[idx] = c < m ? H[idx] : 0.0;
+    A[idx] = c < m ? H[(long long)c * ldH + (idx - c * ldh)] : 0.0;
   }
   for (int idx = tid; idx < m * mm; idx += bd) {
     const int r = idx % m, c = idx / m;
@@ -186,7 +189,8 @@ __global__ __launch_bounds__(kSvdThreads) void k_svd(const double *__restrict__ 
   // Column norms = singular values.
   for (int c = hw; c < m; c += nhw) {
     const double s = half_sumsq(A + (long long)c * ldh, ldh, hl);
-    if (hl == 0) sg[c] = sqrt(s);
+    // zeroLast: an exactly zero column (a zero basis vector of a block cycle) is never selected.
+    if (hl == 0) sg[c] = (zeroLast && s == 0.0) ? INFINITY : sqrt(s);
   }
   __syncthreads();
   for (int c = tid; c < m; c += bd) {
@@ -223,15 +227,24 @@ void lsq(const double *H, int m, const double *t, double *y, double *z, hipStrea
   hipLaunchKernelGGL(k_lsq, dim3(1), dim3(kLsqThreads), lds, s, H, m, t, y, z);
 }
 
-void svd_smallest(const double *H, int m, int k, double tol, int maxSweeps, double *C, double *sig,
-                  int *sweeps, hipStream_t s) {
+size_t svd_lds(int rows, int m) {
   const int mm = m + (m & 1);
-  const size_t lds = sizeof(double) * ((size_t)(m + 1) * mm + (size_t)m * mm + mm);
+  return sizeof(double) * ((size_t)rows * mm + (size_t)m * mm + mm);
+}
+
+void svd_smallest_rect(const double *H, int rows, int ldH, int m, int k, double tol, int maxSweeps,
+                       int zeroLast, double *C, double *sig, int *sweeps, hipStream_t s) {
+  const size_t lds = svd_lds(rows, m);
   if (lds > 64 * 1024)
     hipFuncSetAttribute(reinterpret_cast<const void *>(k_svd),
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(k_svd, dim3(1), dim3(kSvdThreads), lds, s, H, m, k, tol, maxSweeps, C, sig,
-                     sweeps);
+  hipLaunchKernelGGL(k_svd, dim3(1), dim3(kSvdThreads), lds, s, H, rows, ldH, m, k, tol, maxSweeps,
+                     zeroLast, C, sig, sweeps);
+}
+
+void svd_smallest(const double *H, int m, int k, double tol, int maxSweeps, double *C, double *sig,
+                  int *sweeps, hipStream_t s) {
+  svd_smallest_rect(H, m + 1, m + 1, m, k, tol, maxSweeps, 0, C, sig, sweeps, s);
 }
 
 }  // namespace dmr

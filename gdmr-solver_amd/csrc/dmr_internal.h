@@ -52,6 +52,10 @@ template <typename T> inline int red_blocks(long long n) {
   return (int)g;
 }
 
+// The thread-local error channel behind dmrGetLastError / dmrGetLastErrorString.
+int set_error(int code, const char *msg);
+void clear_error();
+
 // Leading dimension (elements) of the basis columns for vectors of n elements: n rounded up to
 // 256, plus a pad so that columns streamed together do not start at addresses a power of two
 // apart (which maps them onto the same HBM channels).  DMR_LD_PAD (elements, multiple of 256)
@@ -178,5 +182,11 @@ void lsq(const double *H, int m, const double *t, double *y, double *z, hipStrea
 // singular values in descending order; *sweeps the number of sweeps.  One workgroup.
 void svd_smallest(const double *H, int m, int k, double tol, int maxSweeps, double *C,
                   double *sig, int *sweeps, hipStream_t s);
+// The same for a rows x m matrix with leading dimension ldH (rows >= m).  The matrix and the
+// right vectors live in LDS: svd_lds(rows, m) bytes, at most 160 KiB.
+size_t svd_lds(int rows, int m);
+// zeroLast != 0: exactly zero columns are ranked largest (never among the k selected).
+void svd_smallest_rect(const double *H, int rows, int ldH, int m, int k, double tol, int maxSweeps,
+                       int zeroLast, double *C, double *sig, int *sweeps, hipStream_t s);
 
 }  // namespace dmr

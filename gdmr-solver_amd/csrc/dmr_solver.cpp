@@ -84,6 +84,9 @@ long long env_ll(const char *name, long long dflt, long long lo, long long hi) {
 
 bool env_flag(const char *name, bool dflt) { return env_ll(name, dflt ? 1 : 0, 0, 1) != 0; }
 
+int set_error(int code, const char *msg) { return set_err(code, msg ? msg : ""); }
+void clear_error() { clear_err(); }
+
 int red_blocks_cap() {
   static const int cap = (int)env_ll("DMR_RED_BLOCKS", kDefaultRedBlocks, 1, kMaxRedBlocks);
   return cap;

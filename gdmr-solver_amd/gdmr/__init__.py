@@ -103,6 +103,20 @@ _sig("dmrServerDestroy", C.c_int, [_vp])
 _sig("dmrServerSetRefinement", C.c_int, [_vp, C.c_int, C.c_double, C.c_int])
 _sig("dmrGetStagnationCount", C.c_int, [_vp])
 _sig("dmrSetStagnationCount", C.c_int, [_vp, C.c_int])
+_sig("dmrBlockCreate", C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int])
+_sig("dmrBlockDestroy", C.c_int, [_vp])
+_sig("dmrBlockSetOperatorCsr", C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp])
+_sig("dmrBlockSetOperatorValues", C.c_int, [_vp, _vp, _vp])
+_sig("dmrBlockSetRefinement", C.c_int, [_vp, C.c_double, C.c_int])
+_sig("dmrBlockSolve", C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_longlong, _vp, C.c_longlong, _vp,
+                                C.c_double, C.c_int, _dp])
+_sig("dmrBlockGetStats", C.c_int, [_vp, C.POINTER(Stats), _dp, C.c_int])
+_sig("dmrBlockHasZ", C.c_int, [_vp])
+_sig("dmrBlockImportZ", C.c_int, [_vp, _dp])
+_sig("dmrBlockExportZ", C.c_int, [_vp, _dp])
+_sig("dmrBlockSetProfiling", C.c_int, [_vp, C.c_int])
+_sig("dmrBlockGetProfile", C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_longlong)])
+_sig("dmrBlockFootprint", C.c_size_t, [C.c_int, C.c_int, C.c_int, C.c_int])
 _sig("dmrSetProfiling", C.c_int, [_vp, C.c_int])
 _sig("dmrGetProfile", C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_longlong)])
 PHASES = ("spmv", "gemvt", "update", "scale", "combo", "zgemm", "lsq", "svd", "orth")
@@ -131,6 +145,9 @@ HEADER_SYMBOLS = (
     "dmrHasZ dmrGetStatsH dmrSetRefinement dmrGlobalHandle dmrFootprint dmrSetProfiling "
     "dmrGetProfile dmrSubspaceBytes dmrSubspaceToDevice dmrSubspaceFromDevice "
     "dmrGetStagnationCount dmrSetStagnationCount "
+    "dmrBlockCreate dmrBlockDestroy dmrBlockSetOperatorCsr dmrBlockSetOperatorValues "
+    "dmrBlockSetRefinement dmrBlockSolve dmrBlockGetStats dmrBlockHasZ dmrBlockImportZ "
+    "dmrBlockExportZ dmrBlockSetProfiling dmrBlockGetProfile dmrBlockFootprint "
     "dmrServerCreate dmrServerAddFamily dmrServerSetRefinement dmrServerSubmit dmrServerPoll "
     "dmrServerWait dmrServerDrain dmrServerNow dmrServerStats dmrServerDestroy "
     "initLS freeLS impImgLS expImgLS impSrcLS expSrcLS impOptLS expOptLS solveLS linOptLS "
@@ -350,6 +367,79 @@ class Solver:
         Z = np.zeros(self.n * self.k)
         _check(lib.dmrExportZ(self.h, _ptr(Z, C.c_double)), "dmrExportZ")
         return Z.reshape(self.k, self.n)
+
+
+class BlockSolver:
+    """Block DMR context: up to max_rhs right-hand sides share one block Krylov space and one
+    recycled subspace Z (dmrBlock* in gDMR.h)."""
+
+    def __init__(self, n: int, block_steps: int, k: int, max_rhs: int, device: int = 0):
+        self.n, self.mb, self.k, self.max_rhs = n, block_steps, k, max_rhs
+        h = C.c_void_p()
+        _check(lib.dmrBlockCreate(C.byref(h), device, n, block_steps, k, max_rhs),
+               "dmrBlockCreate")
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            lib.dmrBlockDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_csr(self, rowptr_ptr, col_ptr, val_ptr, nnz, val32_ptr=0, keep=()):
+        self._keep = list(keep)
+        _check(lib.dmrBlockSetOperatorCsr(self.h, nnz, rowptr_ptr, col_ptr, val_ptr,
+                                          val32_ptr or None), "dmrBlockSetOperatorCsr")
+
+    def set_values(self, val_ptr, val32_ptr=0, keep=()):
+        self._keep_vals = list(keep)
+        _check(lib.dmrBlockSetOperatorValues(self.h, val_ptr, val32_ptr or None),
+               "dmrBlockSetOperatorValues")
+
+    def set_refinement(self, inner_tol: float, max_outer: int = 30):
+        _check(lib.dmrBlockSetRefinement(self.h, inner_tol, max_outer), "dmrBlockSetRefinement")
+
+    def solve(self, x_ptr, ldx, b_ptr, ldb, nrhs, tol, max_cycles=1000, mode=0, stream=0):
+        """Returns (status code, per-column FP64-checked relative residuals)."""
+        rel = np.zeros(nrhs)
+        rc = lib.dmrBlockSolve(self.h, mode, nrhs, x_ptr, ldx, b_ptr, ldb, stream or None, tol,
+                               max_cycles, _ptr(rel, C.c_double))
+        return rc, rel
+
+    def stats(self, cap: int = 4096):
+        st = Stats()
+        hist = np.zeros(cap)
+        cnt = lib.dmrBlockGetStats(self.h, C.byref(st), _ptr(hist, C.c_double), cap)
+        return st, hist[:min(cnt, cap)].copy()
+
+    def has_z(self) -> bool:
+        return bool(lib.dmrBlockHasZ(self.h))
+
+    def import_z(self, Z: np.ndarray):
+        Z = np.ascontiguousarray(Z, np.float64)
+        assert Z.size == self.n * self.k
+        _check(lib.dmrBlockImportZ(self.h, _ptr(Z, C.c_double)), "dmrBlockImportZ")
+
+    def export_z(self) -> np.ndarray:
+        Z = np.zeros(self.n * self.k)
+        _check(lib.dmrBlockExportZ(self.h, _ptr(Z, C.c_double)), "dmrBlockExportZ")
+        return Z.reshape(self.k, self.n)
+
+    def set_profiling(self, on: bool = True):
+        _check(lib.dmrBlockSetProfiling(self.h, int(on)), "dmrBlockSetProfiling")
+
+    def profile(self) -> dict:
+        ms, by = np.zeros(len(PHASES)), np.zeros(len(PHASES))
+        cnt = np.zeros(len(PHASES), np.int64)
+        lib.dmrBlockGetProfile(self.h, _ptr(ms, C.c_double), _ptr(by, C.c_double),
+                               cnt.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return {p: (float(ms[i]), float(by[i]), int(cnt[i])) for i, p in enumerate(PHASES)}
 
 
 def global_stats(cap: int = 4096):

@@ -160,6 +160,50 @@ int dmrGetProfile(dmrHandle h, double *ms, double *bytes, long long *launches);
 /* Bytes of HBM a handle of this shape allocates (for admission by free HBM). */
 size_t dmrFootprint(int elements, int basisDim, int deflatDim, int precision);
 
+/* ---- block (multi-right-hand-side) DMR ----
+ * Up to 32 right-hand sides of one operator share one block Krylov space and one recycled
+ * deflation subspace Z (n x k, k <= 32).  Per restart cycle: the deflation image A Z = W R_Z,
+ * blockSteps block Arnoldi steps with one-stage block classical Gram-Schmidt (tall-skinny GEMMs
+ * on the matrix cores) and Cholesky-QR normalisation, the full-projection least squares on the
+ * block Hessenberg, and the refresh of Z from the smallest right singular vectors of its leading
+ * block.  Storage and matrix-core arithmetic are FP32 inside FP64 iterative refinement per
+ * column; least squares, Cholesky factors and SVD are FP64.  Basis columns k + (blockSteps+1) *
+ * maxRhs must not exceed 384. */
+typedef struct dmrBlockContext *dmrBlockHandle;
+int dmrBlockCreate(dmrBlockHandle *h, int device, int elements, int blockSteps, int deflatDim,
+                   int maxRhs);
+int dmrBlockDestroy(dmrBlockHandle h);
+/* Same contracts as dmrSetOperatorCsr / dmrSetOperatorValues; without val32 the FP32 values are
+ * converted on the next solve's stream. */
+int dmrBlockSetOperatorCsr(dmrBlockHandle h, int nnz, const int *rowPtr, const int *colInd,
+                           const double *val, const float *val32);
+int dmrBlockSetOperatorValues(dmrBlockHandle h, const double *val, const float *val32);
+/* Relative tolerance of the inner FP32 block solves (default 1e-5) and the maximum number of
+ * refinement steps (default 30). */
+int dmrBlockSetRefinement(dmrBlockHandle h, double innerTol, int maxOuter);
+/* Solve A X = B for nrhs <= maxRhs columns.  X: device, column-major n x nrhs (leading dimension
+ * ldx), x0 in / solution out.  B: device, ldb.  mode != 0 starts from the stored subspace (else
+ * the first cycle is undeflated).  maxCycles caps the restart cycles of each inner solve.
+ * relRes (host, nrhs): the FP64-checked ||b - A x|| / ||b|| of each column.  Returns 0 when every
+ * column meets solTol, else a DMR_ERR_* code. */
+int dmrBlockSolve(dmrBlockHandle h, int mode, int nrhs, double *X, long long ldx, const double *B,
+                  long long ldb, hipStream_t stream, double solTol, int maxCycles,
+                  double *relRes);
+/* cycles: inner restart cycles; matvecs: operator applications to single columns; breakdowns:
+ * cycles with a dependent block column; rho: largest final relative residual; residHist: per
+ * cycle, the largest estimated relative residual. */
+int dmrBlockGetStats(dmrBlockHandle h, dmrStats *st, double *residHist, int cap);
+int dmrBlockHasZ(dmrBlockHandle h);
+int dmrBlockImportZ(dmrBlockHandle h, const double *hostZ);
+int dmrBlockExportZ(dmrBlockHandle h, double *hostZ);
+/* Phase profile as dmrSetProfiling / dmrGetProfile: SPMV = SpMM and FP64 residuals, GEMVT =
+ * projections and Gram matrices (gemm_tn), UPDATE = block updates and normalisations (gemm_nn),
+ * COMBO = residual and solution updates, ZGEMM = subspace refresh, LSQ = least squares and
+ * Cholesky factors, SVD = Jacobi SVD, SCALE = column norms and scaling. */
+int dmrBlockSetProfiling(dmrBlockHandle h, int on);
+int dmrBlockGetProfile(dmrBlockHandle h, double *ms, double *bytes, long long *launches);
+size_t dmrBlockFootprint(int elements, int blockSteps, int deflatDim, int maxRhs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -162,10 +162,16 @@ def main():
     ap.add_argument("--serve", action="store_true",
                     help="run the serving load generator (gdmr.serve_bench) instead; the "
                          "remaining arguments go to it")
+    ap.add_argument("--batched", action="store_true",
+                    help="run the batched multi-RHS benchmark (gdmr.batch_bench) instead; the "
+                         "remaining arguments go to it")
     args, rest = ap.parse_known_args()
-    if args.serve:
-        from gdmr import serve_bench
-        serve_bench.main(rest)
+    if args.serve or args.batched:
+        if args.gpus != 1:
+            print("bench.py: this build measures one GPU (--gpus must be 1)", file=sys.stderr)
+            sys.exit(2)
+        from gdmr import batch_bench, serve_bench
+        (serve_bench if args.serve else batch_bench).main(rest)
         return
     if rest:
         ap.error(f"unrecognized arguments: {' '.join(rest)}")

@@ -75,10 +75,11 @@ struct dmrBlockContext {
     bool on = false;
     double ms[DMR_NPHASES] = {};
     double by[DMR_NPHASES] = {};
+    double fl[DMR_NPHASES] = {};
     long long cnt[DMR_NPHASES] = {};
     std::vector<hipEvent_t> ev;
     std::vector<int> phase;
-    std::vector<double> bytes;
+    std::vector<double> bytes, flops;
     int used = 0;
   } prof;
 };
@@ -98,7 +99,7 @@ int tn_rows_target() {
 }
 
 template <typename F>
-void timed(dmrBlockContext *c, hipStream_t s, int phase, double bytes, F &&f) {
+void timedf(dmrBlockContext *c, hipStream_t s, int phase, double bytes, double flops, F &&f) {
   auto &p = c->prof;
   if (!p.on || p.used >= kMaxEvents) {
     f();
@@ -109,6 +110,7 @@ void timed(dmrBlockContext *c, hipStream_t s, int phase, double bytes, F &&f) {
     for (auto &e : p.ev) hipEventCreate(&e);
     p.phase.resize(kMaxEvents);
     p.bytes.resize(kMaxEvents);
+    p.flops.resize(kMaxEvents);
   }
   const int i = p.used++;
   hipEventRecord(p.ev[2 * i], s);
@@ -116,6 +118,12 @@ void timed(dmrBlockContext *c, hipStream_t s, int phase, double bytes, F &&f) {
   hipEventRecord(p.ev[2 * i + 1], s);
   p.phase[i] = phase;
   p.bytes[i] = bytes;
+  p.flops[i] = flops;
+}
+
+template <typename F>
+void timed(dmrBlockContext *c, hipStream_t s, int phase, double bytes, F &&f) {
+  timedf(c, s, phase, bytes, 0.0, f);
 }
 
 void prof_collect(dmrBlockContext *c) {
@@ -125,6 +133,7 @@ void prof_collect(dmrBlockContext *c) {
     if (hipEventElapsedTime(&ms, p.ev[2 * i], p.ev[2 * i + 1]) == hipSuccess) {
       p.ms[p.phase[i]] += ms;
       p.by[p.phase[i]] += p.bytes[i];
+      p.fl[p.phase[i]] += p.flops[i];
       ++p.cnt[p.phase[i]];
     }
   }
@@ -229,14 +238,14 @@ int ctx_alloc(dmrBlockContext *c) {
 void cholqr2(dmrBlockContext *c, float *U, int sc, double *Hdst, int ldh, hipStream_t st) {
   const long long n = c->n, ld = c->ld;
   for (int pass = 0; pass < 2; ++pass) {
-    timed(c, st, DMR_PH_GEMVT, 4.0 * n * 2 * sc, [&] {
+    timedf(c, st, DMR_PH_GEMVT, 4.0 * n * sc, 2.0 * n * sc * sc, [&] {
       dmr::blk::gemm_tn(U, ld, sc, U, ld, sc, n, c->G, c->part, c->G64, sc, nullptr, st);
     });
     timed(c, st, DMR_PH_ORTH, 0.0, [&] {
       dmr::blk::chol_qr(c->G64, sc, kDepTol, c->Sinv32, c->Sacc, pass == 0,
                         pass == 1 ? Hdst : nullptr, ldh, c->flags, st);
     });
-    timed(c, st, DMR_PH_UPDATE, 4.0 * n * 2 * sc, [&] {
+    timedf(c, st, DMR_PH_UPDATE, 4.0 * n * 2 * sc, 2.0 * n * sc * sc, [&] {
       dmr::blk::gemm_nn(U, ld, sc, c->Sinv32, nullptr, 0, 0.0f, 1.0f, U, ld, sc, n, st);
     });
   }
@@ -258,17 +267,17 @@ bool cycle(dmrBlockContext *c, int s, bool deflate, hipStream_t st) {
   dmr::fill<double>(c->H, (long long)rows * cols, 0.0, st);
   dmr::blk::zero_ints(c->flags, 2, st);
   if (deflate) {
-    timed(c, st, DMR_PH_SPMV, spmm_bytes(c, k),
+    timedf(c, st, DMR_PH_SPMV, spmm_bytes(c, k), 2.0 * c->csr.nnz * k,
           [&] { dmr::blk::spmm32(c->csr, c->Z, ld, P, ld, k, st); });
     c->st.matvecs += k;
     cholqr2(c, P, k, c->H, rows, st);
   }
   float *V0 = P + (long long)p0 * ld;
   if (deflate) {
-    timed(c, st, DMR_PH_GEMVT, vb * (k + s), [&] {
+    timedf(c, st, DMR_PH_GEMVT, vb * (k + s), 2.0 * n * k * s, [&] {
       dmr::blk::gemm_tn(P, ld, k, c->R, ld, s, n, G, c->part, nullptr, 0, c->C32, st);
     });
-    timed(c, st, DMR_PH_UPDATE, vb * (k + 2 * s), [&] {
+    timedf(c, st, DMR_PH_UPDATE, vb * (k + 2 * s), 2.0 * n * k * s, [&] {
       dmr::blk::gemm_nn(P, ld, k, c->C32, c->R, ld, 1.0f, -1.0f, V0, ld, s, n, st);
     });
   } else {
@@ -280,31 +289,32 @@ bool cycle(dmrBlockContext *c, int s, bool deflate, hipStream_t st) {
     float *U = P + (long long)(p0 + (j + 1) * s) * ld;
     const int pj = p0 + (j + 1) * s;
     double *Hcol = c->H + (long long)(p0 + j * s) * rows;
-    timed(c, st, DMR_PH_SPMV, spmm_bytes(c, s),
+    timedf(c, st, DMR_PH_SPMV, spmm_bytes(c, s), 2.0 * c->csr.nnz * s,
           [&] { dmr::blk::spmm32(c->csr, Vj, ld, U, ld, s, st); });
     c->st.matvecs += s;
-    timed(c, st, DMR_PH_GEMVT, vb * (pj + s), [&] {
+    timedf(c, st, DMR_PH_GEMVT, vb * (pj + s), 2.0 * n * pj * s, [&] {
       dmr::blk::gemm_tn(P, ld, pj, U, ld, s, n, G, c->part, Hcol, rows, c->C32, st);
     });
-    timed(c, st, DMR_PH_UPDATE, vb * (pj + 2 * s), [&] {
+    timedf(c, st, DMR_PH_UPDATE, vb * (pj + 2 * s), 2.0 * n * pj * s, [&] {
       dmr::blk::gemm_nn(P, ld, pj, c->C32, U, ld, 1.0f, -1.0f, U, ld, s, n, st);
     });
     cholqr2(c, U, s, Hcol + pj, rows, st);
   }
   // Full projection of the residual block, least squares, residual and solution updates.
-  timed(c, st, DMR_PH_GEMVT, vb * (rows + s), [&] {
+  timedf(c, st, DMR_PH_GEMVT, vb * (rows + s), 2.0 * n * rows * s, [&] {
     dmr::blk::gemm_tn(P, ld, rows, c->R, ld, s, n, G, c->part, c->T, rows, nullptr, st);
   });
   timed(c, st, DMR_PH_LSQ, 0.0, [&] {
     dmr::blk::lsq_block(c->H, rows, cols, s, c->T, s, c->rho, c->Hw, c->Tw, c->Y, c->Yr32,
                         c->ZH32, st);
   });
-  timed(c, st, DMR_PH_COMBO, vb * (rows + 2 * s), [&] {
+  timedf(c, st, DMR_PH_COMBO, vb * (rows + 2 * s), 2.0 * n * rows * s, [&] {
     dmr::blk::gemm_nn(P, ld, rows, c->ZH32, c->R, ld, 1.0f, -1.0f, c->R, ld, s, n, st);
   });
   timed(c, st, DMR_PH_SCALE, vb * s,
         [&] { dmr::blk::colnorms<float>(c->R, ld, s, n, c->cpart, c->q, 1, st); });
-  timed(c, st, DMR_PH_COMBO, vb * ((deflate ? k : 0) + mb * s + 2 * s * (deflate ? 2 : 1)), [&] {
+  timedf(c, st, DMR_PH_COMBO, vb * ((deflate ? k : 0) + mb * s + 2 * s * (deflate ? 2 : 1)),
+         2.0 * n * ((deflate ? k : 0) + mb * s) * s, [&] {
     if (deflate)
       dmr::blk::gemm_nn(c->Z, ld, k, c->Yr32, c->Xa, ld, 1.0f, 1.0f, c->Xa, ld, s, n, st);
     dmr::blk::gemm_nn(V0, ld, mb * s, c->Yr32 + (long long)p0 * 32, c->Xa, ld, 1.0f, 1.0f,
@@ -605,17 +615,20 @@ int dmrBlockSetProfiling(dmrBlockHandle h, int on) {
   for (int i = 0; i < DMR_NPHASES; ++i) {
     h->prof.ms[i] = 0.0;
     h->prof.by[i] = 0.0;
+    h->prof.fl[i] = 0.0;
     h->prof.cnt[i] = 0;
   }
   h->prof.used = 0;
   return DMR_OK;
 }
 
-int dmrBlockGetProfile(dmrBlockHandle h, double *ms, double *bytes, long long *launches) {
+int dmrBlockGetProfile(dmrBlockHandle h, double *ms, double *bytes, double *flops,
+                       long long *launches) {
   if (!h) return dmr::set_error(DMR_ERR_ARG, "null handle");
   for (int i = 0; i < DMR_NPHASES; ++i) {
     if (ms) ms[i] = h->prof.ms[i];
     if (bytes) bytes[i] = h->prof.by[i];
+    if (flops) flops[i] = h->prof.fl[i];
     if (launches) launches[i] = h->prof.cnt[i];
   }
   return DMR_NPHASES;

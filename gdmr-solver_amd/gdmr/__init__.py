@@ -115,7 +115,7 @@ _sig("dmrBlockHasZ", C.c_int, [_vp])
 _sig("dmrBlockImportZ", C.c_int, [_vp, _dp])
 _sig("dmrBlockExportZ", C.c_int, [_vp, _dp])
 _sig("dmrBlockSetProfiling", C.c_int, [_vp, C.c_int])
-_sig("dmrBlockGetProfile", C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_longlong)])
+_sig("dmrBlockGetProfile", C.c_int, [_vp, _dp, _dp, _dp, C.POINTER(C.c_longlong)])
 _sig("dmrBlockFootprint", C.c_size_t, [C.c_int, C.c_int, C.c_int, C.c_int])
 _sig("dmrSetProfiling", C.c_int, [_vp, C.c_int])
 _sig("dmrGetProfile", C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_longlong)])
@@ -435,11 +435,13 @@ class BlockSolver:
         _check(lib.dmrBlockSetProfiling(self.h, int(on)), "dmrBlockSetProfiling")
 
     def profile(self) -> dict:
-        ms, by = np.zeros(len(PHASES)), np.zeros(len(PHASES))
+        """{phase: (ms, algorithmic bytes, launches, flops)} since set_profiling(True)."""
+        ms, by, fl = np.zeros(len(PHASES)), np.zeros(len(PHASES)), np.zeros(len(PHASES))
         cnt = np.zeros(len(PHASES), np.int64)
         lib.dmrBlockGetProfile(self.h, _ptr(ms, C.c_double), _ptr(by, C.c_double),
-                               cnt.ctypes.data_as(C.POINTER(C.c_longlong)))
-        return {p: (float(ms[i]), float(by[i]), int(cnt[i])) for i, p in enumerate(PHASES)}
+                               _ptr(fl, C.c_double), cnt.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return {p: (float(ms[i]), float(by[i]), int(cnt[i]), float(fl[i]))
+                for i, p in enumerate(PHASES)}
 
 
 def global_stats(cap: int = 4096):

@@ -199,9 +199,11 @@ int dmrBlockExportZ(dmrBlockHandle h, double *hostZ);
 /* Phase profile as dmrSetProfiling / dmrGetProfile: SPMV = SpMM and FP64 residuals, GEMVT =
  * projections and Gram matrices (gemm_tn), UPDATE = block updates and normalisations (gemm_nn),
  * COMBO = residual and solution updates, ZGEMM = subspace refresh, LSQ = least squares and
- * Cholesky factors, SVD = Jacobi SVD, SCALE = column norms and scaling. */
+ * Cholesky factors, SVD = Jacobi SVD, SCALE = column norms and scaling, ORTH = Cholesky factors
+ * of the block normalisation.  flops: the matrix-core and SpMM floating-point operations. */
 int dmrBlockSetProfiling(dmrBlockHandle h, int on);
-int dmrBlockGetProfile(dmrBlockHandle h, double *ms, double *bytes, long long *launches);
+int dmrBlockGetProfile(dmrBlockHandle h, double *ms, double *bytes, double *flops,
+                       long long *launches);
 size_t dmrBlockFootprint(int elements, int blockSteps, int deflatDim, int maxRhs);
 
 #ifdef __cplusplus

@@ -1,0 +1,149 @@
+"""Block (multi-right-hand-side) DMR on the GPU (dmrBlock* in gDMR.h).
+
+The block solver runs FP32 storage and matrix-core arithmetic inside FP64 iterative refinement,
+so it is not bitwise comparable with the FP64 oracle.  The bar, per column:
+  * the true FP64 residual ||b - A x|| / ||b|| (recomputed on the host) meets the tolerance;
+  * the solution agrees with the FP64 oracle's solution of the same system to the accuracy the
+    residual allows: ||x - x_oracle|| / ||x_oracle|| <= XTOL (stated below, measured with margin
+    on these systems);
+  * results are bitwise reproducible (fixed-order reductions, deterministic MFMA) from run to run.
+"""
+import numpy as np
+import pytest
+
+import gdmr
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+XTOL = 1e-6
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def _block(torch, A, B, mb, k, smax=None, ld=None, tol=1e-10, mode=0, solver=None, x0=None):
+    """Solves A X = B (B: s x n host array, one right-hand side per row) with a block handle."""
+    s, n = B.shape
+    ld = ld or n
+    Bp = np.zeros((s, ld))
+    Bp[:, :n] = B
+    Bd = _dev(torch, Bp)
+    X = np.zeros((s, ld))
+    if x0 is not None:
+        X[:, :n] = x0
+    Xd = _dev(torch, X)
+    own = solver is None
+    if own:
+        solver = gdmr.BlockSolver(n, mb, k, smax or s)
+        rp, ci, va = _dev(torch, A.rowptr), _dev(torch, A.col), _dev(torch, A.val)
+        solver.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A.nnz, keep=(rp, ci, va))
+    torch.cuda.synchronize()
+    rc, rel = solver.solve(Xd.data_ptr(), ld, Bd.data_ptr(), ld, s, tol, max_cycles=500, mode=mode)
+    st, hist = solver.stats()
+    Xh = Xd.cpu().numpy()[:, :n].copy()
+    if own:
+        solver.close()
+    return rc, rel, st, Xh
+
+
+def _true_res(A, b, x):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(b - A.matvec(x)) / (nb if nb > 0 else 1.0)
+
+
+def test_block_matches_oracle(torch_gpu):
+    """Anisotropic 3-D diffusion 16^3 (coefficients 1, 0.1, 0.01), 8 right-hand sides, 4 block
+    steps, k = 8: every column meets 1e-10 in FP64 and matches the FP64 oracle's solution."""
+    A = gdmr.aniso_3d(16, coef=(1.0, 0.1, 0.01))
+    n, s, tol = A.n, 8, 1e-10
+    B = np.stack([gdmr.rand_normal(31, j, n) for j in range(s)])
+    rc, rel, st, X = _block(torch_gpu, A, B, mb=4, k=8)
+    print(f"block: rc {rc} cycles {st.cycles} breakdowns {st.breakdowns} refreshes {st.svdCalls} "
+          f"max rel {rel.max():.2e}", flush=True)
+    assert rc == 0
+    ora = oracle.Oracle(A, 30, 8, gdmr.reduction_blocks(n))
+    try:
+        for j in range(s):
+            tr = _true_res(A, B[j], X[j])
+            assert tr <= tol * 1.001, (j, tr)
+            assert abs(tr - rel[j]) <= 1e-3 * tol
+            xo, reso, itso, sto = ora.solve(B[j], tol=1e-12, mode=j)
+            err = np.linalg.norm(X[j] - xo) / np.linalg.norm(xo)
+            print(f"  column {j}: true residual {tr:.2e}, error vs oracle {err:.2e}", flush=True)
+            assert err <= XTOL, (j, err)
+    finally:
+        ora.close()
+
+
+def test_block_bitwise_reproducible(torch_gpu):
+    A = gdmr.conv_diff_3d(20, jitter=0.01, seed=5, stream=1)
+    B = np.stack([gdmr.rand_normal(32, j, A.n) for j in range(12)])
+    r1 = _block(torch_gpu, A, B, mb=5, k=12, smax=16)
+    r2 = _block(torch_gpu, A, B, mb=5, k=12, smax=16)
+    assert r1[0] == 0 and r2[0] == 0
+    assert r1[2].cycles == r2[2].cycles
+    assert np.array_equal(r1[3], r2[3])
+    assert np.array_equal(r1[1], r2[1])
+
+
+def test_block_shapes_recycling_and_padding(torch_gpu):
+    """Partial block (5 of 32 columns), padded leading dimension, k = 0, and a recycled second
+    block started from the subspace of the first (mode 1) on jittered coefficients."""
+    torch = torch_gpu
+    A0 = gdmr.conv_diff_3d(24, jitter=0.01, seed=8, stream=0)
+    A1 = gdmr.conv_diff_3d(24, jitter=0.01, seed=8, stream=1)
+    n, tol = A0.n, 1e-10
+    B0 = np.stack([gdmr.rand_normal(33, j, n) for j in range(5)])
+    B1 = np.stack([gdmr.rand_normal(34, j, n) for j in range(5)])
+    # k = 0, padded ld.
+    rc, rel, st, X = _block(torch, A0, B0, mb=6, k=0, smax=32, ld=n + 100)
+    assert rc == 0 and st.svdCalls == 0
+    for j in range(5):
+        assert _true_res(A0, B0[j], X[j]) <= tol * 1.001
+    # Recycling across two blocks of a sequence (same pattern, new values).
+    bs = gdmr.BlockSolver(n, 6, 16, 32)
+    rp, ci = _dev(torch, A0.rowptr), _dev(torch, A0.col)
+    v0, v1 = _dev(torch, A0.val), _dev(torch, A1.val)
+    bs.set_csr(rp.data_ptr(), ci.data_ptr(), v0.data_ptr(), A0.nnz, keep=(rp, ci, v0))
+    rc0, rel0, st0, X0 = _block(torch, A0, B0, 6, 16, solver=bs)
+    assert rc0 == 0 and bs.has_z()
+    bs.set_values(v1.data_ptr(), keep=(v1,))
+    rc1, rel1, st1, X1 = _block(torch, A1, B1, 6, 16, solver=bs, mode=1)
+    bs.close()
+    assert rc1 == 0
+    print(f"first block {st0.cycles} cycles, recycled block {st1.cycles} cycles", flush=True)
+    for j in range(5):
+        assert _true_res(A1, B1[j], X1[j]) <= tol * 1.001
+
+
+def test_block_dependent_and_zero_columns(torch_gpu):
+    """Duplicate right-hand sides make the block rank deficient (dependent columns are dropped
+    from the basis); a zero right-hand side has the zero solution; a nonzero initial guess is
+    refined from."""
+    A = gdmr.conv_diff_2d(48, jitter=0.01, seed=3, stream=2)
+    n, tol = A.n, 1e-10
+    b = gdmr.rand_normal(35, 0, n)
+    c = gdmr.rand_normal(35, 1, n)
+    B = np.stack([b, c, b, np.zeros(n), 2.0 * c])
+    x0 = np.zeros((5, n))
+    x0[1] = 0.5 * gdmr.rand_normal(36, 0, n)
+    rc, rel, st, X = _block(torch_gpu, A, B, mb=6, k=8, x0=x0)
+    print(f"rc {rc} cycles {st.cycles} breakdowns {st.breakdowns} rel {rel}", flush=True)
+    assert rc == 0 and st.breakdowns > 0
+    for j in (0, 1, 2, 4):
+        assert _true_res(A, B[j], X[j]) <= tol * 1.001, j
+    assert np.all(X[3] == 0.0) and rel[3] == 0.0
+    assert np.linalg.norm(X[0] - X[2]) <= 1e-8 * np.linalg.norm(X[0])
+
+
+def test_block_rejects_bad_shapes(torch_gpu):
+    with pytest.raises(gdmr.SolverError):
+        gdmr.BlockSolver(1000, 12, 32, 32)  # 32 + 13 * 32 > 384 basis columns
+    with pytest.raises(gdmr.SolverError):
+        gdmr.BlockSolver(1000, 4, 8, 33)
+    bs = gdmr.BlockSolver(1000, 4, 8, 8)
+    rc, rel = bs.solve(0, 1000, 0, 1000, 4, 1e-8)
+    assert rc == gdmr.DMR_ERR_STATE  # no operator
+    bs.close()

@@ -27,6 +27,11 @@ namespace {
 
 struct Family {
   int n = 0, nnz = 0, m = 0, k = 0, prec = DMR_PREC_FP64;
+  // Block family: block handle, block steps, widest block, staging of a batched launch.
+  bool block = false;
+  int mb = 0, sMax = 0;
+  dmrBlockHandle bh = nullptr;
+  double *stage = nullptr;  // 2 * n * sMax doubles: B then X
   const int *rowPtr = nullptr;
   const int *colInd = nullptr;
   size_t footprint = 0;
@@ -87,9 +92,30 @@ size_t free_hbm(int device) {
   return fr;
 }
 
+bool resident(const Family *F) { return F->block ? F->bh != nullptr : F->h != nullptr; }
+
+void release_block(Family *F) {
+  if (F->bh) dmrBlockDestroy(F->bh);
+  F->bh = nullptr;
+  if (F->stage) hipFree(F->stage);
+  F->stage = nullptr;
+}
+
 // Releases the handle of an idle family, keeping its recycled state on the host.  Called with the
 // family marked busy by the caller (so no slot picks it meanwhile) and the server mutex released.
 void evict(dmrServer *S, Family *F) {
+  if (F->block) {
+    if (F->k > 0 && dmrBlockHasZ(F->bh)) {
+      F->hostZ.resize((size_t)F->n * F->k);
+      F->hostHasZ = dmrBlockExportZ(F->bh, F->hostZ.data()) == DMR_OK;
+    } else {
+      F->hostHasZ = false;
+    }
+    release_block(F);
+    std::lock_guard<std::mutex> lk(S->mu);
+    --S->handles;
+    return;
+  }
   if (F->k > 0 && dmrHasZ(F->h)) {
     F->hostZ.resize((size_t)F->n * F->k);
     F->hostHasZ = dmrExportZ(F->h, F->hostZ.data()) == DMR_OK;
@@ -117,7 +143,8 @@ int admit(dmrServer *S, Family *F, const double *val) {
       }
       Family *victim = nullptr;
       for (auto &G : S->families)
-        if (G.get() != F && !G->busy && G->h && (!victim || G->lastUse < victim->lastUse))
+        if (G.get() != F && !G->busy && resident(G.get()) &&
+            (!victim || G->lastUse < victim->lastUse))
           victim = G.get();
       if (victim) {
         victim->busy = true;
@@ -134,7 +161,33 @@ int admit(dmrServer *S, Family *F, const double *val) {
       S->cvDone.wait_for(lk, std::chrono::milliseconds(20));
     }
   }
-  int rc = dmrCreate(&F->h, S->device, F->n, F->m, F->k, F->prec);
+  int rc = DMR_OK;
+  if (F->block) {
+    rc = dmrBlockCreate(&F->bh, S->device, F->n, F->mb, F->k, F->sMax);
+    if (rc == DMR_OK)
+      rc = dmrBlockSetOperatorCsr(F->bh, F->nnz, F->rowPtr, F->colInd, val, nullptr);
+    if (rc == DMR_OK &&
+        hipMalloc(&F->stage, sizeof(double) * 2 * (size_t)F->n * F->sMax) != hipSuccess)
+      rc = DMR_ERR_ALLOC;
+    if (rc == DMR_OK && F->hostHasZ) rc = dmrBlockImportZ(F->bh, F->hostZ.data());
+    if (rc == DMR_OK && (F->innerTol > 0.0 || F->maxOuter > 0))
+      rc = dmrBlockSetRefinement(F->bh, F->innerTol > 0.0 ? F->innerTol : 1e-5,
+                                 F->maxOuter > 0 ? F->maxOuter : 30);
+    if (rc != DMR_OK) release_block(F);
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->pendingBytes -= F->footprint;
+      if (F->bh) ++S->handles;
+    }
+    S->cvDone.notify_all();
+    if (F->bh) {
+      F->hostZ.clear();
+      F->hostZ.shrink_to_fit();
+      F->hostHasZ = false;
+    }
+    return rc;
+  }
+  rc = dmrCreate(&F->h, S->device, F->n, F->m, F->k, F->prec);
   if (rc == DMR_OK) rc = dmrSetOperatorCsr(F->h, F->nnz, F->rowPtr, F->colInd, val, nullptr);
   if (rc == DMR_OK && F->hostHasZ) rc = dmrImportZ(F->h, F->hostZ.data());
   if (rc == DMR_OK) rc = dmrSetStagnationCount(F->h, F->flagCount);
@@ -161,12 +214,60 @@ int admit(dmrServer *S, Family *F, const double *val) {
 
 // A worker slot: takes the oldest queued request whose family is free, runs its right-hand sides
 // in order on the slot's stream, and frees the family for its next request.
+// Solves one launch of a block family: the right-hand sides of every request in `batch` as one
+// block (gathered into the family's staging block, solved, scattered back).
+void run_block(Family *F, std::vector<Request> &batch, hipStream_t stream,
+               std::vector<dmrRequestResult> &res) {
+  int total = 0;
+  for (auto &r : batch) total += (int)r.b.size();
+  const size_t nb = sizeof(double) * (size_t)F->n;
+  double *B = F->stage, *X = F->stage + (size_t)F->n * F->sMax;
+  int rc = dmrBlockSetOperatorValues(F->bh, batch[0].val, nullptr);
+  int col = 0;
+  for (auto &r : batch)
+    for (size_t i = 0; i < r.b.size() && rc == DMR_OK; ++i, ++col) {
+      if (hipMemcpyAsync(B + (size_t)col * F->n, r.b[i], nb, hipMemcpyDeviceToDevice, stream) !=
+              hipSuccess ||
+          hipMemcpyAsync(X + (size_t)col * F->n, r.x[i], nb, hipMemcpyDeviceToDevice, stream) !=
+              hipSuccess)
+        rc = DMR_ERR_HIP;
+    }
+  const bool have = dmrBlockHasZ(F->bh) != 0;
+  std::vector<double> rel(total, 0.0);
+  if (rc == DMR_OK) {
+    const int maxCycles = std::max(1, std::min(batch[0].matRank, 100000));
+    rc = dmrBlockSolve(F->bh, have ? 1 : 0, total, X, F->n, B, F->n, stream, batch[0].tol,
+                       maxCycles, rel.data());
+  }
+  dmrStats st{};
+  dmrBlockGetStats(F->bh, &st, nullptr, 0);
+  col = 0;
+  for (size_t q = 0; q < batch.size(); ++q) {
+    auto &r = batch[q];
+    res[q].recycled = have ? 1 : 0;
+    res[q].cycles = st.cycles;
+    res[q].batch = total;
+    for (size_t i = 0; i < r.b.size(); ++i, ++col) {
+      res[q].maxRelRes = std::max(res[q].maxRelRes, rel[col]);
+      if (hipMemcpyAsync(r.x[i], X + (size_t)col * F->n, nb, hipMemcpyDeviceToDevice, stream) !=
+              hipSuccess &&
+          rc == DMR_OK)
+        rc = DMR_ERR_HIP;
+    }
+  }
+  if (hipStreamSynchronize(stream) != hipSuccess && rc == DMR_OK) rc = DMR_ERR_HIP;
+  for (auto &r : res) r.status = rc;
+}
+
+// A worker slot: takes the oldest queued request whose family is free (for a block family also
+// the later queued requests it can batch with), runs it on the slot's stream, and frees the
+// family for its next launch.
 void worker(dmrServer *S, int slot) {
   hipSetDevice(S->device);
   hipStream_t stream = nullptr;
   hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
   for (;;) {
-    Request req;
+    std::vector<Request> batch;
     Family *F = nullptr;
     {
       std::unique_lock<std::mutex> lk(S->mu);
@@ -178,47 +279,78 @@ void worker(dmrServer *S, int slot) {
       S->cvWork.wait(lk, [&] { return (S->stop && S->queue.empty()) || ready() != S->queue.end(); });
       if (S->stop && S->queue.empty()) break;
       auto it = ready();
-      req = std::move(*it);
-      S->queue.erase(it);
-      F = S->families[req.family].get();
-      F->busy = true;
-      ++S->running;
-    }
-    dmrRequestResult res{};
-    res.id = req.id;
-    res.family = req.family;
-    res.nrhs = (int)req.b.size();
-    res.slot = slot;
-    res.submitTime = req.submitTime;
-    res.startTime = S->now();
-    int rc = F->h ? DMR_OK : admit(S, F, req.val);
-    if (rc == DMR_OK) rc = dmrSetOperatorValues(F->h, req.val, nullptr);
-    if (rc == DMR_OK) {
-      const bool have = dmrHasZ(F->h) != 0;
-      res.recycled = have ? 1 : 0;
-      for (size_t i = 0; i < req.b.size() && rc == DMR_OK; ++i) {
-        int its = 0;
-        const int mode = (have || i > 0) ? 1 : 0;
-        const double r = dmrSolve(F->h, mode, req.x[i], req.b[i], stream, req.tol, req.matRank,
-                                  &its);
-        rc = dmrGetLastError();
-        dmrStats st{};
-        dmrGetStatsH(F->h, &st, nullptr, 0);
-        res.cycles += st.cycles;
-        res.maxRelRes = std::max(res.maxRelRes, r);
+      F = S->families[it->family].get();
+      batch.push_back(std::move(*it));
+      it = S->queue.erase(it);
+      if (F->block) {
+        // Later requests of the family with the same operator values and tolerance join the
+        // launch while the block has room (in submission order).
+        int total = (int)batch[0].b.size();
+        while (it != S->queue.end()) {
+          const Request &r = *it;
+          if (r.family == batch[0].family && r.val == batch[0].val && r.tol == batch[0].tol &&
+              total + (int)r.b.size() <= F->sMax) {
+            total += (int)r.b.size();
+            batch.push_back(std::move(*it));
+            it = S->queue.erase(it);
+          } else {
+            ++it;
+          }
+        }
       }
-      // The next request of this family may run on another slot's stream.
-      if (hipStreamSynchronize(stream) != hipSuccess && rc == DMR_OK) rc = DMR_ERR_HIP;
+      F->busy = true;
+      S->running += (int)batch.size();
     }
-    res.status = rc;
-    res.endTime = S->now();
+    std::vector<dmrRequestResult> res(batch.size());
+    const double t0 = S->now();
+    for (size_t q = 0; q < batch.size(); ++q) {
+      res[q].id = batch[q].id;
+      res[q].family = batch[q].family;
+      res[q].nrhs = (int)batch[q].b.size();
+      res[q].slot = slot;
+      res[q].submitTime = batch[q].submitTime;
+      res[q].startTime = t0;
+      res[q].batch = res[q].nrhs;
+    }
+    int rc = resident(F) ? DMR_OK : admit(S, F, batch[0].val);
+    if (rc != DMR_OK) {
+      for (auto &r : res) r.status = rc;
+    } else if (F->block) {
+      run_block(F, batch, stream, res);
+    } else {
+      Request &req = batch[0];
+      dmrRequestResult &r0 = res[0];
+      rc = dmrSetOperatorValues(F->h, req.val, nullptr);
+      if (rc == DMR_OK) {
+        const bool have = dmrHasZ(F->h) != 0;
+        r0.recycled = have ? 1 : 0;
+        for (size_t i = 0; i < req.b.size() && rc == DMR_OK; ++i) {
+          int its = 0;
+          const int mode = (have || i > 0) ? 1 : 0;
+          const double r = dmrSolve(F->h, mode, req.x[i], req.b[i], stream, req.tol, req.matRank,
+                                    &its);
+          rc = dmrGetLastError();
+          dmrStats st{};
+          dmrGetStatsH(F->h, &st, nullptr, 0);
+          r0.cycles += st.cycles;
+          r0.maxRelRes = std::max(r0.maxRelRes, r);
+        }
+        // The next request of this family may run on another slot's stream.
+        if (hipStreamSynchronize(stream) != hipSuccess && rc == DMR_OK) rc = DMR_ERR_HIP;
+      }
+      r0.status = rc;
+    }
+    const double t1 = S->now();
     {
       std::lock_guard<std::mutex> lk(S->mu);
-      F->lastUse = res.endTime;
+      F->lastUse = t1;
       F->busy = false;
-      S->results.push_back(res);
-      --S->running;
-      ++S->completed;
+      for (auto &r : res) {
+        r.endTime = t1;
+        S->results.push_back(r);
+      }
+      S->running -= (int)batch.size();
+      S->completed += (long long)batch.size();
     }
     S->cvDone.notify_all();
     S->cvWork.notify_all();
@@ -266,12 +398,37 @@ int dmrServerAddFamily(dmrServerHandle s, int n, int nnz, const int *rowPtr, con
   return DMR_OK;
 }
 
+int dmrServerAddBlockFamily(dmrServerHandle s, int n, int nnz, const int *rowPtr,
+                            const int *colInd, int blockSteps, int deflatDim, int maxRhs,
+                            int *family) {
+  if (!s || !rowPtr || !colInd || n < 1 || nnz < 0 || !family) return DMR_ERR_ARG;
+  if (blockSteps < 1 || deflatDim < 0 || deflatDim > 32 || maxRhs < 1 || maxRhs > 32 ||
+      deflatDim + (blockSteps + 1) * maxRhs > 384)
+    return DMR_ERR_ARG;
+  auto F = std::make_unique<Family>();
+  F->block = true;
+  F->n = n;
+  F->nnz = nnz;
+  F->rowPtr = rowPtr;
+  F->colInd = colInd;
+  F->mb = blockSteps;
+  F->k = deflatDim;
+  F->sMax = maxRhs;
+  F->prec = DMR_PREC_FP32;
+  F->footprint = dmrBlockFootprint(n, blockSteps, deflatDim, maxRhs) +
+                 sizeof(double) * 2 * (size_t)n * maxRhs + (size_t)nnz * 4 + (8u << 20);
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->families.push_back(std::move(F));
+  *family = (int)s->families.size() - 1;
+  return DMR_OK;
+}
+
 int dmrServerSetRefinement(dmrServerHandle s, int family, double innerTol, int maxOuter) {
   if (!s || !(innerTol > 0.0 && innerTol < 1.0) || maxOuter < 1) return DMR_ERR_ARG;
   std::lock_guard<std::mutex> lk(s->mu);
   if (family < 0 || family >= (int)s->families.size()) return DMR_ERR_ARG;
   Family *F = s->families[family].get();
-  if (F->busy || F->h) return DMR_ERR_STATE;  // settings are fixed once the family has run
+  if (F->busy || resident(F)) return DMR_ERR_STATE;  // fixed once the family has run
   F->innerTol = innerTol;
   F->maxOuter = maxOuter;
   return DMR_OK;
@@ -353,8 +510,10 @@ int dmrServerDestroy(dmrServerHandle s) {
   s->cvWork.notify_all();
   s->cvDone.notify_all();
   for (auto &t : s->workers) t.join();
-  for (auto &F : s->families)
+  for (auto &F : s->families) {
     if (F->h) dmrDestroy(F->h);
+    release_block(F.get());
+  }
   delete s;
   return DMR_OK;
 }

@@ -101,6 +101,8 @@ _sig("dmrServerDrain", C.c_int, [_vp])
 _sig("dmrServerStats", C.c_int, [_vp, _ip, _ip, _ip, C.POINTER(C.c_size_t)])
 _sig("dmrServerDestroy", C.c_int, [_vp])
 _sig("dmrServerSetRefinement", C.c_int, [_vp, C.c_int, C.c_double, C.c_int])
+_sig("dmrServerAddBlockFamily", C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int,
+                                          C.c_int, _ip])
 _sig("dmrGetStagnationCount", C.c_int, [_vp])
 _sig("dmrSetStagnationCount", C.c_int, [_vp, C.c_int])
 _sig("dmrBlockCreate", C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int])
@@ -148,7 +150,8 @@ HEADER_SYMBOLS = (
     "dmrBlockCreate dmrBlockDestroy dmrBlockSetOperatorCsr dmrBlockSetOperatorValues "
     "dmrBlockSetRefinement dmrBlockSolve dmrBlockGetStats dmrBlockHasZ dmrBlockImportZ "
     "dmrBlockExportZ dmrBlockSetProfiling dmrBlockGetProfile dmrBlockFootprint "
-    "dmrServerCreate dmrServerAddFamily dmrServerSetRefinement dmrServerSubmit dmrServerPoll "
+    "dmrServerCreate dmrServerAddFamily dmrServerAddBlockFamily dmrServerSetRefinement "
+    "dmrServerSubmit dmrServerPoll "
     "dmrServerWait dmrServerDrain dmrServerNow dmrServerStats dmrServerDestroy "
     "initLS freeLS impImgLS expImgLS impSrcLS expSrcLS impOptLS expOptLS solveLS linOptLS "
     "impCsrLS nnzLS dmrReadMatrixMarket dmrWriteMatrixMarket impMtxLS dmrGenConvDiff2D dmrGenConvDiff3D dmrGenAniso3D dmrRandNormal "
@@ -488,7 +491,7 @@ class RequestResult(C.Structure):
     _fields_ = [("id", C.c_longlong), ("family", C.c_int), ("nrhs", C.c_int), ("status", C.c_int),
                 ("cycles", C.c_int), ("slot", C.c_int), ("recycled", C.c_int),
                 ("maxRelRes", C.c_double), ("submitTime", C.c_double),
-                ("startTime", C.c_double), ("endTime", C.c_double)]
+                ("startTime", C.c_double), ("endTime", C.c_double), ("batch", C.c_int)]
 
 
 _sig("dmrServerPoll", C.c_int, [_vp, C.POINTER(RequestResult), C.c_int])
@@ -509,6 +512,14 @@ class Server:
         fam = C.c_int()
         _check(lib.dmrServerAddFamily(self.h, n, nnz, rowptr_ptr, col_ptr, m, k, precision,
                                       C.byref(fam)), "dmrServerAddFamily")
+        self._keep[("family", fam.value)] = keep
+        return fam.value
+
+    def add_block_family(self, n, nnz, rowptr_ptr, col_ptr, block_steps, k, max_rhs,
+                         keep=()) -> int:
+        fam = C.c_int()
+        _check(lib.dmrServerAddBlockFamily(self.h, n, nnz, rowptr_ptr, col_ptr, block_steps, k,
+                                           max_rhs, C.byref(fam)), "dmrServerAddBlockFamily")
         self._keep[("family", fam.value)] = keep
         return fam.value
 

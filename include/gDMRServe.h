@@ -15,7 +15,8 @@
  * when the device has its footprint (dmrFootprint) plus hbmReserve free; while it does not, the
  * least recently used idle family is evicted (its subspace and stagnation counter move to host
  * memory and come back, bit for bit, when it next runs).  A family's results therefore depend on
- * its own request sequence only: they are bitwise the same served alone or in any mix.
+ * its own request sequence only: they are bitwise the same served alone or in any mix (block
+ * families batch queued requests into one launch; see dmrServerAddBlockFamily).
  *
  * All device pointers stay owned by the caller and must stay valid until the request's result
  * has been returned.  The server's streams do not wait on the caller's: a request's values,
@@ -45,6 +46,7 @@ typedef struct {
   double submitTime;   /* server clock (s) at submission                         */
   double startTime;    /* server clock (s) when a slot admitted it               */
   double endTime;      /* server clock (s) at completion                         */
+  int batch;           /* right-hand sides in the launch that solved it (>= nrhs) */
 } dmrRequestResult;
 
 /* slots: worker threads (concurrent requests).  hbmReserve: bytes of HBM kept free. */
@@ -52,6 +54,15 @@ int dmrServerCreate(dmrServerHandle *s, int device, int slots, size_t hbmReserve
 /* rowPtr / colInd: device CSR pattern (int32, n+1 / nnz), shared by all requests of the family. */
 int dmrServerAddFamily(dmrServerHandle s, int n, int nnz, const int *rowPtr, const int *colInd,
                        int basisDim, int deflatDim, int precision, int *family);
+/* A block family: its requests run on a block DMR handle (dmrBlockCreate: FP32 block Krylov
+ * space inside FP64 refinement, recycled subspace of k <= 32 vectors) and the requests of the
+ * family that are queued when a slot takes it are solved together as one block of up to maxRhs
+ * right-hand sides (continuous batching: requests that arrive while the family runs join its
+ * next launch).  A batched result depends on which requests shared its launch, so it agrees with
+ * the same request served alone to the solve tolerance, not bit for bit. */
+int dmrServerAddBlockFamily(dmrServerHandle s, int n, int nnz, const int *rowPtr,
+                            const int *colInd, int blockSteps, int deflatDim, int maxRhs,
+                            int *family);
 /* Refinement settings of a mixed-precision family (dmrSetRefinement), before its first request. */
 int dmrServerSetRefinement(dmrServerHandle s, int family, double innerTol, int maxOuter);
 /* val: device values (nnz doubles) of this request's operator; b[i], x[i] (i < nrhs): device

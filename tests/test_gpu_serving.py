@@ -5,7 +5,8 @@ The invariant under test is the serving rule: a family's results depend on its o
 sequence only.  Each family's requests are served once alone (one slot, no other family) and once
 in a mix (several slots, three families interleaved, one of them mixed precision); the solutions
 must agree bit for bit, and each must meet its tolerance in a true FP64 residual.  Eviction moves a
-family's recycled state to the host and back; that must not change a bit either.
+family's recycled state to the host and back; that must not change a bit either.  Block families
+batch queued requests into one launch; there the bar is agreement within a stated tolerance.
 """
 import numpy as np
 import pytest
@@ -168,3 +169,76 @@ def test_eviction_keeps_the_chain(torch_gpu):
         for a, b in zip(alone[i][1], mix[i][1]):
             assert np.array_equal(a, b), f"request {i} changed by eviction"
         assert alone[i][0].cycles == mix[i][0].cycles
+
+
+XTOL_BATCH = 1e-6  # relative solution agreement, alone vs batched (both at 1e-10 residual)
+
+
+def test_block_family_batches_within_tolerance(torch_gpu):
+    """Block families solve queued requests that share an operator as one block launch
+    (continuous batching).  Served alone (one request at a time: no batching) and in a mix (every
+    request of three families queued at once: requests batch), every solution meets its tolerance
+    in FP64 and the alone and mixed solutions agree to XTOL_BATCH."""
+    torch = torch_gpu
+    tol = 1e-10
+    A1 = gdmr.conv_diff_3d(24, jitter=0.01, seed=51, stream=0)
+    A2 = [gdmr.aniso_3d(20, jitter=0.01, seed=52, stream=t) for t in range(2)]
+    A3 = gdmr.conv_diff_2d(64, jitter=0.01, seed=53, stream=0)
+    pats = [(A1, _dev(torch, A1.rowptr), _dev(torch, A1.col)),
+            (A2[0], _dev(torch, A2[0].rowptr), _dev(torch, A2[0].col)),
+            (A3, _dev(torch, A3.rowptr), _dev(torch, A3.col))]
+    vals = [[_dev(torch, A1.val)], [_dev(torch, A.val) for A in A2], [_dev(torch, A3.val)]]
+    mats = [[A1], A2, [A3]]
+    reqs = []  # (family, value draw, host RHS list)
+    for q, nr in enumerate((1, 3, 2, 1, 2, 3)):
+        reqs.append((0, 0, [gdmr.rand_normal(60, 10 * q + j, A1.n) for j in range(nr)]))
+    for q in range(4):
+        reqs.append((1, q % 2, [gdmr.rand_normal(61, 10 * q + j, A2[0].n) for j in range(2)]))
+    for q in range(2):
+        reqs.append((2, 0, [gdmr.rand_normal(62, q, A3.n)]))
+    bdev = [[_dev(torch, b) for b in r[2]] for r in reqs]
+
+    def serve(slots, one_by_one):
+        srv = gdmr.Server(slots=slots, hbm_reserve=1 << 28)
+        try:
+            fams = [srv.add_block_family(A1.n, A1.nnz, pats[0][1].data_ptr(), pats[0][2].data_ptr(),
+                                         6, 16, 16),
+                    srv.add_block_family(A2[0].n, A2[0].nnz, pats[1][1].data_ptr(),
+                                         pats[1][2].data_ptr(), 6, 8, 8),
+                    srv.add_family(A3.n, A3.nnz, pats[2][1].data_ptr(), pats[2][2].data_ptr(), 30,
+                                   10)]
+            xs, ids, results = {}, {}, []
+            for i, (f, d, B) in enumerate(reqs):
+                xs[i] = [torch.zeros(len(b), dtype=torch.float64, device="cuda") for b in B]
+                torch.cuda.synchronize()
+                rid = srv.submit(fams[f], vals[f][d].data_ptr(), [b.data_ptr() for b in bdev[i]],
+                                 [x.data_ptr() for x in xs[i]], tol, 100000)
+                ids[rid] = i
+                if one_by_one:
+                    srv.drain()
+            srv.drain()
+            results = srv.wait(timeout=0.0, cap=64)
+        finally:
+            srv.close()
+        assert len(results) == len(reqs)
+        return {ids[r.id]: (r, [x.cpu().numpy() for x in xs[ids[r.id]]]) for r in results}
+
+    alone = serve(1, True)
+    mix = serve(3, False)
+    batched = 0
+    for i, (f, d, B) in enumerate(reqs):
+        ra, xa = alone[i]
+        rm, xm = mix[i]
+        assert ra.status == 0 and rm.status == 0, (i, ra.status, rm.status)
+        assert ra.batch == ra.nrhs
+        batched += rm.batch > rm.nrhs
+        A = mats[f][d]
+        for b, a, m in zip(B, xa, xm):
+            for x in (a, m):
+                tr = np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b)
+                assert tr <= tol * 1.05, (i, tr)
+            err = np.linalg.norm(a - m) / np.linalg.norm(a)
+            print(f"request {i} (family {f}, batch {rm.batch}): alone vs mix {err:.2e}",
+                  flush=True)
+            assert err <= XTOL_BATCH, (i, err)
+    assert batched > 0

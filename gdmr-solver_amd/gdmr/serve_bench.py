@@ -14,7 +14,7 @@ measures capacity (requests/s with the queue never empty); then an open loop of 
 Poisson arrivals at rate lambda (default 0.8 x capacity).  Latency = completion - submission on
 the server clock.  Solutions of a sample of requests are checked on the host in FP64.
 
-Run: python -m gdmr.serve_bench [--requests N] [--rate R] [--slots S] [--prec fp64|mixed]
+Run: python -m gdmr.serve_bench [--requests N] [--rate R] [--slots S] [--prec fp64|mixed] [--block]
 """
 from __future__ import annotations
 
@@ -65,10 +65,17 @@ class Load:
         self.srv = gdmr.Server(slots=args.slots)
         self.fam_id = []
         for f in self.fams:
-            fid = self.srv.add_family(f["n"], f["nnz"], f["rp"].data_ptr(), f["ci"].data_ptr(),
-                                      args.m, args.k, prec, keep=(f["rp"], f["ci"]))
-            if prec != gdmr.PREC_FP64:
-                self.srv.set_refinement(fid, args.inner_tol, 60)
+            if args.block:
+                fid = self.srv.add_block_family(f["n"], f["nnz"], f["rp"].data_ptr(),
+                                                f["ci"].data_ptr(), args.block_steps, args.k,
+                                                args.block_rhs, keep=(f["rp"], f["ci"]))
+                self.srv.set_refinement(fid, args.inner_tol, 30)
+            else:
+                fid = self.srv.add_family(f["n"], f["nnz"], f["rp"].data_ptr(),
+                                          f["ci"].data_ptr(), args.m, args.k, prec,
+                                          keep=(f["rp"], f["ci"]))
+                if prec != gdmr.PREC_FP64:
+                    self.srv.set_refinement(fid, args.inner_tol, 60)
             self.fam_id.append(fid)
         self.rng = np.random.default_rng(args.seed)
         self.live = {}  # id -> (spec, xs)
@@ -134,6 +141,11 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--check", type=int, default=2, help="requests per family checked in FP64")
     ap.add_argument("--small", action="store_true", help="small families (quick runs)")
+    ap.add_argument("--block", action="store_true",
+                    help="block families: multi-RHS requests as one block solve, and queued "
+                         "requests with the same operator batched into one launch")
+    ap.add_argument("--block-steps", type=int, default=10)
+    ap.add_argument("--block-rhs", type=int, default=16)
     args = ap.parse_args(argv)
     import torch
     assert torch.cuda.is_available(), "the serving benchmark needs a HIP device"
@@ -174,6 +186,7 @@ def main(argv=None):
             _log(f"open loop: {len(L.done)}/{args.requests} done, queue {srv.stats()['queued']}")
     wall = time.perf_counter() - t_start
     res = [r for r, _ in L.done]
+    mean_batch = float(np.mean([r.batch for r in res])) if res else 0.0
     lat = [1e3 * (r.endTime - r.submitTime) for r in res]
     svc = [1e3 * (r.endTime - r.startTime) for r in res]
     per_fam = {}
@@ -200,14 +213,15 @@ def main(argv=None):
         "service_p50_ms": _pct(svc, 50), "service_p99_ms": _pct(svc, 99),
         "families": per_fam,
         "all_status_ok": status_ok,
+        "mean_rhs_per_launch": mean_batch,
         "checked_requests": len(L.checked),
         "max_true_rel_residual": worst,
-        "dtype": "f64" if args.prec == "fp64" else "fp32+fp64",
+        "dtype": "f64" if (args.prec == "fp64" and not args.block) else "fp32+fp64",
         "data": "synthetic",
         "config": {"workload": "serving: 3 families (2-D 512^2, 3-D 64^3, 3-D 160^3 conv-diff), "
                                f"Poisson arrivals, 1-8 RHS, m={args.m} k={args.k} "
                                f"tol {args.tol:g}" + (" [small families]" if args.small else ""),
-                   "slots": args.slots, "prec": args.prec},
+                   "slots": args.slots, "prec": "block" if args.block else args.prec},
     }
     srv.close()
     print(json.dumps(line), flush=True)

@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -67,6 +68,26 @@ struct dmrBlockContext {
   double *host = nullptr;  // pinned staging
   bool hasZ = false;
   int krMax = 64;  // cap on the leading block steps of the subspace refresh
+  // Launch shapes chosen by the autotuner (per block width s), and whether it ran.
+  struct Tuning {
+    int tnRowsProj = 512, tnVarProj = 0;  // gemm_tn for projections (p > s)
+    int tnRowsGram = 512, tnVarGram = 0;  // gemm_tn for Gram matrices (p = s)
+    int nnUnroll = 4;                     // gemm_nn
+    int spmmCols = 8;                     // SpMM columns per thread
+    int s = 0;
+  } tune;
+  bool autotune = true;
+  // Captured restart cycles, keyed by (s, deflated, which buffer holds Z).
+  bool graphs = true;
+  hipStream_t gst = nullptr;
+  struct Cap {
+    int s, deflate, parity;
+    hipGraphExec_t exec;
+    bool refreshed;
+    int matvecs;
+  };
+  std::vector<Cap> caps;
+  float *zbuf0 = nullptr;
   double innerTol = 1e-5;
   int maxOuter = 30;
   dmrStats st{};
@@ -161,10 +182,17 @@ Sizes sizes(long long n, int mb, int k, int sMax, int G, long long ld) {
   return z;
 }
 
+void drop_graphs(dmrBlockContext *c) {
+  for (auto &g : c->caps) hipGraphExecDestroy(g.exec);
+  c->caps.clear();
+}
+
 void ctx_release(dmrBlockContext *c) {
   if (!c) return;
   hipSetDevice(c->device);
   hipDeviceSynchronize();
+  drop_graphs(c);
+  if (c->gst) hipStreamDestroy(c->gst);
   if (c->big) hipFree(c->big);
   if (c->small) hipFree(c->small);
   if (c->ownedVal32) hipFree(c->ownedVal32);
@@ -233,20 +261,141 @@ int ctx_alloc(dmrBlockContext *c) {
   return DMR_OK;
 }
 
+int tn_blocks(const dmrBlockContext *c, int rowsTarget) {
+  return dmr::blk::gemm_tn_blocks(c->n, dmr::blk::kMaxTnBlocks, rowsTarget);
+}
+
+// C = P^T U with the tuned shape of its class (Gram matrix or projection).
+void tn(dmrBlockContext *c, const float *P, int p, const float *U, int s, double *C64, int ldc,
+        float *C32, bool gram, hipStream_t st) {
+  const auto &t = c->tune;
+  dmr::blk::gemm_tn(P, c->ld, p, U, c->ld, s, c->n,
+                    tn_blocks(c, gram ? t.tnRowsGram : t.tnRowsProj),
+                    gram ? t.tnVarGram : t.tnVarProj, c->part, C64, ldc, C32, st);
+}
+
+void nn(dmrBlockContext *c, const float *P, int p, const float *C, const float *In, float a,
+        float b, float *Out, int s, hipStream_t st) {
+  dmr::blk::gemm_nn(P, c->ld, p, C, In, c->ld, a, b, Out, c->ld, s, c->n, c->tune.nnUnroll, st);
+}
+
+void spmm(dmrBlockContext *c, const float *X, float *Y, int s, hipStream_t st) {
+  dmr::blk::spmm32(c->csr, X, c->ld, Y, c->ld, s, c->tune.spmmCols, st);
+}
+
+// Process-wide table of autotuned shapes, keyed by the problem shape.
+struct TuneKey {
+  long long n, nnz;
+  int s, k, mb;
+  bool operator==(const TuneKey &o) const {
+    return n == o.n && nnz == o.nnz && s == o.s && k == o.k && mb == o.mb;
+  }
+};
+std::mutex g_tuneMu;
+std::vector<std::pair<TuneKey, dmrBlockContext::Tuning>> g_tuned;
+
+// Times each candidate launch shape on the handle's own buffers (HIP events, best of 3 after a
+// warm-up launch) and keeps the fastest: gemm_tn rows per workgroup and variant, separately for
+// projections over the full basis and for s x s Gram matrices; the gemm_nn load-ahead depth; the
+// SpMM columns per thread.  Results are cached per shape for the process, so every handle of a
+// shape launches the same kernels (and computes the same bits).  The buffers it writes (Xa, the
+// partials) are re-initialised by the solve that follows.
+void autotune(dmrBlockContext *c, int s, hipStream_t st) {
+  const TuneKey key{c->n, c->csr.nnz, s, c->k, c->mb};
+  {
+    std::lock_guard<std::mutex> lk(g_tuneMu);
+    for (auto &e : g_tuned)
+      if (e.first == key) {
+        c->tune = e.second;
+        return;
+      }
+  }
+  dmrBlockContext::Tuning t;
+  t.s = s;
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  auto best_of = [&](auto &&launch) {
+    launch();
+    float best = 1e30f;
+    for (int r = 0; r < 3; ++r) {
+      hipEventRecord(e0, st);
+      launch();
+      hipEventRecord(e1, st);
+      hipEventSynchronize(e1);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      best = std::min(best, ms);
+    }
+    return best;
+  };
+  const int pFull = c->k + (c->mb + 1) * s;
+  const int rowsCand[] = {256, 512, 1024, 2048};
+  float bp = 1e30f, bg = 1e30f;
+  for (int rows : rowsCand)
+    for (int var = 0; var < 2; ++var) {
+      const int G = tn_blocks(c, rows);
+      const float tp = best_of([&] {
+        dmr::blk::gemm_tn(c->P, c->ld, pFull, c->R, c->ld, s, c->n, G, var, c->part, nullptr, 0,
+                          c->C32, st);
+      });
+      if (tp < bp) {
+        bp = tp;
+        t.tnRowsProj = rows;
+        t.tnVarProj = var;
+      }
+      const float tg = best_of([&] {
+        dmr::blk::gemm_tn(c->R, c->ld, s, c->R, c->ld, s, c->n, G, var, c->part, c->G64, s,
+                          nullptr, st);
+      });
+      if (tg < bg) {
+        bg = tg;
+        t.tnRowsGram = rows;
+        t.tnVarGram = var;
+      }
+    }
+  float bn = 1e30f;
+  for (int un : {1, 4, 8}) {
+    const float tt = best_of([&] {
+      dmr::blk::gemm_nn(c->P, c->ld, pFull, c->C32, c->Xa, c->ld, 1.0f, 0.0f, c->Xa, c->ld, s,
+                        c->n, un, st);
+    });
+    if (tt < bn) {
+      bn = tt;
+      t.nnUnroll = un;
+    }
+  }
+  float bs = 1e30f;
+  for (int cg : {4, 8, 16}) {
+    const float tt = best_of([&] {
+      dmr::blk::spmm32(c->csr, c->R, c->ld, c->Xa, c->ld, s, cg, st);
+    });
+    if (tt < bs) {
+      bs = tt;
+      t.spmmCols = cg;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  c->tune = t;
+  std::lock_guard<std::mutex> lk(g_tuneMu);
+  g_tuned.emplace_back(key, t);
+}
+
 // Two Cholesky QR passes on the sc columns of U (in place).  The combined triangular factor goes
 // to Hdst (ld ldh) when given.
 void cholqr2(dmrBlockContext *c, float *U, int sc, double *Hdst, int ldh, hipStream_t st) {
-  const long long n = c->n, ld = c->ld;
+  const long long n = c->n;
   for (int pass = 0; pass < 2; ++pass) {
     timedf(c, st, DMR_PH_GEMVT, 4.0 * n * sc, 2.0 * n * sc * sc, [&] {
-      dmr::blk::gemm_tn(U, ld, sc, U, ld, sc, n, c->G, c->part, c->G64, sc, nullptr, st);
+      tn(c, U, sc, U, sc, c->G64, sc, nullptr, true, st);
     });
     timed(c, st, DMR_PH_ORTH, 0.0, [&] {
       dmr::blk::chol_qr(c->G64, sc, kDepTol, c->Sinv32, c->Sacc, pass == 0,
                         pass == 1 ? Hdst : nullptr, ldh, c->flags, st);
     });
     timedf(c, st, DMR_PH_UPDATE, 4.0 * n * 2 * sc, 2.0 * n * sc * sc, [&] {
-      dmr::blk::gemm_nn(U, ld, sc, c->Sinv32, nullptr, 0, 0.0f, 1.0f, U, ld, sc, n, st);
+      nn(c, U, sc, c->Sinv32, nullptr, 0.0f, 1.0f, U, sc, st);
     });
   }
 }
@@ -259,7 +408,7 @@ double spmm_bytes(const dmrBlockContext *c, int s) {
 // refreshed subspace was written to c->Zn.
 bool cycle(dmrBlockContext *c, int s, bool deflate, hipStream_t st) {
   const long long n = c->n, ld = c->ld;
-  const int k = c->k, mb = c->mb, G = c->G;
+  const int k = c->k, mb = c->mb;
   const int p0 = deflate ? k : 0;
   const int rows = p0 + (mb + 1) * s, cols = p0 + mb * s;
   float *P = c->P;
@@ -268,17 +417,17 @@ bool cycle(dmrBlockContext *c, int s, bool deflate, hipStream_t st) {
   dmr::blk::zero_ints(c->flags, 2, st);
   if (deflate) {
     timedf(c, st, DMR_PH_SPMV, spmm_bytes(c, k), 2.0 * c->csr.nnz * k,
-          [&] { dmr::blk::spmm32(c->csr, c->Z, ld, P, ld, k, st); });
+          [&] { spmm(c, c->Z, P, k, st); });
     c->st.matvecs += k;
     cholqr2(c, P, k, c->H, rows, st);
   }
   float *V0 = P + (long long)p0 * ld;
   if (deflate) {
     timedf(c, st, DMR_PH_GEMVT, vb * (k + s), 2.0 * n * k * s, [&] {
-      dmr::blk::gemm_tn(P, ld, k, c->R, ld, s, n, G, c->part, nullptr, 0, c->C32, st);
+      tn(c, P, k, c->R, s, nullptr, 0, c->C32, false, st);
     });
     timedf(c, st, DMR_PH_UPDATE, vb * (k + 2 * s), 2.0 * n * k * s, [&] {
-      dmr::blk::gemm_nn(P, ld, k, c->C32, c->R, ld, 1.0f, -1.0f, V0, ld, s, n, st);
+      nn(c, P, k, c->C32, c->R, 1.0f, -1.0f, V0, s, st);
     });
   } else {
     hipMemcpyAsync(V0, c->R, sizeof(float) * ld * s, hipMemcpyDeviceToDevice, st);
@@ -290,35 +439,34 @@ bool cycle(dmrBlockContext *c, int s, bool deflate, hipStream_t st) {
     const int pj = p0 + (j + 1) * s;
     double *Hcol = c->H + (long long)(p0 + j * s) * rows;
     timedf(c, st, DMR_PH_SPMV, spmm_bytes(c, s), 2.0 * c->csr.nnz * s,
-          [&] { dmr::blk::spmm32(c->csr, Vj, ld, U, ld, s, st); });
+          [&] { spmm(c, Vj, U, s, st); });
     c->st.matvecs += s;
     timedf(c, st, DMR_PH_GEMVT, vb * (pj + s), 2.0 * n * pj * s, [&] {
-      dmr::blk::gemm_tn(P, ld, pj, U, ld, s, n, G, c->part, Hcol, rows, c->C32, st);
+      tn(c, P, pj, U, s, Hcol, rows, c->C32, false, st);
     });
     timedf(c, st, DMR_PH_UPDATE, vb * (pj + 2 * s), 2.0 * n * pj * s, [&] {
-      dmr::blk::gemm_nn(P, ld, pj, c->C32, U, ld, 1.0f, -1.0f, U, ld, s, n, st);
+      nn(c, P, pj, c->C32, U, 1.0f, -1.0f, U, s, st);
     });
     cholqr2(c, U, s, Hcol + pj, rows, st);
   }
   // Full projection of the residual block, least squares, residual and solution updates.
   timedf(c, st, DMR_PH_GEMVT, vb * (rows + s), 2.0 * n * rows * s, [&] {
-    dmr::blk::gemm_tn(P, ld, rows, c->R, ld, s, n, G, c->part, c->T, rows, nullptr, st);
+    tn(c, P, rows, c->R, s, c->T, rows, nullptr, false, st);
   });
   timed(c, st, DMR_PH_LSQ, 0.0, [&] {
     dmr::blk::lsq_block(c->H, rows, cols, s, c->T, s, c->rho, c->Hw, c->Tw, c->Y, c->Yr32,
                         c->ZH32, st);
   });
   timedf(c, st, DMR_PH_COMBO, vb * (rows + 2 * s), 2.0 * n * rows * s, [&] {
-    dmr::blk::gemm_nn(P, ld, rows, c->ZH32, c->R, ld, 1.0f, -1.0f, c->R, ld, s, n, st);
+    nn(c, P, rows, c->ZH32, c->R, 1.0f, -1.0f, c->R, s, st);
   });
   timed(c, st, DMR_PH_SCALE, vb * s,
         [&] { dmr::blk::colnorms<float>(c->R, ld, s, n, c->cpart, c->q, 1, st); });
   timedf(c, st, DMR_PH_COMBO, vb * ((deflate ? k : 0) + mb * s + 2 * s * (deflate ? 2 : 1)),
          2.0 * n * ((deflate ? k : 0) + mb * s) * s, [&] {
     if (deflate)
-      dmr::blk::gemm_nn(c->Z, ld, k, c->Yr32, c->Xa, ld, 1.0f, 1.0f, c->Xa, ld, s, n, st);
-    dmr::blk::gemm_nn(V0, ld, mb * s, c->Yr32 + (long long)p0 * 32, c->Xa, ld, 1.0f, 1.0f,
-                      c->Xa, ld, s, n, st);
+      nn(c, c->Z, k, c->Yr32, c->Xa, 1.0f, 1.0f, c->Xa, s, st);
+    nn(c, V0, mb * s, c->Yr32 + (long long)p0 * 32, c->Xa, 1.0f, 1.0f, c->Xa, s, st);
   });
   // Subspace refresh from the leading block of H over [Z V_0 .. V_{kr-1}].
   bool refreshed = false;
@@ -340,12 +488,10 @@ bool cycle(dmrBlockContext *c, int s, bool deflate, hipStream_t st) {
         });
         timed(c, st, DMR_PH_ZGEMM, vb * (cR + k + 2 * k), [&] {
           if (deflate) {
-            dmr::blk::gemm_nn(c->Z, ld, k, c->Cz32, nullptr, 0, 0.0f, 1.0f, c->Zn, ld, k, n, st);
-            dmr::blk::gemm_nn(V0, ld, kr * s, c->Cz32 + (long long)p0 * 32, c->Zn, ld, 1.0f,
-                              1.0f, c->Zn, ld, k, n, st);
+            nn(c, c->Z, k, c->Cz32, nullptr, 0.0f, 1.0f, c->Zn, k, st);
+            nn(c, V0, kr * s, c->Cz32 + (long long)p0 * 32, c->Zn, 1.0f, 1.0f, c->Zn, k, st);
           } else {
-            dmr::blk::gemm_nn(V0, ld, kr * s, c->Cz32, nullptr, 0, 0.0f, 1.0f, c->Zn, ld, k, n,
-                              st);
+            nn(c, V0, kr * s, c->Cz32, nullptr, 0.0f, 1.0f, c->Zn, k, st);
           }
           dmr::blk::colnorms<float>(c->Zn, ld, k, n, c->cpart, c->zq, 1, st);
           dmr::blk::recip(c->zq, c->zq, k, nullptr, st);
@@ -361,6 +507,35 @@ bool cycle(dmrBlockContext *c, int s, bool deflate, hipStream_t st) {
   return refreshed;
 }
 
+// One restart cycle as a replay of its captured graph.  The cycle's launch sequence depends only
+// on (s, deflated, which of the two subspace buffers holds Z) once the shapes are tuned, so each
+// such variant is captured once (on the handle's own stream, in thread-local capture mode) and
+// replayed on the caller's stream.
+int cycle_graph(dmrBlockContext *c, int s, bool deflate, hipStream_t st, bool *refreshed) {
+  const int parity = c->Z == c->zbuf0 ? 0 : 1;
+  const dmrBlockContext::Cap *cap = nullptr;
+  for (const auto &g : c->caps)
+    if (g.s == s && g.deflate == (int)deflate && g.parity == parity) cap = &g;
+  if (!cap) {
+    const int mv0 = c->st.matvecs;
+    BCHK(hipStreamBeginCapture(c->gst, hipStreamCaptureModeThreadLocal));
+    const bool r = cycle(c, s, deflate, c->gst);
+    hipGraph_t g = nullptr;
+    BCHK(hipStreamEndCapture(c->gst, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    BCHK(e);
+    c->caps.push_back({s, (int)deflate, parity, ex, r, c->st.matvecs - mv0});
+    c->st.matvecs = mv0;
+    cap = &c->caps.back();
+  }
+  BCHK(hipGraphLaunch(cap->exec, st));
+  c->st.matvecs += cap->matvecs;
+  *refreshed = cap->refreshed;
+  return DMR_OK;
+}
+
 int block_solve(dmrBlockContext *c, int mode, int s, double *X, long long ldx, const double *B,
                 long long ldb, hipStream_t st, double tol, int maxCycles, double *relRes) {
   const long long n = c->n, ld = c->ld;
@@ -369,6 +544,7 @@ int block_solve(dmrBlockContext *c, int mode, int s, double *X, long long ldx, c
     dmr::convert<float, double>(c->ownedVal32, c->csr.val, c->csr.nnz, st);
     c->val32Stale = false;
   }
+  if (c->autotune && c->tune.s != s) autotune(c, s, st);
   dmr::blk::colnorms<double>(B, ldb, s, n, c->cpart, c->bn, 1, st);
   BCHK(hipMemcpyAsync(hb, c->bn, sizeof(double) * s, hipMemcpyDeviceToHost, st));
   bool useZ = mode != 0 && c->hasZ;
@@ -410,7 +586,13 @@ int block_solve(dmrBlockContext *c, int mode, int s, double *X, long long ldx, c
     dmr::blk::recip(c->f, c->scal, s, c->rho, st);  // rho = 1
     for (int cyc = 0; cyc < maxCycles; ++cyc) {
       const bool deflate = c->k > 0 && c->hasZ && (useZ || cycled);
-      const bool refreshed = cycle(c, s, deflate, st);
+      bool refreshed = false;
+      if (c->graphs && !c->prof.on) {
+        const int rc = cycle_graph(c, s, deflate, st, &refreshed);
+        if (rc != DMR_OK) return rc;
+      } else {
+        refreshed = cycle(c, s, deflate, st);
+      }
       cycled = true;
       BCHK(hipMemcpyAsync(hrho, c->rho, sizeof(double) * s, hipMemcpyDeviceToHost, st));
       BCHK(hipMemcpyAsync(hf + 32, c->flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
@@ -450,7 +632,7 @@ extern "C" {
 size_t dmrBlockFootprint(int elements, int blockSteps, int deflatDim, int maxRhs) {
   if (elements < 1 || blockSteps < 1 || deflatDim < 0 || maxRhs < 1) return 0;
   const long long ld = dmr::basis_ld(elements);
-  const int G = dmr::blk::gemm_tn_blocks(elements, dmr::blk::kMaxTnBlocks, tn_rows_target());
+  const int G = dmr::blk::gemm_tn_blocks(elements, dmr::blk::kMaxTnBlocks, 256);
   const Sizes z = sizes(elements, blockSteps, deflatDim, maxRhs, G, ld);
   return z.big + z.small + 80 * 256;
 }
@@ -479,16 +661,24 @@ int dmrBlockCreate(dmrBlockHandle *h, int device, int elements, int blockSteps, 
   c->ld = dmr::basis_ld(elements);
   c->rowsMax = deflatDim + (blockSteps + 1) * maxRhs;
   c->colsMax = deflatDim + blockSteps * maxRhs;
-  c->G = dmr::blk::gemm_tn_blocks(elements, dmr::blk::kMaxTnBlocks, tn_rows_target());
+  // Partials for the most row ranges any tuned shape uses (256 rows per workgroup).
+  c->G = dmr::blk::gemm_tn_blocks(elements, dmr::blk::kMaxTnBlocks, 256);
+  c->tune.tnRowsProj = c->tune.tnRowsGram = tn_rows_target();
+  if (const char *e = std::getenv("DMR_BLOCK_GRAPHS")) c->graphs = e[0] != '0';
+  if (const char *e = std::getenv("DMR_BLOCK_TUNE")) c->autotune = e[0] != '0';
+  dmr::blk::prepare();
   if (const char *e = std::getenv("DMR_BLOCK_KR")) {
     const int v = std::atoi(e);
     if (v >= 1) c->krMax = v;
   }
-  const int rc = ctx_alloc(c);
+  int rc = ctx_alloc(c);
+  if (rc == DMR_OK && hipStreamCreateWithFlags(&c->gst, hipStreamNonBlocking) != hipSuccess)
+    rc = dmr::set_error(DMR_ERR_HIP, "stream creation failed");
   if (rc != DMR_OK) {
     ctx_release(c);
     return rc;
   }
+  c->zbuf0 = c->Z;
   *h = c;
   return DMR_OK;
 }
@@ -505,6 +695,7 @@ int dmrBlockSetOperatorCsr(dmrBlockHandle h, int nnz, const int *rowPtr, const i
   if (!h || !rowPtr || !colInd || !val || nnz < 0)
     return dmr::set_error(DMR_ERR_ARG, "invalid CSR arguments");
   BCHK(hipSetDevice(h->device));
+  drop_graphs(h);
   h->csr.n = h->n;
   h->csr.nnz = nnz;
   h->csr.rowPtr = rowPtr;
@@ -529,6 +720,7 @@ int dmrBlockSetOperatorCsr(dmrBlockHandle h, int nnz, const int *rowPtr, const i
 int dmrBlockSetOperatorValues(dmrBlockHandle h, const double *val, const float *val32) {
   dmr::clear_error();
   if (!h || !h->haveOp || !val) return dmr::set_error(DMR_ERR_STATE, "no CSR operator to update");
+  if ((val32 ? val32 : h->ownedVal32) != h->csr.val32) drop_graphs(h);
   h->csr.val = val;
   if (val32) {
     h->csr.val32 = val32;
@@ -582,6 +774,29 @@ int dmrBlockGetStats(dmrBlockHandle h, dmrStats *st, double *residHist, int cap)
 }
 
 int dmrBlockHasZ(dmrBlockHandle h) { return h && h->hasZ ? 1 : 0; }
+
+int dmrBlockSetOptions(dmrBlockHandle h, int graphs, int autotune) {
+  if (!h) return dmr::set_error(DMR_ERR_ARG, "null handle");
+  if (graphs >= 0) {
+    if (!graphs) drop_graphs(h);
+    h->graphs = graphs != 0;
+  }
+  if (autotune >= 0) h->autotune = autotune != 0;
+  return DMR_OK;
+}
+
+int dmrBlockGetTuning(dmrBlockHandle h, int *out) {
+  if (!h || !out) return dmr::set_error(DMR_ERR_ARG, "null argument");
+  const auto &t = h->tune;
+  out[0] = t.tnRowsProj;
+  out[1] = t.tnVarProj;
+  out[2] = t.tnRowsGram;
+  out[3] = t.tnVarGram;
+  out[4] = t.nnUnroll;
+  out[5] = t.spmmCols;
+  out[6] = (int)h->caps.size();
+  return DMR_OK;
+}
 
 int dmrBlockImportZ(dmrBlockHandle h, const double *hostZ) {
   dmr::clear_error();

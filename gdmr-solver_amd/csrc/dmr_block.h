@@ -19,9 +19,10 @@ constexpr int kMaxCols = 32 * kMaxTiles;
 // Cap on the row ranges (workgroups) of gemm_tn.
 constexpr int kMaxTnBlocks = 1024;
 
-// Y[:, 0:s] = A X[:, 0:s] with FP32 values and vectors, FP32 accumulation.
+// Y[:, 0:s] = A X[:, 0:s] with FP32 values and vectors, FP32 accumulation; cg (4, 8 or 16)
+// columns per thread.
 void spmm32(const CsrDev &A, const float *X, long long ldx, float *Y, long long ldy, int s,
-            hipStream_t st);
+            int cg, hipStream_t st);
 // R = B - A X in FP64 (s columns).
 void resid_block(const CsrDev &A, const double *X, long long ldx, const double *B, long long ldb,
                  double *R, long long ldr, int s, hipStream_t st);
@@ -36,11 +37,15 @@ void colnorms(const T *X, long long ld, int s, long long n, double *partial, dou
 // row-major p x 32, zero beyond s.  part: G * roundup32(p) * 32 floats.
 int gemm_tn_blocks(long long n, int cap, int rowsTarget);
 size_t gemm_tn_lds(int pPad);
+// variant 0: register-direct; 1: LDS-staged.
 void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu, int s,
-             long long n, int G, float *part, double *C64, int ldc, float *C32, hipStream_t st);
+             long long n, int G, int variant, float *part, double *C64, int ldc, float *C32,
+             hipStream_t st);
 // Out = a In + b P[:, 0:p] C (C row-major p x 32; In unused when a == 0).  Out may alias In or P.
+// un (1, 4 or 8): column pairs of P loaded ahead of their MFMAs.
 void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *In, long long ldi,
-             float a, float b, float *Out, long long ldo, int s, long long n, hipStream_t st);
+             float a, float b, float *Out, long long ldo, int s, long long n, int un,
+             hipStream_t st);
 // Cholesky QR step on the Gram matrix G (s x s, ld s): see k_chol_qr.
 void chol_qr(const double *G, int s, double relTol, float *Sinv32, double *Sacc, int first,
              double *Hdst, int ldh, int *brk, hipStream_t st);
@@ -59,6 +64,8 @@ void scale_to32(const double *R, long long ldr, const double *f, float *D, long 
                 long long n, hipStream_t st);
 // out[j] = 1 / q[j] (0 where q[j] = 0); rho[j] = 1 when rho is given (s <= 64).
 void recip(const double *q, double *out, int s, double *rho, hipStream_t st);
+// Sets the LDS limits of the one-workgroup kernels; call before any stream capture.
+void prepare();
 // p[0..cnt) = 0 (cnt <= 64), a kernel rather than a memset node.
 void zero_ints(int *p, int cnt, hipStream_t st);
 

@@ -41,30 +41,31 @@ __device__ inline f32x16 mfma32(float a, float b, f32x16 c) {
 __device__ inline int mfma_row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
 
 // ---------------------------------------------------------------------------------------------
-template <typename TV, typename TX, typename TY, typename TA>
+// Thread (row, column group): blockIdx.y selects a group of CG columns, so one launch has
+// ceil(s / CG) threads per row and each thread reuses its row's nonzeros across CG columns.
+template <typename TV, typename TX, typename TY, typename TA, int CG>
 __global__ __launch_bounds__(kThreads) void k_spmm(int n, const int *__restrict__ rowPtr,
                                                    const int *__restrict__ colInd,
                                                    const TV *__restrict__ val,
                                                    const TX *__restrict__ X, long long ldx,
                                                    TY *__restrict__ Y, long long ldy, int s) {
   const int row = blockIdx.x * kThreads + threadIdx.x;
+  const int c0 = blockIdx.y * CG;
   if (row >= n) return;
   const int b = rowPtr[row], e = rowPtr[row + 1];
-  for (int c0 = 0; c0 < s; c0 += 8) {
-    TA acc[8];
+  TA acc[CG];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) acc[c] = (TA)0;
-    for (int q = b; q < e; ++q) {
-      const long long col = colInd[q];
-      const TA v = (TA)val[q];
+  for (int c = 0; c < CG; ++c) acc[c] = (TA)0;
+  for (int q = b; q < e; ++q) {
+    const long long col = colInd[q];
+    const TA v = (TA)val[q];
 #pragma unroll
-      for (int c = 0; c < 8; ++c)
-        if (c0 + c < s) acc[c] = fma(v, (TA)X[col + (c0 + c) * ldx], acc[c]);
-    }
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-      if (c0 + c < s) Y[row + (c0 + c) * ldy] = (TY)acc[c];
+    for (int c = 0; c < CG; ++c)
+      if (c0 + c < s) acc[c] = fma(v, (TA)X[col + (c0 + c) * ldx], acc[c]);
   }
+#pragma unroll
+  for (int c = 0; c < CG; ++c)
+    if (c0 + c < s) Y[row + (c0 + c) * ldy] = (TY)acc[c];
 }
 
 // R = B - A X in FP64.
@@ -99,39 +100,47 @@ __global__ __launch_bounds__(kThreads) void k_resid_block(int n, const int *__re
 // ---------------------------------------------------------------------------------------------
 // Column sums of squares, fixed order: block g sums rows [g*rows, (g+1)*rows) of every column
 // (each thread in row order, then the block's threads in a fixed tree), partial[g * s + c].
+// Block (g, c) = (blockIdx.x, blockIdx.y) sums rows [g*rows, (g+1)*rows) of column c: each thread
+// in row order, then a wavefront xor butterfly, then the four wavefronts in order.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_colsumsq(const T *__restrict__ X, long long ld,
                                                        int s, long long n, long long rows,
                                                        double *__restrict__ partial) {
-  __shared__ double red[kThreads];
+  __shared__ double red[kThreads / 64];
+  const int c = blockIdx.y;
   const long long r0 = (long long)blockIdx.x * rows;
   const long long r1 = r0 + rows < n ? r0 + rows : n;
-  for (int c = 0; c < s; ++c) {
-    double acc = 0.0;
-    const T *col = X + c * ld;
-    for (long long r = r0 + threadIdx.x; r < r1; r += kThreads) {
-      const double x = (double)col[r];
-      acc = fma(x, x, acc);
-    }
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = kThreads / 2; w > 0; w >>= 1) {
-      if (threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[(long long)blockIdx.x * s + c] = red[0];
-    __syncthreads();
+  double acc = 0.0;
+  const T *col = X + c * ld;
+  for (long long r = r0 + threadIdx.x; r < r1; r += kThreads) {
+    const double x = (double)col[r];
+    acc = fma(x, x, acc);
   }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    partial[(long long)blockIdx.x * s + c] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-// out[c] = sum_g partial[g * s + c] (in order); mode 0: out = ss; mode 1: out = sqrt(ss).
-__global__ void k_colsumsq_final(const double *__restrict__ partial, int G, int s,
-                                 double *__restrict__ out, int mode) {
-  const int c = threadIdx.x;
-  if (c >= s) return;
+// out[c] = sum_g partial[g * s + c]: thread (c, q) = (t % 32, t / 32) sums g = q, q + 8, ... in
+// order, then the eight sums in order of q.  mode 0: out = ss; mode 1: out = sqrt(ss).
+__global__ __launch_bounds__(kThreads) void k_colsumsq_final(const double *__restrict__ partial,
+                                                             int G, int s,
+                                                             double *__restrict__ out, int mode) {
+  __shared__ double red[kThreads];
+  const int t = threadIdx.x, c = t & 31, q = t >> 5;
   double a = 0.0;
-  for (int g = 0; g < G; ++g) a = a + partial[(long long)g * s + c];
-  out[c] = mode ? sqrt(a) : a;
+  if (c < s)
+    for (int g = q; g < G; g += 8) a = a + partial[(long long)g * s + c];
+  red[t] = a;
+  __syncthreads();
+  if (q != 0 || c >= s) return;
+  double sum = 0.0;
+#pragma unroll
+  for (int w = 0; w < 8; ++w) sum = sum + red[w * 32 + c];
+  out[c] = mode ? sqrt(sum) : sum;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -330,7 +339,9 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn_reduce(const float *__rest
 
 // ---------------------------------------------------------------------------------------------
 // Out[r, j] = a In[r, j] + b sum_k P[r, k] C[k][j] for j < s, r < n.  C: p x 32 row-major.
-// Out may alias In or P (each row depends on its own row only).
+// Out may alias In or P (each row depends on its own row only).  UN: column pairs of P whose
+// loads are issued ahead of their MFMAs.
+template <int UN>
 __global__ __launch_bounds__(kThreads) void k_gemm_nn(const float *P, long long ldp, int p,
                                                       const float *__restrict__ C,
                                                       const float *In, long long ldi, float a,
@@ -349,6 +360,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nn(const float *P, long long 
   for (int v = 0; v < 16; ++v) acc0[v] = acc1[v] = acc2[v] = acc3[v] = 0.0f;
   const long long rl = r0 + 4 * (lane & 31);
   const int kh = lane >> 5;
+#pragma unroll UN
   for (int c = 0; c < p; c += 2) {
     const int cc = c + kh;
     float x0 = 0.f, x1 = 0.f, x2 = 0.f, x3 = 0.f;
@@ -404,76 +416,106 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nn(const float *P, long long 
 }
 
 // ---------------------------------------------------------------------------------------------
+// Broadcast of lane `l`'s value to the wavefront (l uniform).
+__device__ inline double bcast(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
 // Cholesky QR factor of the Gram matrix G (s x s, column-major ld s, FP64): G = S^T S with S
 // upper triangular.  A pivot below relTol * max diag marks the column dependent: its row of S
 // and its column of S^-1 are zero (the normalised block gets a zero column) and *brk is set.
 // Writes Sinv (FP32 row-major s x 32 for gemm_nn) and the combined factor Sacc <- S * Sacc
 // (Sacc upper triangular s x s, ld s; S itself on the first pass), optionally copied into the
-// block Hessenberg at Hdst (ld ldh).  1024 threads, thread (i, j) = (t % 32, t / 32) owns element
-// (i, j) in every phase.
-__global__ __launch_bounds__(1024) void k_chol_qr(const double *__restrict__ Gin, int s,
-                                                  double relTol, float *__restrict__ Sinv32,
-                                                  double *__restrict__ Sacc, int first,
-                                                  double *__restrict__ Hdst, int ldh,
-                                                  int *__restrict__ brk) {
-  __shared__ double G[32 * 33];   // element (i, j) at j * 33 + i; becomes S (upper)
-  __shared__ double X[32 * 33];   // S^-1
-  __shared__ double A2[32 * 33];  // previous Sacc
-  __shared__ int dep[32];
-  __shared__ double s_dmax;
-  const int t = threadIdx.x, i = t & 31, j = t >> 5;
-  const bool in = i < s && j < s;
-  G[j * 33 + i] = in ? Gin[j * s + i] : 0.0;
-  X[j * 33 + i] = 0.0;
-  if (!first) A2[j * 33 + i] = in ? Sacc[j * s + i] : 0.0;
-  __syncthreads();
-  if (t == 0) {
-    double d = 0.0;
-    for (int c = 0; c < s; ++c) d = fmax(d, G[c * 33 + c]);
-    s_dmax = d;
-  }
-  __syncthreads();
-  const double dmax = s_dmax;
-  for (int k = 0; k < s; ++k) {
-    const double d = G[k * 33 + k];
-    const bool bad = !(d > relTol * dmax);
-    const double skk = bad ? 0.0 : sqrt(d);
-    __syncthreads();
-    if (i == k && j >= k && j < s) G[j * 33 + k] = j == k ? skk : (bad ? 0.0 : G[j * 33 + k] / skk);
-    if (t == 0) dep[k] = bad ? 1 : 0;
-    __syncthreads();
-    if (i > k && j >= i && j < s) G[j * 33 + i] = G[j * 33 + i] - G[i * 33 + k] * G[j * 33 + k];
-    __syncthreads();
-  }
-  // S^-1 by rows from the bottom: X(r, j) = -(sum_{l = r+1..j} S(r, l) X(l, j)) / S(r, r).
-  for (int r = s - 1; r >= 0; --r) {
-    if (i == r && j >= r && j < s && !dep[r] && !dep[j]) {
-      if (j == r) {
-        X[j * 33 + r] = 1.0 / G[r * 33 + r];
-      } else {
-        double a = 0.0;
-        for (int l = r + 1; l <= j; ++l) a = fma(G[l * 33 + r], X[j * 33 + l], a);
-        X[j * 33 + r] = -a / G[r * 33 + r];
+// block Hessenberg at Hdst (ld ldh).  One wavefront: lane j keeps column j of G (then of S, of
+// S^-1 and of the product) in registers; every cross-lane operand is a broadcast of one lane's
+// register, so the factorisation needs no LDS and no barrier.
+__global__ __launch_bounds__(64) void k_chol_qr(const double *__restrict__ Gin, int s,
+                                                double relTol, float *__restrict__ Sinv32,
+                                                double *__restrict__ Sacc, int first,
+                                                double *__restrict__ Hdst, int ldh,
+                                                int *__restrict__ brk) {
+  const int j = threadIdx.x;
+  const bool own = j < s;
+  double g[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) g[i] = (own && i < s) ? Gin[j * s + i] : 0.0;
+  double dmax = 0.0;
+#pragma unroll
+  for (int i = 0; i < 32; ++i)
+    if (i < s) dmax = fmax(dmax, bcast(g[i], i));
+  unsigned dep = 0u;
+  // Right-looking factorisation: after step k lane j holds S(k, j) in g[k].
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    if (k < s) {
+      const double d = bcast(g[k], k);
+      const bool bad = !(d > relTol * dmax);
+      const double skk = bad ? 0.0 : sqrt(d);
+      dep |= bad ? (1u << k) : 0u;
+      if (j == k) g[k] = skk;
+      if (j > k) g[k] = bad ? 0.0 : g[k] / skk;
+      const double skj = g[k];
+#pragma unroll
+      for (int i = k + 1; i < 32; ++i) {
+        if (i < s) {
+          const double ski = bcast(g[k], i);
+          if (j >= i) g[i] = g[i] - ski * skj;
+        }
       }
     }
-    __syncthreads();
   }
-  Sinv32[i * 32 + j] = in ? (float)X[j * 33 + i] : 0.0f;
-  if (in) {
-    double a = 0.0;
-    if (first) {
-      a = i <= j ? G[j * 33 + i] : 0.0;
-    } else {
-      for (int l = i; l <= j; ++l) a = fma(G[l * 33 + i], A2[j * 33 + l], a);
+  // Column j of S^-1 by back substitution, columns of S broadcast from their lanes.
+  const bool depj = own ? ((dep >> j) & 1u) != 0u : true;
+  double x[32], acc[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) {
+    x[i] = 0.0;
+    acc[i] = (i == j) ? 1.0 : 0.0;
+  }
+#pragma unroll
+  for (int l = 31; l >= 0; --l) {
+    if (l < s) {
+      const double sll = bcast(g[l], l);
+      const bool skip = depj || ((dep >> l) & 1u) != 0u || l > j;
+      const double xl = skip ? 0.0 : acc[l] / sll;
+      x[l] = xl;
+#pragma unroll
+      for (int r = 0; r < l; ++r) acc[r] = fma(-bcast(g[r], l), xl, acc[r]);
     }
-    Sacc[j * s + i] = a;
-    if (Hdst) Hdst[(long long)j * ldh + i] = a;
   }
-  if (t == 0) {
-    int any = 0;
-    for (int c = 0; c < s; ++c) any |= dep[c];
-    if (any) atomicOr(brk, 1);
+  if (j < 32) {
+#pragma unroll
+    for (int i = 0; i < 32; ++i) Sinv32[i * 32 + j] = own && i < s ? (float)x[i] : 0.0f;
   }
+  // Sacc <- S * Sacc: column j of the product, S(i, l) broadcast from lane l.
+#pragma unroll
+  for (int i = 0; i < 32; ++i) acc[i] = (!first && own && i < s) ? Sacc[j * s + i] : 0.0;
+#pragma unroll
+  for (int i = 0; i < 32; ++i) {
+    if (i < s) {
+      double a = 0.0;
+      if (first) {
+        a = i <= j ? g[i] : 0.0;
+      } else {
+#pragma unroll
+        for (int l = i; l < 32; ++l)
+          if (l < s) a = fma(bcast(g[i], l), l <= j ? acc[l] : 0.0, a);
+      }
+      x[i] = a;
+    }
+  }
+  if (own) {
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+      if (i < s) {
+        Sacc[j * s + i] = x[i];
+        if (Hdst) Hdst[(long long)j * ldh + i] = x[i];
+      }
+    }
+  }
+  if (j == 0 && dep != 0u) atomicOr(brk, 1);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -482,8 +524,10 @@ __global__ __launch_bounds__(1024) void k_chol_qr(const double *__restrict__ Gin
 // Only rows c..c+bw change at step c, so they live in an LDS window of bw + 1 rows of [H | T]
 // (a ring: row r in slot r % (bw + 1)); after step c row c is final (written to R, column-major
 // ld cols, and Qt = Q^T T) and its slot takes row c + bw + 1 from H, which no reflector has
-// touched yet.  Then back substitution, Yr32 = Y diag(rho) (FP32 row-major cols x 32) and
-// ZH32 = H Y (FP32 row-major rows x 32).  One workgroup of 1024 threads.
+// touched yet (fetched into registers while the reflector is applied).  Back substitution runs in
+// 32-row blocks: one wavefront solves the diagonal block from LDS, then all threads update the
+// rows above with the block's columns of R.  Then Yr32 = Y diag(rho) (FP32 row-major cols x 32)
+// and ZH32 = H Y (FP32 row-major rows x 32).  One workgroup of 1024 threads.
 __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H, int rows,
                                                     int cols, int bw,
                                                     const double *__restrict__ T, int s,
@@ -495,20 +539,23 @@ __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H
                                                     float *__restrict__ ZH32) {
   extern __shared__ double sm[];
   __shared__ double s_tau, s_rmax;
-  __shared__ double yrow[32];
   const int tid = threadIdx.x, bd = blockDim.x;
   const int W = bw + 1, nc = cols + s;
   double *win = sm;  // W x nc
+  auto src = [&](int r, int col) -> double {
+    return col < cols ? H[(long long)col * rows + r] : T[(long long)(col - cols) * rows + r];
+  };
   for (int e = tid; e < W * nc; e += bd) {
     const int r = e / nc, col = e % nc;
-    win[e] = r >= rows ? 0.0
-                       : (col < cols ? H[(long long)col * rows + r]
-                                     : T[(long long)(col - cols) * rows + r]);
+    win[e] = r >= rows ? 0.0 : src(r, col);
   }
   __syncthreads();
   for (int c = 0; c < cols; ++c) {
     const int L = (c + bw < rows - 1 ? c + bw : rows - 1) - c;  // reflector tail length
     double *rowc = win + (c % W) * nc;
+    // The row entering the window after this step, fetched now.
+    const int rn = c + W;
+    const double incoming = (tid < nc && rn < rows) ? src(rn, tid) : 0.0;
     if (tid < 64) {
       double x = 0.0;
       if (tid >= 1 && tid <= L) x = win[((c + tid) % W) * nc + c];
@@ -531,34 +578,36 @@ __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H
     __syncthreads();
     const double tau = s_tau;
     if (tau != 0.0) {
+      const int c0 = c % W;
       for (int col = c + 1 + tid; col < nc; col += bd) {
         double w = rowc[col];
         for (int l = 1; l <= L; ++l) {
-          const double *rl = win + ((c + l) % W) * nc;
+          const int sl = c0 + l < W ? c0 + l : c0 + l - W;
+          const double *rl = win + sl * nc;
           w = fma(rl[c], rl[col], w);
         }
         const double tw = tau * w;
         rowc[col] = rowc[col] - tw;
         for (int l = 1; l <= L; ++l) {
-          double *rl = win + ((c + l) % W) * nc;
+          const int sl = c0 + l < W ? c0 + l : c0 + l - W;
+          double *rl = win + sl * nc;
           rl[col] = fma(-tw, rl[c], rl[col]);
         }
       }
     }
     __syncthreads();
-    const int rn = c + W;
     for (int col = tid; col < nc; col += bd) {
       const double v = rowc[col];
       if (col >= c && col < cols) R[(long long)col * cols + c] = v;
       if (col >= cols) Qt[(long long)c * 32 + (col - cols)] = v;
-      rowc[col] = rn >= rows ? 0.0
-                             : (col < cols ? H[(long long)col * rows + rn]
-                                           : T[(long long)(col - cols) * rows + rn]);
+      rowc[col] = col == tid ? incoming : (rn < rows ? src(rn, col) : 0.0);
     }
     __syncthreads();
   }
-  // Back substitution R Y = Qt on LDS copies of Qt; a negligible pivot leaves its unknown at 0.
-  double *Tl = sm;  // cols x 32
+  // Back substitution R Y = Qt in 32-row blocks; a negligible pivot leaves its unknown at zero.
+  double *Tl = sm;              // cols x 32: right-hand sides, updated in place
+  double *Dg = sm + cols * 32;  // 32 x 33: diagonal block of R (row-major, padded)
+  __shared__ double Yb[32 * 32];
   for (int e = tid; e < cols * 32; e += bd) Tl[e] = (e & 31) < s ? Qt[e] : 0.0;
   if (tid < 64) {
     double m = 0.0;
@@ -568,36 +617,62 @@ __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H
     if (tid == 0) s_rmax = m;
   }
   __syncthreads();
-  const double rmax = s_rmax;
-  for (int i = cols - 1; i >= 0; --i) {
-    const double *Ri = R + (long long)i * cols;  // column i of R
-    if (tid < 32) {
-      const double rii = Ri[i];
-      const double yv = (tid < s && fabs(rii) > 1e-14 * rmax) ? Tl[i * 32 + tid] / rii : 0.0;
-      yrow[tid] = yv;
-      Y[(long long)i * 32 + tid] = yv;
+  const double piv = 1e-14 * s_rmax;
+  for (int i1 = cols; i1 > 0; i1 -= 32) {
+    const int i0 = i1 - 32 > 0 ? i1 - 32 : 0, b = i1 - i0;
+    for (int e = tid; e < b * b; e += bd) {
+      const int r = e % b, l = e / b;  // R(i0 + r, i0 + l)
+      Dg[r * 33 + l] = l >= r ? R[(long long)(i0 + l) * cols + i0 + r] : 0.0;
     }
     __syncthreads();
-    for (int e = tid; e < i * 32; e += bd) {
-      const int r = e >> 5, jr = e & 31;
-      Tl[e] = fma(-Ri[r], yrow[jr], Tl[e]);
+    if (tid < 32) {
+      // Lane jr solves column jr of the diagonal block, bottom up.
+      const int jr = tid;
+      for (int r = b - 1; r >= 0; --r) {
+        double a = Tl[(i0 + r) * 32 + jr];
+        for (int l = r + 1; l < b; ++l) a = fma(-Dg[r * 33 + l], Yb[l * 32 + jr], a);
+        const double d = Dg[r * 33 + r];
+        const double yv = (jr < s && fabs(d) > piv) ? a / d : 0.0;
+        Yb[r * 32 + jr] = yv;
+        Y[(long long)(i0 + r) * 32 + jr] = yv;
+      }
+    }
+    __syncthreads();
+    // Rows above the block: Tl[r][:] -= sum_l R(r, i0 + l) Y(i0 + l, :), a thread per row (the
+    // loads of a column of R are contiguous across threads; Y is read by broadcast).
+    for (int r = tid; r < i0; r += bd) {
+      double a[32];
+#pragma unroll
+      for (int jr = 0; jr < 32; ++jr) a[jr] = Tl[r * 32 + jr];
+      for (int l = 0; l < b; ++l) {
+        const double rv = R[(long long)(i0 + l) * cols + r];
+#pragma unroll
+        for (int jr = 0; jr < 32; ++jr) a[jr] = fma(-rv, Yb[l * 32 + jr], a[jr]);
+      }
+#pragma unroll
+      for (int jr = 0; jr < 32; ++jr) Tl[r * 32 + jr] = a[jr];
     }
     __syncthreads();
   }
   for (int e = tid; e < cols * 32; e += bd) {
-    const int i = e >> 5, j = e & 31;
+    const int j = e & 31;
     Yr32[e] = j < s ? (float)(Y[e] * rho[j]) : 0.0f;
   }
   // ZH = H Y with Y staged in LDS.
   double *Yl = sm;
   for (int e = tid; e < cols * 32; e += bd) Yl[e] = Y[e];
   __syncthreads();
-  for (int e = tid; e < rows * 32; e += bd) {
-    const int r = e >> 5, j = e & 31;
-    double z = 0.0;
-    if (j < s)
-      for (int c = 0; c < cols; ++c) z = fma(H[(long long)c * rows + r], Yl[c * 32 + j], z);
-    ZH32[e] = (float)z;
+  for (int r = tid; r < rows; r += bd) {
+    double z[32];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) z[j] = 0.0;
+    for (int c = 0; c < cols; ++c) {
+      const double hv = H[(long long)c * rows + r];
+#pragma unroll
+      for (int j = 0; j < 32; ++j) z[j] = fma(hv, Yl[c * 32 + j], z[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 32; ++j) ZH32[r * 32 + j] = j < s ? (float)z[j] : 0.0f;
   }
 }
 
@@ -660,6 +735,16 @@ inline unsigned grid1(long long n) { return (unsigned)((n + kThreads - 1) / kThr
 
 }  // namespace
 
+void prepare() {
+  // Once per process, before any stream capture: the LDS limits of the one-workgroup kernels.
+  static const bool done = [] {
+    dmr::allow_max_lds(reinterpret_cast<const void *>(k_lsq_block));
+    dmr::prepare_dense();
+    return true;
+  }();
+  (void)done;
+}
+
 void recip(const double *q, double *out, int s, double *rho, hipStream_t st) {
   hipLaunchKernelGGL(k_recip, dim3(1), dim3(64), 0, st, q, out, s, rho);
 }
@@ -669,9 +754,17 @@ void zero_ints(int *p, int cnt, hipStream_t st) {
 }
 
 void spmm32(const CsrDev &A, const float *X, long long ldx, float *Y, long long ldy, int s,
-            hipStream_t st) {
-  hipLaunchKernelGGL((k_spmm<float, float, float, float>), dim3(grid1(A.n)), dim3(kThreads), 0, st,
-                     A.n, A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
+            int cg, hipStream_t st) {
+  const dim3 grid(grid1(A.n), (unsigned)((s + cg - 1) / cg));
+  if (cg == 4)
+    hipLaunchKernelGGL((k_spmm<float, float, float, float, 4>), grid, dim3(kThreads), 0, st, A.n,
+                       A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
+  else if (cg == 16)
+    hipLaunchKernelGGL((k_spmm<float, float, float, float, 16>), grid, dim3(kThreads), 0, st, A.n,
+                       A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
+  else
+    hipLaunchKernelGGL((k_spmm<float, float, float, float, 8>), grid, dim3(kThreads), 0, st, A.n,
+                       A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
 }
 
 void resid_block(const CsrDev &A, const double *X, long long ldx, const double *B, long long ldb,
@@ -692,9 +785,10 @@ void colnorms(const T *X, long long ld, int s, long long n, double *partial, dou
               int sqrtMode, hipStream_t st) {
   const int G = colnorm_blocks(n);
   const long long rows = (n + G - 1) / G;
-  hipLaunchKernelGGL((k_colsumsq<T>), dim3(G), dim3(kThreads), 0, st, X, ld, s, n, rows,
+  hipLaunchKernelGGL((k_colsumsq<T>), dim3(G, s), dim3(kThreads), 0, st, X, ld, s, n, rows,
                      partial);
-  hipLaunchKernelGGL(k_colsumsq_final, dim3(1), dim3(64), 0, st, partial, G, s, out, sqrtMode);
+  hipLaunchKernelGGL(k_colsumsq_final, dim3(1), dim3(kThreads), 0, st, partial, G, s, out,
+                     sqrtMode);
 }
 template void colnorms<float>(const float *, long long, int, long long, double *, double *, int,
                               hipStream_t);
@@ -710,20 +804,13 @@ int gemm_tn_blocks(long long n, int cap, int rowsTarget) {
 
 size_t gemm_tn_lds(int pPad) { return sizeof(float) * (size_t)(pPad + 32) * kLdsStride; }
 
-static bool tn_lds_variant() {
-  static const bool v = [] {
-    const char *e = std::getenv("DMR_BLOCK_TN_LDS");
-    return e && e[0] == '1';
-  }();
-  return v;
-}
-
 void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu, int s,
-             long long n, int G, float *part, double *C64, int ldc, float *C32, hipStream_t st) {
+             long long n, int G, int variant, float *part, double *C64, int ldc, float *C32,
+             hipStream_t st) {
   const int pPad = (p + 31) & ~31;
   long long rows = (n + G - 1) / G;
   rows = (rows + 63) / 64 * 64;
-  if (tn_lds_variant()) {
+  if (variant == 1) {
     const size_t lds = gemm_tn_lds(pPad);
     hipLaunchKernelGGL(k_gemm_tn_lds, dim3(G), dim3(kThreads), lds, st, P, ldp, p, pPad, U, ldu,
                        s, n, rows, part);
@@ -736,22 +823,30 @@ void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu
 }
 
 void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *In, long long ldi,
-             float a, float b, float *Out, long long ldo, int s, long long n, hipStream_t st) {
+             float a, float b, float *Out, long long ldo, int s, long long n, int un,
+             hipStream_t st) {
   const long long waves = (n + 127) / 128;
   const unsigned grid = (unsigned)((waves + 3) / 4);
   const size_t lds = sizeof(float) * (size_t)(((p + 1) & ~1) * 32);
-  hipLaunchKernelGGL(k_gemm_nn, dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C, In, ldi, a, b,
-                     Out, ldo, s, n);
+  if (un >= 8)
+    hipLaunchKernelGGL(k_gemm_nn<8>, dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C, In, ldi, a,
+                       b, Out, ldo, s, n);
+  else if (un >= 4)
+    hipLaunchKernelGGL(k_gemm_nn<4>, dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C, In, ldi, a,
+                       b, Out, ldo, s, n);
+  else
+    hipLaunchKernelGGL(k_gemm_nn<1>, dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C, In, ldi, a,
+                       b, Out, ldo, s, n);
 }
 
 void chol_qr(const double *G, int s, double relTol, float *Sinv32, double *Sacc, int first,
              double *Hdst, int ldh, int *brk, hipStream_t st) {
-  hipLaunchKernelGGL(k_chol_qr, dim3(1), dim3(1024), 0, st, G, s, relTol, Sinv32, Sacc, first,
+  hipLaunchKernelGGL(k_chol_qr, dim3(1), dim3(64), 0, st, G, s, relTol, Sinv32, Sacc, first,
                      Hdst, ldh, brk);
 }
 
 size_t lsq_block_lds(int cols, int bw, int s) {
-  const size_t win = (size_t)(bw + 1) * (cols + s), back = (size_t)cols * 32;
+  const size_t win = (size_t)(bw + 1) * (cols + s), back = (size_t)cols * 32 + 32 * 33;
   return sizeof(double) * (win > back ? win : back);
 }
 
@@ -759,9 +854,7 @@ void lsq_block(const double *H, int rows, int cols, int bw, const double *T, int
                const double *rho, double *R, double *Qt, double *Y, float *Yr32, float *ZH32,
                hipStream_t st) {
   const size_t lds = lsq_block_lds(cols, bw, s);
-  if (lds > 64 * 1024)
-    hipFuncSetAttribute(reinterpret_cast<const void *>(k_lsq_block),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  prepare();
   hipLaunchKernelGGL(k_lsq_block, dim3(1), dim3(1024), lds, st, H, rows, cols, bw, T, s, rho, R,
                      Qt, Y, Yr32, ZH32);
 }

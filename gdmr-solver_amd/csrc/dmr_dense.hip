@@ -232,12 +232,30 @@ size_t svd_lds(int rows, int m) {
   return sizeof(double) * ((size_t)rows * mm + (size_t)m * mm + mm);
 }
 
+void allow_max_lds(const void *fn) {
+  hipFuncAttributes fa{};
+  if (hipFuncGetAttributes(&fa, fn) == hipSuccess)
+    hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)(160 * 1024 - fa.sharedSizeBytes));
+  // Leave no error behind for the caller's next error check.
+  (void)hipGetLastError();
+}
+
+void prepare_dense() {
+  // Once per process: the largest dynamic LDS (160 KiB less the static LDS) of the
+  // one-workgroup kernels.
+  static const bool done = [] {
+    allow_max_lds(reinterpret_cast<const void *>(k_svd));
+    allow_max_lds(reinterpret_cast<const void *>(k_lsq));
+    return true;
+  }();
+  (void)done;
+}
+
 void svd_smallest_rect(const double *H, int rows, int ldH, int m, int k, double tol, int maxSweeps,
                        int zeroLast, double *C, double *sig, int *sweeps, hipStream_t s) {
   const size_t lds = svd_lds(rows, m);
-  if (lds > 64 * 1024)
-    hipFuncSetAttribute(reinterpret_cast<const void *>(k_svd),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (lds > 64 * 1024) prepare_dense();
   hipLaunchKernelGGL(k_svd, dim3(1), dim3(kSvdThreads), lds, s, H, rows, ldH, m, k, tol, maxSweeps,
                      zeroLast, C, sig, sweeps);
 }

@@ -185,6 +185,10 @@ void svd_smallest(const double *H, int m, int k, double tol, int maxSweeps, doub
 // The same for a rows x m matrix with leading dimension ldH (rows >= m).  The matrix and the
 // right vectors live in LDS: svd_lds(rows, m) bytes, at most 160 KiB.
 size_t svd_lds(int rows, int m);
+// Sets the dynamic LDS limits of the one-workgroup dense kernels (once per process).
+void prepare_dense();
+// Allows a kernel the dynamic LDS left by its static LDS within 160 KiB; clears the error slot.
+void allow_max_lds(const void *fn);
 // zeroLast != 0: exactly zero columns are ranked largest (never among the k selected).
 void svd_smallest_rect(const double *H, int rows, int ldH, int m, int k, double tol, int maxSweeps,
                        int zeroLast, double *C, double *sig, int *sweeps, hipStream_t s);

@@ -119,6 +119,8 @@ _sig("dmrBlockExportZ", C.c_int, [_vp, _dp])
 _sig("dmrBlockSetProfiling", C.c_int, [_vp, C.c_int])
 _sig("dmrBlockGetProfile", C.c_int, [_vp, _dp, _dp, _dp, C.POINTER(C.c_longlong)])
 _sig("dmrBlockFootprint", C.c_size_t, [C.c_int, C.c_int, C.c_int, C.c_int])
+_sig("dmrBlockSetOptions", C.c_int, [_vp, C.c_int, C.c_int])
+_sig("dmrBlockGetTuning", C.c_int, [_vp, _ip])
 _sig("dmrSetProfiling", C.c_int, [_vp, C.c_int])
 _sig("dmrGetProfile", C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_longlong)])
 PHASES = ("spmv", "gemvt", "update", "scale", "combo", "zgemm", "lsq", "svd", "orth")
@@ -150,6 +152,7 @@ HEADER_SYMBOLS = (
     "dmrBlockCreate dmrBlockDestroy dmrBlockSetOperatorCsr dmrBlockSetOperatorValues "
     "dmrBlockSetRefinement dmrBlockSolve dmrBlockGetStats dmrBlockHasZ dmrBlockImportZ "
     "dmrBlockExportZ dmrBlockSetProfiling dmrBlockGetProfile dmrBlockFootprint "
+    "dmrBlockSetOptions dmrBlockGetTuning "
     "dmrServerCreate dmrServerAddFamily dmrServerAddBlockFamily dmrServerSetRefinement "
     "dmrServerSubmit dmrServerPoll "
     "dmrServerWait dmrServerDrain dmrServerNow dmrServerStats dmrServerDestroy "
@@ -433,6 +436,15 @@ class BlockSolver:
         Z = np.zeros(self.n * self.k)
         _check(lib.dmrBlockExportZ(self.h, _ptr(Z, C.c_double)), "dmrBlockExportZ")
         return Z.reshape(self.k, self.n)
+
+    def set_options(self, graphs: int = -1, autotune: int = -1):
+        _check(lib.dmrBlockSetOptions(self.h, graphs, autotune), "dmrBlockSetOptions")
+
+    def tuning(self) -> dict:
+        out = (C.c_int * 7)()
+        _check(lib.dmrBlockGetTuning(self.h, out), "dmrBlockGetTuning")
+        return dict(tn_rows_proj=out[0], tn_lds_proj=out[1], tn_rows_gram=out[2],
+                    tn_lds_gram=out[3], nn_unroll=out[4], spmm_cols=out[5], graphs=out[6])
 
     def set_profiling(self, on: bool = True):
         _check(lib.dmrBlockSetProfiling(self.h, int(on)), "dmrBlockSetProfiling")

@@ -194,6 +194,14 @@ int dmrBlockSolve(dmrBlockHandle h, int mode, int nrhs, double *X, long long ldx
  * cycle, the largest estimated relative residual. */
 int dmrBlockGetStats(dmrBlockHandle h, dmrStats *st, double *residHist, int cap);
 int dmrBlockHasZ(dmrBlockHandle h);
+/* graphs: replay each restart cycle as a captured HIP graph (default on; off while profiling).
+ * autotune: time candidate launch shapes on the first solve of each block width and keep the
+ * fastest (default on; cached per problem shape for the process).  A negative value leaves the
+ * setting unchanged.  DMR_BLOCK_GRAPHS=0 / DMR_BLOCK_TUNE=0 change the defaults. */
+int dmrBlockSetOptions(dmrBlockHandle h, int graphs, int autotune);
+/* out[7]: gemm_tn rows per workgroup and variant for projections, the same for Gram matrices,
+ * gemm_nn load-ahead depth, SpMM columns per thread, number of captured cycle graphs. */
+int dmrBlockGetTuning(dmrBlockHandle h, int *out);
 int dmrBlockImportZ(dmrBlockHandle h, const double *hostZ);
 int dmrBlockExportZ(dmrBlockHandle h, double *hostZ);
 /* Phase profile as dmrSetProfiling / dmrGetProfile: SPMV = SpMM and FP64 residuals, GEMVT =

@@ -147,3 +147,35 @@ def test_block_rejects_bad_shapes(torch_gpu):
     rc, rel = bs.solve(0, 1000, 0, 1000, 4, 1e-8)
     assert rc == gdmr.DMR_ERR_STATE  # no operator
     bs.close()
+
+
+def test_block_graph_replays_match_eager(torch_gpu):
+    """Restart cycles replayed from captured HIP graphs give the eager launches' bits: three
+    solves in a row (each replaying its cycle graphs many times, with the eager refinement work
+    between the replays) on a graph handle and on an eager handle of the same (tuned) shape."""
+    torch = torch_gpu
+    A = gdmr.conv_diff_3d(24, jitter=0.01, seed=9, stream=3)
+    n, s = A.n, 8
+    rp, ci, va = _dev(torch, A.rowptr), _dev(torch, A.col), _dev(torch, A.val)
+    hs = []
+    for graphs in (1, 0):
+        bs = gdmr.BlockSolver(n, 6, 16, s)
+        bs.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A.nnz)
+        bs.set_options(graphs=graphs)
+        hs.append(bs)
+    for t in range(3):
+        B = np.stack([gdmr.rand_normal(37, 10 * t + j, n) for j in range(s)])
+        outs = [_block(torch, A, B, 6, 16, solver=bs, mode=t) for bs in hs]
+        (rc_g, rel_g, st_g, X_g), (rc_e, rel_e, st_e, X_e) = outs
+        assert rc_g == 0 and rc_e == 0
+        assert st_g.cycles == st_e.cycles and st_g.matvecs == st_e.matvecs
+        assert np.array_equal(X_g, X_e), f"solve {t}: graph replay differs from eager launches"
+        for j in range(s):
+            assert _true_res(A, B[j], X_g[j]) <= 1e-10 * 1.001
+    tg, te = hs[0].tuning(), hs[1].tuning()
+    print(f"tuning {tg}; eager handle {te}", flush=True)
+    assert tg["graphs"] > 0 and te["graphs"] == 0
+    assert {k: v for k, v in tg.items() if k != "graphs"} == \
+        {k: v for k, v in te.items() if k != "graphs"}
+    for bs in hs:
+        bs.close()

@@ -110,6 +110,8 @@ int spmv_partition(const int *rowPtrHost, int n, int *br);
 void spmv(const CsrDev &A, const double *x, double *y, hipStream_t s);
 // FP32 values and vectors, FP32 accumulation.
 void spmv32(const CsrDev &A, const float *x, float *y, hipStream_t s);
+// FP32 values, bf16 vectors (FP32 products and sums, rounded to bf16 on store).
+void spmv16(const CsrDev &A, const bf16 *x, bf16 *y, hipStream_t s);
 // r = b - A x in FP64 with ss = ||r||^2 finalised by FIN_NORM into scal.
 void residual64(const CsrDev &A, const double *x, const double *b, double *r, RedWork w,
                 double *scal, hipStream_t s);

@@ -791,6 +791,16 @@ __global__ __launch_bounds__(kBlock) void k_breakdown_fixup(TS *P, long long ld,
 // products in column order.  A row with more than kSpmvNnzCap nonzeros has a block of its own
 // and is reduced by the block.
 // =============================================================================================
+// Element loads / stores of vectors in their storage type (bf16: round to nearest even).
+__device__ inline double ldv(const double *p, long long i) { return p[i]; }
+__device__ inline float ldv(const float *p, long long i) { return p[i]; }
+__device__ inline float ldv(const bf16 *p, long long i) {
+  return __uint_as_float((uint32_t)p[i].bits << 16);
+}
+__device__ inline void stv(double *p, long long i, double v) { p[i] = v; }
+__device__ inline void stv(float *p, long long i, float v) { p[i] = v; }
+__device__ inline void stv(bf16 *p, long long i, float v) { p[i].bits = (uint16_t)bf_bits(v); }
+
 template <typename TV, typename TX, typename TA>
 __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
                                                  const int *__restrict__ ci,
@@ -804,21 +814,22 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
   if (ne > kSpmvNnzCap) {
     TA s = 0;
     for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlock) {
-      const TA t = (TA)va[e] * (TA)x[ci[e]];
+      const TA t = (TA)va[e] * (TA)ldv(x, ci[e]);
       s = s + t;
     }
     s = block_sum(s);
-    if (threadIdx.x == 0) y[rs] = (TX)s;
+    if (threadIdx.x == 0) stv(y, rs, s);
     return;
   }
-  for (int e = threadIdx.x; e < ne; e += kBlock) prod[e] = (TA)va[e0 + e] * (TA)x[ci[e0 + e]];
+  for (int e = threadIdx.x; e < ne; e += kBlock)
+    prod[e] = (TA)va[e0 + e] * (TA)ldv(x, ci[e0 + e]);
   __syncthreads();
   if ((int)threadIdx.x < nr) {
     const int r = rs + threadIdx.x;
     const int a = rp[r] - e0, b = rp[r + 1] - e0;
     TA s = 0;
     for (int e = a; e < b; ++e) s = s + prod[e];
-    y[r] = (TX)s;
+    stv(y, r, s);
   }
 }
 
@@ -912,6 +923,12 @@ void spmv(const CsrDev &A, const double *x, double *y, hipStream_t s) {
 void spmv32(const CsrDev &A, const float *x, float *y, hipStream_t s) {
   if (A.nBlocks == 0) return;
   hipLaunchKernelGGL((k_spmv<float, float, float>), dim3(A.nBlocks), dim3(kBlock), 0, s,
+                     A.rowPtr, A.colInd, A.val32, A.blockRows, x, y);
+}
+
+void spmv16(const CsrDev &A, const bf16 *x, bf16 *y, hipStream_t s) {
+  if (A.nBlocks == 0) return;
+  hipLaunchKernelGGL((k_spmv<float, bf16, float>), dim3(A.nBlocks), dim3(kBlock), 0, s,
                      A.rowPtr, A.colInd, A.val32, A.blockRows, x, y);
 }
 

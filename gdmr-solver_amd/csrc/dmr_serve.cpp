@@ -380,7 +380,8 @@ int dmrServerAddFamily(dmrServerHandle s, int n, int nnz, const int *rowPtr, con
                        int basisDim, int deflatDim, int precision, int *family) {
   if (!s || !rowPtr || !colInd || n < 1 || nnz < 0 || !family) return DMR_ERR_ARG;
   if (basisDim < 1 || deflatDim < 0 || deflatDim >= basisDim) return DMR_ERR_ARG;
-  if (precision != DMR_PREC_FP64 && precision != DMR_PREC_FP32) return DMR_ERR_NOTBUILT;
+  if (precision != DMR_PREC_FP64 && precision != DMR_PREC_FP32 && precision != DMR_PREC_BF16)
+    return DMR_ERR_ARG;
   auto F = std::make_unique<Family>();
   F->n = n;
   F->nnz = nnz;

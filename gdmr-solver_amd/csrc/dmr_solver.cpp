@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -121,6 +122,7 @@ struct dmrContext {
   double *hostStatus = nullptr;  // pinned
   Work<double> w64;
   Work<float> w32;
+  Work<dmr::bf16> w16;  // bf16 basis storage (DMR_PREC_BF16)
   double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr;  // refinement vectors (mixed)
   double *H = nullptr, *t = nullptr, *y = nullptr, *z = nullptr, *C = nullptr, *sig = nullptr,
          *cv = nullptr, *scal = nullptr, *scal2 = nullptr, *scalZ = nullptr, *one = nullptr;
@@ -163,7 +165,7 @@ size_t footprint(int n, int m, int k, int prec) {
   const long long ld = pad_ld(n);
   size_t big = sizeof(double) * (size_t)ld * (size_t)(m + 1 + 2 * k + 4);
   if (prec != DMR_PREC_FP64) {
-    big += sizeof(float) * (size_t)ld * (size_t)(m + 1 + 2 * k + 4);
+    big += (prec == DMR_PREC_BF16 ? 2 : 4) * (size_t)ld * (size_t)(m + 1 + 2 * k + 4);
     big += sizeof(double) * (size_t)ld * 3;
   }
   const size_t small = sizeof(double) * ((size_t)(m + 1) * m * 2 + 8 * (m + 2) + 64 + 2 * k) +
@@ -178,8 +180,9 @@ int ctx_alloc(dmrContext *c) {
   (void)n;
   size_t big = sizeof(double) * (size_t)ld * (size_t)(m + 1 + 2 * k + 4);
   size_t big32 = 0;
+  const size_t es = c->prec == DMR_PREC_BF16 ? 2 : 4;  // inner basis element size
   if (c->prec != DMR_PREC_FP64) {
-    big32 = sizeof(float) * (size_t)ld * (size_t)(m + 1 + 2 * k + 4) + sizeof(double) * ld * 3;
+    big32 = es * (size_t)ld * (size_t)(m + 1 + 2 * k + 4) + sizeof(double) * ld * 3;
     big = 0;  // the FP64 Krylov basis is not needed by a mixed-precision context
   }
   const size_t total = big + big32;
@@ -210,21 +213,25 @@ int ctx_alloc(dmrContext *c) {
     d += ld;
     c->r64 = d;
     d += ld;
-    float *f = (float *)d;
-    c->w32.P = f;
-    f += ld * (m + 1);
-    c->w32.Z = f;
-    f += ld * k;
-    c->w32.Zn = f;
-    f += ld * k;
-    c->w32.r = f;
-    f += ld;
-    c->w32.xa = f;
-    f += ld;
-    c->w32.g = f;
-    f += ld;
-    c->w32.u = f;
-    f += ld;
+    auto carve = [&](auto &w, auto *f) {
+      w.P = f;
+      f += ld * (m + 1);
+      w.Z = f;
+      f += ld * k;
+      w.Zn = f;
+      f += ld * k;
+      w.r = f;
+      f += ld;
+      w.xa = f;
+      f += ld;
+      w.g = f;
+      f += ld;
+      w.u = f;
+    };
+    if (c->prec == DMR_PREC_BF16)
+      carve(c->w16, (dmr::bf16 *)d);
+    else
+      carve(c->w32, (float *)d);
   }
   // small arrays
   const size_t nsmall = (size_t)(m + 1) * m   // H
@@ -344,6 +351,20 @@ struct Op32 {
 template <typename TS> Work<TS> &work_of(dmrContext *c);
 template <> Work<double> &work_of<double>(dmrContext *c) { return c->w64; }
 template <> Work<float> &work_of<float>(dmrContext *c) { return c->w32; }
+template <> Work<dmr::bf16> &work_of<dmr::bf16>(dmrContext *c) { return c->w16; }
+
+struct Op16 {
+  dmrContext *c;
+  hipStream_t s;
+  int operator()(const dmr::bf16 *src, dmr::bf16 *dst) const {
+    dmr::spmv16(c->csr, src, dst, s);
+    return DMR_OK;
+  }
+};
+
+template <typename TS> struct LowOp;
+template <> struct LowOp<float> { using type = Op32; };
+template <> struct LowOp<dmr::bf16> { using type = Op16; };
 
 dmr::Fin fin_none() { return dmr::Fin{dmr::FIN_NONE, nullptr, nullptr, 0.0, 0, nullptr}; }
 
@@ -682,9 +703,12 @@ double solve_fp64(dmrContext *c, int mode, double *x, const double *b, hipStream
 // FP64.  Each outer step computes r = b - A x with the FP64 operator, solves A d = r with the
 // inner DMR to a relative tolerance max(innerTol, 0.5 * tol * ||b|| / ||r||) starting from the
 // recycled FP32 subspace, and adds d to x.  Returns the FP64-checked ||b - A x|| / ||b||.
+// Mixed precision: FP64 refinement around an inner DMR whose basis is stored as TS (FP32 or
+// bf16) with FP32 SpMV values and FP32-accumulated reductions.
+template <typename TS>
 double solve_mixed(dmrContext *c, int mode, double *x, const double *b, hipStream_t s, double tol,
                    int matRank, int *numIts) {
-  Work<float> &w = c->w32;
+  Work<TS> &w = work_of<TS>(c);
   const long long n = c->n, ld = c->ld;
   const int k = c->k;
   *numIts = 0;
@@ -713,7 +737,7 @@ double solve_mixed(dmrContext *c, int mode, double *x, const double *b, hipStrea
   }
   const double beta = std::sqrt(c->hostStatus[4]);
   c->st.beta = beta;
-  Op32 op{c, s};
+  typename LowOp<TS>::type op{c, s};
   int totalCycles = 0;
   double relres = 1.0;
   c->kryLoopFlag = 0;
@@ -737,19 +761,19 @@ double solve_mixed(dmrContext *c, int mode, double *x, const double *b, hipStrea
     }
     // r32 = r / ||r||, x_a = 0.
     dmr::scale<double>(c->r64, n, c->scal + 1, nullptr, s);
-    dmr::convert<float, double>(w.r, c->r64, n, s);
-    dmr::fill<float>(w.xa, ld, 0.0, s);
+    dmr::convert<TS, double>(w.r, c->r64, n, s);
+    dmr::fill<TS>(w.xa, ld, 0.0, s);
     double tolIn = std::max(c->innerTol, 0.5 * tol / relres);
     if (tolIn > 0.5) tolIn = 0.5;
     double rho = rn;
     int grdItr = 0;
     const int startDef = useZ ? k : 0;
-    const int rc = run_cycles<float>(c, op, s, startDef, tolIn, matRank, rn, rho, grdItr,
-                                     rn / beta);
+    const int rc = run_cycles<TS>(c, op, s, startDef, tolIn, matRank, rn, rho, grdItr,
+                                  rn / beta);
     totalCycles += grdItr;
     if (rc != DMR_OK && rc != DMR_ERR_NOCONV && rc != DMR_ERR_RUNAWAY) return relres;
     useZ = (k > 0) && w.hasZ;
-    dmr::add_to<float>(c->x64, w.xa, n, s);
+    dmr::add_to<TS>(c->x64, w.xa, n, s);
   }
   *numIts = totalCycles;
   if (c->st.status == 0) c->st.status = relres <= tol ? 0 : DMR_ERR_NOCONV;
@@ -766,8 +790,8 @@ int create_ctx(dmrContext **out, int device, int n, int m, int k, int prec) {
   *out = nullptr;
   int rc = validate_shape(n, m, k);
   if (rc) return rc;
-  if (prec != DMR_PREC_FP64 && prec != DMR_PREC_FP32)
-    return set_err(DMR_ERR_NOTBUILT, "precision not available (FP64 or FP32 basis)");
+  if (prec != DMR_PREC_FP64 && prec != DMR_PREC_FP32 && prec != DMR_PREC_BF16)
+    return set_err(DMR_ERR_ARG, "unknown precision");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return set_err(DMR_ERR_HIP, "no HIP device");
@@ -925,15 +949,65 @@ double dmrSolve(dmrHandle h, int mode, double *x, const double *b, hipStream_t s
   double res;
   if (h->prec == DMR_PREC_FP64)
     res = solve_fp64(h, mode, x, b, stream, solTol, matRank, numIts);
+  else if (h->prec == DMR_PREC_BF16)
+    res = solve_mixed<dmr::bf16>(h, mode, x, b, stream, solTol, matRank, numIts);
   else
-    res = solve_mixed(h, mode, x, b, stream, solTol, matRank, numIts);
+    res = solve_mixed<float>(h, mode, x, b, stream, solTol, matRank, numIts);
   h->st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return res;
 }
 
+namespace {
+// The recycled subspace of a handle in its storage precision: device pointer, element size,
+// and its hasZ flag.
+struct ZView {
+  void *Z;
+  size_t es;
+  bool *has;
+};
+ZView zview(dmrHandle h) {
+  if (h->prec == DMR_PREC_FP64) return {h->w64.Z, sizeof(double), &h->w64.hasZ};
+  if (h->prec == DMR_PREC_BF16) return {h->w16.Z, 2, &h->w16.hasZ};
+  return {h->w32.Z, sizeof(float), &h->w32.hasZ};
+}
+// Host conversions to and from the storage precision (bf16: round to nearest even).
+void to_storage(const double *src, size_t cnt, int prec, void *dst) {
+  if (prec == DMR_PREC_FP64) {
+    std::memcpy(dst, src, sizeof(double) * cnt);
+  } else if (prec == DMR_PREC_BF16) {
+    uint16_t *d = (uint16_t *)dst;
+    for (size_t i = 0; i < cnt; ++i) {
+      const float f = (float)src[i];
+      uint32_t u;
+      std::memcpy(&u, &f, 4);
+      d[i] = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+    }
+  } else {
+    float *d = (float *)dst;
+    for (size_t i = 0; i < cnt; ++i) d[i] = (float)src[i];
+  }
+}
+void from_storage(const void *src, size_t cnt, int prec, double *dst) {
+  if (prec == DMR_PREC_FP64) {
+    std::memcpy(dst, src, sizeof(double) * cnt);
+  } else if (prec == DMR_PREC_BF16) {
+    const uint16_t *s16 = (const uint16_t *)src;
+    for (size_t i = 0; i < cnt; ++i) {
+      const uint32_t u = (uint32_t)s16[i] << 16;
+      float f;
+      std::memcpy(&f, &u, 4);
+      dst[i] = f;
+    }
+  } else {
+    const float *s32 = (const float *)src;
+    for (size_t i = 0; i < cnt; ++i) dst[i] = s32[i];
+  }
+}
+}  // namespace
+
 int dmrHasZ(dmrHandle h) {
   if (!h) return 0;
-  return h->prec == DMR_PREC_FP64 ? (int)h->w64.hasZ : (int)h->w32.hasZ;
+  return *zview(h).has ? 1 : 0;
 }
 
 int dmrImportZ(dmrHandle h, const double *hostZ) {
@@ -941,17 +1015,12 @@ int dmrImportZ(dmrHandle h, const double *hostZ) {
   if (!h || !hostZ) return set_err(DMR_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(h->device));
   if (h->k == 0) return DMR_OK;
-  if (h->prec == DMR_PREC_FP64) {
-    HIPCHK(hipMemcpy2D(h->w64.Z, sizeof(double) * h->ld, hostZ, sizeof(double) * h->n,
-                       sizeof(double) * h->n, h->k, hipMemcpyHostToDevice));
-    h->w64.hasZ = true;
-  } else {
-    std::vector<float> tmp((size_t)h->n * h->k);
-    for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = (float)hostZ[i];
-    HIPCHK(hipMemcpy2D(h->w32.Z, sizeof(float) * h->ld, tmp.data(), sizeof(float) * h->n,
-                       sizeof(float) * h->n, h->k, hipMemcpyHostToDevice));
-    h->w32.hasZ = true;
-  }
+  const ZView z = zview(h);
+  std::vector<char> tmp(z.es * (size_t)h->n * h->k);
+  to_storage(hostZ, (size_t)h->n * h->k, h->prec, tmp.data());
+  HIPCHK(hipMemcpy2D(z.Z, z.es * h->ld, tmp.data(), z.es * h->n, z.es * h->n, h->k,
+                     hipMemcpyHostToDevice));
+  *z.has = true;
   return DMR_OK;
 }
 
@@ -961,43 +1030,34 @@ int dmrExportZ(dmrHandle h, double *hostZ) {
   HIPCHK(hipSetDevice(h->device));
   if (h->k == 0) return DMR_OK;
   HIPCHK(hipDeviceSynchronize());
-  if (h->prec == DMR_PREC_FP64) {
-    HIPCHK(hipMemcpy2D(hostZ, sizeof(double) * h->n, h->w64.Z, sizeof(double) * h->ld,
-                       sizeof(double) * h->n, h->k, hipMemcpyDeviceToHost));
-  } else {
-    std::vector<float> tmp((size_t)h->n * h->k);
-    HIPCHK(hipMemcpy2D(tmp.data(), sizeof(float) * h->n, h->w32.Z, sizeof(float) * h->ld,
-                       sizeof(float) * h->n, h->k, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < tmp.size(); ++i) hostZ[i] = tmp[i];
-  }
+  const ZView z = zview(h);
+  std::vector<char> tmp(z.es * (size_t)h->n * h->k);
+  HIPCHK(hipMemcpy2D(tmp.data(), z.es * h->n, z.Z, z.es * h->ld, z.es * h->n, h->k,
+                     hipMemcpyDeviceToHost));
+  from_storage(tmp.data(), (size_t)h->n * h->k, h->prec, hostZ);
   return DMR_OK;
 }
 
 size_t dmrSubspaceBytes(dmrHandle h) {
   if (!h) return 0;
-  const size_t es = h->prec == DMR_PREC_FP64 ? sizeof(double) : sizeof(float);
-  return es * (size_t)h->ld * (size_t)h->k;
+  return zview(h).es * (size_t)h->ld * (size_t)h->k;
 }
 
 int dmrSubspaceToDevice(dmrHandle h, void *dst, hipStream_t stream) {
   if (!h || !dst) return set_err(DMR_ERR_ARG, "null argument");
-  const void *src = h->prec == DMR_PREC_FP64 ? (const void *)h->w64.Z : (const void *)h->w32.Z;
   if (h->k == 0) return DMR_OK;
   HIPCHK(hipSetDevice(h->device));
-  HIPCHK(hipMemcpyAsync(dst, src, dmrSubspaceBytes(h), hipMemcpyDeviceToDevice, stream));
+  HIPCHK(hipMemcpyAsync(dst, zview(h).Z, dmrSubspaceBytes(h), hipMemcpyDeviceToDevice, stream));
   return DMR_OK;
 }
 
 int dmrSubspaceFromDevice(dmrHandle h, const void *src, hipStream_t stream) {
   if (!h || !src) return set_err(DMR_ERR_ARG, "null argument");
   if (h->k == 0) return DMR_OK;
-  void *dst = h->prec == DMR_PREC_FP64 ? (void *)h->w64.Z : (void *)h->w32.Z;
+  const ZView z = zview(h);
   HIPCHK(hipSetDevice(h->device));
-  HIPCHK(hipMemcpyAsync(dst, src, dmrSubspaceBytes(h), hipMemcpyDeviceToDevice, stream));
-  if (h->prec == DMR_PREC_FP64)
-    h->w64.hasZ = true;
-  else
-    h->w32.hasZ = true;
+  HIPCHK(hipMemcpyAsync(z.Z, src, dmrSubspaceBytes(h), hipMemcpyDeviceToDevice, stream));
+  *z.has = true;
   return DMR_OK;
 }
 

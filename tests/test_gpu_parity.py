@@ -215,13 +215,16 @@ def test_operator_error_channel(torch_gpu):
     s.close()
 
 
-def test_mixed_precision_reaches_fp64_tolerance(torch_gpu):
-    """FP32 basis + FP32 SpMV inside FP64 refinement reaches 1e-10 FP64-checked, and its solution
-    agrees with the FP64 oracle's to the conditioning of the problem."""
+@pytest.mark.parametrize("prec", [gdmr.PREC_FP32, gdmr.PREC_BF16], ids=["fp32", "bf16"])
+def test_mixed_precision_reaches_fp64_tolerance(torch_gpu, prec):
+    """FP32 (or bf16) basis storage with FP32 SpMV values and FP32-accumulated reductions inside
+    FP64 refinement reaches 1e-10 FP64-checked, and its solution agrees with the FP64 oracle's to
+    the conditioning of the problem.  The recycled subspace survives an export / import round
+    trip through FP64 host memory bit for bit."""
     torch = torch_gpu
     A = gdmr.conv_diff_2d(64)
     n, m, k, tol = A.n, 30, 10, 1e-10
-    s = gdmr.Solver(n, m, k, precision=gdmr.PREC_FP32)
+    s = gdmr.Solver(n, m, k, precision=prec)
     rp, ci, va = _dev(torch, A.rowptr), _dev(torch, A.col), _dev(torch, A.val)
     s.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A.nnz, keep=(rp, ci, va))
     ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n))
@@ -239,6 +242,11 @@ def test_mixed_precision_reaches_fp64_tolerance(torch_gpu):
               flush=True)
         assert tr <= tol and abs(res - tr) <= 1e-3 * tr + 1e-16
         assert err < 1e-7
+    Z = s.export_z()
+    s2 = gdmr.Solver(n, m, k, precision=prec)
+    s2.import_z(Z)
+    assert np.array_equal(s2.export_z(), Z)
+    s2.close()
     s.close()
 
 

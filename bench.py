@@ -9,9 +9,14 @@ Two legs run on the same sequence, each in a fresh solver context:
   * fp64  -- the plain FP64 DMR (the parity-checked path);
   * mixed -- FP32 Krylov basis and SpMV with FP32-accumulated reductions inside FP64 iterative
              refinement; Hessenberg LSQ / SVD in FP64.
-A step is one solve of one system.  `value` is the mixed leg's solves/sec when every system of it
-reaches 1e-10 checked in FP64 (else the fp64 leg's).  Every solve's true residual
+A step is one solve of one system.  `value` is the solves/sec of the fastest leg whose every
+system reaches 1e-10 checked in FP64 (the mixed leg).  Every solve's true residual
 ||b - A x|| / ||b|| is recomputed on the host in FP64 after the timed region.
+
+The line also carries the batched form of the metric ("batched"): 3-D anisotropic diffusion 96^3
+with 32 right-hand sides solved as one block (block DMR on the matrix cores, FP32 inside FP64
+refinement) against the plain FP64 DMR solving them one at a time (gdmr.batch_bench), and the
+FP64 host oracle's rate on the same sequence ("cpu_baseline").
 """
 import argparse
 import json
@@ -162,6 +167,8 @@ def main():
                     help="relative tolerance of each inner FP32 solve of the mixed leg")
     ap.add_argument("--bf16-inner-tol", type=float, default=1e-2,
                     help="relative tolerance of each inner solve of the bf16 leg")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched multi-RHS leg (96^3 anisotropic, 32 RHS)")
     ap.add_argument("--serve", action="store_true",
                     help="run the serving load generator (gdmr.serve_bench) instead; the "
                          "remaining arguments go to it")
@@ -242,6 +249,26 @@ def main():
         line["phase_GBps"] = {nm: {p: round(v[1] / (v[0] * 1e-3) / 1e9, 1)
                                    for p, v in lg["prof"].items() if v[0] > 0 and v[1] > 0}
                               for nm, lg in legs.items() if lg["prof"]}
+    if not args.no_batched:
+        # The batched form of the metric: 32 right-hand sides of one operator as one block solve
+        # (block Gram-Schmidt on the matrix cores, FP32 inside FP64 refinement) against the plain
+        # FP64 DMR solving them one at a time.
+        from gdmr import batch_bench
+        bargs = batch_bench.parse(["--no-profile"])
+        br = batch_bench.run(bargs)
+        line["batched"] = {
+            "workload": br["config"]["workload"],
+            "solves_per_s": br["value"],
+            "first_block_solves_per_s": br["block_first"]["solves_per_s"],
+            "fp64_solves_per_s": br["fp64_sequential"]["solves_per_s"],
+            "fp64_sample": br["fp64_sequential"]["sample"],
+            "speedup_vs_fp64": br["speedup_vs_fp64"],
+            "speedup_first_block_vs_fp64": br["speedup_first_block_vs_fp64"],
+            "max_true_rel_residual": max(br["block_first"]["max_true_rel_residual"],
+                                         br["block_recycled"]["max_true_rel_residual"]),
+            "max_rel_err_vs_fp64_solution": br["max_rel_err_vs_fp64_solution"],
+            "tuning": br["tuning"],
+        }
     if not args.no_cpu:
         ref = out_legs.get("fp64", hs)
         cyc = float(np.mean(ref["cycles"]))

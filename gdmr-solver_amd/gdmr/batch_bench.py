@@ -32,7 +32,7 @@ def _log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def main(argv=None):
+def parse(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--grid", type=int, default=96)
     ap.add_argument("--rhs", type=int, default=32)
@@ -44,7 +44,11 @@ def main(argv=None):
     ap.add_argument("--fp64-sample", type=int, default=4)
     ap.add_argument("--seed", type=int, default=96)
     ap.add_argument("--no-profile", action="store_true")
-    args = ap.parse_args(argv)
+    return ap.parse_args(argv)
+
+
+def run(args) -> dict:
+    """Runs both legs; returns the result record (printed as one JSON line by main)."""
     import torch
     assert torch.cuda.is_available(), "the batched benchmark needs a HIP device"
     N, s, tol = args.grid, args.rhs, args.tol
@@ -145,6 +149,11 @@ def main(argv=None):
                              for p, v in prof.items() if v[0] > 0 and v[1] > 0}
         out["phase_TFLOPs"] = {p: round(v[3] / (v[0] * 1e-3) / 1e12, 2)
                                for p, v in prof.items() if v[0] > 0 and v[3] > 0}
+    return out
+
+
+def main(argv=None):
+    out = run(parse(argv))
     print(json.dumps(out), flush=True)
     return out
 

@@ -120,6 +120,10 @@ struct dmrContext {
   void *bigPool = nullptr;
   void *smallPool = nullptr;
   double *hostStatus = nullptr;  // pinned
+  // Side stream (default priority) on which the Hessenberg SVD of a cycle runs while the main
+  // stream does the least squares and the residual / solution updates.
+  hipStream_t side = nullptr;
+  hipEvent_t evH = nullptr, evSvd = nullptr;
   Work<double> w64;
   Work<float> w32;
   Work<dmr::bf16> w16;  // bf16 basis storage (DMR_PREC_BF16)
@@ -281,6 +285,9 @@ int ctx_alloc(dmrContext *c) {
   const double one = 1.0;
   HIPCHK(hipMemcpy(c->one, &one, sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipHostMalloc((void **)&c->hostStatus, 64 * sizeof(double), hipHostMallocDefault));
+  HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&c->evH, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&c->evSvd, hipEventDisableTiming));
   return DMR_OK;
 }
 
@@ -295,6 +302,9 @@ void ctx_free(dmrContext *c) {
   if (c->ownedBlockRows) hipFree(c->ownedBlockRows);
   if (c->ownedVal32) hipFree(c->ownedVal32);
   if (c->hostStatus) hipHostFree(c->hostStatus);
+  if (c->evH) hipEventDestroy(c->evH);
+  if (c->evSvd) hipEventDestroy(c->evSvd);
+  if (c->side) hipStreamDestroy(c->side);
   delete c;
 }
 
@@ -526,6 +536,20 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
         orthogonalize(w.g, vj, w.P, d > 0 ? k : 0, hcol, Po, j, hcol + d, 3, fin, c->flag);
     }
     dmr::breakdown_fixup<TS>(w.P, ld, n, o, d, m, c->H, c->flag, s);
+    // ---- genDeflatDMR, first half (gDMR.cu:1583-1704): the SVD of H needs H only, so it runs on
+    // the side stream, overlapped with the least squares and the updates below.
+    const bool refresh = k > 0;
+    if (refresh) {
+      if (hipEventRecord(c->evH, s) != hipSuccess ||
+          hipStreamWaitEvent(c->side, c->evH, 0) != hipSuccess)
+        return set_err(DMR_ERR_HIP, "side stream fork failed");
+      timed(c, c->side, DMR_PH_SVD, 0.0, [&] {
+        dmr::svd_smallest(c->H, m, k, kSvdTol, kSvdMaxSweeps, c->C, c->sig, c->sweeps, c->side);
+        return 0;
+      });
+      if (hipEventRecord(c->evSvd, c->side) != hipSuccess)
+        return set_err(DMR_ERR_HIP, "side stream join failed");
+    }
     // ---- updateAppResDMR (gDMR.cu:2411-2582)
     timed(c, s, DMR_PH_GEMVT, vb * (m + 2), [&] {
       dmr::multidot<TS>(w.P, ld, m + 1, w.r, n, c->t, 1, red, nullptr, s);
@@ -554,6 +578,18 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
         return 0;
       });
     }
+    // ---- genDeflatDMR, second half: Z_new = normalize([V Z] C) into Zn, adopted below unless
+    // the cycle broke down.
+    if (refresh) {
+      if (hipStreamWaitEvent(s, c->evSvd, 0) != hipSuccess)
+        return set_err(DMR_ERR_HIP, "side stream join failed");
+      timed(c, s, DMR_PH_ZGEMM, vb * (m + k) + vb * 2 * k, [&] {
+        dmr::combo_multi<TS>(w.Zn, k, Po, m - d, c->C + d, w.Z, d, c->C, m, ld, n, c->scalZ, red,
+                             s);
+        dmr::scale_cols<TS>(w.Zn, k, ld, n, c->scalZ, s);
+        return 0;
+      });
+    }
     if (hipMemcpyAsync(c->hostStatus, c->scal2, 4 * sizeof(double), hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
         hipMemcpyAsync(c->hostStatus + 8, c->flag, sizeof(int), hipMemcpyDeviceToHost, s) !=
@@ -572,29 +608,17 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
     std::memcpy(&sweeps, c->hostStatus + 10, sizeof(int));
     if (derr) return set_err(DMR_ERR_HIP, "device grid barrier exceeded its wait bound");
     if (bflag) ++c->st.breakdowns;
-    // The sweep count read here belongs to the previous cycle's refresh (0 before the first).
-    if (sweeps > 0 && c->st.cycles > 0) {
-      c->st.svdSweeps += sweeps;
-      ++c->st.svdCalls;
-    }
-    // ---- genDeflatDMR (gDMR.cu:1583-1704): Z <- normalize([V Z] C), C = smallest right
-    // singular vectors of H.  After a breakdown the basis beyond the breakdown step is zero and H
-    // is padded with identity columns, so singular vectors from the padding combine zero vectors
-    // (the reference then divides by sqrt(0)); the refresh is skipped for such a cycle and the
-    // previous subspace is kept.
-    if (k > 0 && bflag == 0) {
-      timed(c, s, DMR_PH_SVD, 0.0, [&] {
-        dmr::svd_smallest(c->H, m, k, kSvdTol, kSvdMaxSweeps, c->C, c->sig, c->sweeps, s);
-        return 0;
-      });
-      timed(c, s, DMR_PH_ZGEMM, vb * (m + k) + vb * 2 * k, [&] {
-        dmr::combo_multi<TS>(w.Zn, k, Po, m - d, c->C + d, w.Z, d, c->C, m, ld, n, c->scalZ, red,
-                             s);
-        dmr::scale_cols<TS>(w.Zn, k, ld, n, c->scalZ, s);
-        return 0;
-      });
+    // Z <- the refreshed subspace (genDeflatDMR).  After a breakdown the basis beyond the
+    // breakdown step is zero and H is padded with identity columns, so singular vectors from the
+    // padding combine zero vectors (the reference then divides by sqrt(0)); such a refresh is
+    // discarded and the previous subspace is kept.
+    if (refresh && bflag == 0) {
       std::swap(w.Z, w.Zn);
       w.hasZ = true;
+      if (sweeps > 0) {
+        c->st.svdSweeps += sweeps;
+        ++c->st.svdCalls;
+      }
     }
     if (q > kStagnation) {
       c->kryLoopFlag = 1;

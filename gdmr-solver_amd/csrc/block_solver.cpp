@@ -72,7 +72,7 @@ struct dmrBlockContext {
   struct Tuning {
     int tnRowsProj = 512, tnVarProj = 0;  // gemm_tn for projections (p > s)
     int tnRowsGram = 512, tnVarGram = 0;  // gemm_tn for Gram matrices (p = s)
-    int nnUnroll = 4;                     // gemm_nn
+    int nnUnroll = 8;                     // gemm_nn loads in flight per wavefront (KiB)
     int spmmCols = 8;                     // SpMM columns per thread
     int s = 0;
   } tune;
@@ -355,7 +355,7 @@ void autotune(dmrBlockContext *c, int s, hipStream_t st) {
       }
     }
   float bn = 1e30f;
-  for (int un : {1, 4, 8}) {
+  for (int un : {4, 8, 16}) {
     const float tt = best_of([&] {
       dmr::blk::gemm_nn(c->P, c->ld, pFull, c->C32, c->Xa, c->ld, 1.0f, 0.0f, c->Xa, c->ld, s,
                         c->n, un, st);

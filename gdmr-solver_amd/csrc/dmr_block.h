@@ -42,7 +42,7 @@ void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu
              long long n, int G, int variant, float *part, double *C64, int ldc, float *C32,
              hipStream_t st);
 // Out = a In + b P[:, 0:p] C (C row-major p x 32; In unused when a == 0).  Out may alias In or P.
-// un (1, 4 or 8): column pairs of P loaded ahead of their MFMAs.
+// un (2, 4, 8 or 16): column pairs of P loaded ahead of their MFMAs.
 void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *In, long long ldi,
              float a, float b, float *Out, long long ldo, int s, long long n, int un,
              hipStream_t st);

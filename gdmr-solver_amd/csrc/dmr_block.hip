@@ -240,7 +240,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn(const float *__restrict__ 
   }
 }
 
-// LDS-staged variant (DMR_BLOCK_TN_LDS=1): each 32-row chunk of P and U is transposed through LDS.
+// LDS-staged variant: each 32-row chunk of P and U is transposed through LDS.
 __global__ __launch_bounds__(kThreads) void k_gemm_tn_lds(const float *__restrict__ P, long long ldp,
                                                       int p, int pPad,
                                                       const float *__restrict__ U, long long ldu,
@@ -339,17 +339,35 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn_reduce(const float *__rest
 
 // ---------------------------------------------------------------------------------------------
 // Out[r, j] = a In[r, j] + b sum_k P[r, k] C[k][j] for j < s, r < n.  C: p x 32 row-major.
-// Out may alias In or P (each row depends on its own row only).  UN: column pairs of P whose
-// loads are issued ahead of their MFMAs.
-template <int UN>
+// Out may alias In or P (each row depends on its own row only).  A wavefront owns 128 rows as
+// four interleaved 32-row MFMA tiles; lane (i, h) loads rows r0 + 4i .. r0 + 4i + 3 of column
+// c + h as one 16-byte vector, and the loads run D column pairs ahead of their MFMAs (a register
+// ring), so each wavefront keeps D KiB in flight.
+__device__ inline float4 load_rows4(const float *P, long long ldp, int cc, int p, long long rl,
+                                    long long n) {
+  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (cc < p) {
+    const float *src = P + cc * ldp + rl;
+    if (rl + 3 < n) {
+      v = *reinterpret_cast<const float4 *>(src);
+    } else {
+      if (rl < n) v.x = src[0];
+      if (rl + 1 < n) v.y = src[1];
+      if (rl + 2 < n) v.z = src[2];
+    }
+  }
+  return v;
+}
+
+template <int D>
 __global__ __launch_bounds__(kThreads) void k_gemm_nn(const float *P, long long ldp, int p,
                                                       const float *__restrict__ C,
                                                       const float *In, long long ldi, float a,
                                                       float b, float *Out, long long ldo, int s,
                                                       long long n) {
   extern __shared__ float Cs[];
-  const int pEven = (p + 1) & ~1;
-  for (int idx = threadIdx.x; idx < pEven * 32; idx += kThreads)
+  const int pPad = (p + 2 * D - 1) / (2 * D) * (2 * D) + 2 * D;  // ring reads past p are zero
+  for (int idx = threadIdx.x; idx < pPad * 32; idx += kThreads)
     Cs[idx] = (idx >> 5) < p ? C[idx] : 0.0f;
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -359,32 +377,23 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nn(const float *P, long long 
 #pragma unroll
   for (int v = 0; v < 16; ++v) acc0[v] = acc1[v] = acc2[v] = acc3[v] = 0.0f;
   const long long rl = r0 + 4 * (lane & 31);
-  const int kh = lane >> 5;
-#pragma unroll UN
-  for (int c = 0; c < p; c += 2) {
-    const int cc = c + kh;
-    float x0 = 0.f, x1 = 0.f, x2 = 0.f, x3 = 0.f;
-    if (cc < p) {
-      const float *src = P + cc * ldp + rl;
-      if (rl + 3 < n) {
-        const float4 v = *reinterpret_cast<const float4 *>(src);
-        x0 = v.x;
-        x1 = v.y;
-        x2 = v.z;
-        x3 = v.w;
-      } else {
-        if (rl < n) x0 = src[0];
-        if (rl + 1 < n) x1 = src[1];
-        if (rl + 2 < n) x2 = src[2];
-      }
+  const int kh = lane >> 5, jl = lane & 31;
+  float4 ring[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) ring[i] = load_rows4(P, ldp, 2 * i + kh, p, rl, n);
+  for (int c = 0; c < p; c += 2 * D) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const float4 x = ring[i];
+      ring[i] = load_rows4(P, ldp, c + 2 * (i + D) + kh, p, rl, n);
+      const float bb = Cs[(c + 2 * i + kh) * 32 + jl];
+      acc0 = mfma32(x.x, bb, acc0);
+      acc1 = mfma32(x.y, bb, acc1);
+      acc2 = mfma32(x.z, bb, acc2);
+      acc3 = mfma32(x.w, bb, acc3);
     }
-    const float bb = Cs[cc * 32 + (lane & 31)];
-    acc0 = mfma32(x0, bb, acc0);
-    acc1 = mfma32(x1, bb, acc1);
-    acc2 = mfma32(x2, bb, acc2);
-    acc3 = mfma32(x3, bb, acc3);
   }
-  const int j = lane & 31;
+  const int j = jl;
   if (j >= s) return;
 #pragma unroll
   for (int v = 0; v < 16; ++v) {
@@ -827,15 +836,20 @@ void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *
              hipStream_t st) {
   const long long waves = (n + 127) / 128;
   const unsigned grid = (unsigned)((waves + 3) / 4);
-  const size_t lds = sizeof(float) * (size_t)(((p + 1) & ~1) * 32);
-  if (un >= 8)
+  const int D = un >= 16 ? 16 : (un >= 8 ? 8 : (un >= 4 ? 4 : 2));
+  const int pPad = (p + 2 * D - 1) / (2 * D) * (2 * D) + 2 * D;
+  const size_t lds = sizeof(float) * (size_t)pPad * 32;
+  if (D == 16)
+    hipLaunchKernelGGL(k_gemm_nn<16>, dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C, In, ldi,
+                       a, b, Out, ldo, s, n);
+  else if (D == 8)
     hipLaunchKernelGGL(k_gemm_nn<8>, dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C, In, ldi, a,
                        b, Out, ldo, s, n);
-  else if (un >= 4)
+  else if (D == 4)
     hipLaunchKernelGGL(k_gemm_nn<4>, dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C, In, ldi, a,
                        b, Out, ldo, s, n);
   else
-    hipLaunchKernelGGL(k_gemm_nn<1>, dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C, In, ldi, a,
+    hipLaunchKernelGGL(k_gemm_nn<2>, dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C, In, ldi, a,
                        b, Out, ldo, s, n);
 }
 

@@ -29,6 +29,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gdmr-solver_amd"))
 
+PROFILE_SYSTEMS = 4  # systems re-solved with per-launch profiling after the timed sequence
 METRIC = ("solves/sec @1e-10 rel residual (FP64-checked), mixed+batched vs FP64; p50/p99")
 HBM_PEAK = 8.0e12
 
@@ -69,8 +70,6 @@ def run_leg(name, prec, mats, rhs, dev_vals, dev_vals32, rp, ci, m, k, tol, warm
               dev_vals32[0].data_ptr() if prec != gdmr.PREC_FP64 else 0)
     if prec != gdmr.PREC_FP64:
         s.set_refinement(inner_tol)
-    if profile:
-        s.set_profiling(True)
     bs = [torch.from_numpy(b).to("cuda") for b in rhs]
     xs = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in rhs]
     lat, cycles, reported = [], [], []
@@ -89,7 +88,20 @@ def run_leg(name, prec, mats, rhs, dev_vals, dev_vals32, rp, ci, m, k, tol, warm
             f"status {st.status}, svd sweeps/call {st.svdSweeps / max(st.svdCalls, 1):.1f}")
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
-    prof = s.profile() if profile else None
+    prof = None
+    if profile:
+        # Phase profile (HIP events around every launch, which slows the launches down) on a
+        # re-solve of the first systems after the timed sequence; not part of the timed numbers.
+        s.set_profiling(True)
+        xp = torch.zeros(n, dtype=torch.float64, device="cuda")
+        for t in range(min(PROFILE_SYSTEMS, len(mats))):
+            s.set_values(dev_vals[t].data_ptr(),
+                         dev_vals32[t].data_ptr() if prec != gdmr.PREC_FP64 else 0)
+            xp.zero_()
+            torch.cuda.synchronize()
+            s.solve(xp.data_ptr(), bs[t].data_ptr(), tol, 100000, mode=1)
+        prof = s.profile()
+        s.set_profiling(False)
     s.close()
     true = []
     for t, A in enumerate(mats):
@@ -98,6 +110,53 @@ def run_leg(name, prec, mats, rhs, dev_vals, dev_vals32, rp, ci, m, k, tol, warm
     xs_host = [x.cpu().numpy() for x in xs]
     return dict(total=total, lat=lat, cycles=cycles, reported=reported, true=true, prof=prof,
                 x=xs_host)
+
+
+def run_block1_leg(mats, rhs, dev_vals, rp, ci, m, k, tol, warmup, inner_tol=1e-5):
+    """The sequence through the block solver with one right-hand side per system: block steps
+    m - k (the same basis size per cycle), recycled subspace of k vectors, FP32 inside FP64
+    refinement."""
+    import torch
+
+    import gdmr
+    n, mb = mats[0].n, m - k
+    for w in range(warmup):
+        s = gdmr.BlockSolver(n, mb, k, 1)
+        s.set_csr(rp.data_ptr(), ci.data_ptr(), dev_vals[0].data_ptr(), mats[0].nnz)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        b = torch.from_numpy(rhs[0]).to("cuda")
+        torch.cuda.synchronize()
+        s.solve(x.data_ptr(), n, b.data_ptr(), n, 1, tol, max_cycles=100000)
+        s.close()
+        log(f"[block1] warm-up {w} done")
+    s = gdmr.BlockSolver(n, mb, k, 1)
+    s.set_csr(rp.data_ptr(), ci.data_ptr(), dev_vals[0].data_ptr(), mats[0].nnz)
+    s.set_refinement(inner_tol, 60)
+    bs = [torch.from_numpy(b).to("cuda") for b in rhs]
+    xs = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in rhs]
+    lat, cycles, reported = [], [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(len(mats)):
+        ts = time.perf_counter()
+        s.set_values(dev_vals[t].data_ptr())
+        rc, rel = s.solve(xs[t].data_ptr(), n, bs[t].data_ptr(), n, 1, tol, max_cycles=100000,
+                          mode=1 if t else 0)
+        lat.append(time.perf_counter() - ts)
+        st, _ = s.stats()
+        cycles.append(st.cycles)
+        reported.append(float(rel[0]))
+        log(f"[block1] system {t}: {lat[-1]*1e3:.1f} ms, cycles {st.cycles}, residual {rel[0]:.3e}, "
+            f"status {rc}")
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    s.close()
+    true = []
+    for t, A in enumerate(mats):
+        x = xs[t].cpu().numpy()
+        true.append(float(np.linalg.norm(rhs[t] - A.matvec(x)) / np.linalg.norm(rhs[t])))
+    return dict(total=total, lat=lat, cycles=cycles, reported=reported, true=true, prof=None,
+                x=[x.cpu().numpy() for x in xs])
 
 
 def summarize(leg, tol):
@@ -159,7 +218,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--legs", default="fp64,mixed",
-                    help="comma list of fp64, mixed (FP32 basis) and bf16 (bf16 basis storage)")
+                    help="comma list of fp64, mixed (FP32 basis), bf16 (bf16 basis storage) and "
+                         "block1 (the block solver with one right-hand side)")
     ap.add_argument("--cpu-cycles", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -204,6 +264,10 @@ def main():
     torch.cuda.synchronize()
     legs = {}
     for name in args.legs.split(","):
+        if name == "block1":
+            legs[name] = run_block1_leg(mats, rhs, dev_vals, rp, ci, m, k, tol, args.warmup,
+                                        args.inner_tol)
+            continue
         prec = {"fp64": gdmr.PREC_FP64, "mixed": gdmr.PREC_FP32, "bf16": gdmr.PREC_BF16}[name]
         inner = args.bf16_inner_tol if name == "bf16" else args.inner_tol
         legs[name] = run_leg(name, prec, mats, rhs, dev_vals, dev_vals32, rp, ci, m, k, tol,
@@ -229,7 +293,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "mixed": "fp32+fp64", "bf16": "bf16+fp32+fp64"}[head],
+        "dtype": {"fp64": "f64", "mixed": "fp32+fp64", "bf16": "bf16+fp32+fp64",
+                  "block1": "fp32+fp64"}[head],
         "data": "synthetic",
         "config": {"workload": f"3-D conv-diff 7-pt {N}^3 sequence of {args.steps} jittered "
                                f"systems, m={m} k={k} tol {tol:g}, Z recycled",

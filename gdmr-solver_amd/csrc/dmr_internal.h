@@ -15,7 +15,7 @@ constexpr int kBlock = 256;
 constexpr int kMaxRedBlocks = 1024;
 constexpr int kDefaultRedBlocks = 512;
 // Widest column group of the multi-dot (GEMV-T) kernel.
-constexpr int kDotCols = 32;
+constexpr int kDotCols = 64;
 // Largest single-output coefficient count the combination kernels stage in LDS.
 constexpr int kMaxCoef = 256;
 // Row blocks of the CSR SpMV hold at most this many nonzeros (staged in LDS) and 256 rows.

@@ -145,12 +145,55 @@ __device__ inline bool arrive_last(unsigned *counter, unsigned total) {
 }
 
 // Sum of G block partials in block order, one wavefront per column: lane l adds partials l,
-// l+64, ... in order, then the xor butterfly.  Returns the sum in every lane.
+// l+64, ... in order, then the xor butterfly.  Returns the sum in every lane.  The loads are
+// issued sixteen per lane at a time, ahead of the adds, so the last block waits one memory
+// round trip per 1024 partials instead of one per 64.
 template <typename TA> __device__ inline TA wave_sum_partials(const TA *part, int G) {
   const int lane = threadIdx.x & 63;
   TA v = 0;
-  for (int q = lane; q < G; q += 64) v = v + get_partial(part + q);
+  for (int q0 = 0; q0 < G; q0 += 64 * 16) {
+    TA t[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int q = q0 + lane + 64 * i;
+      t[i] = q < G ? get_partial(part + q) : TA(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      if (q0 + lane + 64 * i < G) v = v + t[i];
+  }
   return wave_sum(v);
+}
+
+// Column sums of partial columns c0..c0+pc-1 (column-major, stride G <= 64 * NQ), each in the
+// order of wave_sum_partials.  Wavefront w takes CPB = min(8, 16 / NQ) adjacent columns at a
+// time (columns w*CPB.., then +4*CPB), all their loads in flight together.  Column c's sum lands
+// in res[c].
+template <typename TA, int NQ, typename TR>
+__device__ inline void sum_cols_nq(const TA *part, int G, int c0, int pc, TR *res) {
+  constexpr int CPB = NQ >= 2 ? 16 / NQ : 8;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int cb = wv * CPB; cb < pc; cb += (kBlock / 64) * CPB) {
+    TA t[CPB][NQ];
+#pragma unroll
+    for (int cc = 0; cc < CPB; ++cc)
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) {
+        const int q = lane + 64 * i;
+        t[cc][i] = (cb + cc < pc && q < G)
+                       ? get_partial(part + (long long)(c0 + cb + cc) * G + q)
+                       : TA(0);
+      }
+#pragma unroll
+    for (int cc = 0; cc < CPB; ++cc) {
+      TA v = 0;
+#pragma unroll
+      for (int i = 0; i < NQ; ++i)
+        if (lane + 64 * i < G) v = v + t[cc][i];
+      v = wave_sum(v);
+      if (lane == 0 && cb + cc < pc) res[cb + cc] = (TR)v;
+    }
+  }
 }
 
 __device__ inline void finalize(const Fin &fin, double ss, double sg) {
@@ -186,12 +229,13 @@ __device__ inline void finalize(const Fin &fin, double ss, double sg) {
 template <typename TS, typename TA, int N>
 __device__ inline void fma_cols(const TS *__restrict__ B, int p, long long ld, long long u,
                                 const TA *cs, TA (&o)[N]) {
-  for (int l0 = 0; l0 < p; l0 += 8) {
-    TA b[8][N];
+  // Sixteen columns' loads in flight at a time; the FMAs stay in column order.
+  for (int l0 = 0; l0 < p; l0 += 16) {
+    TA b[16][N];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) ldu(B + (long long)min(l0 + i, p - 1) * ld, u, b[i]);
+    for (int i = 0; i < 16; ++i) ldu(B + (long long)min(l0 + i, p - 1) * ld, u, b[i]);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < 16; ++i) {
       if (l0 + i < p) {
         const TA c = cs[l0 + i];
 #pragma unroll
@@ -226,15 +270,23 @@ __device__ inline void block_sum_cols(TA (&v)[CB], TA *res) {
   __syncthreads();
 }
 
-// Last-block sum of the G partials of columns c0..c0+pc-1 (column-major, stride G): wavefront
-// w takes columns w, w+4, ... (wave_sum_partials each).  Column c's sum lands in res[c].
-template <typename TA>
-__device__ inline void sum_partials_cols(const TA *part, int G, int c0, int pc, TA *res) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int c = wv; c < pc; c += kBlock / 64) {
-    const TA s = wave_sum_partials(part + (long long)(c0 + c) * G, G);
-    if (lane == 0) res[c] = s;
-  }
+// Last-block sum of the G partials (G <= kMaxRedBlocks = 1024) of columns c0..c0+pc-1
+// (column-major, stride G), each in the order of wave_sum_partials.  Column c's sum lands in
+// res[c].
+template <typename TA, typename TR>
+__device__ inline void sum_partials_cols(const TA *part, int G, int c0, int pc, TR *res) {
+  static_assert(kMaxRedBlocks <= 64 * 16, "sixteen partials per lane cover every row block");
+  const int nq = (G + 63) / 64;
+  if (nq <= 1)
+    sum_cols_nq<TA, 1>(part, G, c0, pc, res);
+  else if (nq <= 2)
+    sum_cols_nq<TA, 2>(part, G, c0, pc, res);
+  else if (nq <= 4)
+    sum_cols_nq<TA, 4>(part, G, c0, pc, res);
+  else if (nq <= 8)
+    sum_cols_nq<TA, 8>(part, G, c0, pc, res);
+  else
+    sum_cols_nq<TA, 16>(part, G, c0, pc, res);
   __syncthreads();
 }
 
@@ -257,20 +309,24 @@ __global__ __launch_bounds__(kBlock) void k_multidot(const TS *__restrict__ B, l
   for (long long u = (long long)blockIdx.x * kBlock + threadIdx.x; u < nv; u += stride) {
     TA xv[N];
     ldu(x, u, xv);
-    // CH columns at a time, their loads issued together; indices past the group alias its
-    // last column (a cache hit) and their sums are discarded.
-    constexpr int CH = CB < 8 ? CB : 8;
+    // The loads of CH columns of the unit issued together: all of a group of up to 32 columns
+    // (one memory round trip per unit), else chunks of 16 columns (measured faster than 32 for
+    // FP32 groups of 40-56: fewer registers, more wavefronts), the last chunk cut at CB.
+    // Indices past the group alias its last column (a cache hit) and their sums are discarded.
+    constexpr int CH = CB <= 32 && N <= 4 ? CB : CB < 16 ? CB : 16;
 #pragma unroll
     for (int cc = 0; cc < CB; cc += CH) {
       if (cc < pc) {
         TA bv[CH][N];
 #pragma unroll
         for (int i = 0; i < CH; ++i)
-          ldu(B + (long long)(c0 + min(cc + i, pc - 1)) * ld, u, bv[i]);
+          if (cc + i < CB) ldu(B + (long long)(c0 + min(cc + i, pc - 1)) * ld, u, bv[i]);
 #pragma unroll
         for (int i = 0; i < CH; ++i)
+          if (cc + i < CB) {
 #pragma unroll
-          for (int e = 0; e < N; ++e) acc[cc + i] = fmac(bv[i][e], xv[e], acc[cc + i]);
+            for (int e = 0; e < N; ++e) acc[cc + i] = fmac(bv[i][e], xv[e], acc[cc + i]);
+          }
       }
     }
   }
@@ -548,12 +604,7 @@ __device__ inline void orth_dots(const TS *__restrict__ Bm, int p, long long ld,
 template <typename TA>
 __device__ inline void orth_finish_cols(const TA *partials, int G, int p, double *out) {
   __shared__ double fin[kMaxCoef];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int c = wv; c < p; c += kBlock / 64) {
-    const TA s = wave_sum_partials(partials + (long long)c * G, G);
-    if (lane == 0) fin[c] = (double)s;
-  }
-  __syncthreads();
+  sum_partials_cols<TA, double>(partials, G, 0, p, fin);
   if (threadIdx.x == 0)
     for (int c = 0; c < p; ++c) out[c] = fin[c];
 }
@@ -958,9 +1009,11 @@ void multidot(const TS *B, long long ld, int p, const TS *x, long long n, double
   using TA = typename Acc<TS>::type;
   const int G = red_blocks<TS>(n);
   const long long nv = units<TS>(n);
-  // Column groups of at most kDotCols columns, balanced; a group's width is rounded up to one
-  // of 1, 2, 4 or a multiple of 8 (the kernel template), so x is read once per group.
-  const int groups = (p + kDotCols - 1) / kDotCols;
+  // Column groups of at most kDotCols columns (56 for FP64, whose 64-wide accumulators would
+  // leave one wavefront per SIMD), balanced; a group's width is rounded up to one of 1, 2, 4 or
+  // a multiple of 8 (the kernel template), so x is read once per group.
+  const int widest = sizeof(TS) == 8 ? kDotCols - 8 : kDotCols;
+  const int groups = (p + widest - 1) / widest;
   const int per = (p + groups - 1) / groups;
   const int cb = per <= 4 ? (per <= 1 ? 1 : per <= 2 ? 2 : 4) : (per + 7) / 8 * 8;
   TA *part = (TA *)w.partials;
@@ -974,9 +1027,13 @@ void multidot(const TS *B, long long ld, int p, const TS *x, long long n, double
     case 8: DMR_MDOT(8); break;
     case 16: DMR_MDOT(16); break;
     case 24: DMR_MDOT(24); break;
-    default: DMR_MDOT(32); break;
+    case 32: DMR_MDOT(32); break;
+    case 40: DMR_MDOT(40); break;
+    case 48: DMR_MDOT(48); break;
+    case 56: DMR_MDOT(56); break;
+    default: DMR_MDOT(64); break;
   }
-  static_assert(kDotCols == 32, "multidot launcher covers group widths up to 32");
+  static_assert(kDotCols == 64, "multidot launcher covers group widths up to 64");
 #undef DMR_MDOT
 }
 

@@ -5,13 +5,17 @@ Workload (the 3-D configuration of the DMR north star): 3-D convection-diffusion
 is jittered U(-1%, +1%) per step (seeded), N(0,1) right-hand sides, m = 50, k = 20, tol 1e-10,
 deflation subspace recycled across the sequence (mode = 0 for the first system, 1 after).
 
-Two legs run on the same sequence, each in a fresh solver context:
-  * fp64  -- the plain FP64 DMR (the parity-checked path);
-  * mixed -- FP32 Krylov basis and SpMV with FP32-accumulated reductions inside FP64 iterative
-             refinement; Hessenberg LSQ / SVD in FP64.
+Three legs run on the same sequence, each in a fresh solver context:
+  * fp64     -- the plain FP64 DMR, one system after the other (the parity-checked path);
+  * mixed    -- FP32 Krylov basis and SpMV with FP32-accumulated reductions inside FP64 iterative
+                refinement; Hessenberg LSQ / SVD in FP64; one system after the other;
+  * seqblock -- mixed + batched: the 16 systems as one block solve.  Column t is refined in FP64
+                against its own operator A_t; the inner FP32 block DMR (block Gram-Schmidt as
+                tall-skinny GEMMs on the matrix cores, 16 block steps, recycled subspace k = 20)
+                shares the entrywise mean of the A_t.  Every system's latency is the block's.
 A step is one solve of one system.  `value` is the solves/sec of the fastest leg whose every
-system reaches 1e-10 checked in FP64 (the mixed leg).  Every solve's true residual
-||b - A x|| / ||b|| is recomputed on the host in FP64 after the timed region.
+system reaches 1e-10 checked in FP64.  Every solve's true residual ||b - A x|| / ||b|| (with its
+own operator) is recomputed on the host in FP64 after the timed region.
 
 The line also carries the batched form of the metric ("batched"): 3-D anisotropic diffusion 96^3
 with 32 right-hand sides solved as one block (block DMR on the matrix cores, FP32 inside FP64
@@ -159,6 +163,60 @@ def run_block1_leg(mats, rhs, dev_vals, rp, ci, m, k, tol, warmup, inner_tol=1e-
                 x=[x.cpu().numpy() for x in xs])
 
 
+def run_seqblock_leg(mats, rhs, dev_vals, rp, ci, mb, k, tol, warmup, inner_tol, profile):
+    """The whole sequence as one block solve: column t is system t with its own FP64 operator
+    A_t in the refinement residual, while the inner FP32 block DMR (block Gram-Schmidt on the
+    matrix cores) shares the entrywise mean of the A_t (dmrBlockSetColumnOperators).  Every
+    column's latency is the block's."""
+    import torch
+
+    import gdmr
+    n, nnz, s = mats[0].n, mats[0].nnz, len(mats)
+    vals = torch.stack(dev_vals).contiguous()  # s x nnz, column t's operator at row t
+    B = torch.stack([torch.from_numpy(b) for b in rhs]).to("cuda").contiguous()
+
+    def make():
+        h = gdmr.BlockSolver(n, mb, k, s)
+        h.set_csr(rp.data_ptr(), ci.data_ptr(), vals[0].data_ptr(), nnz)
+        h.set_column_operators(vals.data_ptr(), nnz, s, inner_mean=True)
+        h.set_refinement(inner_tol, 60)
+        return h
+
+    for w in range(warmup):
+        h = make()
+        X = torch.zeros(s, n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        h.solve(X.data_ptr(), n, B.data_ptr(), n, s, tol, max_cycles=100000)
+        h.close()
+        log(f"[seqblock] warm-up {w} done")
+    h = make()
+    X = torch.zeros(s, n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rc, rel = h.solve(X.data_ptr(), n, B.data_ptr(), n, s, tol, max_cycles=100000)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    st, hist = h.stats()
+    log(f"[seqblock] {s} systems as one block: {total*1e3:.1f} ms, rc {rc}, {st.cycles} cycles, "
+        f"max residual {rel.max():.3e}")
+    prof = None
+    if profile:
+        # Phase profile (HIP events around every launch, eager launches instead of graph
+        # replays) on a re-solve from the recycled subspace; not part of the timed numbers.
+        h.set_profiling(True)
+        Xp = torch.zeros(s, n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        h.solve(Xp.data_ptr(), n, B.data_ptr(), n, s, tol, max_cycles=100000, mode=1)
+        prof = h.profile()
+        h.set_profiling(False)
+    h.close()
+    Xh = X.cpu().numpy()
+    true = [float(np.linalg.norm(rhs[t] - A.matvec(Xh[t])) / np.linalg.norm(rhs[t]))
+            for t, A in enumerate(mats)]
+    return dict(total=total, lat=[total] * s, cycles=[st.cycles] * s, reported=list(rel),
+                true=true, prof=prof, x=[Xh[t] for t in range(s)], block=True)
+
+
 def summarize(leg, tol):
     lat = np.array(leg["lat"])
     return {
@@ -172,17 +230,37 @@ def summarize(leg, tol):
     }
 
 
-def roofline(prof):
-    # Dominant streaming kernel of the cycle: the multi-dot (GEMV-T) or the Gram-Schmidt update,
-    # whichever took more time; achieved = algorithmic bytes / kernel time.
-    best = max(("gemvt", "update"), key=lambda p: prof[p][0])
-    ms, by, cnt = prof[best]
+MFMA_FP32_PEAK = 157.3e12
+
+
+def roofline(prof, block=False):
+    """The dominant streaming kernel of the headline leg; achieved = algorithmic bytes / kernel
+    time (HIP events around each launch on the solve stream).  Single-vector legs: the multi-dot
+    (GEMV-T) or the Gram-Schmidt update, whichever took more time.  Block leg: k_gemm_nn (the
+    block updates, normalisations and the residual and solution updates, every launch of which
+    is timed), with its matrix-core rate beside the HBM one."""
+    if block:
+        ms = prof["update"][0] + prof["combo"][0]
+        by = prof["update"][1] + prof["combo"][1]
+        cnt = prof["update"][2] + prof["combo"][2]
+        fl = prof["update"][3] + prof["combo"][3]
+        kernel = "k_gemm_nn"
+    else:
+        best = max(("gemvt", "update"), key=lambda p: prof[p][0])
+        ms, by, cnt = prof[best][:3]
+        fl = None
+        kernel = "k_multidot" if best == "gemvt" else "k_combo(SUB)"
     ach = by / (ms * 1e-3) if ms > 0 else 0.0
-    return {"bound": "hbm", "kernel": "k_multidot" if best == "gemvt" else "k_combo(SUB)",
-            "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": ach / HBM_PEAK, "traffic": None, "launches": cnt,
-            "avg_launch_us": ms * 1e3 / max(cnt, 1),
-            "bytes_per_launch": by / max(cnt, 1)}
+    out = {"bound": "hbm", "kernel": kernel,
+           "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+           "frac": ach / HBM_PEAK, "traffic": None, "launches": cnt,
+           "avg_launch_us": ms * 1e3 / max(cnt, 1),
+           "bytes_per_launch": by / max(cnt, 1)}
+    if fl:
+        tf = fl / (ms * 1e-3)
+        out["mfma_fp32"] = {"achieved_tflops": tf / 1e12, "peak_tflops": MFMA_FP32_PEAK / 1e12,
+                            "frac": tf / MFMA_FP32_PEAK}
+    return out
 
 
 def cpu_baseline(mats, rhs, m, k, tol, gpu_cycles_per_solve, sample_cycles):
@@ -217,9 +295,15 @@ def main():
     ap.add_argument("--k", type=int, default=20)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--legs", default="fp64,mixed",
-                    help="comma list of fp64, mixed (FP32 basis), bf16 (bf16 basis storage) and "
-                         "block1 (the block solver with one right-hand side)")
+    ap.add_argument("--legs", default="fp64,mixed,seqblock",
+                    help="comma list of fp64, mixed (FP32 basis), seqblock (the sequence as one "
+                         "block), bf16 (bf16 basis storage) and block1 (the block solver with one "
+                         "right-hand side)")
+    ap.add_argument("--seq-block-steps", type=int, default=16,
+                    help="block steps per cycle of the seqblock leg")
+    ap.add_argument("--seq-k", type=int, default=20, help="recycled subspace of the seqblock leg")
+    ap.add_argument("--seq-inner-tol", type=float, default=1e-2,
+                    help="relative tolerance of each inner block solve of the seqblock leg")
     ap.add_argument("--cpu-cycles", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -264,6 +348,11 @@ def main():
     torch.cuda.synchronize()
     legs = {}
     for name in args.legs.split(","):
+        if name == "seqblock":
+            legs[name] = run_seqblock_leg(mats, rhs, dev_vals, rp, ci, args.seq_block_steps,
+                                          args.seq_k, tol, args.warmup, args.seq_inner_tol,
+                                          not args.no_profile)
+            continue
         if name == "block1":
             legs[name] = run_block1_leg(mats, rhs, dev_vals, rp, ci, m, k, tol, args.warmup,
                                         args.inner_tol)
@@ -294,7 +383,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "mixed": "fp32+fp64", "bf16": "bf16+fp32+fp64",
-                  "block1": "fp32+fp64"}[head],
+                  "block1": "fp32+fp64", "seqblock": "fp32+fp64"}[head],
         "data": "synthetic",
         "config": {"workload": f"3-D conv-diff 7-pt {N}^3 sequence of {args.steps} jittered "
                                f"systems, m={m} k={k} tol {tol:g}, Z recycled",
@@ -308,7 +397,7 @@ def main():
             line[f"speedup_{nm}_vs_fp64"] = (out_legs[nm]["solves_per_s"] /
                                              out_legs["fp64"]["solves_per_s"])
     if legs[head]["prof"] is not None:
-        line["roofline"] = roofline(legs[head]["prof"])
+        line["roofline"] = roofline(legs[head]["prof"], legs[head].get("block", False))
         line["phase_ms"] = {nm: {p: round(v[0], 3) for p, v in lg["prof"].items()}
                             for nm, lg in legs.items() if lg["prof"]}
         line["phase_GBps"] = {nm: {p: round(v[1] / (v[0] * 1e-3) / 1e9, 1)

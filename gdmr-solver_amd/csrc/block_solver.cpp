@@ -33,7 +33,7 @@
 
 namespace {
 
-constexpr double kSvdTol = 1e-14;
+constexpr double kSvdTol = 1e-14;  // Jacobi stopping tolerance (DMR_BLOCK_SVD_TOL overrides)
 constexpr int kSvdMaxSweeps = 128;
 constexpr double kDepTol = 1e-12;  // Cholesky pivot / max pivot below which a column is dependent
 constexpr int kMaxEvents = 8192;
@@ -55,6 +55,13 @@ struct dmrBlockContext {
   float *ownedVal32 = nullptr;
   bool val32Stale = false;
   bool haveOp = false;
+  // Per-column FP64 operators of the refinement residual (dmrBlockSetColumnOperators): column c
+  // uses colVal + c * colLd; colCount columns at most.  innerMean: the FP32 inner operator is
+  // their entrywise mean, recomputed at the start of every solve.
+  const double *colVal = nullptr;
+  long long colLd = 0;
+  int colCount = 0;
+  bool innerMean = false;
   void *big = nullptr, *small = nullptr;
   float *P = nullptr, *Z = nullptr, *Zn = nullptr, *R = nullptr, *Xa = nullptr;
   double *R64 = nullptr;
@@ -68,6 +75,7 @@ struct dmrBlockContext {
   double *host = nullptr;  // pinned staging
   bool hasZ = false;
   int krMax = 64;  // cap on the leading block steps of the subspace refresh
+  double svdTol = kSvdTol;
   // Launch shapes chosen by the autotuner (per block width s), and whether it ran.
   struct Tuning {
     int tnRowsProj = 512, tnVarProj = 0;  // gemm_tn for projections (p > s)
@@ -80,6 +88,10 @@ struct dmrBlockContext {
   // Captured restart cycles, keyed by (s, deflated, which buffer holds Z).
   bool graphs = true;
   hipStream_t gst = nullptr;
+  // The subspace refresh's SVD runs on a side stream, overlapped with the least squares and the
+  // residual and solution updates.
+  hipStream_t side = nullptr;
+  hipEvent_t evFork = nullptr, evJoin = nullptr;
   struct Cap {
     int s, deflate, parity;
     hipGraphExec_t exec;
@@ -193,6 +205,9 @@ void ctx_release(dmrBlockContext *c) {
   hipDeviceSynchronize();
   drop_graphs(c);
   if (c->gst) hipStreamDestroy(c->gst);
+  if (c->side) hipStreamDestroy(c->side);
+  if (c->evFork) hipEventDestroy(c->evFork);
+  if (c->evJoin) hipEventDestroy(c->evJoin);
   if (c->big) hipFree(c->big);
   if (c->small) hipFree(c->small);
   if (c->ownedVal32) hipFree(c->ownedVal32);
@@ -333,7 +348,7 @@ void autotune(dmrBlockContext *c, int s, hipStream_t st) {
   const int rowsCand[] = {256, 512, 1024, 2048};
   float bp = 1e30f, bg = 1e30f;
   for (int rows : rowsCand)
-    for (int var = 0; var < 2; ++var) {
+    for (int var = 0; var < 3; ++var) {
       const int G = tn_blocks(c, rows);
       const float tp = best_of([&] {
         dmr::blk::gemm_tn(c->P, c->ld, pFull, c->R, c->ld, s, c->n, G, var, c->part, nullptr, 0,
@@ -366,7 +381,7 @@ void autotune(dmrBlockContext *c, int s, hipStream_t st) {
     }
   }
   float bs = 1e30f;
-  for (int cg : {4, 8, 16}) {
+  for (int cg : {4, 8, 16, 104, 108, 116}) {
     const float tt = best_of([&] {
       dmr::blk::spmm32(c->csr, c->R, c->ld, c->Xa, c->ld, s, cg, st);
     });
@@ -449,6 +464,34 @@ bool cycle(dmrBlockContext *c, int s, bool deflate, hipStream_t st) {
     });
     cholqr2(c, U, s, Hcol + pj, rows, st);
   }
+  // Subspace refresh, part 1 (side stream, overlapped with the least squares and the updates):
+  // the k smallest right singular vectors of the leading block of H over [Z V_0 .. V_{kr-1}].
+  bool refreshed = false;
+  int cR = 0;
+  if (k > 0) {
+    // Widest leading block whose SVD fits in LDS (at least k columns).
+    int kr = deflate ? 1 : std::max(1, (k + s - 1) / s);
+    while (kr < std::min(mb, c->krMax) &&
+           dmr::svd_lds(p0 + (kr + 2) * s, p0 + (kr + 1) * s) <= 160 * 1024)
+      ++kr;
+    if (kr <= mb) {
+      cR = p0 + kr * s;
+      const int rR = cR + s;
+      if (cR >= k && dmr::svd_lds(rR, cR) <= 160 * 1024) {
+        hipEventRecord(c->evFork, st);
+        hipStreamWaitEvent(c->side, c->evFork, 0);
+        hipMemcpy2DAsync(c->Hred, sizeof(double) * rR, c->H, sizeof(double) * rows,
+                         sizeof(double) * rR, cR, hipMemcpyDeviceToDevice, c->side);
+        timed(c, c->side, DMR_PH_SVD, 0.0, [&] {
+          dmr::svd_smallest_rect(c->Hred, rR, rR, cR, k, c->svdTol, kSvdMaxSweeps, 1, c->Csvd,
+                                 c->sig, c->flags + 1, c->side);
+          dmr::blk::pack_coef(c->Csvd, cR, k, c->Cz32, c->side);
+        });
+        hipEventRecord(c->evJoin, c->side);
+        refreshed = true;
+      }
+    }
+  }
   // Full projection of the residual block, least squares, residual and solution updates.
   timedf(c, st, DMR_PH_GEMVT, vb * (rows + s), 2.0 * n * rows * s, [&] {
     tn(c, P, rows, c->R, s, c->T, rows, nullptr, false, st);
@@ -468,38 +511,21 @@ bool cycle(dmrBlockContext *c, int s, bool deflate, hipStream_t st) {
       nn(c, c->Z, k, c->Yr32, c->Xa, 1.0f, 1.0f, c->Xa, s, st);
     nn(c, V0, mb * s, c->Yr32 + (long long)p0 * 32, c->Xa, 1.0f, 1.0f, c->Xa, s, st);
   });
-  // Subspace refresh from the leading block of H over [Z V_0 .. V_{kr-1}].
-  bool refreshed = false;
-  if (k > 0) {
-    // Widest leading block whose SVD fits in LDS (at least k columns).
-    int kr = deflate ? 1 : std::max(1, (k + s - 1) / s);
-    while (kr < std::min(mb, c->krMax) &&
-           dmr::svd_lds(p0 + (kr + 2) * s, p0 + (kr + 1) * s) <= 160 * 1024)
-      ++kr;
-    if (kr <= mb) {
-      const int cR = p0 + kr * s, rR = cR + s;
-      if (cR >= k && dmr::svd_lds(rR, cR) <= 160 * 1024) {
-        hipMemcpy2DAsync(c->Hred, sizeof(double) * rR, c->H, sizeof(double) * rows,
-                         sizeof(double) * rR, cR, hipMemcpyDeviceToDevice, st);
-        timed(c, st, DMR_PH_SVD, 0.0, [&] {
-          dmr::svd_smallest_rect(c->Hred, rR, rR, cR, k, kSvdTol, kSvdMaxSweeps, 1, c->Csvd,
-                                 c->sig, c->flags + 1, st);
-          dmr::blk::pack_coef(c->Csvd, cR, k, c->Cz32, st);
-        });
-        timed(c, st, DMR_PH_ZGEMM, vb * (cR + k + 2 * k), [&] {
-          if (deflate) {
-            nn(c, c->Z, k, c->Cz32, nullptr, 0.0f, 1.0f, c->Zn, k, st);
-            nn(c, V0, kr * s, c->Cz32 + (long long)p0 * 32, c->Zn, 1.0f, 1.0f, c->Zn, k, st);
-          } else {
-            nn(c, V0, kr * s, c->Cz32, nullptr, 0.0f, 1.0f, c->Zn, k, st);
-          }
-          dmr::blk::colnorms<float>(c->Zn, ld, k, n, c->cpart, c->zq, 1, st);
-          dmr::blk::recip(c->zq, c->zq, k, nullptr, st);
-          dmr::scale_cols<float>(c->Zn, k, ld, n, c->zq, st);
-        });
-        refreshed = true;
+  // Subspace refresh, part 2: Z <- normalize([Z V_0 .. V_{kr-1}] C) once the SVD has joined.
+  if (refreshed) {
+    hipStreamWaitEvent(st, c->evJoin, 0);
+    const int kv = cR - p0;  // kr * s Krylov columns
+    timed(c, st, DMR_PH_ZGEMM, vb * (cR + k + 2 * k), [&] {
+      if (deflate) {
+        nn(c, c->Z, k, c->Cz32, nullptr, 0.0f, 1.0f, c->Zn, k, st);
+        nn(c, V0, kv, c->Cz32 + (long long)p0 * 32, c->Zn, 1.0f, 1.0f, c->Zn, k, st);
+      } else {
+        nn(c, V0, kv, c->Cz32, nullptr, 0.0f, 1.0f, c->Zn, k, st);
       }
-    }
+      dmr::blk::colnorms<float>(c->Zn, ld, k, n, c->cpart, c->zq, 1, st);
+      dmr::blk::recip(c->zq, c->zq, k, nullptr, st);
+      dmr::scale_cols<float>(c->Zn, k, ld, n, c->zq, st);
+    });
   }
   dmr::blk::track(c->rho, c->q, c->scal, s, st);
   timed(c, st, DMR_PH_SCALE, vb * 2 * s,
@@ -540,7 +566,10 @@ int block_solve(dmrBlockContext *c, int mode, int s, double *X, long long ldx, c
                 long long ldb, hipStream_t st, double tol, int maxCycles, double *relRes) {
   const long long n = c->n, ld = c->ld;
   double *hb = c->host, *hr = c->host + 64, *hrho = c->host + 128, *hf = c->host + 192;
-  if (c->val32Stale) {
+  if (c->innerMean) {
+    dmr::blk::mean_values(c->colVal, c->colLd, c->colCount, c->csr.nnz, c->ownedVal32, st);
+    c->val32Stale = false;
+  } else if (c->val32Stale) {
     dmr::convert<float, double>(c->ownedVal32, c->csr.val, c->csr.nnz, st);
     c->val32Stale = false;
   }
@@ -551,8 +580,9 @@ int block_solve(dmrBlockContext *c, int mode, int s, double *X, long long ldx, c
   bool cycled = false;
   double bmax = 0.0;
   for (int outer = 0;; ++outer) {
-    timed(c, st, DMR_PH_SPMV, (double)c->csr.nnz * 12.0 + 4.0 * (n + 1) + 24.0 * n * s, [&] {
-      dmr::blk::resid_block(c->csr, X, ldx, B, ldb, c->R64, ld, s, st);
+    const double valBytes = 8.0 * c->csr.nnz * (c->colVal ? s : 1);
+    timed(c, st, DMR_PH_SPMV, valBytes + 4.0 * c->csr.nnz + 4.0 * (n + 1) + 24.0 * n * s, [&] {
+      dmr::blk::resid_block(c->csr, c->colVal, c->colLd, X, ldx, B, ldb, c->R64, ld, s, st);
     });
     c->st.matvecs += s;
     dmr::blk::colnorms<double>(c->R64, ld, s, n, c->cpart, c->rn, 1, st);
@@ -667,12 +697,19 @@ int dmrBlockCreate(dmrBlockHandle *h, int device, int elements, int blockSteps, 
   if (const char *e = std::getenv("DMR_BLOCK_GRAPHS")) c->graphs = e[0] != '0';
   if (const char *e = std::getenv("DMR_BLOCK_TUNE")) c->autotune = e[0] != '0';
   dmr::blk::prepare();
+  if (const char *e = std::getenv("DMR_BLOCK_SVD_TOL")) {
+    const double v = std::atof(e);
+    if (v > 0.0 && v < 1.0) c->svdTol = v;
+  }
   if (const char *e = std::getenv("DMR_BLOCK_KR")) {
     const int v = std::atoi(e);
     if (v >= 1) c->krMax = v;
   }
   int rc = ctx_alloc(c);
-  if (rc == DMR_OK && hipStreamCreateWithFlags(&c->gst, hipStreamNonBlocking) != hipSuccess)
+  if (rc == DMR_OK && (hipStreamCreateWithFlags(&c->gst, hipStreamNonBlocking) != hipSuccess ||
+                       hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+                       hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess ||
+                       hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming) != hipSuccess))
     rc = dmr::set_error(DMR_ERR_HIP, "stream creation failed");
   if (rc != DMR_OK) {
     ctx_release(c);
@@ -703,6 +740,10 @@ int dmrBlockSetOperatorCsr(dmrBlockHandle h, int nnz, const int *rowPtr, const i
   h->csr.val = val;
   h->csr.blockRows = nullptr;
   h->csr.nBlocks = 0;
+  h->colVal = nullptr;
+  h->colLd = 0;
+  h->colCount = 0;
+  h->innerMean = false;
   if (!val32) {
     if (h->ownedVal32) hipFree(h->ownedVal32);
     h->ownedVal32 = nullptr;
@@ -722,6 +763,7 @@ int dmrBlockSetOperatorValues(dmrBlockHandle h, const double *val, const float *
   if (!h || !h->haveOp || !val) return dmr::set_error(DMR_ERR_STATE, "no CSR operator to update");
   if ((val32 ? val32 : h->ownedVal32) != h->csr.val32) drop_graphs(h);
   h->csr.val = val;
+  h->innerMean = false;
   if (val32) {
     h->csr.val32 = val32;
     h->val32Stale = false;
@@ -731,6 +773,36 @@ int dmrBlockSetOperatorValues(dmrBlockHandle h, const double *val, const float *
     }
     h->csr.val32 = h->ownedVal32;
     h->val32Stale = true;
+  }
+  return DMR_OK;
+}
+
+int dmrBlockSetColumnOperators(dmrBlockHandle h, const double *val, long long ldv, int count,
+                               int innerMean) {
+  dmr::clear_error();
+  if (!h || !h->haveOp) return dmr::set_error(DMR_ERR_STATE, "no CSR operator to extend");
+  if (!val) {
+    h->colVal = nullptr;
+    h->colLd = 0;
+    h->colCount = 0;
+    if (h->innerMean) {
+      h->innerMean = false;
+      h->val32Stale = h->csr.val32 == h->ownedVal32;
+    }
+    return DMR_OK;
+  }
+  if (count < 1 || count > h->sMax || ldv < h->csr.nnz)
+    return dmr::set_error(DMR_ERR_ARG, "column operators need 1 <= count <= maxRhs, ldv >= nnz");
+  BCHK(hipSetDevice(h->device));
+  h->colVal = val;
+  h->colLd = ldv;
+  h->colCount = count;
+  h->innerMean = innerMean != 0;
+  if (h->innerMean) {
+    if (!h->ownedVal32)
+      BCHK(hipMalloc(&h->ownedVal32, sizeof(float) * std::max(h->csr.nnz, 1)));
+    if (h->csr.val32 != h->ownedVal32) drop_graphs(h);
+    h->csr.val32 = h->ownedVal32;
   }
   return DMR_OK;
 }
@@ -752,6 +824,8 @@ int dmrBlockSolve(dmrBlockHandle h, int mode, int nrhs, double *X, long long ldx
   if (nrhs < 1 || nrhs > h->sMax || !X || !B || !relRes || ldx < h->n || ldb < h->n ||
       maxCycles < 1 || !(solTol >= 0.0))
     return dmr::set_error(DMR_ERR_ARG, "invalid block solve arguments");
+  if (h->colVal && nrhs > h->colCount)
+    return dmr::set_error(DMR_ERR_ARG, "more right-hand sides than column operators");
   BCHK(hipSetDevice(h->device));
   h->st = dmrStats{};
   h->hist.clear();

@@ -20,12 +20,17 @@ constexpr int kMaxCols = 32 * kMaxTiles;
 constexpr int kMaxTnBlocks = 1024;
 
 // Y[:, 0:s] = A X[:, 0:s] with FP32 values and vectors, FP32 accumulation; cg (4, 8 or 16)
-// columns per thread.
+// columns per thread, or 100 + cg for the variant that loads each row's indices up front.
 void spmm32(const CsrDev &A, const float *X, long long ldx, float *Y, long long ldy, int s,
             int cg, hipStream_t st);
-// R = B - A X in FP64 (s columns).
-void resid_block(const CsrDev &A, const double *X, long long ldx, const double *B, long long ldb,
-                 double *R, long long ldr, int s, hipStream_t st);
+// R = B - A X in FP64 (s columns).  With val, column c uses the values val + c * ldv (same
+// sparsity as A); without, A's FP64 values.
+void resid_block(const CsrDev &A, const double *val, long long ldv, const double *X,
+                 long long ldx, const double *B, long long ldb, double *R, long long ldr, int s,
+                 hipStream_t st);
+// out = FP32 entrywise mean of count value arrays (val + c * ldv, nnz each), summed in order.
+void mean_values(const double *val, long long ldv, int count, long long nnz, float *out,
+                 hipStream_t st);
 // out[c] = ||X[:, c]||^2 (sqrtMode 0) or ||X[:, c]|| (1), FP64 accumulation in a fixed order.
 // partial: colnorm_blocks(n) * s doubles.
 int colnorm_blocks(long long n);
@@ -37,7 +42,7 @@ void colnorms(const T *X, long long ld, int s, long long n, double *partial, dou
 // row-major p x 32, zero beyond s.  part: G * roundup32(p) * 32 floats.
 int gemm_tn_blocks(long long n, int cap, int rowsTarget);
 size_t gemm_tn_lds(int pPad);
-// variant 0: register-direct; 1: LDS-staged.
+// variant 0: register-direct 32x32x2 MFMA; 1: LDS-staged; 2: register-direct 16x16x4 MFMA.
 void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu, int s,
              long long n, int G, int variant, float *part, double *C64, int ldc, float *C32,
              hipStream_t st);

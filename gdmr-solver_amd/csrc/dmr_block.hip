@@ -68,10 +68,63 @@ __global__ __launch_bounds__(kThreads) void k_spmm(int n, const int *__restrict_
     if (c0 + c < s) Y[row + (c0 + c) * ldy] = (TY)acc[c];
 }
 
+// Same product with every row's first W column indices and values loaded up front (one memory
+// latency), then all W x CG gathers of the row issued independently; rows with more than W entries
+// take the loop of k_spmm.  Each column's sum runs over the row's entries in order, as in k_spmm.
+template <typename TV, typename TX, typename TY, typename TA, int CG, int W>
+__global__ __launch_bounds__(kThreads) void k_spmm_w(int n, const int *__restrict__ rowPtr,
+                                                     const int *__restrict__ colInd,
+                                                     const TV *__restrict__ val,
+                                                     const TX *__restrict__ X, long long ldx,
+                                                     TY *__restrict__ Y, long long ldy, int s) {
+  const int row = blockIdx.x * kThreads + threadIdx.x;
+  const int c0 = blockIdx.y * CG;
+  if (row >= n) return;
+  const int b = rowPtr[row], e = rowPtr[row + 1];
+  TA acc[CG];
+#pragma unroll
+  for (int c = 0; c < CG; ++c) acc[c] = (TA)0;
+  if (e - b <= W) {
+    int col[W];
+    TA v[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      const bool in = b + q < e;
+      col[q] = in ? colInd[b + q] : row;
+      v[q] = in ? (TA)val[b + q] : (TA)0;
+    }
+#pragma unroll
+    for (int c = 0; c < CG; ++c) {
+      if (c0 + c < s) {
+        const TX *xc = X + (long long)(c0 + c) * ldx;
+        TA xv[W];
+#pragma unroll
+        for (int q = 0; q < W; ++q) xv[q] = (TA)xc[col[q]];
+#pragma unroll
+        for (int q = 0; q < W; ++q)
+          if (b + q < e) acc[c] = fma(v[q], xv[q], acc[c]);
+      }
+    }
+  } else {
+    for (int q = b; q < e; ++q) {
+      const long long col = colInd[q];
+      const TA v = (TA)val[q];
+#pragma unroll
+      for (int c = 0; c < CG; ++c)
+        if (c0 + c < s) acc[c] = fma(v, (TA)X[col + (c0 + c) * ldx], acc[c]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CG; ++c)
+    if (c0 + c < s) Y[row + (c0 + c) * ldy] = (TY)acc[c];
+}
+
 // R = B - A X in FP64.
+// Column c uses the values val + c * ldv (ldv = 0: one operator for every column).
 __global__ __launch_bounds__(kThreads) void k_resid_block(int n, const int *__restrict__ rowPtr,
                                                           const int *__restrict__ colInd,
                                                           const double *__restrict__ val,
+                                                          long long ldv,
                                                           const double *__restrict__ X,
                                                           long long ldx,
                                                           const double *__restrict__ B,
@@ -86,14 +139,26 @@ __global__ __launch_bounds__(kThreads) void k_resid_block(int n, const int *__re
     for (int c = 0; c < 8; ++c) acc[c] = 0.0;
     for (int q = b; q < e; ++q) {
       const long long col = colInd[q];
-      const double v = val[q];
 #pragma unroll
       for (int c = 0; c < 8; ++c)
-        if (c0 + c < s) acc[c] = fma(v, X[col + (c0 + c) * ldx], acc[c]);
+        if (c0 + c < s) acc[c] = fma(val[q + (c0 + c) * ldv], X[col + (c0 + c) * ldx], acc[c]);
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c)
       if (c0 + c < s) R[row + (c0 + c) * ldr] = B[row + (c0 + c) * ldb] - acc[c];
+  }
+}
+
+// out[q] = (float)(sum_c val[q + c * ldv] / count), the sum in column order in FP64: the entrywise
+// mean of count operators with one sparsity pattern.
+__global__ __launch_bounds__(kThreads) void k_mean_values(const double *__restrict__ val,
+                                                          long long ldv, int count, long long nnz,
+                                                          float *__restrict__ out) {
+  for (long long q = (long long)blockIdx.x * kThreads + threadIdx.x; q < nnz;
+       q += (long long)gridDim.x * kThreads) {
+    double acc = 0.0;
+    for (int c = 0; c < count; ++c) acc += val[q + c * ldv];
+    out[q] = (float)(acc / count);
   }
 }
 
@@ -236,6 +301,135 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn(const float *__restrict__ 
         for (int g = 0; g < nRG; ++g) a = a + red[((g * nTG + tg) * kTPW + t) * 1024 + v * 64 + lane];
         out[(tile * 32 + mfma_row(v, lane)) * 32 + i] = a;
       }
+    }
+  }
+}
+
+// 16 x 16 x 4 MFMA variant (no half-empty tiles for s <= 16).  P's columns form nT = ceil(p/16)
+// tiles of 16 and U's columns NU = ceil(s/16) tiles.  The four wavefronts are nTG tile groups (of
+// at most TPW tiles) x nRG = 4 / nTG row groups.  A row group walks 32-row rounds; in a round lane
+// (m, kk) = (l % 16, l / 16) loads rows r0 + 8kk .. r0 + 8kk + 7 of column m of each of its tiles
+// and of U as two 16-byte loads, and MFMA k-step e takes row r0 + 8kk + e of both operands.  The
+// next round's loads are issued before the current round's MFMAs.  The row groups' sums are
+// added through LDS in a fixed order.  Output layout of v_mfma_f32_16x16x4f32: lane l, register v
+// holds D[4 (l / 16) + v][l % 16].
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline f32x4 mfma16(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+__device__ inline void load_rows8(const float *col, long long r, long long rEnd, float4 &a,
+                                  float4 &b) {
+  a = make_float4(0.f, 0.f, 0.f, 0.f);
+  b = a;
+  if (!col) return;
+  if (r + 7 < rEnd) {
+    a = *reinterpret_cast<const float4 *>(col + r);
+    b = *reinterpret_cast<const float4 *>(col + r + 4);
+  } else {
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = r + e < rEnd ? col[r + e] : 0.0f;
+    a = make_float4(v[0], v[1], v[2], v[3]);
+    b = make_float4(v[4], v[5], v[6], v[7]);
+  }
+}
+
+template <int TPW, int NU>
+__global__ __launch_bounds__(kThreads) void k_gemm_tn16(const float *__restrict__ P, long long ldp,
+                                                        int p, int pPad,
+                                                        const float *__restrict__ U,
+                                                        long long ldu, int s, long long n,
+                                                        long long rowsPerBlock,
+                                                        float *__restrict__ part) {
+  __shared__ float red[4 * TPW * NU * 256];
+  const int nT = (p + 15) / 16;
+  const int nTG = (nT + TPW - 1) / TPW;
+  const int nRG = 4 / nTG;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tg = wave % nTG, rg = wave / nTG;
+  const int m = lane & 15, kk = lane >> 4;
+  const long long rBeg = (long long)blockIdx.x * rowsPerBlock;
+  const long long rEnd = rBeg + rowsPerBlock < n ? rBeg + rowsPerBlock : n;
+  f32x4 acc[TPW][NU];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int u = 0; u < NU; ++u) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const float *pc[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int tile = tg * TPW + t, c = tile * 16 + m;
+    pc[t] = (tile < nT && c < p) ? P + c * ldp : nullptr;
+  }
+  const float *uc[NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) uc[u] = u * 16 + m < s ? U + (u * 16 + m) * ldu : nullptr;
+  if (rg < nRG) {
+    const long long step = 32LL * nRG;
+    long long r0 = rBeg + 32 * rg;
+    float4 pa[TPW], pb[TPW], ua[NU], ub[NU];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) load_rows8(pc[t], r0 + 8 * kk, rEnd, pa[t], pb[t]);
+#pragma unroll
+    for (int u = 0; u < NU; ++u) load_rows8(uc[u], r0 + 8 * kk, rEnd, ua[u], ub[u]);
+    while (r0 < rEnd) {
+      const long long rn = r0 + step;
+      float4 qa[TPW], qb[TPW], va[NU], vb[NU];
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) load_rows8(rn < rEnd ? pc[t] : nullptr, rn + 8 * kk, rEnd,
+                                               qa[t], qb[t]);
+#pragma unroll
+      for (int u = 0; u < NU; ++u) load_rows8(rn < rEnd ? uc[u] : nullptr, rn + 8 * kk, rEnd,
+                                              va[u], vb[u]);
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const float bu[8] = {ua[u].x, ua[u].y, ua[u].z, ua[u].w,
+                             ub[u].x, ub[u].y, ub[u].z, ub[u].w};
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+          const float at[8] = {pa[t].x, pa[t].y, pa[t].z, pa[t].w,
+                               pb[t].x, pb[t].y, pb[t].z, pb[t].w};
+#pragma unroll
+          for (int e = 0; e < 8; ++e) acc[t][u] = mfma16(at[e], bu[e], acc[t][u]);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) {
+        pa[t] = qa[t];
+        pb[t] = qb[t];
+      }
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        ua[u] = va[u];
+        ub[u] = vb[u];
+      }
+      r0 = rn;
+    }
+#pragma unroll
+    for (int t = 0; t < TPW; ++t)
+#pragma unroll
+      for (int u = 0; u < NU; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[((wave * TPW + t) * NU + u) * 256 + v * 64 + lane] = acc[t][u][v];
+  }
+  __syncthreads();
+  if (rg != 0) return;
+  float *out = part + (long long)blockIdx.x * pPad * 32;
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int tile = tg * TPW + t;
+    if (tile < nT) {
+#pragma unroll
+      for (int u = 0; u < NU; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          float a = 0.0f;
+          for (int g = 0; g < nRG; ++g)
+            a = a + red[(((g * nTG + tg) * TPW + t) * NU + u) * 256 + v * 64 + lane];
+          out[(tile * 16 + 4 * kk + v) * 32 + u * 16 + m] = a;
+        }
     }
   }
 }
@@ -764,6 +958,21 @@ void zero_ints(int *p, int cnt, hipStream_t st) {
 
 void spmm32(const CsrDev &A, const float *X, long long ldx, float *Y, long long ldy, int s,
             int cg, hipStream_t st) {
+  if (cg >= 100) {
+    // Up-front index variant: cg - 100 columns per thread, rows of up to 8 entries unrolled.
+    const int g = cg - 100;
+    const dim3 grid(grid1(A.n), (unsigned)((s + g - 1) / g));
+    if (g == 4)
+      hipLaunchKernelGGL((k_spmm_w<float, float, float, float, 4, 8>), grid, dim3(kThreads), 0, st,
+                         A.n, A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
+    else if (g == 16)
+      hipLaunchKernelGGL((k_spmm_w<float, float, float, float, 16, 8>), grid, dim3(kThreads), 0,
+                         st, A.n, A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
+    else
+      hipLaunchKernelGGL((k_spmm_w<float, float, float, float, 8, 8>), grid, dim3(kThreads), 0, st,
+                         A.n, A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
+    return;
+  }
   const dim3 grid(grid1(A.n), (unsigned)((s + cg - 1) / cg));
   if (cg == 4)
     hipLaunchKernelGGL((k_spmm<float, float, float, float, 4>), grid, dim3(kThreads), 0, st, A.n,
@@ -776,10 +985,20 @@ void spmm32(const CsrDev &A, const float *X, long long ldx, float *Y, long long 
                        A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
 }
 
-void resid_block(const CsrDev &A, const double *X, long long ldx, const double *B, long long ldb,
-                 double *R, long long ldr, int s, hipStream_t st) {
+void resid_block(const CsrDev &A, const double *val, long long ldv, const double *X,
+                 long long ldx, const double *B, long long ldb, double *R, long long ldr, int s,
+                 hipStream_t st) {
   hipLaunchKernelGGL(k_resid_block, dim3(grid1(A.n)), dim3(kThreads), 0, st, A.n, A.rowPtr,
-                     A.colInd, A.val, X, ldx, B, ldb, R, ldr, s);
+                     A.colInd, val ? val : A.val, val ? ldv : 0LL, X, ldx, B, ldb, R, ldr, s);
+}
+
+void mean_values(const double *val, long long ldv, int count, long long nnz, float *out,
+                 hipStream_t st) {
+  long long g = (nnz + kThreads - 1) / kThreads;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL(k_mean_values, dim3((unsigned)g), dim3(kThreads), 0, st, val, ldv, count,
+                     nnz, out);
 }
 
 int colnorm_blocks(long long n) {
@@ -819,7 +1038,29 @@ void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu
   const int pPad = (p + 31) & ~31;
   long long rows = (n + G - 1) / G;
   rows = (rows + 63) / 64 * 64;
-  if (variant == 1) {
+  if (variant == 2) {
+    // Tile groups: one group of up to 6 tiles (four row groups), else two, else four.
+    const int nT = (p + 15) / 16;
+    const int nTG = nT <= 6 ? 1 : (nT <= 12 ? 2 : 4);
+    const int tpw = (nT + nTG - 1) / nTG;
+    const int nu = s <= 16 ? 1 : 2;
+#define DMR_TN16(T)                                                                              \
+  if (tpw == T) {                                                                                \
+    if (nu == 1)                                                                                 \
+      hipLaunchKernelGGL((k_gemm_tn16<T, 1>), dim3(G), dim3(kThreads), 0, st, P, ldp, p, pPad, U, \
+                         ldu, s, n, rows, part);                                                 \
+    else                                                                                         \
+      hipLaunchKernelGGL((k_gemm_tn16<T, 2>), dim3(G), dim3(kThreads), 0, st, P, ldp, p, pPad, U, \
+                         ldu, s, n, rows, part);                                                 \
+  }
+    DMR_TN16(1)
+    DMR_TN16(2)
+    DMR_TN16(3)
+    DMR_TN16(4)
+    DMR_TN16(5)
+    DMR_TN16(6)
+#undef DMR_TN16
+  } else if (variant == 1) {
     const size_t lds = gemm_tn_lds(pPad);
     hipLaunchKernelGGL(k_gemm_tn_lds, dim3(G), dim3(kThreads), lds, st, P, ldp, p, pPad, U, ldu,
                        s, n, rows, part);

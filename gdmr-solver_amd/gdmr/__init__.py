@@ -110,6 +110,7 @@ _sig("dmrBlockDestroy", C.c_int, [_vp])
 _sig("dmrBlockSetOperatorCsr", C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp])
 _sig("dmrBlockSetOperatorValues", C.c_int, [_vp, _vp, _vp])
 _sig("dmrBlockSetRefinement", C.c_int, [_vp, C.c_double, C.c_int])
+_sig("dmrBlockSetColumnOperators", C.c_int, [_vp, _vp, C.c_longlong, C.c_int, C.c_int])
 _sig("dmrBlockSolve", C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_longlong, _vp, C.c_longlong, _vp,
                                 C.c_double, C.c_int, _dp])
 _sig("dmrBlockGetStats", C.c_int, [_vp, C.POINTER(Stats), _dp, C.c_int])
@@ -150,7 +151,7 @@ HEADER_SYMBOLS = (
     "dmrGetProfile dmrSubspaceBytes dmrSubspaceToDevice dmrSubspaceFromDevice "
     "dmrGetStagnationCount dmrSetStagnationCount "
     "dmrBlockCreate dmrBlockDestroy dmrBlockSetOperatorCsr dmrBlockSetOperatorValues "
-    "dmrBlockSetRefinement dmrBlockSolve dmrBlockGetStats dmrBlockHasZ dmrBlockImportZ "
+    "dmrBlockSetRefinement dmrBlockSetColumnOperators dmrBlockSolve dmrBlockGetStats dmrBlockHasZ dmrBlockImportZ "
     "dmrBlockExportZ dmrBlockSetProfiling dmrBlockGetProfile dmrBlockFootprint "
     "dmrBlockSetOptions dmrBlockGetTuning "
     "dmrServerCreate dmrServerAddFamily dmrServerAddBlockFamily dmrServerSetRefinement "
@@ -407,6 +408,14 @@ class BlockSolver:
         self._keep_vals = list(keep)
         _check(lib.dmrBlockSetOperatorValues(self.h, val_ptr, val32_ptr or None),
                "dmrBlockSetOperatorValues")
+
+    def set_column_operators(self, val_ptr, ldv: int, count: int, inner_mean: bool = True,
+                             keep=()):
+        """Column c solves with the FP64 values at val_ptr + 8 * c * ldv (same CSR pattern);
+        the inner FP32 operator is their mean when inner_mean.  val_ptr = 0 clears."""
+        self._keep_cols = list(keep)
+        _check(lib.dmrBlockSetColumnOperators(self.h, val_ptr or None, ldv, count,
+                                              int(inner_mean)), "dmrBlockSetColumnOperators")
 
     def set_refinement(self, inner_tol: float, max_outer: int = 30):
         _check(lib.dmrBlockSetRefinement(self.h, inner_tol, max_outer), "dmrBlockSetRefinement")

@@ -178,6 +178,19 @@ int dmrBlockDestroy(dmrBlockHandle h);
 int dmrBlockSetOperatorCsr(dmrBlockHandle h, int nnz, const int *rowPtr, const int *colInd,
                            const double *val, const float *val32);
 int dmrBlockSetOperatorValues(dmrBlockHandle h, const double *val, const float *val32);
+/* Per-column operators (no counterpart in the reference): column c of the following solves is
+ * the system A_c X_c = B_c with A_c = the CSR pattern of dmrBlockSetOperatorCsr and the FP64
+ * values val + c * ldv (device, nnz each, ldv >= nnz, count <= maxRhs columns; the caller keeps
+ * them alive).  The FP64 refinement residual of column c uses A_c; the inner FP32 block solve
+ * shares one operator for all columns: the entrywise mean of the count operators when innerMean
+ * is nonzero (recomputed at the start of every solve), else the values of
+ * dmrBlockSetOperatorCsr / dmrBlockSetOperatorValues.  This solves a sequence of systems that
+ * differ in their values (a jittered or time-stepped family) as one block: the refinement
+ * converges while the inner operator is close to every A_c.  val == NULL clears the setting;
+ * dmrBlockSetOperatorCsr clears it and dmrBlockSetOperatorValues turns innerMean off.  A solve
+ * with more columns than count is rejected. */
+int dmrBlockSetColumnOperators(dmrBlockHandle h, const double *val, long long ldv, int count,
+                               int innerMean);
 /* Relative tolerance of the inner FP32 block solves (default 1e-5) and the maximum number of
  * refinement steps (default 30). */
 int dmrBlockSetRefinement(dmrBlockHandle h, double innerTol, int maxOuter);

@@ -179,3 +179,94 @@ def test_block_graph_replays_match_eager(torch_gpu):
         {k: v for k, v in te.items() if k != "graphs"}
     for bs in hs:
         bs.close()
+
+
+def _column_operator_solve(torch, mats, B, mb, k, inner_mean, tol=1e-10, solver=None):
+    """Solves A_t x_t = b_t for a sequence of systems with one sparsity pattern as one block:
+    column t uses its own FP64 operator (dmrBlockSetColumnOperators)."""
+    s, n = B.shape
+    nnz = mats[0].nnz
+    vals = _dev(torch, np.stack([A.val for A in mats]))
+    rp, ci = _dev(torch, mats[0].rowptr), _dev(torch, mats[0].col)
+    bs = solver or gdmr.BlockSolver(n, mb, k, s)
+    bs.set_csr(rp.data_ptr(), ci.data_ptr(), vals[0].data_ptr(), nnz, keep=(rp, ci))
+    bs.set_column_operators(vals.data_ptr(), nnz, s, inner_mean=inner_mean, keep=(vals,))
+    Bd = _dev(torch, B)
+    Xd = torch.zeros(s, n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    rc, rel = bs.solve(Xd.data_ptr(), n, Bd.data_ptr(), n, s, tol, max_cycles=500)
+    st, _ = bs.stats()
+    X = Xd.cpu().numpy()
+    if solver is None:
+        bs.close()
+    return rc, rel, st, X
+
+
+@pytest.mark.parametrize("inner_mean", [True, False])
+def test_column_operators_solve_a_jittered_sequence(torch_gpu, inner_mean):
+    """Six systems of a jittered 3-D convection-diffusion sequence (16^3, per-cell diffusion
+    U(+-1%) per system) solved as one block: column t is refined against its own FP64 operator,
+    the inner FP32 solve shares the mean operator (or system 0's).  Every column meets 1e-10 with
+    its own operator and matches the FP64 oracle's solution of its own system to XTOL."""
+    mats = [gdmr.conv_diff_3d(16, jitter=0.01, seed=41, stream=t) for t in range(6)]
+    n, s, tol = mats[0].n, len(mats), 1e-10
+    B = np.stack([gdmr.rand_normal(42, t, n) for t in range(s)])
+    rc, rel, st, X = _column_operator_solve(torch_gpu, mats, B, mb=6, k=8, inner_mean=inner_mean)
+    print(f"inner_mean {inner_mean}: rc {rc} cycles {st.cycles} max rel {rel.max():.2e}",
+          flush=True)
+    assert rc == 0
+    for t, A in enumerate(mats):
+        tr = _true_res(A, B[t], X[t])
+        assert tr <= tol * 1.001, (t, tr)
+        assert abs(tr - rel[t]) <= 1e-3 * tol
+        # Against another system's operator the residual is far above the tolerance.
+        other = mats[(t + 1) % s]
+        assert _true_res(other, B[t], X[t]) > 1e3 * tol
+        ora = oracle.Oracle(A, 30, 8, gdmr.reduction_blocks(n))
+        try:
+            xo, _, _, _ = ora.solve(B[t], tol=1e-12, mode=0)
+        finally:
+            ora.close()
+        err = np.linalg.norm(X[t] - xo) / np.linalg.norm(xo)
+        print(f"  system {t}: true residual {tr:.2e}, error vs oracle {err:.2e}", flush=True)
+        assert err <= XTOL, (t, err)
+
+
+def test_column_operators_equal_to_the_operator_match_plain_block(torch_gpu):
+    """Column operators that all equal the block operator (inner operator unchanged) give the
+    plain block solve's bits; clearing them restores the plain behaviour on the same handle."""
+    torch = torch_gpu
+    A = gdmr.conv_diff_3d(18, jitter=0.01, seed=43, stream=0)
+    n, s = A.n, 5
+    B = np.stack([gdmr.rand_normal(44, j, n) for j in range(s)])
+    rc0, rel0, st0, X0 = _block(torch, A, B, mb=5, k=8)
+    rc1, rel1, st1, X1 = _column_operator_solve(torch, [A] * s, B, mb=5, k=8, inner_mean=False)
+    assert rc0 == 0 and rc1 == 0
+    assert st0.cycles == st1.cycles
+    assert np.array_equal(X0, X1) and np.array_equal(rel0, rel1)
+
+
+def test_column_operators_reject_bad_arguments(torch_gpu):
+    torch = torch_gpu
+    mats = [gdmr.conv_diff_3d(10, jitter=0.01, seed=45, stream=t) for t in range(3)]
+    n, nnz = mats[0].n, mats[0].nnz
+    vals = _dev(torch, np.stack([A.val for A in mats]))
+    rp, ci = _dev(torch, mats[0].rowptr), _dev(torch, mats[0].col)
+    bs = gdmr.BlockSolver(n, 4, 4, 4)
+    with pytest.raises(gdmr.SolverError):  # no operator yet
+        bs.set_column_operators(vals.data_ptr(), nnz, 3)
+    bs.set_csr(rp.data_ptr(), ci.data_ptr(), vals[0].data_ptr(), nnz)
+    with pytest.raises(gdmr.SolverError):  # more than maxRhs
+        bs.set_column_operators(vals.data_ptr(), nnz, 5)
+    with pytest.raises(gdmr.SolverError):  # ldv < nnz
+        bs.set_column_operators(vals.data_ptr(), nnz - 1, 3)
+    bs.set_column_operators(vals.data_ptr(), nnz, 3)
+    B = _dev(torch, np.stack([gdmr.rand_normal(46, j, n) for j in range(4)]))
+    X = torch.zeros(4, n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    rc, _ = bs.solve(X.data_ptr(), n, B.data_ptr(), n, 4, 1e-10)  # 4 columns, 3 operators
+    assert rc == gdmr.DMR_ERR_ARG
+    bs.set_column_operators(0, 0, 0)  # cleared: a 4-column solve with the block operator
+    rc, rel = bs.solve(X.data_ptr(), n, B.data_ptr(), n, 4, 1e-10, max_cycles=500)
+    assert rc == 0
+    bs.close()

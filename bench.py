@@ -10,9 +10,11 @@ Three legs run on the same sequence, each in a fresh solver context:
   * mixed    -- FP32 Krylov basis and SpMV with FP32-accumulated reductions inside FP64 iterative
                 refinement; Hessenberg LSQ / SVD in FP64; one system after the other;
   * seqblock -- mixed + batched: the 16 systems as one block solve.  Column t is refined in FP64
-                against its own operator A_t; the inner FP32 block DMR (block Gram-Schmidt as
-                tall-skinny GEMMs on the matrix cores, 16 block steps, recycled subspace k = 20)
-                shares the entrywise mean of the A_t.  Every system's latency is the block's.
+                against its own operator A_t; the inner block DMR (block Gram-Schmidt as
+                tall-skinny GEMMs on the matrix cores, 16 block steps, recycled subspace k = 20,
+                basis stored as bf16 and multiplied on the bf16 matrix cores with FP32 operands
+                split into hi + lo parts, FP32 accumulation) shares the entrywise mean of the A_t.
+                Every system's latency is the block's.
 A step is one solve of one system.  `value` is the solves/sec of the fastest leg whose every
 system reaches 1e-10 checked in FP64.  Every solve's true residual ||b - A x|| / ||b|| (with its
 own operator) is recomputed on the host in FP64 after the timed region.
@@ -163,7 +165,8 @@ def run_block1_leg(mats, rhs, dev_vals, rp, ci, m, k, tol, warmup, inner_tol=1e-
                 x=[x.cpu().numpy() for x in xs])
 
 
-def run_seqblock_leg(mats, rhs, dev_vals, rp, ci, mb, k, tol, warmup, inner_tol, profile):
+def run_seqblock_leg(mats, rhs, dev_vals, rp, ci, mb, k, tol, warmup, inner_tol, profile,
+                     prec="bf16"):
     """The whole sequence as one block solve: column t is system t with its own FP64 operator
     A_t in the refinement residual, while the inner FP32 block DMR (block Gram-Schmidt on the
     matrix cores) shares the entrywise mean of the A_t (dmrBlockSetColumnOperators).  Every
@@ -180,6 +183,7 @@ def run_seqblock_leg(mats, rhs, dev_vals, rp, ci, mb, k, tol, warmup, inner_tol,
         h.set_csr(rp.data_ptr(), ci.data_ptr(), vals[0].data_ptr(), nnz)
         h.set_column_operators(vals.data_ptr(), nnz, s, inner_mean=True)
         h.set_refinement(inner_tol, 60)
+        h.set_precision(gdmr.PREC_BF16 if prec == "bf16" else gdmr.PREC_FP32)
         return h
 
     for w in range(warmup):
@@ -302,8 +306,10 @@ def main():
     ap.add_argument("--seq-block-steps", type=int, default=16,
                     help="block steps per cycle of the seqblock leg")
     ap.add_argument("--seq-k", type=int, default=20, help="recycled subspace of the seqblock leg")
-    ap.add_argument("--seq-inner-tol", type=float, default=1e-2,
+    ap.add_argument("--seq-inner-tol", type=float, default=3e-2,
                     help="relative tolerance of each inner block solve of the seqblock leg")
+    ap.add_argument("--seq-prec", default="bf16", choices=("bf16", "fp32"),
+                    help="basis storage of the seqblock leg")
     ap.add_argument("--cpu-cycles", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -351,7 +357,7 @@ def main():
         if name == "seqblock":
             legs[name] = run_seqblock_leg(mats, rhs, dev_vals, rp, ci, args.seq_block_steps,
                                           args.seq_k, tol, args.warmup, args.seq_inner_tol,
-                                          not args.no_profile)
+                                          not args.no_profile, args.seq_prec)
             continue
         if name == "block1":
             legs[name] = run_block1_leg(mats, rhs, dev_vals, rp, ci, m, k, tol, args.warmup,
@@ -383,7 +389,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "mixed": "fp32+fp64", "bf16": "bf16+fp32+fp64",
-                  "block1": "fp32+fp64", "seqblock": "fp32+fp64"}[head],
+                  "block1": "fp32+fp64",
+                  "seqblock": ("bf16+fp32+fp64" if args.seq_prec == "bf16" else
+                               "fp32+fp64")}[head],
         "data": "synthetic",
         "config": {"workload": f"3-D conv-diff 7-pt {N}^3 sequence of {args.steps} jittered "
                                f"systems, m={m} k={k} tol {tol:g}, Z recycled",

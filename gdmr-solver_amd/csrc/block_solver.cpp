@@ -26,6 +26,7 @@
 #include <cstdlib>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "dmr_block.h"
@@ -63,7 +64,11 @@ struct dmrBlockContext {
   int colCount = 0;
   bool innerMean = false;
   void *big = nullptr, *small = nullptr;
-  float *P = nullptr, *Z = nullptr, *Zn = nullptr, *R = nullptr, *Xa = nullptr;
+  // P: the basis [W V_0 V_1 ...], FP32 or (bfBasis) bf16 storage in the same allocation.  Ub: the
+  // newest block in FP32 while it is orthogonalised (the product A V_j, or A Z, or the start
+  // block), written into its slot of P by the last normalisation.
+  float *P = nullptr, *Z = nullptr, *Zn = nullptr, *R = nullptr, *Xa = nullptr, *Ub = nullptr;
+  bool bfBasis = false;
   double *R64 = nullptr;
   double *H = nullptr, *T = nullptr, *Hw = nullptr, *Tw = nullptr, *Y = nullptr, *Hred = nullptr;
   float *C32 = nullptr, *Yr32 = nullptr, *ZH32 = nullptr, *Sinv32 = nullptr, *Cz32 = nullptr;
@@ -76,11 +81,13 @@ struct dmrBlockContext {
   bool hasZ = false;
   int krMax = 64;  // cap on the leading block steps of the subspace refresh
   double svdTol = kSvdTol;
+  int cholPasses = 0;  // Cholesky QR passes per block normalisation; 0: by storage type
   // Launch shapes chosen by the autotuner (per block width s), and whether it ran.
   struct Tuning {
     int tnRowsProj = 512, tnVarProj = 0;  // gemm_tn for projections (p > s)
     int tnRowsGram = 512, tnVarGram = 0;  // gemm_tn for Gram matrices (p = s)
     int nnUnroll = 8;                     // gemm_nn loads in flight per wavefront (KiB)
+    int nnBasis = 8;                      // the same for a bf16 basis; >= 100: bf16 matrix cores
     int spmmCols = 8;                     // SpMM columns per thread
     int s = 0;
   } tune;
@@ -182,7 +189,7 @@ Sizes sizes(long long n, int mb, int k, int sMax, int G, long long ld) {
   const int pPadMax = (rowsMax + 31) & ~31;
   const int rRed = rowsMax, cRed = colsMax;
   Sizes z;
-  z.big = sizeof(float) * (size_t)ld * (size_t)(rowsMax + 2 * k + 2 * sMax) +
+  z.big = sizeof(float) * (size_t)ld * (size_t)(rowsMax + 2 * k + 2 * sMax + std::max(sMax, k)) +
           sizeof(double) * (size_t)ld * sMax;
   z.small = sizeof(double) * ((size_t)rowsMax * colsMax * 2 + (size_t)rowsMax * 32 * 2 +
                               (size_t)colsMax * 32 + (size_t)rRed * cRed + 32 * 32 * 2 +
@@ -244,6 +251,7 @@ int ctx_alloc(dmrBlockContext *c) {
   c->R = b.take<float>((size_t)ld * s);
   c->Xa = b.take<float>((size_t)ld * s);
   c->R64 = b.take<double>((size_t)ld * s);
+  c->Ub = b.take<float>((size_t)ld * std::max(s, std::max(k, 1)));
   Carver m{(char *)c->small};
   const int rowsMax = c->rowsMax, colsMax = c->colsMax;
   const int pPadMax = (rowsMax + 31) & ~31;
@@ -280,8 +288,11 @@ int tn_blocks(const dmrBlockContext *c, int rowsTarget) {
   return dmr::blk::gemm_tn_blocks(c->n, dmr::blk::kMaxTnBlocks, rowsTarget);
 }
 
+using dmr::bf16;
+
 // C = P^T U with the tuned shape of its class (Gram matrix or projection).
-void tn(dmrBlockContext *c, const float *P, int p, const float *U, int s, double *C64, int ldc,
+template <typename TP>
+void tn(dmrBlockContext *c, const TP *P, int p, const float *U, int s, double *C64, int ldc,
         float *C32, bool gram, hipStream_t st) {
   const auto &t = c->tune;
   dmr::blk::gemm_tn(P, c->ld, p, U, c->ld, s, c->n,
@@ -289,21 +300,24 @@ void tn(dmrBlockContext *c, const float *P, int p, const float *U, int s, double
                     gram ? t.tnVarGram : t.tnVarProj, c->part, C64, ldc, C32, st);
 }
 
-void nn(dmrBlockContext *c, const float *P, int p, const float *C, const float *In, float a,
-        float b, float *Out, int s, hipStream_t st) {
-  dmr::blk::gemm_nn(P, c->ld, p, C, In, c->ld, a, b, Out, c->ld, s, c->n, c->tune.nnUnroll, st);
+template <typename TP, typename TO>
+void nn(dmrBlockContext *c, const TP *P, int p, const float *C, const float *In, float a,
+        float b, TO *Out, int s, hipStream_t st) {
+  const int un = std::is_same<TP, bf16>::value ? c->tune.nnBasis : c->tune.nnUnroll;
+  dmr::blk::gemm_nn(P, c->ld, p, C, In, c->ld, a, b, Out, c->ld, s, c->n, un, st);
 }
 
-void spmm(dmrBlockContext *c, const float *X, float *Y, int s, hipStream_t st) {
+template <typename TX>
+void spmm(dmrBlockContext *c, const TX *X, float *Y, int s, hipStream_t st) {
   dmr::blk::spmm32(c->csr, X, c->ld, Y, c->ld, s, c->tune.spmmCols, st);
 }
 
 // Process-wide table of autotuned shapes, keyed by the problem shape.
 struct TuneKey {
   long long n, nnz;
-  int s, k, mb;
+  int s, k, mb, bf;
   bool operator==(const TuneKey &o) const {
-    return n == o.n && nnz == o.nnz && s == o.s && k == o.k && mb == o.mb;
+    return n == o.n && nnz == o.nnz && s == o.s && k == o.k && mb == o.mb && bf == o.bf;
   }
 };
 std::mutex g_tuneMu;
@@ -315,8 +329,10 @@ std::vector<std::pair<TuneKey, dmrBlockContext::Tuning>> g_tuned;
 // SpMM columns per thread.  Results are cached per shape for the process, so every handle of a
 // shape launches the same kernels (and computes the same bits).  The buffers it writes (Xa, the
 // partials) are re-initialised by the solve that follows.
-void autotune(dmrBlockContext *c, int s, hipStream_t st) {
-  const TuneKey key{c->n, c->csr.nnz, s, c->k, c->mb};
+template <typename TP>
+void autotune_t(dmrBlockContext *c, TP *P, int s, hipStream_t st) {
+  constexpr bool bf = std::is_same<TP, bf16>::value;
+  const TuneKey key{c->n, c->csr.nnz, s, c->k, c->mb, bf ? 1 : 0};
   {
     std::lock_guard<std::mutex> lk(g_tuneMu);
     for (auto &e : g_tuned)
@@ -348,10 +364,11 @@ void autotune(dmrBlockContext *c, int s, hipStream_t st) {
   const int rowsCand[] = {256, 512, 1024, 2048};
   float bp = 1e30f, bg = 1e30f;
   for (int rows : rowsCand)
-    for (int var = 0; var < 3; ++var) {
+    for (int var = 0; var < 4; ++var) {
       const int G = tn_blocks(c, rows);
-      const float tp = best_of([&] {
-        dmr::blk::gemm_tn(c->P, c->ld, pFull, c->R, c->ld, s, c->n, G, var, c->part, nullptr, 0,
+      // A bf16 basis: the FP32 (2) or bf16 (3) matrix-core kernels; FP32 Gram matrices: 0, 1, 2.
+      const float tp = (bf ? var < 2 : var == 3) ? 1e30f : best_of([&] {
+        dmr::blk::gemm_tn(P, c->ld, pFull, c->R, c->ld, s, c->n, G, var, c->part, nullptr, 0,
                           c->C32, st);
       });
       if (tp < bp) {
@@ -359,7 +376,7 @@ void autotune(dmrBlockContext *c, int s, hipStream_t st) {
         t.tnRowsProj = rows;
         t.tnVarProj = var;
       }
-      const float tg = best_of([&] {
+      const float tg = var == 3 ? 1e30f : best_of([&] {
         dmr::blk::gemm_tn(c->R, c->ld, s, c->R, c->ld, s, c->n, G, var, c->part, c->G64, s,
                           nullptr, st);
       });
@@ -370,20 +387,36 @@ void autotune(dmrBlockContext *c, int s, hipStream_t st) {
       }
     }
   float bn = 1e30f;
-  for (int un : {4, 8, 16}) {
+  for (int un : {4, 8, 16, 100}) {
+    if (!bf && un >= 100) continue;
     const float tt = best_of([&] {
-      dmr::blk::gemm_nn(c->P, c->ld, pFull, c->C32, c->Xa, c->ld, 1.0f, 0.0f, c->Xa, c->ld, s,
+      dmr::blk::gemm_nn(P, c->ld, pFull, c->C32, c->Xa, c->ld, 1.0f, 0.0f, c->Xa, c->ld, s,
                         c->n, un, st);
     });
     if (tt < bn) {
       bn = tt;
-      t.nnUnroll = un;
+      t.nnUnroll = t.nnBasis = un;
+    }
+  }
+  if (bf) {
+    // FP32 operands of a bf16-basis cycle (normalisations, subspace products): p = s columns.
+    float bq = 1e30f;
+    for (int un : {4, 8, 16}) {
+      const float tt = best_of([&] {
+        dmr::blk::gemm_nn(c->Ub, c->ld, s, c->C32, c->Xa, c->ld, 1.0f, 0.0f, c->Xa, c->ld, s,
+                          c->n, un, st);
+      });
+      if (tt < bq) {
+        bq = tt;
+        t.nnUnroll = un;
+      }
     }
   }
   float bs = 1e30f;
   for (int cg : {4, 8, 16, 104, 108, 116}) {
+    if (bf && cg < 100) continue;  // bf16 vectors: the up-front variant only
     const float tt = best_of([&] {
-      dmr::blk::spmm32(c->csr, c->R, c->ld, c->Xa, c->ld, s, cg, st);
+      dmr::blk::spmm32(c->csr, P, c->ld, c->Xa, c->ld, s, cg, st);
     });
     if (tt < bs) {
       bs = tt;
@@ -397,20 +430,36 @@ void autotune(dmrBlockContext *c, int s, hipStream_t st) {
   g_tuned.emplace_back(key, t);
 }
 
+void autotune(dmrBlockContext *c, int s, hipStream_t st) {
+  if (c->bfBasis)
+    autotune_t(c, reinterpret_cast<bf16 *>(c->P), s, st);
+  else
+    autotune_t(c, c->P, s, st);
+}
+
 // Two Cholesky QR passes on the sc columns of U (in place).  The combined triangular factor goes
 // to Hdst (ld ldh) when given.
-void cholqr2(dmrBlockContext *c, float *U, int sc, double *Hdst, int ldh, hipStream_t st) {
+template <typename TO>
+void cholqr2(dmrBlockContext *c, float *U, int sc, double *Hdst, int ldh, TO *dst,
+             hipStream_t st) {
   const long long n = c->n;
-  for (int pass = 0; pass < 2; ++pass) {
+  // A bf16-stored block is orthogonal only to bf16 rounding (about 4e-3), so one Cholesky QR
+  // pass, orthogonal to about kappa^2 * 6e-8, is enough there; FP32 storage keeps two.
+  const int passes = c->cholPasses ? c->cholPasses : (c->bfBasis ? 1 : 2);
+  for (int pass = 0; pass < passes; ++pass) {
     timedf(c, st, DMR_PH_GEMVT, 4.0 * n * sc, 2.0 * n * sc * sc, [&] {
       tn(c, U, sc, U, sc, c->G64, sc, nullptr, true, st);
     });
     timed(c, st, DMR_PH_ORTH, 0.0, [&] {
       dmr::blk::chol_qr(c->G64, sc, kDepTol, c->Sinv32, c->Sacc, pass == 0,
-                        pass == 1 ? Hdst : nullptr, ldh, c->flags, st);
+                        pass == passes - 1 ? Hdst : nullptr, ldh, c->flags, st);
     });
-    timedf(c, st, DMR_PH_UPDATE, 4.0 * n * 2 * sc, 2.0 * n * sc * sc, [&] {
-      nn(c, U, sc, c->Sinv32, nullptr, 0.0f, 1.0f, U, sc, st);
+    timedf(c, st, DMR_PH_UPDATE, 4.0 * n * sc + (double)sizeof(TO) * n * sc,
+           2.0 * n * sc * sc, [&] {
+      if (pass < passes - 1)
+        nn(c, U, sc, c->Sinv32, nullptr, 0.0f, 1.0f, U, sc, st);
+      else
+        nn(c, U, sc, c->Sinv32, nullptr, 0.0f, 1.0f, dst, sc, st);
     });
   }
 }
@@ -419,50 +468,52 @@ double spmm_bytes(const dmrBlockContext *c, int s) {
   return (double)c->csr.nnz * 8.0 + 4.0 * (c->n + 1) + 8.0 * (double)c->n * s;
 }
 
-// One restart cycle on the normalised residual block c->R (s columns).  Returns whether the
-// refreshed subspace was written to c->Zn.
-bool cycle(dmrBlockContext *c, int s, bool deflate, hipStream_t st) {
+// One restart cycle on the normalised residual block c->R (s columns), with the basis stored as
+// TP (float or bf16).  Returns whether the refreshed subspace was written to c->Zn.
+template <typename TP>
+bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
   const long long n = c->n, ld = c->ld;
   const int k = c->k, mb = c->mb;
   const int p0 = deflate ? k : 0;
   const int rows = p0 + (mb + 1) * s, cols = p0 + mb * s;
-  float *P = c->P;
-  const double vb = 4.0 * (double)n;
+  float *U = c->Ub;
+  const double vb = 4.0 * (double)n;              // one FP32 column
+  const double pb = (double)sizeof(TP) * (double)n;  // one basis column
   dmr::fill<double>(c->H, (long long)rows * cols, 0.0, st);
   dmr::blk::zero_ints(c->flags, 2, st);
   if (deflate) {
     timedf(c, st, DMR_PH_SPMV, spmm_bytes(c, k), 2.0 * c->csr.nnz * k,
-          [&] { spmm(c, c->Z, P, k, st); });
+          [&] { spmm(c, c->Z, U, k, st); });
     c->st.matvecs += k;
-    cholqr2(c, P, k, c->H, rows, st);
+    cholqr2(c, U, k, c->H, rows, P, st);
   }
-  float *V0 = P + (long long)p0 * ld;
+  TP *V0 = P + (long long)p0 * ld;
   if (deflate) {
-    timedf(c, st, DMR_PH_GEMVT, vb * (k + s), 2.0 * n * k * s, [&] {
+    timedf(c, st, DMR_PH_GEMVT, pb * k + vb * s, 2.0 * n * k * s, [&] {
       tn(c, P, k, c->R, s, nullptr, 0, c->C32, false, st);
     });
-    timedf(c, st, DMR_PH_UPDATE, vb * (k + 2 * s), 2.0 * n * k * s, [&] {
-      nn(c, P, k, c->C32, c->R, 1.0f, -1.0f, V0, s, st);
+    timedf(c, st, DMR_PH_UPDATE, pb * k + vb * 2 * s, 2.0 * n * k * s, [&] {
+      nn(c, P, k, c->C32, c->R, 1.0f, -1.0f, U, s, st);
     });
   } else {
-    hipMemcpyAsync(V0, c->R, sizeof(float) * ld * s, hipMemcpyDeviceToDevice, st);
+    hipMemcpyAsync(U, c->R, sizeof(float) * ld * s, hipMemcpyDeviceToDevice, st);
   }
-  cholqr2(c, V0, s, nullptr, 0, st);
+  cholqr2(c, U, s, (double *)nullptr, 0, V0, st);
   for (int j = 0; j < mb; ++j) {
-    float *Vj = P + (long long)(p0 + j * s) * ld;
-    float *U = P + (long long)(p0 + (j + 1) * s) * ld;
+    const TP *Vj = P + (long long)(p0 + j * s) * ld;
     const int pj = p0 + (j + 1) * s;
+    TP *slot = P + (long long)pj * ld;
     double *Hcol = c->H + (long long)(p0 + j * s) * rows;
-    timedf(c, st, DMR_PH_SPMV, spmm_bytes(c, s), 2.0 * c->csr.nnz * s,
-          [&] { spmm(c, Vj, U, s, st); });
+    timedf(c, st, DMR_PH_SPMV, spmm_bytes(c, s) - (4.0 - sizeof(TP)) * n * s,
+           2.0 * c->csr.nnz * s, [&] { spmm(c, Vj, U, s, st); });
     c->st.matvecs += s;
-    timedf(c, st, DMR_PH_GEMVT, vb * (pj + s), 2.0 * n * pj * s, [&] {
+    timedf(c, st, DMR_PH_GEMVT, pb * pj + vb * s, 2.0 * n * pj * s, [&] {
       tn(c, P, pj, U, s, Hcol, rows, c->C32, false, st);
     });
-    timedf(c, st, DMR_PH_UPDATE, vb * (pj + 2 * s), 2.0 * n * pj * s, [&] {
+    timedf(c, st, DMR_PH_UPDATE, pb * pj + vb * 2 * s, 2.0 * n * pj * s, [&] {
       nn(c, P, pj, c->C32, U, 1.0f, -1.0f, U, s, st);
     });
-    cholqr2(c, U, s, Hcol + pj, rows, st);
+    cholqr2(c, U, s, Hcol + pj, rows, slot, st);
   }
   // Subspace refresh, part 1 (side stream, overlapped with the least squares and the updates):
   // the k smallest right singular vectors of the leading block of H over [Z V_0 .. V_{kr-1}].
@@ -493,19 +544,20 @@ bool cycle(dmrBlockContext *c, int s, bool deflate, hipStream_t st) {
     }
   }
   // Full projection of the residual block, least squares, residual and solution updates.
-  timedf(c, st, DMR_PH_GEMVT, vb * (rows + s), 2.0 * n * rows * s, [&] {
+  timedf(c, st, DMR_PH_GEMVT, pb * rows + vb * s, 2.0 * n * rows * s, [&] {
     tn(c, P, rows, c->R, s, c->T, rows, nullptr, false, st);
   });
   timed(c, st, DMR_PH_LSQ, 0.0, [&] {
     dmr::blk::lsq_block(c->H, rows, cols, s, c->T, s, c->rho, c->Hw, c->Tw, c->Y, c->Yr32,
                         c->ZH32, st);
   });
-  timedf(c, st, DMR_PH_COMBO, vb * (rows + 2 * s), 2.0 * n * rows * s, [&] {
+  timedf(c, st, DMR_PH_COMBO, pb * rows + vb * 2 * s, 2.0 * n * rows * s, [&] {
     nn(c, P, rows, c->ZH32, c->R, 1.0f, -1.0f, c->R, s, st);
   });
   timed(c, st, DMR_PH_SCALE, vb * s,
         [&] { dmr::blk::colnorms<float>(c->R, ld, s, n, c->cpart, c->q, 1, st); });
-  timedf(c, st, DMR_PH_COMBO, vb * ((deflate ? k : 0) + mb * s + 2 * s * (deflate ? 2 : 1)),
+  timedf(c, st, DMR_PH_COMBO,
+         vb * ((deflate ? k : 0) + 2 * s * (deflate ? 2 : 1)) + pb * mb * s,
          2.0 * n * ((deflate ? k : 0) + mb * s) * s, [&] {
     if (deflate)
       nn(c, c->Z, k, c->Yr32, c->Xa, 1.0f, 1.0f, c->Xa, s, st);
@@ -515,7 +567,7 @@ bool cycle(dmrBlockContext *c, int s, bool deflate, hipStream_t st) {
   if (refreshed) {
     hipStreamWaitEvent(st, c->evJoin, 0);
     const int kv = cR - p0;  // kr * s Krylov columns
-    timed(c, st, DMR_PH_ZGEMM, vb * (cR + k + 2 * k), [&] {
+    timed(c, st, DMR_PH_ZGEMM, vb * (p0 + k + 2 * k) + pb * kv, [&] {
       if (deflate) {
         nn(c, c->Z, k, c->Cz32, nullptr, 0.0f, 1.0f, c->Zn, k, st);
         nn(c, V0, kv, c->Cz32 + (long long)p0 * 32, c->Zn, 1.0f, 1.0f, c->Zn, k, st);
@@ -531,6 +583,11 @@ bool cycle(dmrBlockContext *c, int s, bool deflate, hipStream_t st) {
   timed(c, st, DMR_PH_SCALE, vb * 2 * s,
         [&] { dmr::scale_cols<float>(c->R, s, ld, n, c->scal, st); });
   return refreshed;
+}
+
+bool cycle(dmrBlockContext *c, int s, bool deflate, hipStream_t st) {
+  return c->bfBasis ? cycle_t(c, reinterpret_cast<bf16 *>(c->P), s, deflate, st)
+                    : cycle_t(c, c->P, s, deflate, st);
 }
 
 // One restart cycle as a replay of its captured graph.  The cycle's launch sequence depends only
@@ -697,6 +754,10 @@ int dmrBlockCreate(dmrBlockHandle *h, int device, int elements, int blockSteps, 
   if (const char *e = std::getenv("DMR_BLOCK_GRAPHS")) c->graphs = e[0] != '0';
   if (const char *e = std::getenv("DMR_BLOCK_TUNE")) c->autotune = e[0] != '0';
   dmr::blk::prepare();
+  if (const char *e = std::getenv("DMR_BLOCK_CHOLQR_PASSES")) {
+    const int v = std::atoi(e);
+    if (v == 1 || v == 2) c->cholPasses = v;
+  }
   if (const char *e = std::getenv("DMR_BLOCK_SVD_TOL")) {
     const double v = std::atof(e);
     if (v > 0.0 && v < 1.0) c->svdTol = v;
@@ -807,6 +868,21 @@ int dmrBlockSetColumnOperators(dmrBlockHandle h, const double *val, long long ld
   return DMR_OK;
 }
 
+int dmrBlockSetPrecision(dmrBlockHandle h, int precision) {
+  dmr::clear_error();
+  if (!h) return dmr::set_error(DMR_ERR_ARG, "null handle");
+  if (precision != DMR_PREC_FP32 && precision != DMR_PREC_BF16)
+    return dmr::set_error(DMR_ERR_ARG, "block basis storage is DMR_PREC_FP32 or DMR_PREC_BF16");
+  const bool bf = precision == DMR_PREC_BF16;
+  if (bf != h->bfBasis) {
+    BCHK(hipSetDevice(h->device));
+    drop_graphs(h);
+    h->bfBasis = bf;
+    h->tune.s = 0;  // launch shapes are tuned per storage type
+  }
+  return DMR_OK;
+}
+
 int dmrBlockSetRefinement(dmrBlockHandle h, double innerTol, int maxOuter) {
   if (!h || !(innerTol > 0.0 && innerTol < 1.0) || maxOuter < 1)
     return dmr::set_error(DMR_ERR_ARG, "invalid refinement settings");
@@ -866,7 +942,7 @@ int dmrBlockGetTuning(dmrBlockHandle h, int *out) {
   out[1] = t.tnVarProj;
   out[2] = t.tnRowsGram;
   out[3] = t.tnVarGram;
-  out[4] = t.nnUnroll;
+  out[4] = h->bfBasis ? t.nnBasis : t.nnUnroll;
   out[5] = t.spmmCols;
   out[6] = (int)h->caps.size();
   return DMR_OK;

@@ -23,6 +23,9 @@ constexpr int kMaxTnBlocks = 1024;
 // columns per thread, or 100 + cg for the variant that loads each row's indices up front.
 void spmm32(const CsrDev &A, const float *X, long long ldx, float *Y, long long ldy, int s,
             int cg, hipStream_t st);
+// bf16 X (basis storage): the up-front variant only (cg or 100 + cg).
+void spmm32(const CsrDev &A, const bf16 *X, long long ldx, float *Y, long long ldy, int s, int cg,
+            hipStream_t st);
 // R = B - A X in FP64 (s columns).  With val, column c uses the values val + c * ldv (same
 // sparsity as A); without, A's FP64 values.
 void resid_block(const CsrDev &A, const double *val, long long ldv, const double *X,
@@ -46,10 +49,23 @@ size_t gemm_tn_lds(int pPad);
 void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu, int s,
              long long n, int G, int variant, float *part, double *C64, int ldc, float *C32,
              hipStream_t st);
+// bf16 P (basis storage), FP32 U: variant 3 uses the bf16 matrix cores (U split into hi + lo
+// bf16 parts), any other the FP32 ones on unpacked values.
+void gemm_tn(const bf16 *P, long long ldp, int p, const float *U, long long ldu, int s,
+             long long n, int G, int variant, float *part, double *C64, int ldc, float *C32,
+             hipStream_t st);
 // Out = a In + b P[:, 0:p] C (C row-major p x 32; In unused when a == 0).  Out may alias In or P.
 // un (2, 4, 8 or 16): column pairs of P loaded ahead of their MFMAs.
 void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *In, long long ldi,
              float a, float b, float *Out, long long ldo, int s, long long n, int un,
+             hipStream_t st);
+// bf16 basis P (un >= 100: the bf16 matrix cores with C split into hi + lo parts, else the FP32
+// ones on unpacked values); or FP32 P written to a bf16 Out (rounded to nearest).
+void gemm_nn(const bf16 *P, long long ldp, int p, const float *C, const float *In, long long ldi,
+             float a, float b, float *Out, long long ldo, int s, long long n, int un,
+             hipStream_t st);
+void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *In, long long ldi,
+             float a, float b, bf16 *Out, long long ldo, int s, long long n, int un,
              hipStream_t st);
 // Cholesky QR step on the Gram matrix G (s x s, ld s): see k_chol_qr.
 void chol_qr(const double *G, int s, double relTol, float *Sinv32, double *Sacc, int first,

@@ -40,6 +40,34 @@ __device__ inline f32x16 mfma32(float a, float b, f32x16 c) {
 // Row i of the 32 x 32 MFMA result held in accumulator v by lane l (column l % 32).
 __device__ inline int mfma_row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
 
+// bf16 storage: the upper 16 bits of a binary32, rounded to nearest even on store.
+__device__ inline float bf_to_f(uint32_t bits16) { return __uint_as_float(bits16 << 16); }
+__device__ inline uint32_t f_to_bf(float f) {  // finite inputs
+  const uint32_t x = __float_as_uint(f);
+  return (x + 0x7fffu + ((x >> 16) & 1u)) >> 16;
+}
+__device__ inline float ldf(const float *p, long long i) { return p[i]; }
+__device__ inline float ldf(const bf16 *p, long long i) { return bf_to_f(p[i].bits); }
+
+// Four consecutive outputs (row .. row + 3 of one column; full when all four are below n).
+__device__ inline void store_rows4(float *dst, const float (&w)[4], int cnt) {
+  if (cnt == 4) {
+    *reinterpret_cast<float4 *>(dst) = make_float4(w[0], w[1], w[2], w[3]);
+  } else {
+    for (int q = 0; q < cnt; ++q) dst[q] = w[q];
+  }
+}
+__device__ inline void store_rows4(bf16 *dst, const float (&w)[4], int cnt) {
+  if (cnt == 4) {
+    uint2 t;
+    t.x = f_to_bf(w[0]) | (f_to_bf(w[1]) << 16);
+    t.y = f_to_bf(w[2]) | (f_to_bf(w[3]) << 16);
+    *reinterpret_cast<uint2 *>(dst) = t;
+  } else {
+    for (int q = 0; q < cnt; ++q) dst[q].bits = (uint16_t)f_to_bf(w[q]);
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Thread (row, column group): blockIdx.y selects a group of CG columns, so one launch has
 // ceil(s / CG) threads per row and each thread reuses its row's nonzeros across CG columns.
@@ -99,7 +127,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_w(int n, const int *__restric
         const TX *xc = X + (long long)(c0 + c) * ldx;
         TA xv[W];
 #pragma unroll
-        for (int q = 0; q < W; ++q) xv[q] = (TA)xc[col[q]];
+        for (int q = 0; q < W; ++q) xv[q] = (TA)ldf(xc, col[q]);
 #pragma unroll
         for (int q = 0; q < W; ++q)
           if (b + q < e) acc[c] = fma(v[q], xv[q], acc[c]);
@@ -111,7 +139,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_w(int n, const int *__restric
       const TA v = (TA)val[q];
 #pragma unroll
       for (int c = 0; c < CG; ++c)
-        if (c0 + c < s) acc[c] = fma(v, (TA)X[col + (c0 + c) * ldx], acc[c]);
+        if (c0 + c < s) acc[c] = fma(v, (TA)ldf(X, col + (c0 + c) * ldx), acc[c]);
     }
   }
 #pragma unroll
@@ -319,25 +347,55 @@ __device__ inline f32x4 mfma16(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-__device__ inline void load_rows8(const float *col, long long r, long long rEnd, float4 &a,
-                                  float4 &b) {
-  a = make_float4(0.f, 0.f, 0.f, 0.f);
-  b = a;
+// Eight rows of one column as loaded (raw storage, unpacked only where consumed, so the loads of
+// a prefetch ring stay in flight): two float4 for FP32, one 16-byte word for bf16.
+template <typename TP> struct Raw8;
+template <> struct Raw8<float> { float4 a, b; };
+template <> struct Raw8<bf16> { uint4 w; };
+
+__device__ inline void load_raw8(const float *col, long long r, long long rEnd, Raw8<float> &o) {
+  o.a = make_float4(0.f, 0.f, 0.f, 0.f);
+  o.b = o.a;
   if (!col) return;
   if (r + 7 < rEnd) {
-    a = *reinterpret_cast<const float4 *>(col + r);
-    b = *reinterpret_cast<const float4 *>(col + r + 4);
+    o.a = *reinterpret_cast<const float4 *>(col + r);
+    o.b = *reinterpret_cast<const float4 *>(col + r + 4);
   } else {
     float v[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) v[e] = r + e < rEnd ? col[r + e] : 0.0f;
-    a = make_float4(v[0], v[1], v[2], v[3]);
-    b = make_float4(v[4], v[5], v[6], v[7]);
+    o.a = make_float4(v[0], v[1], v[2], v[3]);
+    o.b = make_float4(v[4], v[5], v[6], v[7]);
+  }
+}
+__device__ inline void load_raw8(const bf16 *col, long long r, long long rEnd, Raw8<bf16> &o) {
+  o.w = make_uint4(0u, 0u, 0u, 0u);
+  if (!col) return;
+  if (r + 7 < rEnd) {
+    o.w = *reinterpret_cast<const uint4 *>(col + r);
+  } else {
+    uint32_t h[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) h[e] = r + e < rEnd ? (uint32_t)col[r + e].bits : 0u;
+    o.w = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16),
+                     h[6] | (h[7] << 16));
+  }
+}
+__device__ inline void unpack8(const Raw8<float> &o, float (&v)[8]) {
+  v[0] = o.a.x; v[1] = o.a.y; v[2] = o.a.z; v[3] = o.a.w;
+  v[4] = o.b.x; v[5] = o.b.y; v[6] = o.b.z; v[7] = o.b.w;
+}
+__device__ inline void unpack8(const Raw8<bf16> &o, float (&v)[8]) {
+  const uint32_t w[4] = {o.w.x, o.w.y, o.w.z, o.w.w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    v[2 * i] = __uint_as_float(w[i] << 16);
+    v[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
   }
 }
 
-template <int TPW, int NU>
-__global__ __launch_bounds__(kThreads) void k_gemm_tn16(const float *__restrict__ P, long long ldp,
+template <typename TP, int TPW, int NU>
+__global__ __launch_bounds__(kThreads) void k_gemm_tn16(const TP *__restrict__ P, long long ldp,
                                                         int p, int pPad,
                                                         const float *__restrict__ U,
                                                         long long ldu, int s, long long n,
@@ -357,7 +415,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn16(const float *__restrict_
   for (int t = 0; t < TPW; ++t)
 #pragma unroll
     for (int u = 0; u < NU; ++u) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const float *pc[TPW];
+  const TP *pc[TPW];
 #pragma unroll
   for (int t = 0; t < TPW; ++t) {
     const int tile = tg * TPW + t, c = tile * 16 + m;
@@ -369,42 +427,36 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn16(const float *__restrict_
   if (rg < nRG) {
     const long long step = 32LL * nRG;
     long long r0 = rBeg + 32 * rg;
-    float4 pa[TPW], pb[TPW], ua[NU], ub[NU];
+    Raw8<TP> pr[TPW];
+    Raw8<float> ur[NU];
 #pragma unroll
-    for (int t = 0; t < TPW; ++t) load_rows8(pc[t], r0 + 8 * kk, rEnd, pa[t], pb[t]);
+    for (int t = 0; t < TPW; ++t) load_raw8(pc[t], r0 + 8 * kk, rEnd, pr[t]);
 #pragma unroll
-    for (int u = 0; u < NU; ++u) load_rows8(uc[u], r0 + 8 * kk, rEnd, ua[u], ub[u]);
+    for (int u = 0; u < NU; ++u) load_raw8(uc[u], r0 + 8 * kk, rEnd, ur[u]);
     while (r0 < rEnd) {
       const long long rn = r0 + step;
-      float4 qa[TPW], qb[TPW], va[NU], vb[NU];
+      Raw8<TP> qr[TPW];
+      Raw8<float> vr[NU];
 #pragma unroll
-      for (int t = 0; t < TPW; ++t) load_rows8(rn < rEnd ? pc[t] : nullptr, rn + 8 * kk, rEnd,
-                                               qa[t], qb[t]);
+      for (int t = 0; t < TPW; ++t) load_raw8(rn < rEnd ? pc[t] : nullptr, rn + 8 * kk, rEnd, qr[t]);
 #pragma unroll
-      for (int u = 0; u < NU; ++u) load_rows8(rn < rEnd ? uc[u] : nullptr, rn + 8 * kk, rEnd,
-                                              va[u], vb[u]);
+      for (int u = 0; u < NU; ++u) load_raw8(rn < rEnd ? uc[u] : nullptr, rn + 8 * kk, rEnd, vr[u]);
 #pragma unroll
       for (int u = 0; u < NU; ++u) {
-        const float bu[8] = {ua[u].x, ua[u].y, ua[u].z, ua[u].w,
-                             ub[u].x, ub[u].y, ub[u].z, ub[u].w};
+        float bu[8];
+        unpack8(ur[u], bu);
 #pragma unroll
         for (int t = 0; t < TPW; ++t) {
-          const float at[8] = {pa[t].x, pa[t].y, pa[t].z, pa[t].w,
-                               pb[t].x, pb[t].y, pb[t].z, pb[t].w};
+          float at[8];
+          unpack8(pr[t], at);
 #pragma unroll
           for (int e = 0; e < 8; ++e) acc[t][u] = mfma16(at[e], bu[e], acc[t][u]);
         }
       }
 #pragma unroll
-      for (int t = 0; t < TPW; ++t) {
-        pa[t] = qa[t];
-        pb[t] = qb[t];
-      }
+      for (int t = 0; t < TPW; ++t) pr[t] = qr[t];
 #pragma unroll
-      for (int u = 0; u < NU; ++u) {
-        ua[u] = va[u];
-        ub[u] = vb[u];
-      }
+      for (int u = 0; u < NU; ++u) ur[u] = vr[u];
       r0 = rn;
     }
 #pragma unroll
@@ -430,6 +482,243 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn16(const float *__restrict_
             a = a + red[(((g * nTG + tg) * TPW + t) * NU + u) * 256 + v * 64 + lane];
           out[(tile * 16 + 4 * kk + v) * 32 + u * 16 + m] = a;
         }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// bf16 basis on the bf16 matrix cores (v_mfma_f32_16x16x16bf16_1k, FP32 accumulation).  The FP32
+// operand (U, or the coefficients C) is split into hi + lo bf16 parts (hi = RNE(x), lo =
+// RNE(x - hi), |x - hi - lo| <= 2^-17 |x|), and both parts are multiplied, so the products keep
+// about FP32 accuracy while the basis is read as stored.  Operand layouts: A (16 x 16): lane l
+// holds A[l % 16][4 (l / 16) + 0..3]; B (16 x 16): lane l holds B[4 (l / 16) + 0..3][l % 16];
+// D: lane l, register v holds D[4 (l / 16) + v][l % 16].
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline f32x4 mfma16b(s16x4 a, s16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ inline s16x4 pack_s16x4(uint32_t lo, uint32_t hi) {
+  s16x4 r;
+  r[0] = (short)(lo & 0xffffu);
+  r[1] = (short)(lo >> 16);
+  r[2] = (short)(hi & 0xffffu);
+  r[3] = (short)(hi >> 16);
+  return r;
+}
+// hi and lo bf16 parts of four floats.
+__device__ inline void split4(float x0, float x1, float x2, float x3, s16x4 &hi, s16x4 &lo) {
+  const uint32_t h0 = f_to_bf(x0), h1 = f_to_bf(x1), h2 = f_to_bf(x2), h3 = f_to_bf(x3);
+  const uint32_t l0 = f_to_bf(x0 - bf_to_f(h0)), l1 = f_to_bf(x1 - bf_to_f(h1)),
+                 l2 = f_to_bf(x2 - bf_to_f(h2)), l3 = f_to_bf(x3 - bf_to_f(h3));
+  hi = pack_s16x4(h0 | (h1 << 16), h2 | (h3 << 16));
+  lo = pack_s16x4(l0 | (l1 << 16), l2 | (l3 << 16));
+}
+
+// C = P^T U with bf16 P: the structure of k_gemm_tn16 (tile groups x row groups, 32-row rounds,
+// lane (m, kk) holding rows r0 + 8kk .. r0 + 8kk + 7 of its columns), with two bf16 MFMAs per
+// 8 rows: K index 4kk + e is row r0 + 8kk + e in the first and r0 + 8kk + 4 + e in the second.
+template <int TPW, int NU>
+__global__ __launch_bounds__(kThreads) void k_gemm_tn16b(const bf16 *__restrict__ P, long long ldp,
+                                                         int p, int pPad,
+                                                         const float *__restrict__ U,
+                                                         long long ldu, int s, long long n,
+                                                         long long rowsPerBlock,
+                                                         float *__restrict__ part) {
+  __shared__ float red[4 * TPW * NU * 256];
+  const int nT = (p + 15) / 16;
+  const int nTG = (nT + TPW - 1) / TPW;
+  const int nRG = 4 / nTG;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tg = wave % nTG, rg = wave / nTG;
+  const int m = lane & 15, kk = lane >> 4;
+  const long long rBeg = (long long)blockIdx.x * rowsPerBlock;
+  const long long rEnd = rBeg + rowsPerBlock < n ? rBeg + rowsPerBlock : n;
+  f32x4 acc[TPW][NU];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int u = 0; u < NU; ++u) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16 *pc[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int tile = tg * TPW + t, c = tile * 16 + m;
+    pc[t] = (tile < nT && c < p) ? P + c * ldp : nullptr;
+  }
+  const float *uc[NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) uc[u] = u * 16 + m < s ? U + (u * 16 + m) * ldu : nullptr;
+  if (rg < nRG) {
+    const long long step = 32LL * nRG;
+    long long r0 = rBeg + 32 * rg;
+    Raw8<bf16> pr[TPW];
+    Raw8<float> ur[NU];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) load_raw8(pc[t], r0 + 8 * kk, rEnd, pr[t]);
+#pragma unroll
+    for (int u = 0; u < NU; ++u) load_raw8(uc[u], r0 + 8 * kk, rEnd, ur[u]);
+    while (r0 < rEnd) {
+      const long long rn = r0 + step;
+      Raw8<bf16> qr[TPW];
+      Raw8<float> vr[NU];
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) load_raw8(rn < rEnd ? pc[t] : nullptr, rn + 8 * kk, rEnd, qr[t]);
+#pragma unroll
+      for (int u = 0; u < NU; ++u) load_raw8(rn < rEnd ? uc[u] : nullptr, rn + 8 * kk, rEnd, vr[u]);
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        s16x4 bh0, bl0, bh1, bl1;
+        split4(ur[u].a.x, ur[u].a.y, ur[u].a.z, ur[u].a.w, bh0, bl0);
+        split4(ur[u].b.x, ur[u].b.y, ur[u].b.z, ur[u].b.w, bh1, bl1);
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+          const s16x4 a0 = pack_s16x4(pr[t].w.x, pr[t].w.y);
+          const s16x4 a1 = pack_s16x4(pr[t].w.z, pr[t].w.w);
+          acc[t][u] = mfma16b(a0, bh0, acc[t][u]);
+          acc[t][u] = mfma16b(a0, bl0, acc[t][u]);
+          acc[t][u] = mfma16b(a1, bh1, acc[t][u]);
+          acc[t][u] = mfma16b(a1, bl1, acc[t][u]);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) pr[t] = qr[t];
+#pragma unroll
+      for (int u = 0; u < NU; ++u) ur[u] = vr[u];
+      r0 = rn;
+    }
+#pragma unroll
+    for (int t = 0; t < TPW; ++t)
+#pragma unroll
+      for (int u = 0; u < NU; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[((wave * TPW + t) * NU + u) * 256 + v * 64 + lane] = acc[t][u][v];
+  }
+  __syncthreads();
+  if (rg != 0) return;
+  float *out = part + (long long)blockIdx.x * pPad * 32;
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int tile = tg * TPW + t;
+    if (tile < nT) {
+#pragma unroll
+      for (int u = 0; u < NU; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          float a = 0.0f;
+          for (int g = 0; g < nRG; ++g)
+            a = a + red[(((g * nTG + tg) * TPW + t) * NU + u) * 256 + v * 64 + lane];
+          out[(tile * 16 + 4 * kk + v) * 32 + u * 16 + m] = a;
+        }
+    }
+  }
+}
+
+// Out[r, j] = a In[r, j] + b sum_k P[r, k] C[k][j] with bf16 P (C: p x 32 row-major FP32, staged
+// in LDS as hi and lo bf16 parts, column-major by output column).  A wavefront owns 128 rows:
+// lane (m, kg) loads rows r0 + 8m .. r0 + 8m + 7 of the four columns c0 + 4kg .. c0 + 4kg + 3 of
+// each 16-column chunk (16-byte loads, a ring of two chunks in flight); MFMA i (i = 0..7) covers
+// rows r0 + 8m + i, m = 0..15, so its A operand is element i of the four loaded words.  Output
+// row r0 + 8 (4kg + v) + i, column 16t + m of N-tile t comes out in lane (m, kg), register v of
+// accumulator (i, t): eight consecutive rows of one column per lane.
+template <int NT, typename TO>
+__global__ __launch_bounds__(kThreads) void k_gemm_nn_b(const bf16 *__restrict__ P, long long ldp,
+                                                        int p, const float *__restrict__ C,
+                                                        const float *In, long long ldi, float a,
+                                                        float b, TO *Out, long long ldo, int s,
+                                                        long long n) {
+  extern __shared__ uint16_t Cl[];  // [hi | lo][16 NT output columns][KP]
+  const int KP = (p + 15) & ~15;
+  uint16_t *Ch = Cl, *Clo = Cl + (size_t)16 * NT * KP;
+  for (int idx = threadIdx.x; idx < 16 * NT * KP; idx += kThreads) {
+    const int col = idx / KP, k = idx % KP;
+    const float x = (k < p && col < s) ? C[k * 32 + col] : 0.0f;
+    const uint32_t h = f_to_bf(x);
+    Ch[idx] = (uint16_t)h;
+    Clo[idx] = (uint16_t)f_to_bf(x - bf_to_f(h));
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long r0 = ((long long)blockIdx.x * 4 + wave) * 128;
+  if (r0 >= n) return;
+  const int m = lane & 15, kg = lane >> 4;
+  const long long rl = r0 + 8 * m;
+  f32x4 acc[8][NT];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[i][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  auto load_chunk = [&](int c0, Raw8<bf16> (&w)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int cc = c0 + 4 * kg + j;
+      load_raw8(cc < p ? P + cc * ldp : nullptr, rl, n, w[j]);
+    }
+  };
+  Raw8<bf16> cur[4], nxt[4];
+  load_chunk(0, cur);
+  for (int c0 = 0; c0 < p; c0 += 16) {
+    load_chunk(c0 + 16, nxt);  // past p: zero words, no loads
+    s16x4 bh[NT], bl[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int base = (16 * t + m) * KP + c0 + 4 * kg;
+      const uint2 h = *reinterpret_cast<const uint2 *>(Ch + base);
+      const uint2 l = *reinterpret_cast<const uint2 *>(Clo + base);
+      bh[t] = pack_s16x4(h.x, h.y);
+      bl[t] = pack_s16x4(l.x, l.y);
+    }
+    const uint32_t w[4][4] = {{cur[0].w.x, cur[0].w.y, cur[0].w.z, cur[0].w.w},
+                              {cur[1].w.x, cur[1].w.y, cur[1].w.z, cur[1].w.w},
+                              {cur[2].w.x, cur[2].w.y, cur[2].w.z, cur[2].w.w},
+                              {cur[3].w.x, cur[3].w.y, cur[3].w.z, cur[3].w.w}};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int q = i >> 1, sh = (i & 1) * 16;
+      const uint32_t e0 = (w[0][q] >> sh) & 0xffffu, e1 = (w[1][q] >> sh) & 0xffffu;
+      const uint32_t e2 = (w[2][q] >> sh) & 0xffffu, e3 = (w[3][q] >> sh) & 0xffffu;
+      const s16x4 av = pack_s16x4(e0 | (e1 << 16), e2 | (e3 << 16));
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        acc[i][t] = mfma16b(av, bh[t], acc[i][t]);
+        acc[i][t] = mfma16b(av, bl[t], acc[i][t]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) cur[j] = nxt[j];
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int j = 16 * t + m;
+    if (j >= s) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const long long row = r0 + 8 * (4 * kg + v);
+      if (row >= n) continue;
+      const int cnt = row + 7 < n ? 8 : (int)(n - row);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int c4 = cnt - 4 * h < 0 ? 0 : (cnt - 4 * h > 4 ? 4 : cnt - 4 * h);
+        if (c4 == 0) continue;
+        float in[4] = {0.f, 0.f, 0.f, 0.f};
+        if (a != 0.0f) {
+          if (c4 == 4) {
+            const float4 iv = *reinterpret_cast<const float4 *>(In + j * ldi + row + 4 * h);
+            in[0] = iv.x;
+            in[1] = iv.y;
+            in[2] = iv.z;
+            in[3] = iv.w;
+          } else {
+            for (int q = 0; q < c4; ++q) in[q] = In[j * ldi + row + 4 * h + q];
+          }
+        }
+        float o[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float pv = acc[4 * h + q][t][v];
+          o[q] = a != 0.0f ? a * in[q] + b * pv : b * pv;
+        }
+        store_rows4(Out + j * ldo + row + 4 * h, o, c4);
+      }
     }
   }
 }
@@ -553,11 +842,41 @@ __device__ inline float4 load_rows4(const float *P, long long ldp, int cc, int p
   return v;
 }
 
-template <int D>
-__global__ __launch_bounds__(kThreads) void k_gemm_nn(const float *P, long long ldp, int p,
+// Four rows of one column as loaded (raw, unpacked where consumed): float4 or two bf16 words.
+template <typename TP> struct Raw4;
+template <> struct Raw4<float> { float4 v; };
+template <> struct Raw4<bf16> { uint2 w; };
+__device__ inline Raw4<float> load_raw4(const float *P, long long ldp, int cc, int p,
+                                        long long rl, long long n) {
+  return Raw4<float>{load_rows4(P, ldp, cc, p, rl, n)};
+}
+__device__ inline Raw4<bf16> load_raw4(const bf16 *P, long long ldp, int cc, int p, long long rl,
+                                       long long n) {
+  Raw4<bf16> o{make_uint2(0u, 0u)};
+  if (cc < p) {
+    const bf16 *src = P + cc * ldp + rl;
+    if (rl + 3 < n) {
+      o.w = *reinterpret_cast<const uint2 *>(src);
+    } else {
+      uint32_t h[4] = {0u, 0u, 0u, 0u};
+      for (int q = 0; q < 3; ++q)
+        if (rl + q < n) h[q] = src[q].bits;
+      o.w = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+    }
+  }
+  return o;
+}
+__device__ inline float4 unpack4(const Raw4<float> &o) { return o.v; }
+__device__ inline float4 unpack4(const Raw4<bf16> &o) {
+  return make_float4(__uint_as_float(o.w.x << 16), __uint_as_float(o.w.x & 0xffff0000u),
+                     __uint_as_float(o.w.y << 16), __uint_as_float(o.w.y & 0xffff0000u));
+}
+
+template <int D, typename TP, typename TO>
+__global__ __launch_bounds__(kThreads) void k_gemm_nn(const TP *P, long long ldp, int p,
                                                       const float *__restrict__ C,
                                                       const float *In, long long ldi, float a,
-                                                      float b, float *Out, long long ldo, int s,
+                                                      float b, TO *Out, long long ldo, int s,
                                                       long long n) {
   extern __shared__ float Cs[];
   const int pPad = (p + 2 * D - 1) / (2 * D) * (2 * D) + 2 * D;  // ring reads past p are zero
@@ -572,14 +891,14 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nn(const float *P, long long 
   for (int v = 0; v < 16; ++v) acc0[v] = acc1[v] = acc2[v] = acc3[v] = 0.0f;
   const long long rl = r0 + 4 * (lane & 31);
   const int kh = lane >> 5, jl = lane & 31;
-  float4 ring[D];
+  Raw4<TP> ring[D];
 #pragma unroll
-  for (int i = 0; i < D; ++i) ring[i] = load_rows4(P, ldp, 2 * i + kh, p, rl, n);
+  for (int i = 0; i < D; ++i) ring[i] = load_raw4(P, ldp, 2 * i + kh, p, rl, n);
   for (int c = 0; c < p; c += 2 * D) {
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-      const float4 x = ring[i];
-      ring[i] = load_rows4(P, ldp, c + 2 * (i + D) + kh, p, rl, n);
+      const float4 x = unpack4(ring[i]);
+      ring[i] = load_raw4(P, ldp, c + 2 * (i + D) + kh, p, rl, n);
       const float bb = Cs[(c + 2 * i + kh) * 32 + jl];
       acc0 = mfma32(x.x, bb, acc0);
       acc1 = mfma32(x.y, bb, acc1);
@@ -593,28 +912,27 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nn(const float *P, long long 
   for (int v = 0; v < 16; ++v) {
     const long long row = r0 + 4 * mfma_row(v, lane);
     const float o[4] = {acc0[v], acc1[v], acc2[v], acc3[v]};
-    if (row + 3 < n) {
-      float4 w;
-      if (a != 0.0f) {
-        const float4 in = *reinterpret_cast<const float4 *>(In + j * ldi + row);
-        w.x = a * in.x + b * o[0];
-        w.y = a * in.y + b * o[1];
-        w.z = a * in.z + b * o[2];
-        w.w = a * in.w + b * o[3];
+    if (row >= n) continue;
+    const int cnt = row + 3 < n ? 4 : (int)(n - row);
+    float w[4];
+    if (a != 0.0f) {
+      float in[4] = {0.f, 0.f, 0.f, 0.f};
+      if (cnt == 4) {
+        const float4 iv = *reinterpret_cast<const float4 *>(In + j * ldi + row);
+        in[0] = iv.x;
+        in[1] = iv.y;
+        in[2] = iv.z;
+        in[3] = iv.w;
       } else {
-        w.x = b * o[0];
-        w.y = b * o[1];
-        w.z = b * o[2];
-        w.w = b * o[3];
+        for (int q = 0; q < cnt; ++q) in[q] = In[j * ldi + row + q];
       }
-      *reinterpret_cast<float4 *>(Out + j * ldo + row) = w;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) w[q] = a * in[q] + b * o[q];
     } else {
-      for (int q = 0; q < 4; ++q)
-        if (row + q < n) {
-          const float in = a != 0.0f ? In[j * ldi + row + q] : 0.0f;
-          Out[j * ldo + row + q] = a * in + b * o[q];
-        }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) w[q] = b * o[q];
     }
+    store_rows4(Out + j * ldo + row, w, cnt);
   }
 }
 
@@ -956,21 +1274,31 @@ void zero_ints(int *p, int cnt, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, p, cnt);
 }
 
+template <typename TX>
+void spmm_upfront(const CsrDev &A, const TX *X, long long ldx, float *Y, long long ldy, int s,
+                  int g, hipStream_t st) {
+  const dim3 grid(grid1(A.n), (unsigned)((s + g - 1) / g));
+  if (g == 4)
+    hipLaunchKernelGGL((k_spmm_w<float, TX, float, float, 4, 8>), grid, dim3(kThreads), 0, st, A.n,
+                       A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
+  else if (g == 16)
+    hipLaunchKernelGGL((k_spmm_w<float, TX, float, float, 16, 8>), grid, dim3(kThreads), 0, st,
+                       A.n, A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
+  else
+    hipLaunchKernelGGL((k_spmm_w<float, TX, float, float, 8, 8>), grid, dim3(kThreads), 0, st, A.n,
+                       A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
+}
+
+void spmm32(const CsrDev &A, const bf16 *X, long long ldx, float *Y, long long ldy, int s, int cg,
+            hipStream_t st) {
+  spmm_upfront<bf16>(A, X, ldx, Y, ldy, s, cg >= 100 ? cg - 100 : cg, st);
+}
+
 void spmm32(const CsrDev &A, const float *X, long long ldx, float *Y, long long ldy, int s,
             int cg, hipStream_t st) {
   if (cg >= 100) {
     // Up-front index variant: cg - 100 columns per thread, rows of up to 8 entries unrolled.
-    const int g = cg - 100;
-    const dim3 grid(grid1(A.n), (unsigned)((s + g - 1) / g));
-    if (g == 4)
-      hipLaunchKernelGGL((k_spmm_w<float, float, float, float, 4, 8>), grid, dim3(kThreads), 0, st,
-                         A.n, A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
-    else if (g == 16)
-      hipLaunchKernelGGL((k_spmm_w<float, float, float, float, 16, 8>), grid, dim3(kThreads), 0,
-                         st, A.n, A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
-    else
-      hipLaunchKernelGGL((k_spmm_w<float, float, float, float, 8, 8>), grid, dim3(kThreads), 0, st,
-                         A.n, A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
+    spmm_upfront<float>(A, X, ldx, Y, ldy, s, cg - 100, st);
     return;
   }
   const dim3 grid(grid1(A.n), (unsigned)((s + cg - 1) / cg));
@@ -1032,34 +1360,77 @@ int gemm_tn_blocks(long long n, int cap, int rowsTarget) {
 
 size_t gemm_tn_lds(int pPad) { return sizeof(float) * (size_t)(pPad + 32) * kLdsStride; }
 
-void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu, int s,
+template <typename TP>
+void gemm_tn16(const TP *P, long long ldp, int p, int pPad, const float *U, long long ldu, int s,
+               long long n, int G, long long rows, float *part, hipStream_t st) {
+  // Tile groups: one group of up to 6 tiles (four row groups), else two, else four.
+  const int nT = (p + 15) / 16;
+  const int nTG = nT <= 6 ? 1 : (nT <= 12 ? 2 : 4);
+  const int tpw = (nT + nTG - 1) / nTG;
+  const int nu = s <= 16 ? 1 : 2;
+#define DMR_TN16(T)                                                                              \
+  if (tpw == T) {                                                                                \
+    if (nu == 1)                                                                                 \
+      hipLaunchKernelGGL((k_gemm_tn16<TP, T, 1>), dim3(G), dim3(kThreads), 0, st, P, ldp, p,     \
+                         pPad, U, ldu, s, n, rows, part);                                        \
+    else                                                                                         \
+      hipLaunchKernelGGL((k_gemm_tn16<TP, T, 2>), dim3(G), dim3(kThreads), 0, st, P, ldp, p,     \
+                         pPad, U, ldu, s, n, rows, part);                                        \
+  }
+  DMR_TN16(1)
+  DMR_TN16(2)
+  DMR_TN16(3)
+  DMR_TN16(4)
+  DMR_TN16(5)
+  DMR_TN16(6)
+#undef DMR_TN16
+}
+
+long long tn_rows(long long n, int G) {
+  long long rows = (n + G - 1) / G;
+  return (rows + 63) / 64 * 64;
+}
+
+void gemm_tn(const bf16 *P, long long ldp, int p, const float *U, long long ldu, int s,
              long long n, int G, int variant, float *part, double *C64, int ldc, float *C32,
              hipStream_t st) {
   const int pPad = (p + 31) & ~31;
-  long long rows = (n + G - 1) / G;
-  rows = (rows + 63) / 64 * 64;
-  if (variant == 2) {
-    // Tile groups: one group of up to 6 tiles (four row groups), else two, else four.
+  const long long rows = tn_rows(n, G);
+  if (variant == 3) {
     const int nT = (p + 15) / 16;
     const int nTG = nT <= 6 ? 1 : (nT <= 12 ? 2 : 4);
     const int tpw = (nT + nTG - 1) / nTG;
     const int nu = s <= 16 ? 1 : 2;
-#define DMR_TN16(T)                                                                              \
+#define DMR_TN16B(T)                                                                             \
   if (tpw == T) {                                                                                \
     if (nu == 1)                                                                                 \
-      hipLaunchKernelGGL((k_gemm_tn16<T, 1>), dim3(G), dim3(kThreads), 0, st, P, ldp, p, pPad, U, \
-                         ldu, s, n, rows, part);                                                 \
+      hipLaunchKernelGGL((k_gemm_tn16b<T, 1>), dim3(G), dim3(kThreads), 0, st, P, ldp, p, pPad,  \
+                         U, ldu, s, n, rows, part);                                              \
     else                                                                                         \
-      hipLaunchKernelGGL((k_gemm_tn16<T, 2>), dim3(G), dim3(kThreads), 0, st, P, ldp, p, pPad, U, \
-                         ldu, s, n, rows, part);                                                 \
+      hipLaunchKernelGGL((k_gemm_tn16b<T, 2>), dim3(G), dim3(kThreads), 0, st, P, ldp, p, pPad,  \
+                         U, ldu, s, n, rows, part);                                              \
   }
-    DMR_TN16(1)
-    DMR_TN16(2)
-    DMR_TN16(3)
-    DMR_TN16(4)
-    DMR_TN16(5)
-    DMR_TN16(6)
-#undef DMR_TN16
+    DMR_TN16B(1)
+    DMR_TN16B(2)
+    DMR_TN16B(3)
+    DMR_TN16B(4)
+    DMR_TN16B(5)
+    DMR_TN16B(6)
+#undef DMR_TN16B
+  } else {
+    gemm_tn16<bf16>(P, ldp, p, pPad, U, ldu, s, n, G, rows, part, st);
+  }
+  hipLaunchKernelGGL(k_gemm_tn_reduce, dim3(p), dim3(kThreads), 0, st, part, G, pPad, p, s, C64,
+                     ldc, C32);
+}
+
+void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu, int s,
+             long long n, int G, int variant, float *part, double *C64, int ldc, float *C32,
+             hipStream_t st) {
+  const int pPad = (p + 31) & ~31;
+  const long long rows = tn_rows(n, G);
+  if (variant == 2) {
+    gemm_tn16<float>(P, ldp, p, pPad, U, ldu, s, n, G, rows, part, st);
   } else if (variant == 1) {
     const size_t lds = gemm_tn_lds(pPad);
     hipLaunchKernelGGL(k_gemm_tn_lds, dim3(G), dim3(kThreads), lds, st, P, ldp, p, pPad, U, ldu,
@@ -1072,26 +1443,58 @@ void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu
                      ldc, C32);
 }
 
-void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *In, long long ldi,
-             float a, float b, float *Out, long long ldo, int s, long long n, int un,
-             hipStream_t st) {
+template <typename TP, typename TO>
+void gemm_nn_t(const TP *P, long long ldp, int p, const float *C, const float *In, long long ldi,
+               float a, float b, TO *Out, long long ldo, int s, long long n, int un,
+               hipStream_t st) {
   const long long waves = (n + 127) / 128;
   const unsigned grid = (unsigned)((waves + 3) / 4);
   const int D = un >= 16 ? 16 : (un >= 8 ? 8 : (un >= 4 ? 4 : 2));
   const int pPad = (p + 2 * D - 1) / (2 * D) * (2 * D) + 2 * D;
   const size_t lds = sizeof(float) * (size_t)pPad * 32;
   if (D == 16)
-    hipLaunchKernelGGL(k_gemm_nn<16>, dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C, In, ldi,
-                       a, b, Out, ldo, s, n);
+    hipLaunchKernelGGL((k_gemm_nn<16, TP, TO>), dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C,
+                       In, ldi, a, b, Out, ldo, s, n);
   else if (D == 8)
-    hipLaunchKernelGGL(k_gemm_nn<8>, dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C, In, ldi, a,
-                       b, Out, ldo, s, n);
+    hipLaunchKernelGGL((k_gemm_nn<8, TP, TO>), dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C,
+                       In, ldi, a, b, Out, ldo, s, n);
   else if (D == 4)
-    hipLaunchKernelGGL(k_gemm_nn<4>, dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C, In, ldi, a,
-                       b, Out, ldo, s, n);
+    hipLaunchKernelGGL((k_gemm_nn<4, TP, TO>), dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C,
+                       In, ldi, a, b, Out, ldo, s, n);
   else
-    hipLaunchKernelGGL(k_gemm_nn<2>, dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C, In, ldi, a,
-                       b, Out, ldo, s, n);
+    hipLaunchKernelGGL((k_gemm_nn<2, TP, TO>), dim3(grid), dim3(kThreads), lds, st, P, ldp, p, C,
+                       In, ldi, a, b, Out, ldo, s, n);
+}
+
+void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *In, long long ldi,
+             float a, float b, float *Out, long long ldo, int s, long long n, int un,
+             hipStream_t st) {
+  gemm_nn_t<float, float>(P, ldp, p, C, In, ldi, a, b, Out, ldo, s, n, un, st);
+}
+void gemm_nn(const bf16 *P, long long ldp, int p, const float *C, const float *In, long long ldi,
+             float a, float b, float *Out, long long ldo, int s, long long n, int un,
+             hipStream_t st) {
+  if (un >= 100) {
+    // bf16 matrix cores: 128 rows per wavefront, four per workgroup.
+    const long long waves = (n + 127) / 128;
+    const unsigned grid = (unsigned)((waves + 3) / 4);
+    const int KP = (p + 15) & ~15;
+    if (s <= 16)
+      hipLaunchKernelGGL((k_gemm_nn_b<1, float>), dim3(grid), dim3(kThreads),
+                         sizeof(uint16_t) * 2 * 16 * KP, st, P, ldp, p, C, In, ldi, a, b, Out, ldo,
+                         s, n);
+    else
+      hipLaunchKernelGGL((k_gemm_nn_b<2, float>), dim3(grid), dim3(kThreads),
+                         sizeof(uint16_t) * 2 * 32 * KP, st, P, ldp, p, C, In, ldi, a, b, Out, ldo,
+                         s, n);
+    return;
+  }
+  gemm_nn_t<bf16, float>(P, ldp, p, C, In, ldi, a, b, Out, ldo, s, n, un, st);
+}
+void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *In, long long ldi,
+             float a, float b, bf16 *Out, long long ldo, int s, long long n, int un,
+             hipStream_t st) {
+  gemm_nn_t<float, bf16>(P, ldp, p, C, In, ldi, a, b, Out, ldo, s, n, un, st);
 }
 
 void chol_qr(const double *G, int s, double relTol, float *Sinv32, double *Sacc, int first,

@@ -111,6 +111,7 @@ _sig("dmrBlockSetOperatorCsr", C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp])
 _sig("dmrBlockSetOperatorValues", C.c_int, [_vp, _vp, _vp])
 _sig("dmrBlockSetRefinement", C.c_int, [_vp, C.c_double, C.c_int])
 _sig("dmrBlockSetColumnOperators", C.c_int, [_vp, _vp, C.c_longlong, C.c_int, C.c_int])
+_sig("dmrBlockSetPrecision", C.c_int, [_vp, C.c_int])
 _sig("dmrBlockSolve", C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_longlong, _vp, C.c_longlong, _vp,
                                 C.c_double, C.c_int, _dp])
 _sig("dmrBlockGetStats", C.c_int, [_vp, C.POINTER(Stats), _dp, C.c_int])
@@ -151,7 +152,7 @@ HEADER_SYMBOLS = (
     "dmrGetProfile dmrSubspaceBytes dmrSubspaceToDevice dmrSubspaceFromDevice "
     "dmrGetStagnationCount dmrSetStagnationCount "
     "dmrBlockCreate dmrBlockDestroy dmrBlockSetOperatorCsr dmrBlockSetOperatorValues "
-    "dmrBlockSetRefinement dmrBlockSetColumnOperators dmrBlockSolve dmrBlockGetStats dmrBlockHasZ dmrBlockImportZ "
+    "dmrBlockSetRefinement dmrBlockSetColumnOperators dmrBlockSetPrecision dmrBlockSolve dmrBlockGetStats dmrBlockHasZ dmrBlockImportZ "
     "dmrBlockExportZ dmrBlockSetProfiling dmrBlockGetProfile dmrBlockFootprint "
     "dmrBlockSetOptions dmrBlockGetTuning "
     "dmrServerCreate dmrServerAddFamily dmrServerAddBlockFamily dmrServerSetRefinement "
@@ -419,6 +420,10 @@ class BlockSolver:
 
     def set_refinement(self, inner_tol: float, max_outer: int = 30):
         _check(lib.dmrBlockSetRefinement(self.h, inner_tol, max_outer), "dmrBlockSetRefinement")
+
+    def set_precision(self, precision: int):
+        """Basis storage: PREC_FP32 (default) or PREC_BF16."""
+        _check(lib.dmrBlockSetPrecision(self.h, precision), "dmrBlockSetPrecision")
 
     def solve(self, x_ptr, ldx, b_ptr, ldb, nrhs, tol, max_cycles=1000, mode=0, stream=0):
         """Returns (status code, per-column FP64-checked relative residuals)."""

@@ -191,6 +191,13 @@ int dmrBlockSetOperatorValues(dmrBlockHandle h, const double *val, const float *
  * with more columns than count is rejected. */
 int dmrBlockSetColumnOperators(dmrBlockHandle h, const double *val, long long ldv, int count,
                                int innerMean);
+/* Storage of the block basis [W V_0 V_1 ...] (no counterpart in the reference): DMR_PREC_FP32
+ * (default) or DMR_PREC_BF16.  With bf16 the GEMMs and the SpMM read half the bytes and compute
+ * in FP32 (MFMA FP32 accumulation); the newest block, the residual and solution blocks and the
+ * recycled subspace stay FP32, and the refinement, least squares and SVD stay FP64.  Each inner
+ * solve then reaches about 1e-3 relative, so the inner tolerance should be at least that; the
+ * FP64 refinement still delivers the requested tolerance. */
+int dmrBlockSetPrecision(dmrBlockHandle h, int precision);
 /* Relative tolerance of the inner FP32 block solves (default 1e-5) and the maximum number of
  * refinement steps (default 30). */
 int dmrBlockSetRefinement(dmrBlockHandle h, double innerTol, int maxOuter);

@@ -23,7 +23,8 @@ def _dev(torch, a):
     return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
 
 
-def _block(torch, A, B, mb, k, smax=None, ld=None, tol=1e-10, mode=0, solver=None, x0=None):
+def _block(torch, A, B, mb, k, smax=None, ld=None, tol=1e-10, mode=0, solver=None, x0=None,
+           prec=None, inner_tol=None):
     """Solves A X = B (B: s x n host array, one right-hand side per row) with a block handle."""
     s, n = B.shape
     ld = ld or n
@@ -39,6 +40,10 @@ def _block(torch, A, B, mb, k, smax=None, ld=None, tol=1e-10, mode=0, solver=Non
         solver = gdmr.BlockSolver(n, mb, k, smax or s)
         rp, ci, va = _dev(torch, A.rowptr), _dev(torch, A.col), _dev(torch, A.val)
         solver.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A.nnz, keep=(rp, ci, va))
+        if prec is not None:
+            solver.set_precision(prec)
+        if inner_tol is not None:
+            solver.set_refinement(inner_tol, 60)
     torch.cuda.synchronize()
     rc, rel = solver.solve(Xd.data_ptr(), ld, Bd.data_ptr(), ld, s, tol, max_cycles=500, mode=mode)
     st, hist = solver.stats()
@@ -181,7 +186,8 @@ def test_block_graph_replays_match_eager(torch_gpu):
         bs.close()
 
 
-def _column_operator_solve(torch, mats, B, mb, k, inner_mean, tol=1e-10, solver=None):
+def _column_operator_solve(torch, mats, B, mb, k, inner_mean, tol=1e-10, solver=None,
+                           prec=None, inner_tol=None):
     """Solves A_t x_t = b_t for a sequence of systems with one sparsity pattern as one block:
     column t uses its own FP64 operator (dmrBlockSetColumnOperators)."""
     s, n = B.shape
@@ -191,6 +197,10 @@ def _column_operator_solve(torch, mats, B, mb, k, inner_mean, tol=1e-10, solver=
     bs = solver or gdmr.BlockSolver(n, mb, k, s)
     bs.set_csr(rp.data_ptr(), ci.data_ptr(), vals[0].data_ptr(), nnz, keep=(rp, ci))
     bs.set_column_operators(vals.data_ptr(), nnz, s, inner_mean=inner_mean, keep=(vals,))
+    if prec is not None:
+        bs.set_precision(prec)
+    if inner_tol is not None:
+        bs.set_refinement(inner_tol, 60)
     Bd = _dev(torch, B)
     Xd = torch.zeros(s, n, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
@@ -202,8 +212,8 @@ def _column_operator_solve(torch, mats, B, mb, k, inner_mean, tol=1e-10, solver=
     return rc, rel, st, X
 
 
-@pytest.mark.parametrize("inner_mean", [True, False])
-def test_column_operators_solve_a_jittered_sequence(torch_gpu, inner_mean):
+@pytest.mark.parametrize("inner_mean,prec", [(True, "fp32"), (False, "fp32"), (True, "bf16")])
+def test_column_operators_solve_a_jittered_sequence(torch_gpu, inner_mean, prec):
     """Six systems of a jittered 3-D convection-diffusion sequence (16^3, per-cell diffusion
     U(+-1%) per system) solved as one block: column t is refined against its own FP64 operator,
     the inner FP32 solve shares the mean operator (or system 0's).  Every column meets 1e-10 with
@@ -211,8 +221,10 @@ def test_column_operators_solve_a_jittered_sequence(torch_gpu, inner_mean):
     mats = [gdmr.conv_diff_3d(16, jitter=0.01, seed=41, stream=t) for t in range(6)]
     n, s, tol = mats[0].n, len(mats), 1e-10
     B = np.stack([gdmr.rand_normal(42, t, n) for t in range(s)])
-    rc, rel, st, X = _column_operator_solve(torch_gpu, mats, B, mb=6, k=8, inner_mean=inner_mean)
-    print(f"inner_mean {inner_mean}: rc {rc} cycles {st.cycles} max rel {rel.max():.2e}",
+    p = gdmr.PREC_BF16 if prec == "bf16" else gdmr.PREC_FP32
+    rc, rel, st, X = _column_operator_solve(torch_gpu, mats, B, mb=6, k=8, inner_mean=inner_mean,
+                                            prec=p, inner_tol=3e-2 if prec == "bf16" else None)
+    print(f"inner_mean {inner_mean} {prec}: rc {rc} cycles {st.cycles} max rel {rel.max():.2e}",
           flush=True)
     assert rc == 0
     for t, A in enumerate(mats):
@@ -269,4 +281,78 @@ def test_column_operators_reject_bad_arguments(torch_gpu):
     bs.set_column_operators(0, 0, 0)  # cleared: a 4-column solve with the block operator
     rc, rel = bs.solve(X.data_ptr(), n, B.data_ptr(), n, 4, 1e-10, max_cycles=500)
     assert rc == 0
+    bs.close()
+
+
+def test_bf16_basis_matches_oracle_and_fp32_block(torch_gpu):
+    """bf16 basis storage (GEMMs and SpMM read bf16, compute in FP32 on the matrix cores): the
+    anisotropic 16^3 case of test_block_matches_oracle meets 1e-10 in FP64 per column and agrees
+    with the FP64 oracle's solutions (and so with the FP32-basis block solve) to XTOL."""
+    A = gdmr.aniso_3d(16, coef=(1.0, 0.1, 0.01))
+    n, s, tol = A.n, 8, 1e-10
+    B = np.stack([gdmr.rand_normal(31, j, n) for j in range(s)])
+    rc, rel, st, X = _block(torch_gpu, A, B, mb=4, k=8, prec=gdmr.PREC_BF16, inner_tol=3e-2)
+    rc32, rel32, st32, X32 = _block(torch_gpu, A, B, mb=4, k=8)
+    print(f"bf16: rc {rc} cycles {st.cycles}; fp32: rc {rc32} cycles {st32.cycles}", flush=True)
+    assert rc == 0 and rc32 == 0
+    ora = oracle.Oracle(A, 30, 8, gdmr.reduction_blocks(n))
+    try:
+        for j in range(s):
+            tr = _true_res(A, B[j], X[j])
+            assert tr <= tol * 1.001, (j, tr)
+            xo, _, _, _ = ora.solve(B[j], tol=1e-12, mode=j)
+            err = np.linalg.norm(X[j] - xo) / np.linalg.norm(xo)
+            err32 = np.linalg.norm(X[j] - X32[j]) / np.linalg.norm(X32[j])
+            print(f"  column {j}: true residual {tr:.2e}, vs oracle {err:.2e}, vs fp32 block "
+                  f"{err32:.2e}", flush=True)
+            assert err <= XTOL and err32 <= XTOL, (j, err, err32)
+    finally:
+        ora.close()
+
+
+def test_bf16_basis_dependent_columns_graphs_and_reproducibility(torch_gpu):
+    """bf16 basis: dependent and zero right-hand sides (rank-deficient blocks), graph replays
+    against eager launches, and bitwise reproducibility."""
+    torch = torch_gpu
+    A = gdmr.conv_diff_2d(48, jitter=0.01, seed=3, stream=2)
+    n, tol = A.n, 1e-10
+    b = gdmr.rand_normal(35, 0, n)
+    c = gdmr.rand_normal(35, 1, n)
+    B = np.stack([b, c, b, np.zeros(n), 2.0 * c])
+    rc, rel, st, X = _block(torch, A, B, mb=6, k=8, prec=gdmr.PREC_BF16, inner_tol=3e-2)
+    print(f"rc {rc} cycles {st.cycles} breakdowns {st.breakdowns}", flush=True)
+    assert rc == 0 and st.breakdowns > 0
+    for j in (0, 1, 2, 4):
+        assert _true_res(A, B[j], X[j]) <= tol * 1.001, j
+    assert np.all(X[3] == 0.0) and rel[3] == 0.0
+    A3 = gdmr.conv_diff_3d(24, jitter=0.01, seed=9, stream=3)
+    n3, s = A3.n, 8
+    rp, ci, va = _dev(torch, A3.rowptr), _dev(torch, A3.col), _dev(torch, A3.val)
+    hs = []
+    for graphs in (1, 0, 1):
+        bs = gdmr.BlockSolver(n3, 6, 16, s)
+        bs.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A3.nnz)
+        bs.set_precision(gdmr.PREC_BF16)
+        bs.set_refinement(3e-2, 60)
+        bs.set_options(graphs=graphs)
+        hs.append(bs)
+    for t in range(2):
+        B3 = np.stack([gdmr.rand_normal(37, 10 * t + j, n3) for j in range(s)])
+        outs = [_block(torch, A3, B3, 6, 16, solver=bs, mode=t) for bs in hs]
+        for rc_, rel_, st_, X_ in outs:
+            assert rc_ == 0
+            for j in range(s):
+                assert _true_res(A3, B3[j], X_[j]) <= 1e-10 * 1.001
+        assert np.array_equal(outs[0][3], outs[1][3]), "graph replay differs from eager launches"
+        assert np.array_equal(outs[0][3], outs[2][3]), "not reproducible"
+    for bs in hs:
+        bs.close()
+
+
+def test_precision_rejects_fp64(torch_gpu):
+    bs = gdmr.BlockSolver(1000, 4, 8, 8)
+    with pytest.raises(gdmr.SolverError):
+        bs.set_precision(gdmr.PREC_FP64)
+    bs.set_precision(gdmr.PREC_BF16)
+    bs.set_precision(gdmr.PREC_FP32)
     bs.close()

@@ -235,20 +235,22 @@ def summarize(leg, tol):
 
 
 MFMA_FP32_PEAK = 157.3e12
+MFMA_BF16_PEAK = 2.5e15
 
 
-def roofline(prof, block=False):
+def roofline(prof, block=False, bf16=False):
     """The dominant streaming kernel of the headline leg; achieved = algorithmic bytes / kernel
     time (HIP events around each launch on the solve stream).  Single-vector legs: the multi-dot
-    (GEMV-T) or the Gram-Schmidt update, whichever took more time.  Block leg: k_gemm_nn (the
-    block updates, normalisations and the residual and solution updates, every launch of which
-    is timed), with its matrix-core rate beside the HBM one."""
+    (GEMV-T) or the Gram-Schmidt update, whichever took more time.  Block leg: the basis GEMM
+    Out = a In + b P C over the stored basis P (k_gemm_nn_b with a bf16 basis, k_gemm_nn with an
+    FP32 one): the block updates U -= P C and the residual and solution updates, every launch of
+    which is timed on its own; its matrix-core rate is reported beside the HBM one."""
     if block:
         ms = prof["update"][0] + prof["combo"][0]
         by = prof["update"][1] + prof["combo"][1]
         cnt = prof["update"][2] + prof["combo"][2]
         fl = prof["update"][3] + prof["combo"][3]
-        kernel = "k_gemm_nn"
+        kernel = "k_gemm_nn_b" if bf16 else "k_gemm_nn"
     else:
         best = max(("gemvt", "update"), key=lambda p: prof[p][0])
         ms, by, cnt = prof[best][:3]
@@ -261,9 +263,15 @@ def roofline(prof, block=False):
            "avg_launch_us": ms * 1e3 / max(cnt, 1),
            "bytes_per_launch": by / max(cnt, 1)}
     if fl:
+        # The useful flops of the contraction.  The bf16 kernel issues each product twice (hi
+        # and lo parts of the FP32 operand) on the bf16 matrix cores; the FP32 one on the FP32
+        # cores.
         tf = fl / (ms * 1e-3)
-        out["mfma_fp32"] = {"achieved_tflops": tf / 1e12, "peak_tflops": MFMA_FP32_PEAK / 1e12,
-                            "frac": tf / MFMA_FP32_PEAK}
+        peak = MFMA_BF16_PEAK if bf16 else MFMA_FP32_PEAK
+        issued = 2.0 * tf if bf16 else tf
+        out["mfma"] = {"unit": "bf16" if bf16 else "fp32", "useful_tflops": tf / 1e12,
+                       "issued_tflops": issued / 1e12, "peak_tflops": peak / 1e12,
+                       "frac": issued / peak}
     return out
 
 
@@ -405,7 +413,8 @@ def main():
             line[f"speedup_{nm}_vs_fp64"] = (out_legs[nm]["solves_per_s"] /
                                              out_legs["fp64"]["solves_per_s"])
     if legs[head]["prof"] is not None:
-        line["roofline"] = roofline(legs[head]["prof"], legs[head].get("block", False))
+        line["roofline"] = roofline(legs[head]["prof"], legs[head].get("block", False),
+                                    head == "seqblock" and args.seq_prec == "bf16")
         line["phase_ms"] = {nm: {p: round(v[0], 3) for p, v in lg["prof"].items()}
                             for nm, lg in legs.items() if lg["prof"]}
         line["phase_GBps"] = {nm: {p: round(v[1] / (v[0] * 1e-3) / 1e9, 1)

@@ -447,14 +447,14 @@ void cholqr2(dmrBlockContext *c, float *U, int sc, double *Hdst, int ldh, TO *ds
   // pass, orthogonal to about kappa^2 * 6e-8, is enough there; FP32 storage keeps two.
   const int passes = c->cholPasses ? c->cholPasses : (c->bfBasis ? 1 : 2);
   for (int pass = 0; pass < passes; ++pass) {
-    timedf(c, st, DMR_PH_GEMVT, 4.0 * n * sc, 2.0 * n * sc * sc, [&] {
+    timedf(c, st, DMR_PH_ORTH, 4.0 * n * sc, 2.0 * n * sc * sc, [&] {
       tn(c, U, sc, U, sc, c->G64, sc, nullptr, true, st);
     });
     timed(c, st, DMR_PH_ORTH, 0.0, [&] {
       dmr::blk::chol_qr(c->G64, sc, kDepTol, c->Sinv32, c->Sacc, pass == 0,
                         pass == passes - 1 ? Hdst : nullptr, ldh, c->flags, st);
     });
-    timedf(c, st, DMR_PH_UPDATE, 4.0 * n * sc + (double)sizeof(TO) * n * sc,
+    timedf(c, st, DMR_PH_ORTH, 4.0 * n * sc + (double)sizeof(TO) * n * sc,
            2.0 * n * sc * sc, [&] {
       if (pass < passes - 1)
         nn(c, U, sc, c->Sinv32, nullptr, 0.0f, 1.0f, U, sc, st);
@@ -556,11 +556,11 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
   });
   timed(c, st, DMR_PH_SCALE, vb * s,
         [&] { dmr::blk::colnorms<float>(c->R, ld, s, n, c->cpart, c->q, 1, st); });
-  timedf(c, st, DMR_PH_COMBO,
-         vb * ((deflate ? k : 0) + 2 * s * (deflate ? 2 : 1)) + pb * mb * s,
-         2.0 * n * ((deflate ? k : 0) + mb * s) * s, [&] {
-    if (deflate)
+  if (deflate)
+    timedf(c, st, DMR_PH_ZGEMM, vb * (k + 2 * s), 2.0 * n * k * s, [&] {
       nn(c, c->Z, k, c->Yr32, c->Xa, 1.0f, 1.0f, c->Xa, s, st);
+    });
+  timedf(c, st, DMR_PH_COMBO, pb * mb * s + vb * 2 * s, 2.0 * n * mb * s * s, [&] {
     nn(c, V0, mb * s, c->Yr32 + (long long)p0 * 32, c->Xa, 1.0f, 1.0f, c->Xa, s, st);
   });
   // Subspace refresh, part 2: Z <- normalize([Z V_0 .. V_{kr-1}] C) once the SVD has joined.

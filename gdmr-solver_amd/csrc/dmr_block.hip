@@ -515,9 +515,11 @@ __device__ inline void split4(float x0, float x1, float x2, float x3, s16x4 &hi,
   lo = pack_s16x4(l0 | (l1 << 16), l2 | (l3 << 16));
 }
 
-// C = P^T U with bf16 P: the structure of k_gemm_tn16 (tile groups x row groups, 32-row rounds,
-// lane (m, kk) holding rows r0 + 8kk .. r0 + 8kk + 7 of its columns), with two bf16 MFMAs per
-// 8 rows: K index 4kk + e is row r0 + 8kk + e in the first and r0 + 8kk + 4 + e in the second.
+// C = P^T U with bf16 P: the tile-group x row-group structure of k_gemm_tn16 with 64-row rounds:
+// lane (m, kk) holds rows r0 + 16kk .. r0 + 16kk + 15 of its columns (two 16-byte loads per bf16
+// column, so a column's four lanes read 128 contiguous bytes), and MFMA group g (g = 0..3) takes
+// rows r0 + 16kk + 4g .. + 3 as K indices 4kk .. 4kk + 3, once with the hi and once with the lo
+// part of U.
 template <int TPW, int NU>
 __global__ __launch_bounds__(kThreads) void k_gemm_tn16b(const bf16 *__restrict__ P, long long ldp,
                                                          int p, int pPad,
@@ -549,41 +551,64 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn16b(const bf16 *__restrict_
 #pragma unroll
   for (int u = 0; u < NU; ++u) uc[u] = u * 16 + m < s ? U + (u * 16 + m) * ldu : nullptr;
   if (rg < nRG) {
-    const long long step = 32LL * nRG;
-    long long r0 = rBeg + 32 * rg;
-    Raw8<bf16> pr[TPW];
-    Raw8<float> ur[NU];
+    const long long step = 64LL * nRG;
+    long long r0 = rBeg + 64 * rg;
+    Raw8<bf16> pr[TPW][2];
+    Raw8<float> ur[NU][2];
 #pragma unroll
-    for (int t = 0; t < TPW; ++t) load_raw8(pc[t], r0 + 8 * kk, rEnd, pr[t]);
+    for (int t = 0; t < TPW; ++t)
 #pragma unroll
-    for (int u = 0; u < NU; ++u) load_raw8(uc[u], r0 + 8 * kk, rEnd, ur[u]);
+      for (int h = 0; h < 2; ++h) load_raw8(pc[t], r0 + 16 * kk + 8 * h, rEnd, pr[t][h]);
+#pragma unroll
+    for (int u = 0; u < NU; ++u)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) load_raw8(uc[u], r0 + 16 * kk + 8 * h, rEnd, ur[u][h]);
     while (r0 < rEnd) {
       const long long rn = r0 + step;
-      Raw8<bf16> qr[TPW];
-      Raw8<float> vr[NU];
+      Raw8<bf16> qr[TPW][2];
+      Raw8<float> vr[NU][2];
 #pragma unroll
-      for (int t = 0; t < TPW; ++t) load_raw8(rn < rEnd ? pc[t] : nullptr, rn + 8 * kk, rEnd, qr[t]);
+      for (int t = 0; t < TPW; ++t)
 #pragma unroll
-      for (int u = 0; u < NU; ++u) load_raw8(rn < rEnd ? uc[u] : nullptr, rn + 8 * kk, rEnd, vr[u]);
+        for (int h = 0; h < 2; ++h)
+          load_raw8(rn < rEnd ? pc[t] : nullptr, rn + 16 * kk + 8 * h, rEnd, qr[t][h]);
+#pragma unroll
+      for (int u = 0; u < NU; ++u)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          load_raw8(rn < rEnd ? uc[u] : nullptr, rn + 16 * kk + 8 * h, rEnd, vr[u][h]);
 #pragma unroll
       for (int u = 0; u < NU; ++u) {
-        s16x4 bh0, bl0, bh1, bl1;
-        split4(ur[u].a.x, ur[u].a.y, ur[u].a.z, ur[u].a.w, bh0, bl0);
-        split4(ur[u].b.x, ur[u].b.y, ur[u].b.z, ur[u].b.w, bh1, bl1);
+        s16x4 bh[4], bl[4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          split4(ur[u][h].a.x, ur[u][h].a.y, ur[u][h].a.z, ur[u][h].a.w, bh[2 * h], bl[2 * h]);
+          split4(ur[u][h].b.x, ur[u][h].b.y, ur[u][h].b.z, ur[u][h].b.w, bh[2 * h + 1],
+                 bl[2 * h + 1]);
+        }
 #pragma unroll
         for (int t = 0; t < TPW; ++t) {
-          const s16x4 a0 = pack_s16x4(pr[t].w.x, pr[t].w.y);
-          const s16x4 a1 = pack_s16x4(pr[t].w.z, pr[t].w.w);
-          acc[t][u] = mfma16b(a0, bh0, acc[t][u]);
-          acc[t][u] = mfma16b(a0, bl0, acc[t][u]);
-          acc[t][u] = mfma16b(a1, bh1, acc[t][u]);
-          acc[t][u] = mfma16b(a1, bl1, acc[t][u]);
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const s16x4 a0 = pack_s16x4(pr[t][h].w.x, pr[t][h].w.y);
+            const s16x4 a1 = pack_s16x4(pr[t][h].w.z, pr[t][h].w.w);
+            acc[t][u] = mfma16b(a0, bh[2 * h], acc[t][u]);
+            acc[t][u] = mfma16b(a0, bl[2 * h], acc[t][u]);
+            acc[t][u] = mfma16b(a1, bh[2 * h + 1], acc[t][u]);
+            acc[t][u] = mfma16b(a1, bl[2 * h + 1], acc[t][u]);
+          }
         }
       }
 #pragma unroll
-      for (int t = 0; t < TPW; ++t) pr[t] = qr[t];
+      for (int t = 0; t < TPW; ++t) {
+        pr[t][0] = qr[t][0];
+        pr[t][1] = qr[t][1];
+      }
 #pragma unroll
-      for (int u = 0; u < NU; ++u) ur[u] = vr[u];
+      for (int u = 0; u < NU; ++u) {
+        ur[u][0] = vr[u][0];
+        ur[u][1] = vr[u][1];
+      }
       r0 = rn;
     }
 #pragma unroll
@@ -619,14 +644,18 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn16b(const bf16 *__restrict_
 // each 16-column chunk (16-byte loads, a ring of two chunks in flight); MFMA i (i = 0..7) covers
 // rows r0 + 8m + i, m = 0..15, so its A operand is element i of the four loaded words.  Output
 // row r0 + 8 (4kg + v) + i, column 16t + m of N-tile t comes out in lane (m, kg), register v of
-// accumulator (i, t): eight consecutive rows of one column per lane.
+// accumulator (i, t); each N-tile is transposed through LDS so that In and Out move as whole
+// columns (512 contiguous bytes per 128 rows).  Every wavefront runs the whole loop (rows past n
+// load zeros and store nothing), so the workgroup barriers are uniform.
+constexpr int kNnbTile = 128 * 16 + 16 * 4;  // floats per wavefront: 16 columns of 128 (+4 pad)
+
 template <int NT, typename TO>
 __global__ __launch_bounds__(kThreads) void k_gemm_nn_b(const bf16 *__restrict__ P, long long ldp,
                                                         int p, const float *__restrict__ C,
                                                         const float *In, long long ldi, float a,
                                                         float b, TO *Out, long long ldo, int s,
                                                         long long n) {
-  extern __shared__ uint16_t Cl[];  // [hi | lo][16 NT output columns][KP]
+  extern __shared__ uint16_t Cl[];  // [hi | lo][16 NT output columns][KP]; then the epilogue
   const int KP = (p + 15) & ~15;
   uint16_t *Ch = Cl, *Clo = Cl + (size_t)16 * NT * KP;
   for (int idx = threadIdx.x; idx < 16 * NT * KP; idx += kThreads) {
@@ -639,7 +668,6 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nn_b(const bf16 *__restrict__
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const long long r0 = ((long long)blockIdx.x * 4 + wave) * 128;
-  if (r0 >= n) return;
   const int m = lane & 15, kg = lane >> 4;
   const long long rl = r0 + 8 * m;
   f32x4 acc[8][NT];
@@ -686,38 +714,48 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nn_b(const bf16 *__restrict__
 #pragma unroll
     for (int j = 0; j < 4; ++j) cur[j] = nxt[j];
   }
+  // Epilogue: through LDS (the coefficient area is free once every wavefront is done with it).
+  float *tile = reinterpret_cast<float *>(Cl) + wave * kNnbTile;
+  const int stride = 128 + 4;
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    const int j = 16 * t + m;
-    if (j >= s) continue;
+    __syncthreads();
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const long long row = r0 + 8 * (4 * kg + v);
-      if (row >= n) continue;
-      const int cnt = row + 7 < n ? 8 : (int)(n - row);
+      float *dst = tile + m * stride + 8 * (4 * kg + v);
+      *reinterpret_cast<float4 *>(dst) =
+          make_float4(acc[0][t][v], acc[1][t][v], acc[2][t][v], acc[3][t][v]);
+      *reinterpret_cast<float4 *>(dst + 4) =
+          make_float4(acc[4][t][v], acc[5][t][v], acc[6][t][v], acc[7][t][v]);
+    }
+    __syncthreads();
+    // Lane (h, q) = (l / 32, l % 32) moves rows r0 + 4q .. r0 + 4q + 3 of column 2cc + h.
+    const int h = lane >> 5, q4 = (lane & 31) * 4;
+    const long long row = r0 + q4;
+    if (row < n) {
+      const int cnt = row + 3 < n ? 4 : (int)(n - row);
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int c4 = cnt - 4 * h < 0 ? 0 : (cnt - 4 * h > 4 ? 4 : cnt - 4 * h);
-        if (c4 == 0) continue;
+      for (int cc = 0; cc < 8; ++cc) {
+        const int jl = 2 * cc + h, j = 16 * t + jl;
+        if (j >= s) continue;
+        const float4 pv = *reinterpret_cast<const float4 *>(tile + jl * stride + q4);
+        const float pr[4] = {pv.x, pv.y, pv.z, pv.w};
         float in[4] = {0.f, 0.f, 0.f, 0.f};
         if (a != 0.0f) {
-          if (c4 == 4) {
-            const float4 iv = *reinterpret_cast<const float4 *>(In + j * ldi + row + 4 * h);
+          if (cnt == 4) {
+            const float4 iv = *reinterpret_cast<const float4 *>(In + j * ldi + row);
             in[0] = iv.x;
             in[1] = iv.y;
             in[2] = iv.z;
             in[3] = iv.w;
           } else {
-            for (int q = 0; q < c4; ++q) in[q] = In[j * ldi + row + 4 * h + q];
+            for (int e = 0; e < cnt; ++e) in[e] = In[j * ldi + row + e];
           }
         }
         float o[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const float pv = acc[4 * h + q][t][v];
-          o[q] = a != 0.0f ? a * in[q] + b * pv : b * pv;
-        }
-        store_rows4(Out + j * ldo + row + 4 * h, o, c4);
+        for (int e = 0; e < 4; ++e) o[e] = a != 0.0f ? a * in[e] + b * pr[e] : b * pr[e];
+        store_rows4(Out + j * ldo + row, o, cnt);
       }
     }
   }
@@ -1479,14 +1517,14 @@ void gemm_nn(const bf16 *P, long long ldp, int p, const float *C, const float *I
     const long long waves = (n + 127) / 128;
     const unsigned grid = (unsigned)((waves + 3) / 4);
     const int KP = (p + 15) & ~15;
+    const size_t epi = sizeof(float) * 4 * kNnbTile;
+    const size_t l1 = sizeof(uint16_t) * 2 * 16 * KP, l2 = 2 * l1;
     if (s <= 16)
-      hipLaunchKernelGGL((k_gemm_nn_b<1, float>), dim3(grid), dim3(kThreads),
-                         sizeof(uint16_t) * 2 * 16 * KP, st, P, ldp, p, C, In, ldi, a, b, Out, ldo,
-                         s, n);
+      hipLaunchKernelGGL((k_gemm_nn_b<1, float>), dim3(grid), dim3(kThreads), l1 > epi ? l1 : epi,
+                         st, P, ldp, p, C, In, ldi, a, b, Out, ldo, s, n);
     else
-      hipLaunchKernelGGL((k_gemm_nn_b<2, float>), dim3(grid), dim3(kThreads),
-                         sizeof(uint16_t) * 2 * 32 * KP, st, P, ldp, p, C, In, ldi, a, b, Out, ldo,
-                         s, n);
+      hipLaunchKernelGGL((k_gemm_nn_b<2, float>), dim3(grid), dim3(kThreads), l2 > epi ? l2 : epi,
+                         st, P, ldp, p, C, In, ldi, a, b, Out, ldo, s, n);
     return;
   }
   gemm_nn_t<bf16, float>(P, ldp, p, C, In, ldi, a, b, Out, ldo, s, n, un, st);

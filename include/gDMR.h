@@ -225,10 +225,11 @@ int dmrBlockGetTuning(dmrBlockHandle h, int *out);
 int dmrBlockImportZ(dmrBlockHandle h, const double *hostZ);
 int dmrBlockExportZ(dmrBlockHandle h, double *hostZ);
 /* Phase profile as dmrSetProfiling / dmrGetProfile: SPMV = SpMM and FP64 residuals, GEMVT =
- * projections and Gram matrices (gemm_tn), UPDATE = block updates and normalisations (gemm_nn),
- * COMBO = residual and solution updates, ZGEMM = subspace refresh, LSQ = least squares and
- * Cholesky factors, SVD = Jacobi SVD, SCALE = column norms and scaling, ORTH = Cholesky factors
- * of the block normalisation.  flops: the matrix-core and SpMM floating-point operations. */
+ * projections onto the basis (gemm_tn), UPDATE = block updates U -= P C (gemm_nn over the basis),
+ * COMBO = residual and solution updates over the basis (gemm_nn), ZGEMM = products with the
+ * subspace Z (solution update and refresh), LSQ = least squares, SVD = Jacobi SVD, SCALE =
+ * column norms and scaling, ORTH = block normalisation (Cholesky QR: Gram matrices, factors,
+ * scaling).  flops: the matrix-core and SpMM floating-point operations. */
 int dmrBlockSetProfiling(dmrBlockHandle h, int on);
 int dmrBlockGetProfile(dmrBlockHandle h, double *ms, double *bytes, double *flops,
                        long long *launches);

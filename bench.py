@@ -262,6 +262,14 @@ def roofline(prof, block=False, bf16=False):
            "frac": ach / HBM_PEAK, "traffic": None, "launches": cnt,
            "avg_launch_us": ms * 1e3 / max(cnt, 1),
            "bytes_per_launch": by / max(cnt, 1)}
+    # HBM traffic per launch of the same kernel from the calibrated FETCH_SIZE / WRITE_SIZE
+    # passes committed under profiles/ (rocprofv3 --pmc cannot run inside this process).
+    tf_path = os.path.join(ROOT, "profiles", "roofline_traffic.json")
+    if os.path.exists(tf_path):
+        rec = json.load(open(tf_path)).get(kernel)
+        if rec:
+            out["traffic"] = rec["traffic_bytes_per_launch"]
+            out["traffic_source"] = rec["source"]
     if fl:
         # The useful flops of the contraction.  The bf16 kernel issues each product twice (hi
         # and lo parts of the FP32 operand) on the bf16 matrix cores; the FP32 one on the FP32

@@ -40,7 +40,10 @@ def parse(argv=None):
     ap.add_argument("--k", type=int, default=32)
     ap.add_argument("--m", type=int, default=40, help="basis of the FP64 single-vector leg")
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--inner-tol", type=float, default=1e-5)
+    ap.add_argument("--inner-tol", type=float, default=None,
+                    help="inner block-solve tolerance (default 1e-5 for fp32, 3e-2 for bf16)")
+    ap.add_argument("--prec", default="fp32", choices=("fp32", "bf16"),
+                    help="block basis storage")
     ap.add_argument("--fp64-sample", type=int, default=4)
     ap.add_argument("--seed", type=int, default=96)
     ap.add_argument("--no-profile", action="store_true")
@@ -63,15 +66,20 @@ def run(args) -> dict:
 
     # Warm-up on a throwaway handle (code objects, first touch).
     w = gdmr.BlockSolver(n, args.block_steps, args.k, s)
+    w.set_precision(gdmr.PREC_BF16 if args.prec == "bf16" else gdmr.PREC_FP32)
     w.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A.nnz)
     xw = torch.zeros(s, n, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
     w.solve(xw.data_ptr(), n, Bd[0].data_ptr(), n, s, 1e-4, max_cycles=4)
     w.close()
 
+    inner = args.inner_tol if args.inner_tol is not None else (3e-2 if args.prec == "bf16"
+                                                               else 1e-5)
+    prec = gdmr.PREC_BF16 if args.prec == "bf16" else gdmr.PREC_FP32
     bs = gdmr.BlockSolver(n, args.block_steps, args.k, s)
+    bs.set_precision(prec)
     bs.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A.nnz)
-    bs.set_refinement(args.inner_tol, 30)
+    bs.set_refinement(inner, 60)
     legs = []
     prof = None
     for b in range(2):
@@ -129,10 +137,11 @@ def run(args) -> dict:
         "value": legs[1]["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1,
-        "dtype": "fp32+fp64",
+        "dtype": "bf16+fp32+fp64" if args.prec == "bf16" else "fp32+fp64",
         "data": "synthetic",
         "config": {"workload": f"3-D anisotropic diffusion 7-pt {N}^3 (1, 0.1, 0.01), {s} RHS, "
-                               f"block steps {args.block_steps}, k={args.k}, tol {tol:g}",
+                               f"block steps {args.block_steps}, k={args.k}, tol {tol:g}, "
+                               f"{args.prec} basis, inner tol {inner:g}",
                    "n": n, "nnz": A.nnz},
         "block_first": {k: v for k, v in legs[0].items() if k != "X"},
         "block_recycled": {k: v for k, v in legs[1].items() if k != "X"},

@@ -20,9 +20,9 @@ system reaches 1e-10 checked in FP64.  Every solve's true residual ||b - A x|| /
 own operator) is recomputed on the host in FP64 after the timed region.
 
 The line also carries the batched form of the metric ("batched"): 3-D anisotropic diffusion 96^3
-with 32 right-hand sides solved as one block (block DMR on the matrix cores, FP32 inside FP64
-refinement) against the plain FP64 DMR solving them one at a time (gdmr.batch_bench), and the
-FP64 host oracle's rate on the same sequence ("cpu_baseline").
+with 32 right-hand sides of one operator solved as one block (block DMR on the matrix cores, bf16
+basis storage inside FP64 refinement) against the plain FP64 DMR solving them one at a time
+(gdmr.batch_bench), and the FP64 host oracle's rate on the same sequence ("cpu_baseline").
 """
 import argparse
 import json
@@ -433,7 +433,7 @@ def main():
         # (block Gram-Schmidt on the matrix cores, FP32 inside FP64 refinement) against the plain
         # FP64 DMR solving them one at a time.
         from gdmr import batch_bench
-        bargs = batch_bench.parse(["--no-profile"])
+        bargs = batch_bench.parse(["--no-profile", "--prec", "bf16"])
         br = batch_bench.run(bargs)
         line["batched"] = {
             "workload": br["config"]["workload"],

@@ -6,9 +6,10 @@ one deflation subspace of k = 32; tol 1e-10 checked in FP64 per column.
 
 Legs:
   * block  -- dmrBlockSolve: one block Krylov space for the 32 columns (block Gram-Schmidt as
-              tall-skinny GEMMs on the matrix cores, FP32 storage and FP32 accumulation) inside
-              FP64 refinement; blockSteps = 10, so k + 11 * 32 = 384 basis columns.  Two blocks
-              of 32 run back to back: the first from scratch, the second recycling the subspace.
+              tall-skinny GEMMs on the matrix cores, FP32 or bf16 basis storage (--prec) and FP32
+              accumulation) inside FP64 refinement; blockSteps = 10, so k + 11 * 32 = 384 basis
+              columns.  Two blocks of 32 run back to back: the first from scratch, the second
+              recycling the subspace.
   * fp64   -- the plain FP64 DMR (m = 40, k = 32), one right-hand side at a time with the
               subspace recycled (mode = i), on the first `--fp64-sample` columns.
 Reported: solves/s of each leg, the speed-up, per-column true residuals, the block solution's

@@ -481,6 +481,35 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
   const double pb = (double)sizeof(TP) * (double)n;  // one basis column
   dmr::fill<double>(c->H, (long long)rows * cols, 0.0, st);
   dmr::blk::zero_ints(c->flags, 2, st);
+  // The subspace refresh uses the k smallest right singular vectors of the leading block of H
+  // over [Z V_0 .. V_{kr-1}] (the widest whose SVD fits in LDS, at least k columns).  That block
+  // is final once block step kr - 1 is done, so its SVD runs on the side stream from there on,
+  // overlapped with the remaining block steps, the least squares and the updates.
+  bool refreshed = false;
+  int kr = 0, cR = 0, rR = 0;
+  if (k > 0) {
+    kr = deflate ? 1 : std::max(1, (k + s - 1) / s);
+    while (kr < std::min(mb, c->krMax) &&
+           dmr::svd_lds(p0 + (kr + 2) * s, p0 + (kr + 1) * s) <= 160 * 1024)
+      ++kr;
+    if (kr <= mb) {
+      cR = p0 + kr * s;
+      rR = cR + s;
+      refreshed = cR >= k && dmr::svd_lds(rR, cR) <= 160 * 1024;
+    }
+  }
+  auto fork_svd = [&] {
+    hipEventRecord(c->evFork, st);
+    hipStreamWaitEvent(c->side, c->evFork, 0);
+    hipMemcpy2DAsync(c->Hred, sizeof(double) * rR, c->H, sizeof(double) * rows,
+                     sizeof(double) * rR, cR, hipMemcpyDeviceToDevice, c->side);
+    timed(c, c->side, DMR_PH_SVD, 0.0, [&] {
+      dmr::svd_smallest_rect(c->Hred, rR, rR, cR, k, c->svdTol, kSvdMaxSweeps, 1, c->Csvd,
+                             c->sig, c->flags + 1, c->side);
+      dmr::blk::pack_coef(c->Csvd, cR, k, c->Cz32, c->side);
+    });
+    hipEventRecord(c->evJoin, c->side);
+  };
   if (deflate) {
     timedf(c, st, DMR_PH_SPMV, spmm_bytes(c, k), 2.0 * c->csr.nnz * k,
           [&] { spmm(c, c->Z, U, k, st); });
@@ -514,34 +543,7 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
       nn(c, P, pj, c->C32, U, 1.0f, -1.0f, U, s, st);
     });
     cholqr2(c, U, s, Hcol + pj, rows, slot, st);
-  }
-  // Subspace refresh, part 1 (side stream, overlapped with the least squares and the updates):
-  // the k smallest right singular vectors of the leading block of H over [Z V_0 .. V_{kr-1}].
-  bool refreshed = false;
-  int cR = 0;
-  if (k > 0) {
-    // Widest leading block whose SVD fits in LDS (at least k columns).
-    int kr = deflate ? 1 : std::max(1, (k + s - 1) / s);
-    while (kr < std::min(mb, c->krMax) &&
-           dmr::svd_lds(p0 + (kr + 2) * s, p0 + (kr + 1) * s) <= 160 * 1024)
-      ++kr;
-    if (kr <= mb) {
-      cR = p0 + kr * s;
-      const int rR = cR + s;
-      if (cR >= k && dmr::svd_lds(rR, cR) <= 160 * 1024) {
-        hipEventRecord(c->evFork, st);
-        hipStreamWaitEvent(c->side, c->evFork, 0);
-        hipMemcpy2DAsync(c->Hred, sizeof(double) * rR, c->H, sizeof(double) * rows,
-                         sizeof(double) * rR, cR, hipMemcpyDeviceToDevice, c->side);
-        timed(c, c->side, DMR_PH_SVD, 0.0, [&] {
-          dmr::svd_smallest_rect(c->Hred, rR, rR, cR, k, c->svdTol, kSvdMaxSweeps, 1, c->Csvd,
-                                 c->sig, c->flags + 1, c->side);
-          dmr::blk::pack_coef(c->Csvd, cR, k, c->Cz32, c->side);
-        });
-        hipEventRecord(c->evJoin, c->side);
-        refreshed = true;
-      }
-    }
+    if (refreshed && j == kr - 1) fork_svd();
   }
   // Full projection of the residual block, least squares, residual and solution updates.
   timedf(c, st, DMR_PH_GEMVT, pb * rows + vb * s, 2.0 * n * rows * s, [&] {

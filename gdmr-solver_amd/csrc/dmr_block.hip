@@ -1087,6 +1087,10 @@ __global__ __launch_bounds__(64) void k_chol_qr(const double *__restrict__ Gin, 
 // 32-row blocks: one wavefront solves the diagonal block from LDS, then all threads update the
 // rows above with the block's columns of R.  Then Yr32 = Y diag(rho) (FP32 row-major cols x 32)
 // and ZH32 = H Y (FP32 row-major rows x 32).  One workgroup of 1024 threads.
+// BWT: the bandwidth as a compile-time constant (16) for the unrolled reflector application, or
+// 0 for a runtime bandwidth.  Both compute the same bits: rows of the window past the matrix are
+// zero, so a reflector padded to the full bandwidth adds exact zeros.
+template <int BWT>
 __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H, int rows,
                                                     int cols, int bw,
                                                     const double *__restrict__ T, int s,
@@ -1139,18 +1143,39 @@ __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H
     if (tau != 0.0) {
       const int c0 = c % W;
       for (int col = c + 1 + tid; col < nc; col += bd) {
-        double w = rowc[col];
-        for (int l = 1; l <= L; ++l) {
-          const int sl = c0 + l < W ? c0 + l : c0 + l - W;
-          const double *rl = win + sl * nc;
-          w = fma(rl[c], rl[col], w);
-        }
-        const double tw = tau * w;
-        rowc[col] = rowc[col] - tw;
-        for (int l = 1; l <= L; ++l) {
-          const int sl = c0 + l < W ? c0 + l : c0 + l - W;
-          double *rl = win + sl * nc;
-          rl[col] = fma(-tw, rl[c], rl[col]);
+        if constexpr (BWT > 0) {
+          // All loads of the column's window entries issued before the sums.
+          double vr[BWT], xr[BWT];
+#pragma unroll
+          for (int l = 1; l <= BWT; ++l) {
+            const int sl = c0 + l < W ? c0 + l : c0 + l - W;
+            vr[l - 1] = win[sl * nc + c];
+            xr[l - 1] = win[sl * nc + col];
+          }
+          double w = rowc[col];
+#pragma unroll
+          for (int l = 0; l < BWT; ++l) w = fma(vr[l], xr[l], w);
+          const double tw = tau * w;
+          rowc[col] = rowc[col] - tw;
+#pragma unroll
+          for (int l = 1; l <= BWT; ++l) {
+            const int sl = c0 + l < W ? c0 + l : c0 + l - W;
+            win[sl * nc + col] = fma(-tw, vr[l - 1], xr[l - 1]);
+          }
+        } else {
+          double w = rowc[col];
+          for (int l = 1; l <= L; ++l) {
+            const int sl = c0 + l < W ? c0 + l : c0 + l - W;
+            const double *rl = win + sl * nc;
+            w = fma(rl[c], rl[col], w);
+          }
+          const double tw = tau * w;
+          rowc[col] = rowc[col] - tw;
+          for (int l = 1; l <= L; ++l) {
+            const int sl = c0 + l < W ? c0 + l : c0 + l - W;
+            double *rl = win + sl * nc;
+            rl[col] = fma(-tw, rl[c], rl[col]);
+          }
         }
       }
     }
@@ -1185,15 +1210,23 @@ __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H
     }
     __syncthreads();
     if (tid < 32) {
-      // Lane jr solves column jr of the diagonal block, bottom up.
+      // Lane jr solves column jr of the diagonal block, bottom up, its unknowns in registers
+      // (the rows r >= b of a partial block do not exist).
       const int jr = tid;
-      for (int r = b - 1; r >= 0; --r) {
-        double a = Tl[(i0 + r) * 32 + jr];
-        for (int l = r + 1; l < b; ++l) a = fma(-Dg[r * 33 + l], Yb[l * 32 + jr], a);
-        const double d = Dg[r * 33 + r];
-        const double yv = (jr < s && fabs(d) > piv) ? a / d : 0.0;
-        Yb[r * 32 + jr] = yv;
-        Y[(long long)(i0 + r) * 32 + jr] = yv;
+      double yv[32];
+#pragma unroll
+      for (int r = 31; r >= 0; --r) {
+        yv[r] = 0.0;
+        if (r < b) {
+          double a = Tl[(i0 + r) * 32 + jr];
+#pragma unroll
+          for (int l = r + 1; l < 32; ++l)
+            if (l < b) a = fma(-Dg[r * 33 + l], yv[l], a);
+          const double d = Dg[r * 33 + r];
+          yv[r] = (jr < s && fabs(d) > piv) ? a / d : 0.0;
+          Yb[r * 32 + jr] = yv[r];
+          Y[(long long)(i0 + r) * 32 + jr] = yv[r];
+        }
       }
     }
     __syncthreads();
@@ -1203,10 +1236,18 @@ __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H
       double a[32];
 #pragma unroll
       for (int jr = 0; jr < 32; ++jr) a[jr] = Tl[r * 32 + jr];
-      for (int l = 0; l < b; ++l) {
-        const double rv = R[(long long)(i0 + l) * cols + r];
+      for (int l0 = 0; l0 < b; l0 += 8) {
+        double rv[8];  // eight loads of R in flight
 #pragma unroll
-        for (int jr = 0; jr < 32; ++jr) a[jr] = fma(-rv, Yb[l * 32 + jr], a[jr]);
+        for (int q = 0; q < 8; ++q)
+          rv[q] = l0 + q < b ? R[(long long)(i0 + l0 + q) * cols + r] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          if (l0 + q < b) {
+#pragma unroll
+            for (int jr = 0; jr < 32; ++jr) a[jr] = fma(-rv[q], Yb[(l0 + q) * 32 + jr], a[jr]);
+          }
+        }
       }
 #pragma unroll
       for (int jr = 0; jr < 32; ++jr) Tl[r * 32 + jr] = a[jr];
@@ -1225,10 +1266,17 @@ __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H
     double z[32];
 #pragma unroll
     for (int j = 0; j < 32; ++j) z[j] = 0.0;
-    for (int c = 0; c < cols; ++c) {
-      const double hv = H[(long long)c * rows + r];
+    for (int c0 = 0; c0 < cols; c0 += 8) {
+      double hv[8];  // eight loads of H in flight
 #pragma unroll
-      for (int j = 0; j < 32; ++j) z[j] = fma(hv, Yl[c * 32 + j], z[j]);
+      for (int q = 0; q < 8; ++q) hv[q] = c0 + q < cols ? H[(long long)(c0 + q) * rows + r] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if (c0 + q < cols) {
+#pragma unroll
+          for (int j = 0; j < 32; ++j) z[j] = fma(hv[q], Yl[(c0 + q) * 32 + j], z[j]);
+        }
+      }
     }
 #pragma unroll
     for (int j = 0; j < 32; ++j) ZH32[r * 32 + j] = j < s ? (float)z[j] : 0.0f;
@@ -1297,7 +1345,8 @@ inline unsigned grid1(long long n) { return (unsigned)((n + kThreads - 1) / kThr
 void prepare() {
   // Once per process, before any stream capture: the LDS limits of the one-workgroup kernels.
   static const bool done = [] {
-    dmr::allow_max_lds(reinterpret_cast<const void *>(k_lsq_block));
+    dmr::allow_max_lds(reinterpret_cast<const void *>(k_lsq_block<0>));
+    dmr::allow_max_lds(reinterpret_cast<const void *>(k_lsq_block<16>));
     dmr::prepare_dense();
     return true;
   }();
@@ -1551,8 +1600,12 @@ void lsq_block(const double *H, int rows, int cols, int bw, const double *T, int
                hipStream_t st) {
   const size_t lds = lsq_block_lds(cols, bw, s);
   prepare();
-  hipLaunchKernelGGL(k_lsq_block, dim3(1), dim3(1024), lds, st, H, rows, cols, bw, T, s, rho, R,
-                     Qt, Y, Yr32, ZH32);
+  if (bw == 16)
+    hipLaunchKernelGGL(k_lsq_block<16>, dim3(1), dim3(1024), lds, st, H, rows, cols, bw, T, s, rho,
+                       R, Qt, Y, Yr32, ZH32);
+  else
+    hipLaunchKernelGGL(k_lsq_block<0>, dim3(1), dim3(1024), lds, st, H, rows, cols, bw, T, s, rho,
+                       R, Qt, Y, Yr32, ZH32);
 }
 
 void pack_coef(const double *C, int cols, int k, float *C32, hipStream_t st) {

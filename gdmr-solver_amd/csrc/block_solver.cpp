@@ -387,7 +387,7 @@ void autotune_t(dmrBlockContext *c, TP *P, int s, hipStream_t st) {
       }
     }
   float bn = 1e30f;
-  for (int un : {4, 8, 16, 100}) {
+  for (int un : {4, 8, 16, 100, 101}) {
     if (!bf && un >= 100) continue;
     const float tt = best_of([&] {
       dmr::blk::gemm_nn(P, c->ld, pFull, c->C32, c->Xa, c->ld, 1.0f, 0.0f, c->Xa, c->ld, s,

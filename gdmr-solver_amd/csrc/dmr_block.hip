@@ -158,21 +158,23 @@ __global__ __launch_bounds__(kThreads) void k_resid_block(int n, const int *__re
                                                           const double *__restrict__ B,
                                                           long long ldb, double *__restrict__ R,
                                                           long long ldr, int s) {
+  // blockIdx.y selects a group of four columns, so one pass touches the vectors of four columns.
   const int row = blockIdx.x * kThreads + threadIdx.x;
   if (row >= n) return;
   const int b = rowPtr[row], e = rowPtr[row + 1];
-  for (int c0 = 0; c0 < s; c0 += 8) {
-    double acc[8];
+  {
+    const int c0 = blockIdx.y * 4;
+    double acc[4];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) acc[c] = 0.0;
+    for (int c = 0; c < 4; ++c) acc[c] = 0.0;
     for (int q = b; q < e; ++q) {
       const long long col = colInd[q];
 #pragma unroll
-      for (int c = 0; c < 8; ++c)
+      for (int c = 0; c < 4; ++c)
         if (c0 + c < s) acc[c] = fma(val[q + (c0 + c) * ldv], X[col + (c0 + c) * ldx], acc[c]);
     }
 #pragma unroll
-    for (int c = 0; c < 8; ++c)
+    for (int c = 0; c < 4; ++c)
       if (c0 + c < s) R[row + (c0 + c) * ldr] = B[row + (c0 + c) * ldb] - acc[c];
   }
 }
@@ -649,7 +651,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn16b(const bf16 *__restrict_
 // load zeros and store nothing), so the workgroup barriers are uniform.
 constexpr int kNnbTile = 128 * 16 + 16 * 4;  // floats per wavefront: 16 columns of 128 (+4 pad)
 
-template <int NT, typename TO>
+template <int NT, typename TO, int RB>
 __global__ __launch_bounds__(kThreads) void k_gemm_nn_b(const bf16 *__restrict__ P, long long ldp,
                                                         int p, const float *__restrict__ C,
                                                         const float *In, long long ldi, float a,
@@ -658,31 +660,47 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nn_b(const bf16 *__restrict__
   extern __shared__ uint16_t Cl[];  // [hi | lo][16 NT output columns][KP]; then the epilogue
   const int KP = (p + 15) & ~15;
   uint16_t *Ch = Cl, *Clo = Cl + (size_t)16 * NT * KP;
-  for (int idx = threadIdx.x; idx < 16 * NT * KP; idx += kThreads) {
-    const int col = idx / KP, k = idx % KP;
-    const float x = (k < p && col < s) ? C[k * 32 + col] : 0.0f;
-    const uint32_t h = f_to_bf(x);
-    Ch[idx] = (uint16_t)h;
-    Clo[idx] = (uint16_t)f_to_bf(x - bf_to_f(h));
+  // Eight coefficient loads in flight per thread, then their hi/lo parts.
+  for (int i0 = threadIdx.x; i0 < 16 * NT * KP; i0 += 8 * kThreads) {
+    float x[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int idx = i0 + q * kThreads;
+      const int col = idx / KP, k = idx % KP;
+      x[q] = (idx < 16 * NT * KP && k < p && col < s) ? C[k * 32 + col] : 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int idx = i0 + q * kThreads;
+      if (idx < 16 * NT * KP) {
+        const uint32_t h = f_to_bf(x[q]);
+        Ch[idx] = (uint16_t)h;
+        Clo[idx] = (uint16_t)f_to_bf(x[q] - bf_to_f(h));
+      }
+    }
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long long r0 = ((long long)blockIdx.x * 4 + wave) * 128;
+  // RB consecutive 128-row blocks per wavefront: each column is read in RB * 256 contiguous bytes.
+  const long long rw = ((long long)blockIdx.x * 4 + wave) * 128 * RB;
   const int m = lane & 15, kg = lane >> 4;
-  const long long rl = r0 + 8 * m;
-  f32x4 acc[8][NT];
+  f32x4 acc[RB][8][NT];
 #pragma unroll
-  for (int i = 0; i < 8; ++i)
+  for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[i][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  auto load_chunk = [&](int c0, Raw8<bf16> (&w)[4]) {
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[rb][i][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  auto load_chunk = [&](int c0, Raw8<bf16> (&w)[RB][4]) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int cc = c0 + 4 * kg + j;
-      load_raw8(cc < p ? P + cc * ldp : nullptr, rl, n, w[j]);
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+        load_raw8(cc < p ? P + cc * ldp : nullptr, rw + 128 * rb + 8 * m, n, w[rb][j]);
     }
   };
-  Raw8<bf16> cur[4], nxt[4];
+  Raw8<bf16> cur[RB][4], nxt[RB][4];
   load_chunk(0, cur);
   for (int c0 = 0; c0 < p; c0 += 16) {
     load_chunk(c0 + 16, nxt);  // past p: zero words, no loads
@@ -695,38 +713,46 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nn_b(const bf16 *__restrict__
       bh[t] = pack_s16x4(h.x, h.y);
       bl[t] = pack_s16x4(l.x, l.y);
     }
-    const uint32_t w[4][4] = {{cur[0].w.x, cur[0].w.y, cur[0].w.z, cur[0].w.w},
-                              {cur[1].w.x, cur[1].w.y, cur[1].w.z, cur[1].w.w},
-                              {cur[2].w.x, cur[2].w.y, cur[2].w.z, cur[2].w.w},
-                              {cur[3].w.x, cur[3].w.y, cur[3].w.z, cur[3].w.w}};
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int q = i >> 1, sh = (i & 1) * 16;
-      const uint32_t e0 = (w[0][q] >> sh) & 0xffffu, e1 = (w[1][q] >> sh) & 0xffffu;
-      const uint32_t e2 = (w[2][q] >> sh) & 0xffffu, e3 = (w[3][q] >> sh) & 0xffffu;
-      const s16x4 av = pack_s16x4(e0 | (e1 << 16), e2 | (e3 << 16));
+    for (int rb = 0; rb < RB; ++rb) {
+      const uint32_t w[4][4] = {{cur[rb][0].w.x, cur[rb][0].w.y, cur[rb][0].w.z, cur[rb][0].w.w},
+                                {cur[rb][1].w.x, cur[rb][1].w.y, cur[rb][1].w.z, cur[rb][1].w.w},
+                                {cur[rb][2].w.x, cur[rb][2].w.y, cur[rb][2].w.z, cur[rb][2].w.w},
+                                {cur[rb][3].w.x, cur[rb][3].w.y, cur[rb][3].w.z, cur[rb][3].w.w}};
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        acc[i][t] = mfma16b(av, bh[t], acc[i][t]);
-        acc[i][t] = mfma16b(av, bl[t], acc[i][t]);
+      for (int i = 0; i < 8; ++i) {
+        const int q = i >> 1, sh = (i & 1) * 16;
+        const uint32_t e0 = (w[0][q] >> sh) & 0xffffu, e1 = (w[1][q] >> sh) & 0xffffu;
+        const uint32_t e2 = (w[2][q] >> sh) & 0xffffu, e3 = (w[3][q] >> sh) & 0xffffu;
+        const s16x4 av = pack_s16x4(e0 | (e1 << 16), e2 | (e3 << 16));
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          acc[rb][i][t] = mfma16b(av, bh[t], acc[rb][i][t]);
+          acc[rb][i][t] = mfma16b(av, bl[t], acc[rb][i][t]);
+        }
       }
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) cur[j] = nxt[j];
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) cur[rb][j] = nxt[rb][j];
   }
   // Epilogue: through LDS (the coefficient area is free once every wavefront is done with it).
   float *tile = reinterpret_cast<float *>(Cl) + wave * kNnbTile;
   const int stride = 128 + 4;
 #pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
   for (int t = 0; t < NT; ++t) {
+    const long long r0 = rw + 128 * rb;
     __syncthreads();
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       float *dst = tile + m * stride + 8 * (4 * kg + v);
-      *reinterpret_cast<float4 *>(dst) =
-          make_float4(acc[0][t][v], acc[1][t][v], acc[2][t][v], acc[3][t][v]);
-      *reinterpret_cast<float4 *>(dst + 4) =
-          make_float4(acc[4][t][v], acc[5][t][v], acc[6][t][v], acc[7][t][v]);
+      *reinterpret_cast<float4 *>(dst) = make_float4(acc[rb][0][t][v], acc[rb][1][t][v],
+                                                     acc[rb][2][t][v], acc[rb][3][t][v]);
+      *reinterpret_cast<float4 *>(dst + 4) = make_float4(acc[rb][4][t][v], acc[rb][5][t][v],
+                                                         acc[rb][6][t][v], acc[rb][7][t][v]);
     }
     __syncthreads();
     // Lane (h, q) = (l / 32, l % 32) moves rows r0 + 4q .. r0 + 4q + 3 of column 2cc + h.
@@ -1403,8 +1429,9 @@ void spmm32(const CsrDev &A, const float *X, long long ldx, float *Y, long long 
 void resid_block(const CsrDev &A, const double *val, long long ldv, const double *X,
                  long long ldx, const double *B, long long ldb, double *R, long long ldr, int s,
                  hipStream_t st) {
-  hipLaunchKernelGGL(k_resid_block, dim3(grid1(A.n)), dim3(kThreads), 0, st, A.n, A.rowPtr,
-                     A.colInd, val ? val : A.val, val ? ldv : 0LL, X, ldx, B, ldb, R, ldr, s);
+  hipLaunchKernelGGL(k_resid_block, dim3(grid1(A.n), (s + 3) / 4), dim3(kThreads), 0, st, A.n,
+                     A.rowPtr, A.colInd, val ? val : A.val, val ? ldv : 0LL, X, ldx, B, ldb, R,
+                     ldr, s);
 }
 
 void mean_values(const double *val, long long ldv, int count, long long nnz, float *out,
@@ -1562,18 +1589,26 @@ void gemm_nn(const bf16 *P, long long ldp, int p, const float *C, const float *I
              float a, float b, float *Out, long long ldo, int s, long long n, int un,
              hipStream_t st) {
   if (un >= 100) {
-    // bf16 matrix cores: 128 rows per wavefront, four per workgroup.
-    const long long waves = (n + 127) / 128;
+    // bf16 matrix cores: 128 (un = 100) or 256 (un = 101) rows per wavefront, four per workgroup.
+    const int RB = un >= 101 ? 2 : 1;
+    const long long waves = (n + 128 * RB - 1) / (128 * RB);
     const unsigned grid = (unsigned)((waves + 3) / 4);
     const int KP = (p + 15) & ~15;
     const size_t epi = sizeof(float) * 4 * kNnbTile;
     const size_t l1 = sizeof(uint16_t) * 2 * 16 * KP, l2 = 2 * l1;
-    if (s <= 16)
-      hipLaunchKernelGGL((k_gemm_nn_b<1, float>), dim3(grid), dim3(kThreads), l1 > epi ? l1 : epi,
-                         st, P, ldp, p, C, In, ldi, a, b, Out, ldo, s, n);
+    const size_t lds1 = l1 > epi ? l1 : epi, lds2 = l2 > epi ? l2 : epi;
+    if (s <= 16 && RB == 2)
+      hipLaunchKernelGGL((k_gemm_nn_b<1, float, 2>), dim3(grid), dim3(kThreads), lds1, st, P, ldp,
+                         p, C, In, ldi, a, b, Out, ldo, s, n);
+    else if (s <= 16)
+      hipLaunchKernelGGL((k_gemm_nn_b<1, float, 1>), dim3(grid), dim3(kThreads), lds1, st, P, ldp,
+                         p, C, In, ldi, a, b, Out, ldo, s, n);
+    else if (RB == 2)
+      hipLaunchKernelGGL((k_gemm_nn_b<2, float, 2>), dim3(grid), dim3(kThreads), lds2, st, P, ldp,
+                         p, C, In, ldi, a, b, Out, ldo, s, n);
     else
-      hipLaunchKernelGGL((k_gemm_nn_b<2, float>), dim3(grid), dim3(kThreads), l2 > epi ? l2 : epi,
-                         st, P, ldp, p, C, In, ldi, a, b, Out, ldo, s, n);
+      hipLaunchKernelGGL((k_gemm_nn_b<2, float, 1>), dim3(grid), dim3(kThreads), lds2, st, P, ldp,
+                         p, C, In, ldi, a, b, Out, ldo, s, n);
     return;
   }
   gemm_nn_t<bf16, float>(P, ldp, p, C, In, ldi, a, b, Out, ldo, s, n, un, st);

@@ -169,6 +169,7 @@ int admit(dmrServer *S, Family *F, const double *val) {
     if (rc == DMR_OK &&
         hipMalloc(&F->stage, sizeof(double) * 2 * (size_t)F->n * F->sMax) != hipSuccess)
       rc = DMR_ERR_ALLOC;
+    if (rc == DMR_OK) rc = dmrBlockSetPrecision(F->bh, F->prec);
     if (rc == DMR_OK && F->hostHasZ) rc = dmrBlockImportZ(F->bh, F->hostZ.data());
     if (rc == DMR_OK && (F->innerTol > 0.0 || F->maxOuter > 0))
       rc = dmrBlockSetRefinement(F->bh, F->innerTol > 0.0 ? F->innerTol : 1e-5,
@@ -432,6 +433,17 @@ int dmrServerSetRefinement(dmrServerHandle s, int family, double innerTol, int m
   if (F->busy || resident(F)) return DMR_ERR_STATE;  // fixed once the family has run
   F->innerTol = innerTol;
   F->maxOuter = maxOuter;
+  return DMR_OK;
+}
+
+int dmrServerSetBlockPrecision(dmrServerHandle s, int family, int precision) {
+  if (!s || (precision != DMR_PREC_FP32 && precision != DMR_PREC_BF16)) return DMR_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (family < 0 || family >= (int)s->families.size()) return DMR_ERR_ARG;
+  Family *F = s->families[family].get();
+  if (!F->block) return DMR_ERR_ARG;
+  if (F->busy || resident(F)) return DMR_ERR_STATE;  // fixed once the family has run
+  F->prec = precision;
   return DMR_OK;
 }
 

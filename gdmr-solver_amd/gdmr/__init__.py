@@ -101,6 +101,7 @@ _sig("dmrServerDrain", C.c_int, [_vp])
 _sig("dmrServerStats", C.c_int, [_vp, _ip, _ip, _ip, C.POINTER(C.c_size_t)])
 _sig("dmrServerDestroy", C.c_int, [_vp])
 _sig("dmrServerSetRefinement", C.c_int, [_vp, C.c_int, C.c_double, C.c_int])
+_sig("dmrServerSetBlockPrecision", C.c_int, [_vp, C.c_int, C.c_int])
 _sig("dmrServerAddBlockFamily", C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int,
                                           C.c_int, _ip])
 _sig("dmrGetStagnationCount", C.c_int, [_vp])
@@ -156,6 +157,7 @@ HEADER_SYMBOLS = (
     "dmrBlockExportZ dmrBlockSetProfiling dmrBlockGetProfile dmrBlockFootprint "
     "dmrBlockSetOptions dmrBlockGetTuning "
     "dmrServerCreate dmrServerAddFamily dmrServerAddBlockFamily dmrServerSetRefinement "
+    "dmrServerSetBlockPrecision "
     "dmrServerSubmit dmrServerPoll "
     "dmrServerWait dmrServerDrain dmrServerNow dmrServerStats dmrServerDestroy "
     "initLS freeLS impImgLS expImgLS impSrcLS expSrcLS impOptLS expOptLS solveLS linOptLS "
@@ -552,6 +554,10 @@ class Server:
     def set_refinement(self, family: int, inner_tol: float, max_outer: int = 60):
         _check(lib.dmrServerSetRefinement(self.h, family, inner_tol, max_outer),
                "dmrServerSetRefinement")
+
+    def set_block_precision(self, family: int, precision: int):
+        _check(lib.dmrServerSetBlockPrecision(self.h, family, precision),
+               "dmrServerSetBlockPrecision")
 
     def submit(self, family, val_ptr, b_ptrs, x_ptrs, tol, mat_rank, keep=()) -> int:
         nr = len(b_ptrs)

@@ -69,7 +69,12 @@ class Load:
                 fid = self.srv.add_block_family(f["n"], f["nnz"], f["rp"].data_ptr(),
                                                 f["ci"].data_ptr(), args.block_steps, args.k,
                                                 args.block_rhs, keep=(f["rp"], f["ci"]))
-                self.srv.set_refinement(fid, args.inner_tol, 30)
+                if args.block_prec == "bf16":
+                    self.srv.set_block_precision(fid, gdmr.PREC_BF16)
+                    self.srv.set_refinement(fid, 3e-2 if args.inner_tol == 1e-4 else
+                                            args.inner_tol, 60)
+                else:
+                    self.srv.set_refinement(fid, args.inner_tol, 30)
             else:
                 fid = self.srv.add_family(f["n"], f["nnz"], f["rp"].data_ptr(),
                                           f["ci"].data_ptr(), args.m, args.k, prec,
@@ -146,6 +151,9 @@ def main(argv=None):
                          "requests with the same operator batched into one launch")
     ap.add_argument("--block-steps", type=int, default=10)
     ap.add_argument("--block-rhs", type=int, default=16)
+    ap.add_argument("--block-prec", choices=("fp32", "bf16"), default="fp32",
+                    help="basis storage of block families (bf16: inner tolerance 3e-2 unless "
+                         "--inner-tol is given)")
     args = ap.parse_args(argv)
     import torch
     assert torch.cuda.is_available(), "the serving benchmark needs a HIP device"

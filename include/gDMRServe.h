@@ -63,6 +63,9 @@ int dmrServerAddFamily(dmrServerHandle s, int n, int nnz, const int *rowPtr, con
 int dmrServerAddBlockFamily(dmrServerHandle s, int n, int nnz, const int *rowPtr,
                             const int *colInd, int blockSteps, int deflatDim, int maxRhs,
                             int *family);
+/* Basis storage of a block family (dmrBlockSetPrecision: DMR_PREC_FP32, the default, or
+ * DMR_PREC_BF16), before its first request.  No counterpart in the reference. */
+int dmrServerSetBlockPrecision(dmrServerHandle s, int family, int precision);
 /* Refinement settings of a mixed-precision family (dmrSetRefinement), before its first request. */
 int dmrServerSetRefinement(dmrServerHandle s, int family, double innerTol, int maxOuter);
 /* val: device values (nnz doubles) of this request's operator; b[i], x[i] (i < nrhs): device

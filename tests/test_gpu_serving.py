@@ -174,11 +174,13 @@ def test_eviction_keeps_the_chain(torch_gpu):
 XTOL_BATCH = 1e-6  # relative solution agreement, alone vs batched (both at 1e-10 residual)
 
 
-def test_block_family_batches_within_tolerance(torch_gpu):
+@pytest.mark.parametrize("basis", ["fp32", "bf16"])
+def test_block_family_batches_within_tolerance(torch_gpu, basis):
     """Block families solve queued requests that share an operator as one block launch
     (continuous batching).  Served alone (one request at a time: no batching) and in a mix (every
     request of three families queued at once: requests batch), every solution meets its tolerance
-    in FP64 and the alone and mixed solutions agree to XTOL_BATCH."""
+    in FP64 and the alone and mixed solutions agree to XTOL_BATCH; with an FP32 or a bf16 block
+    basis."""
     torch = torch_gpu
     tol = 1e-10
     A1 = gdmr.conv_diff_3d(24, jitter=0.01, seed=51, stream=0)
@@ -207,6 +209,12 @@ def test_block_family_batches_within_tolerance(torch_gpu):
                                          pats[1][2].data_ptr(), 6, 8, 8),
                     srv.add_family(A3.n, A3.nnz, pats[2][1].data_ptr(), pats[2][2].data_ptr(), 30,
                                    10)]
+            if basis == "bf16":
+                for f in fams[:2]:
+                    srv.set_block_precision(f, gdmr.PREC_BF16)
+                    srv.set_refinement(f, 3e-2, 60)
+                with pytest.raises(gdmr.SolverError):
+                    srv.set_block_precision(fams[2], gdmr.PREC_BF16)  # not a block family
             xs, ids, results = {}, {}, []
             for i, (f, d, B) in enumerate(reqs):
                 xs[i] = [torch.zeros(len(b), dtype=torch.float64, device="cuda") for b in B]

@@ -38,6 +38,7 @@ constexpr double kSvdTol = 1e-14;  // Jacobi stopping tolerance (DMR_BLOCK_SVD_T
 constexpr int kSvdMaxSweeps = 128;
 constexpr double kDepTol = 1e-12;  // Cholesky pivot / max pivot below which a column is dependent
 constexpr int kMaxEvents = 8192;
+constexpr int kStallCycles = 25;  // inner cycles without 0.1% progress before a refinement restart
 
 #define BCHK(call)                                                                         \
   do {                                                                                     \
@@ -673,6 +674,11 @@ int block_solve(dmrBlockContext *c, int mode, int s, double *X, long long ldx, c
     dmr::blk::scale_to32(c->R64, ld, c->f, c->R, ld, s, n, st);
     dmr::fill<float>(c->Xa, ld * s, 0.0, st);
     dmr::blk::recip(c->f, c->scal, s, c->rho, st);  // rho = 1
+    // Stagnation exit: an inner solve whose worst estimate has not improved by 0.1% for
+    // kStallCycles cycles returns to the refinement, which recomputes the FP64 residual and
+    // restarts (bounded by maxOuter), instead of running to maxCycles.
+    double bestEst = HUGE_VAL;
+    int stall = 0;
     for (int cyc = 0; cyc < maxCycles; ++cyc) {
       const bool deflate = c->k > 0 && c->hasZ && (useZ || cycled);
       bool refreshed = false;
@@ -707,6 +713,12 @@ int block_solve(dmrBlockContext *c, int mode, int s, double *X, long long ldx, c
       c->hist.push_back(est);
       if (!std::isfinite(est)) return DMR_ERR_RUNAWAY;
       if (innerDone) break;
+      if (est < 0.999 * bestEst) {
+        bestEst = est;
+        stall = 0;
+      } else if (++stall >= kStallCycles) {
+        break;
+      }
     }
     // X += ||R_j|| D_j.
     dmr::blk::add_scaled(X, ldx, c->Xa, ld, c->rn, s, n, st);

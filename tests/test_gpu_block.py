@@ -356,3 +356,30 @@ def test_precision_rejects_fp64(torch_gpu):
     bs.set_precision(gdmr.PREC_BF16)
     bs.set_precision(gdmr.PREC_FP32)
     bs.close()
+
+
+def test_inner_stagnation_returns_to_refinement(torch_gpu):
+    """A cyclic shift with unit-vector right-hand sides e_0, e_300: every Krylov block is
+    orthogonal to the right-hand sides until the space wraps around, so restarted block GMRES
+    makes exactly no progress.  The solve ends with DMR_ERR_NOCONV at the refinement limit: each
+    inner solve gives up after 25 cycles without progress instead of running to the cycle cap."""
+    n = 600
+    A = gdmr.CsrMatrix(n, np.arange(n + 1, dtype=np.int32),
+                       ((np.arange(n) + 1) % n).astype(np.int32), np.ones(n))
+    B = np.zeros((2, n))
+    B[0, 0] = 1.0
+    B[1, 300] = 1.0
+    solver = gdmr.BlockSolver(n, 4, 4, 2)
+    rp, ci, va = _dev(torch_gpu, A.rowptr), _dev(torch_gpu, A.col), _dev(torch_gpu, A.val)
+    solver.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A.nnz, keep=(rp, ci, va))
+    solver.set_refinement(1e-2, 3)
+    Bd = _dev(torch_gpu, B)
+    Xd = torch_gpu.zeros(2, n, dtype=torch_gpu.float64, device="cuda")
+    torch_gpu.cuda.synchronize()
+    rc, rel = solver.solve(Xd.data_ptr(), n, Bd.data_ptr(), n, 2, 1e-10, max_cycles=1000)
+    st, hist = solver.stats()
+    solver.close()
+    print(f"rc {rc} cycles {st.cycles} rel {rel}", flush=True)
+    assert rc == gdmr.DMR_ERR_NOCONV
+    assert st.cycles <= 4 * 26
+    assert np.all(np.isfinite(rel))

@@ -1103,6 +1103,88 @@ __global__ __launch_bounds__(64) void k_chol_qr(const double *__restrict__ Gin, 
   if (j == 0 && dep != 0u) atomicOr(brk, 1);
 }
 
+// The same factorisation with one thread per matrix element (D x D threads, D = 16 or 32 >= s):
+// the operands of every step come from LDS, and the operations and their order per element are
+// those of k_chol_qr, so both give the same bits.
+template <int D>
+__global__ __launch_bounds__(D * D) void k_chol_qr_el(const double *__restrict__ Gin, int s,
+                                                      double relTol, float *__restrict__ Sinv32,
+                                                      double *__restrict__ Sacc, int first,
+                                                      double *__restrict__ Hdst, int ldh,
+                                                      int *__restrict__ brk) {
+  constexpr int LD = D + 1;
+  __shared__ double A[D * LD];  // G, then S in the upper triangle
+  __shared__ double X[D * LD];  // unit columns, then S^-1 (column j of S^-1 in column j)
+  __shared__ double Y[D * LD];  // the previous accumulated factor
+  __shared__ double sh_dmax, sh_skk;
+  __shared__ unsigned sh_dep;
+  __shared__ int sh_bad;
+  const int t = threadIdx.x, i = t / D, j = t % D;
+  const bool own = i < s && j < s;
+  A[i * LD + j] = own ? Gin[j * s + i] : 0.0;
+  X[i * LD + j] = i == j ? 1.0 : 0.0;
+  Y[i * LD + j] = (!first && own) ? Sacc[j * s + i] : 0.0;
+  if (t == 0) sh_dep = 0u;
+  __syncthreads();
+  if (t < 64) {
+    double d = t < s ? A[t * LD + t] : 0.0;
+    d = fmax(d, 0.0);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) d = fmax(d, __shfl_xor(d, off, 64));
+    if (t == 0) sh_dmax = d;
+  }
+  __syncthreads();
+  const double dmax = sh_dmax;
+  // Right-looking factorisation: after step k row k of A holds S(k, :).
+  for (int k = 0; k < s; ++k) {
+    if (t == 0) {
+      const double d = A[k * LD + k];
+      const bool bad = !(d > relTol * dmax);
+      const double skk = bad ? 0.0 : sqrt(d);
+      A[k * LD + k] = skk;
+      sh_skk = skk;
+      sh_bad = bad ? 1 : 0;
+      if (bad) sh_dep |= 1u << k;
+    }
+    __syncthreads();
+    if (i == k && j > k && j < s) A[k * LD + j] = sh_bad ? 0.0 : A[k * LD + j] / sh_skk;
+    __syncthreads();
+    if (i > k && j >= i && j < s) A[i * LD + j] = A[i * LD + j] - A[k * LD + i] * A[k * LD + j];
+    __syncthreads();
+  }
+  const unsigned dep = sh_dep;
+  // Column j of S^-1 by back substitution (row l of X becomes x_l once step l is done).
+  for (int l = s - 1; l >= 0; --l) {
+    if (i == l) {
+      const bool depj = j < s ? ((dep >> j) & 1u) != 0u : true;
+      const bool skip = depj || ((dep >> l) & 1u) != 0u || l > j;
+      X[l * LD + j] = skip ? 0.0 : X[l * LD + j] / A[l * LD + l];
+    }
+    __syncthreads();
+    if (i < l) X[i * LD + j] = fma(-A[i * LD + l], X[l * LD + j], X[i * LD + j]);
+    __syncthreads();
+  }
+  if (i < 32 && j < 32)
+    Sinv32[i * 32 + j] = own ? (float)X[i * LD + j] : 0.0f;
+  for (int q = t; q < 32 * 32; q += D * D) {  // the rest of the 32-wide FP32 factor is zero
+    const int qi = q >> 5, qj = q & 31;
+    if (qi >= D || qj >= D) Sinv32[q] = 0.0f;
+  }
+  // Sacc <- S * Sacc (S alone on the first pass), into Hdst as well.
+  if (own) {
+    double a = 0.0;
+    if (first) {
+      a = i <= j ? A[i * LD + j] : 0.0;
+    } else {
+      for (int l = i; l < s; ++l) a = fma(A[i * LD + l], l <= j ? Y[l * LD + j] : 0.0, a);
+    }
+    Sacc[j * s + i] = a;
+    if (Hdst) Hdst[(long long)j * ldh + i] = a;
+  }
+  if (t == 0 && dep != 0u) atomicOr(brk, 1);
+}
+
+
 // ---------------------------------------------------------------------------------------------
 // Least squares min ||T - H Y|| (per column) on the block Hessenberg H (rows x cols, ld rows,
 // lower bandwidth bw <= 32, rows = cols + bw): Householder QR with reflectors over rows c..c+bw.
@@ -1621,8 +1703,12 @@ void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *
 
 void chol_qr(const double *G, int s, double relTol, float *Sinv32, double *Sacc, int first,
              double *Hdst, int ldh, int *brk, hipStream_t st) {
-  hipLaunchKernelGGL(k_chol_qr, dim3(1), dim3(64), 0, st, G, s, relTol, Sinv32, Sacc, first,
-                     Hdst, ldh, brk);
+  if (s <= 16)
+    hipLaunchKernelGGL(k_chol_qr_el<16>, dim3(1), dim3(256), 0, st, G, s, relTol, Sinv32, Sacc,
+                       first, Hdst, ldh, brk);
+  else
+    hipLaunchKernelGGL(k_chol_qr_el<32>, dim3(1), dim3(1024), 0, st, G, s, relTol, Sinv32, Sacc,
+                       first, Hdst, ldh, brk);
 }
 
 size_t lsq_block_lds(int cols, int bw, int s) {

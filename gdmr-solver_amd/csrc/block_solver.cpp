@@ -74,6 +74,7 @@ struct dmrBlockContext {
   double *H = nullptr, *T = nullptr, *Hw = nullptr, *Tw = nullptr, *Y = nullptr, *Hred = nullptr;
   float *C32 = nullptr, *Yr32 = nullptr, *ZH32 = nullptr, *Sinv32 = nullptr, *Cz32 = nullptr;
   float *part = nullptr;
+  size_t partFloats = 0;
   double *G64 = nullptr, *Sacc = nullptr, *Csvd = nullptr, *sig = nullptr;
   double *rho = nullptr, *q = nullptr, *scal = nullptr, *bn = nullptr, *rn = nullptr,
          *f = nullptr, *zq = nullptr, *cpart = nullptr;
@@ -83,6 +84,7 @@ struct dmrBlockContext {
   int krMax = 64;  // cap on the leading block steps of the subspace refresh
   double svdTol = kSvdTol;
   int cholPasses = 0;  // Cholesky QR passes per block normalisation; 0: by storage type
+  bool fusedGram = true;  // first Gram matrix from the update's epilogue where possible
   // Launch shapes chosen by the autotuner (per block width s), and whether it ran.
   struct Tuning {
     int tnRowsProj = 512, tnVarProj = 0;  // gemm_tn for projections (p > s)
@@ -280,7 +282,8 @@ int ctx_alloc(dmrBlockContext *c) {
   c->Yr32 = m.take<float>((size_t)colsMax * 32);
   c->Sinv32 = m.take<float>(32 * 32);
   c->Cz32 = m.take<float>((size_t)cRed * 32);
-  c->part = m.take<float>((size_t)c->G * pPadMax * 32);
+  c->partFloats = (size_t)c->G * pPadMax * 32;
+  c->part = m.take<float>(c->partFloats);
   c->flags = m.take<int>(64);
   return DMR_OK;
 }
@@ -306,6 +309,21 @@ void nn(dmrBlockContext *c, const TP *P, int p, const float *C, const float *In,
         float b, TO *Out, int s, hipStream_t st) {
   const int un = std::is_same<TP, bf16>::value ? c->tune.nnBasis : c->tune.nnUnroll;
   dmr::blk::gemm_nn(P, c->ld, p, C, In, c->ld, a, b, Out, c->ld, s, c->n, un, st);
+}
+
+// Partial Gram matrices from the epilogue of a bf16-basis update (see gemm_nn): how many, or 0
+// when this block width, launch shape or partial buffer does not allow it.
+int gram_parts(const dmrBlockContext *c, int s) {
+  if (!c->fusedGram || !c->bfBasis || s > 16 || c->tune.nnBasis < 100) return 0;
+  const int G = dmr::blk::gemm_nn_gram_blocks(c->n, c->tune.nnBasis);
+  return (size_t)G * 1024 <= c->partFloats ? G : 0;
+}
+
+// Out = a In + b P C over a bf16 basis, writing the partial Gram matrices of Out (gram_parts).
+void nn_gram(dmrBlockContext *c, const bf16 *P, int p, const float *C, const float *In, float a,
+             float b, float *Out, int s, hipStream_t st) {
+  dmr::blk::gemm_nn(P, c->ld, p, C, In, c->ld, a, b, Out, c->ld, s, c->n, c->tune.nnBasis, st,
+                    c->part);
 }
 
 template <typename TX>
@@ -440,17 +458,24 @@ void autotune(dmrBlockContext *c, int s, hipStream_t st) {
 
 // Two Cholesky QR passes on the sc columns of U (in place).  The combined triangular factor goes
 // to Hdst (ld ldh) when given.
+// gramParts > 0: the first pass's Gram matrix is already in c->part as that many partials
+// (written by the update that produced U).
 template <typename TO>
 void cholqr2(dmrBlockContext *c, float *U, int sc, double *Hdst, int ldh, TO *dst,
-             hipStream_t st) {
+             hipStream_t st, int gramParts = 0) {
   const long long n = c->n;
   // A bf16-stored block is orthogonal only to bf16 rounding (about 4e-3), so one Cholesky QR
   // pass, orthogonal to about kappa^2 * 6e-8, is enough there; FP32 storage keeps two.
   const int passes = c->cholPasses ? c->cholPasses : (c->bfBasis ? 1 : 2);
   for (int pass = 0; pass < passes; ++pass) {
-    timedf(c, st, DMR_PH_ORTH, 4.0 * n * sc, 2.0 * n * sc * sc, [&] {
-      tn(c, U, sc, U, sc, c->G64, sc, nullptr, true, st);
-    });
+    if (pass == 0 && gramParts > 0) {
+      timed(c, st, DMR_PH_ORTH, 0.0,
+            [&] { dmr::blk::gram_reduce(c->part, gramParts, sc, c->G64, st); });
+    } else {
+      timedf(c, st, DMR_PH_ORTH, 4.0 * n * sc, 2.0 * n * sc * sc, [&] {
+        tn(c, U, sc, U, sc, c->G64, sc, nullptr, true, st);
+      });
+    }
     timed(c, st, DMR_PH_ORTH, 0.0, [&] {
       dmr::blk::chol_qr(c->G64, sc, kDepTol, c->Sinv32, c->Sacc, pass == 0,
                         pass == passes - 1 ? Hdst : nullptr, ldh, c->flags, st);
@@ -478,6 +503,7 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
   const int p0 = deflate ? k : 0;
   const int rows = p0 + (mb + 1) * s, cols = p0 + mb * s;
   float *U = c->Ub;
+  const int gp = gram_parts(c, s);  // Gram matrices from the update's epilogue, when possible
   const double vb = 4.0 * (double)n;              // one FP32 column
   const double pb = (double)sizeof(TP) * (double)n;  // one basis column
   dmr::fill<double>(c->H, (long long)rows * cols, 0.0, st);
@@ -523,12 +549,18 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
       tn(c, P, k, c->R, s, nullptr, 0, c->C32, false, st);
     });
     timedf(c, st, DMR_PH_UPDATE, pb * k + vb * 2 * s, 2.0 * n * k * s, [&] {
+      if constexpr (std::is_same<TP, bf16>::value) {
+        if (gp) {
+          nn_gram(c, P, k, c->C32, c->R, 1.0f, -1.0f, U, s, st);
+          return;
+        }
+      }
       nn(c, P, k, c->C32, c->R, 1.0f, -1.0f, U, s, st);
     });
   } else {
     hipMemcpyAsync(U, c->R, sizeof(float) * ld * s, hipMemcpyDeviceToDevice, st);
   }
-  cholqr2(c, U, s, (double *)nullptr, 0, V0, st);
+  cholqr2(c, U, s, (double *)nullptr, 0, V0, st, deflate ? gp : 0);
   for (int j = 0; j < mb; ++j) {
     const TP *Vj = P + (long long)(p0 + j * s) * ld;
     const int pj = p0 + (j + 1) * s;
@@ -541,9 +573,15 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
       tn(c, P, pj, U, s, Hcol, rows, c->C32, false, st);
     });
     timedf(c, st, DMR_PH_UPDATE, pb * pj + vb * 2 * s, 2.0 * n * pj * s, [&] {
+      if constexpr (std::is_same<TP, bf16>::value) {
+        if (gp) {
+          nn_gram(c, P, pj, c->C32, U, 1.0f, -1.0f, U, s, st);
+          return;
+        }
+      }
       nn(c, P, pj, c->C32, U, 1.0f, -1.0f, U, s, st);
     });
-    cholqr2(c, U, s, Hcol + pj, rows, slot, st);
+    cholqr2(c, U, s, Hcol + pj, rows, slot, st, gp);
     if (refreshed && j == kr - 1) fork_svd();
   }
   // Full projection of the residual block, least squares, residual and solution updates.
@@ -772,6 +810,7 @@ int dmrBlockCreate(dmrBlockHandle *h, int device, int elements, int blockSteps, 
     const int v = std::atoi(e);
     if (v == 1 || v == 2) c->cholPasses = v;
   }
+  if (const char *e = std::getenv("DMR_BLOCK_FUSED_GRAM")) c->fusedGram = e[0] != '0';
   if (const char *e = std::getenv("DMR_BLOCK_SVD_TOL")) {
     const double v = std::atof(e);
     if (v > 0.0 && v < 1.0) c->svdTol = v;

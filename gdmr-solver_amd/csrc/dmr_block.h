@@ -61,9 +61,14 @@ void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *
              hipStream_t st);
 // bf16 basis P (un >= 100: the bf16 matrix cores with C split into hi + lo parts, else the FP32
 // ones on unpacked values); or FP32 P written to a bf16 Out (rounded to nearest).
+// gram (un >= 100, s <= 16): each workgroup also writes the partial Gram matrix Out^T Out of its
+// rows in gemm_tn's partial layout (pPad = 32); gemm_nn_gram_blocks partials; gram_reduce sums
+// them in a fixed order into G64 (s x s, ld s).
 void gemm_nn(const bf16 *P, long long ldp, int p, const float *C, const float *In, long long ldi,
              float a, float b, float *Out, long long ldo, int s, long long n, int un,
-             hipStream_t st);
+             hipStream_t st, float *gram = nullptr);
+int gemm_nn_gram_blocks(long long n, int un);
+void gram_reduce(const float *part, int G, int s, double *G64, hipStream_t st);
 void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *In, long long ldi,
              float a, float b, bf16 *Out, long long ldo, int s, long long n, int un,
              hipStream_t st);

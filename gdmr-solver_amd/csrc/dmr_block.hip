@@ -651,12 +651,15 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn16b(const bf16 *__restrict_
 // load zeros and store nothing), so the workgroup barriers are uniform.
 constexpr int kNnbTile = 128 * 16 + 16 * 4;  // floats per wavefront: 16 columns of 128 (+4 pad)
 
+// With gram (NT = 1 only), the workgroup also writes the partial Gram matrix Out^T Out of its
+// rows, gram[blockIdx.x * 1024 + i * 32 + j] for i, j < 16 (FP32 16x16x4 MFMA over the output
+// tile in LDS, the wavefronts added in order): the layout of gemm_tn's partials (pPad = 32).
 template <int NT, typename TO, int RB>
 __global__ __launch_bounds__(kThreads) void k_gemm_nn_b(const bf16 *__restrict__ P, long long ldp,
                                                         int p, const float *__restrict__ C,
                                                         const float *In, long long ldi, float a,
                                                         float b, TO *Out, long long ldo, int s,
-                                                        long long n) {
+                                                        long long n, float *__restrict__ gram) {
   extern __shared__ uint16_t Cl[];  // [hi | lo][16 NT output columns][KP]; then the epilogue
   const int KP = (p + 15) & ~15;
   uint16_t *Ch = Cl, *Clo = Cl + (size_t)16 * NT * KP;
@@ -738,6 +741,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nn_b(const bf16 *__restrict__
       for (int j = 0; j < 4; ++j) cur[rb][j] = nxt[rb][j];
   }
   // Epilogue: through LDS (the coefficient area is free once every wavefront is done with it).
+  f32x4 gacc = f32x4{0.f, 0.f, 0.f, 0.f};
   float *tile = reinterpret_cast<float *>(Cl) + wave * kNnbTile;
   const int stride = 128 + 4;
 #pragma unroll
@@ -782,6 +786,35 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nn_b(const bf16 *__restrict__
 #pragma unroll
         for (int e = 0; e < 4; ++e) o[e] = a != 0.0f ? a * in[e] + b * pr[e] : b * pr[e];
         store_rows4(Out + j * ldo + row, o, cnt);
+        if (NT == 1 && gram)  // the output as stored (rows past n stay zero) for the Gram matrix
+          *reinterpret_cast<float4 *>(tile + jl * stride + q4) = make_float4(o[0], o[1], o[2], o[3]);
+      }
+    }
+    if (NT == 1 && gram) {
+      __syncthreads();
+      // Lane l contributes rows 4q + l / 16 of column l % 16 to k-step q: A = B = this value.
+#pragma unroll 8
+      for (int q = 0; q < 32; ++q) {
+        const float x = tile[(lane & 15) * stride + 4 * q + (lane >> 4)];
+        gacc = mfma16(x, x, gacc);
+      }
+    }
+  }
+  if (NT == 1 && gram) {
+    __syncthreads();
+    float *red = reinterpret_cast<float *>(Cl) + wave * kNnbTile;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red[v * 64 + lane] = gacc[v];
+    __syncthreads();
+    if (wave == 0) {
+      float *outp = gram + (long long)blockIdx.x * 1024;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        float g = 0.0f;
+#pragma unroll
+        for (int w = 0; w < 4; ++w)
+          g = g + reinterpret_cast<const float *>(Cl)[w * kNnbTile + v * 64 + lane];
+        outp[(4 * kg + v) * 32 + m] = g;
       }
     }
   }
@@ -1582,6 +1615,11 @@ void gemm_tn16(const TP *P, long long ldp, int p, int pPad, const float *U, long
 #undef DMR_TN16
 }
 
+void gram_reduce(const float *part, int G, int s, double *G64, hipStream_t st) {
+  hipLaunchKernelGGL(k_gemm_tn_reduce, dim3(s), dim3(kThreads), 0, st, part, G, 32, s, s, G64, s,
+                     (float *)nullptr);
+}
+
 long long tn_rows(long long n, int G) {
   long long rows = (n + G - 1) / G;
   return (rows + 63) / 64 * 64;
@@ -1667,9 +1705,15 @@ void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *
              hipStream_t st) {
   gemm_nn_t<float, float>(P, ldp, p, C, In, ldi, a, b, Out, ldo, s, n, un, st);
 }
+int gemm_nn_gram_blocks(long long n, int un) {
+  const int RB = un >= 101 ? 2 : 1;
+  const long long waves = (n + 128 * RB - 1) / (128 * RB);
+  return (int)((waves + 3) / 4);
+}
+
 void gemm_nn(const bf16 *P, long long ldp, int p, const float *C, const float *In, long long ldi,
              float a, float b, float *Out, long long ldo, int s, long long n, int un,
-             hipStream_t st) {
+             hipStream_t st, float *gram) {
   if (un >= 100) {
     // bf16 matrix cores: 128 (un = 100) or 256 (un = 101) rows per wavefront, four per workgroup.
     const int RB = un >= 101 ? 2 : 1;
@@ -1681,16 +1725,16 @@ void gemm_nn(const bf16 *P, long long ldp, int p, const float *C, const float *I
     const size_t lds1 = l1 > epi ? l1 : epi, lds2 = l2 > epi ? l2 : epi;
     if (s <= 16 && RB == 2)
       hipLaunchKernelGGL((k_gemm_nn_b<1, float, 2>), dim3(grid), dim3(kThreads), lds1, st, P, ldp,
-                         p, C, In, ldi, a, b, Out, ldo, s, n);
+                         p, C, In, ldi, a, b, Out, ldo, s, n, gram);
     else if (s <= 16)
       hipLaunchKernelGGL((k_gemm_nn_b<1, float, 1>), dim3(grid), dim3(kThreads), lds1, st, P, ldp,
-                         p, C, In, ldi, a, b, Out, ldo, s, n);
+                         p, C, In, ldi, a, b, Out, ldo, s, n, gram);
     else if (RB == 2)
       hipLaunchKernelGGL((k_gemm_nn_b<2, float, 2>), dim3(grid), dim3(kThreads), lds2, st, P, ldp,
-                         p, C, In, ldi, a, b, Out, ldo, s, n);
+                         p, C, In, ldi, a, b, Out, ldo, s, n, nullptr);
     else
       hipLaunchKernelGGL((k_gemm_nn_b<2, float, 1>), dim3(grid), dim3(kThreads), lds2, st, P, ldp,
-                         p, C, In, ldi, a, b, Out, ldo, s, n);
+                         p, C, In, ldi, a, b, Out, ldo, s, n, nullptr);
     return;
   }
   gemm_nn_t<bf16, float>(P, ldp, p, C, In, ldi, a, b, Out, ldo, s, n, un, st);

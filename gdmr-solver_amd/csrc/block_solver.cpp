@@ -85,6 +85,7 @@ struct dmrBlockContext {
   double svdTol = kSvdTol;
   int cholPasses = 0;  // Cholesky QR passes per block normalisation; 0: by storage type
   bool fusedGram = true;  // first Gram matrix from the update's epilogue where possible
+  int spmmForce = 0;      // SpMM variant pinned by DMR_BLOCK_SPMM (0: the tuned one)
   // Launch shapes chosen by the autotuner (per block width s), and whether it ran.
   struct Tuning {
     int tnRowsProj = 512, tnVarProj = 0;  // gemm_tn for projections (p > s)
@@ -328,7 +329,8 @@ void nn_gram(dmrBlockContext *c, const bf16 *P, int p, const float *C, const flo
 
 template <typename TX>
 void spmm(dmrBlockContext *c, const TX *X, float *Y, int s, hipStream_t st) {
-  dmr::blk::spmm32(c->csr, X, c->ld, Y, c->ld, s, c->tune.spmmCols, st);
+  dmr::blk::spmm32(c->csr, X, c->ld, Y, c->ld, s, c->spmmForce ? c->spmmForce : c->tune.spmmCols,
+                   st);
 }
 
 // Process-wide table of autotuned shapes, keyed by the problem shape.
@@ -432,7 +434,7 @@ void autotune_t(dmrBlockContext *c, TP *P, int s, hipStream_t st) {
     }
   }
   float bs = 1e30f;
-  for (int cg : {4, 8, 16, 104, 108, 116}) {
+  for (int cg : {4, 8, 16, 104, 108, 116, 208, 216}) {
     if (bf && cg < 100) continue;  // bf16 vectors: the up-front variant only
     const float tt = best_of([&] {
       dmr::blk::spmm32(c->csr, P, c->ld, c->Xa, c->ld, s, cg, st);
@@ -482,10 +484,12 @@ void cholqr2(dmrBlockContext *c, float *U, int sc, double *Hdst, int ldh, TO *ds
     });
     timedf(c, st, DMR_PH_ORTH, 4.0 * n * sc + (double)sizeof(TO) * n * sc,
            2.0 * n * sc * sc, [&] {
-      if (pass < passes - 1)
-        nn(c, U, sc, c->Sinv32, nullptr, 0.0f, 1.0f, U, sc, st);
-      else
+      if (pass < passes - 1) {
+        if (!dmr::blk::scale_block(U, c->ld, c->Sinv32, U, c->ld, sc, n, st))
+          nn(c, U, sc, c->Sinv32, nullptr, 0.0f, 1.0f, U, sc, st);
+      } else if (!dmr::blk::scale_block(U, c->ld, c->Sinv32, dst, c->ld, sc, n, st)) {
         nn(c, U, sc, c->Sinv32, nullptr, 0.0f, 1.0f, dst, sc, st);
+      }
     });
   }
 }
@@ -811,6 +815,11 @@ int dmrBlockCreate(dmrBlockHandle *h, int device, int elements, int blockSteps, 
     if (v == 1 || v == 2) c->cholPasses = v;
   }
   if (const char *e = std::getenv("DMR_BLOCK_FUSED_GRAM")) c->fusedGram = e[0] != '0';
+  if (const char *e = std::getenv("DMR_BLOCK_SPMM")) {
+    const int v = std::atoi(e);
+    for (int ok : {4, 8, 16, 104, 108, 116, 208, 216})
+      if (v == ok) c->spmmForce = v;
+  }
   if (const char *e = std::getenv("DMR_BLOCK_SVD_TOL")) {
     const double v = std::atof(e);
     if (v > 0.0 && v < 1.0) c->svdTol = v;

@@ -20,10 +20,12 @@ constexpr int kMaxCols = 32 * kMaxTiles;
 constexpr int kMaxTnBlocks = 1024;
 
 // Y[:, 0:s] = A X[:, 0:s] with FP32 values and vectors, FP32 accumulation; cg (4, 8 or 16)
-// columns per thread, or 100 + cg for the variant that loads each row's indices up front.
+// columns per thread, or 100 + cg for the variant that loads each row's indices up front, or
+// 200 + cg (cg 8 or 16) for the one that stages the workgroup's indices in LDS.  Every variant
+// sums each row's entries in order: the results are identical.
 void spmm32(const CsrDev &A, const float *X, long long ldx, float *Y, long long ldy, int s,
             int cg, hipStream_t st);
-// bf16 X (basis storage): the up-front variant only (cg or 100 + cg).
+// bf16 X (basis storage): the up-front variants only (cg or 100 + cg; 200 + cg).
 void spmm32(const CsrDev &A, const bf16 *X, long long ldx, float *Y, long long ldy, int s, int cg,
             hipStream_t st);
 // R = B - A X in FP64 (s columns).  With val, column c uses the values val + c * ldv (same
@@ -72,6 +74,12 @@ void gram_reduce(const float *part, int G, int s, double *G64, hipStream_t st);
 void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *In, long long ldi,
              float a, float b, bf16 *Out, long long ldo, int s, long long n, int un,
              hipStream_t st);
+// Out[:, 0:s] = U[:, 0:s] C (C row-major s x 32), s <= 16, even ldu and ldo (else returns false
+// and launches nothing).  Out may alias U.
+bool scale_block(const float *U, long long ldu, const float *C, float *Out, long long ldo, int s,
+                 long long n, hipStream_t st);
+bool scale_block(const float *U, long long ldu, const float *C, bf16 *Out, long long ldo, int s,
+                 long long n, hipStream_t st);
 // Cholesky QR step on the Gram matrix G (s x s, ld s): see k_chol_qr.
 void chol_qr(const double *G, int s, double relTol, float *Sinv32, double *Sacc, int first,
              double *Hdst, int ldh, int *brk, hipStream_t st);

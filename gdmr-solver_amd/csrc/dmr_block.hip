@@ -147,6 +147,70 @@ __global__ __launch_bounds__(kThreads) void k_spmm_w(int n, const int *__restric
     if (c0 + c < s) Y[row + (c0 + c) * ldy] = (TY)acc[c];
 }
 
+// Same product with the workgroup's nonzeros (its rows' index range of colInd and val, when it
+// holds at most W entries per row on average) staged in LDS by coalesced loads first; each row of
+// up to W entries then reads its indices from LDS and issues its W x CG gathers at once.  Rows
+// with more entries, and workgroups whose range does not fit, read colInd and val directly.  The
+// sums run over each row's entries in order, as in k_spmm.
+template <typename TV, typename TX, typename TY, typename TA, int CG, int W>
+__global__ __launch_bounds__(kThreads) void k_spmm_lds(int n, const int *__restrict__ rowPtr,
+                                                       const int *__restrict__ colInd,
+                                                       const TV *__restrict__ val,
+                                                       const TX *__restrict__ X, long long ldx,
+                                                       TY *__restrict__ Y, long long ldy, int s) {
+  __shared__ int sCol[kThreads * W];
+  __shared__ TV sVal[kThreads * W];
+  const int r0 = blockIdx.x * kThreads, row = r0 + threadIdx.x;
+  const int c0 = blockIdx.y * CG;
+  const int gb = rowPtr[r0], ge = rowPtr[r0 + kThreads < n ? r0 + kThreads : n];
+  const bool staged = ge - gb <= kThreads * W;  // uniform over the workgroup
+  if (staged) {
+    for (int i = threadIdx.x; i < ge - gb; i += kThreads) {
+      sCol[i] = colInd[gb + i];
+      sVal[i] = val[gb + i];
+    }
+  }
+  __syncthreads();
+  if (row >= n) return;
+  const int b = rowPtr[row], e = rowPtr[row + 1];
+  TA acc[CG];
+#pragma unroll
+  for (int c = 0; c < CG; ++c) acc[c] = (TA)0;
+  if (staged && e - b <= W) {
+    int col[W];
+    TA v[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      const bool in = b + q < e;
+      col[q] = in ? sCol[b - gb + q] : row;
+      v[q] = in ? (TA)sVal[b - gb + q] : (TA)0;
+    }
+#pragma unroll
+    for (int c = 0; c < CG; ++c) {
+      if (c0 + c < s) {
+        const TX *xc = X + (long long)(c0 + c) * ldx;
+        TA xv[W];
+#pragma unroll
+        for (int q = 0; q < W; ++q) xv[q] = (TA)ldf(xc, col[q]);
+#pragma unroll
+        for (int q = 0; q < W; ++q)
+          if (b + q < e) acc[c] = fma(v[q], xv[q], acc[c]);
+      }
+    }
+  } else {
+    for (int q = b; q < e; ++q) {
+      const long long col = colInd[q];
+      const TA v = (TA)val[q];
+#pragma unroll
+      for (int c = 0; c < CG; ++c)
+        if (c0 + c < s) acc[c] = fma(v, (TA)ldf(X, col + (c0 + c) * ldx), acc[c]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CG; ++c)
+    if (c0 + c < s) Y[row + (c0 + c) * ldy] = (TY)acc[c];
+}
+
 // R = B - A X in FP64.
 // Column c uses the values val + c * ldv (ldv = 0: one operator for every column).
 __global__ __launch_bounds__(kThreads) void k_resid_block(int n, const int *__restrict__ rowPtr,
@@ -818,6 +882,64 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nn_b(const bf16 *__restrict__
       }
     }
   }
+}
+
+// Out[r, j] = sum_{k < s} U[r, k] C[k][j] for j < s <= 16 (C row-major, 32 wide): the product of
+// a block by a small square factor (Cholesky QR's normalisation).  Thread: rows 2t and 2t + 1,
+// one 8-byte load and one store of two rows per column; C is uniform (scalar loads).  Each sum
+// runs over k in order with fused multiply-adds.  ldu and ldo are even (the basis pitch).  Out
+// may alias U: a thread reads all of its rows before it writes them.
+__device__ inline void store_rows2(float *dst, float v0, float v1, bool two) {
+  if (two)
+    *reinterpret_cast<float2 *>(dst) = make_float2(v0, v1);
+  else
+    dst[0] = v0;
+}
+__device__ inline void store_rows2(bf16 *dst, float v0, float v1, bool two) {
+  if (two)
+    *reinterpret_cast<uint32_t *>(dst) = f_to_bf(v0) | (f_to_bf(v1) << 16);
+  else
+    dst[0].bits = (uint16_t)f_to_bf(v0);
+}
+
+template <typename TO>
+__global__ __launch_bounds__(kThreads) void k_scale_block(const float *U, long long ldu,
+                                                          const float *__restrict__ C, TO *Out,
+                                                          long long ldo, int s, long long n) {
+  const long long r = 2 * ((long long)blockIdx.x * kThreads + threadIdx.x);
+  if (r >= n) return;
+  const bool two = r + 1 < n;
+  float u0[16], u1[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    u0[k] = u1[k] = 0.0f;
+    if (k < s) {
+      if (two) {
+        const float2 v = *reinterpret_cast<const float2 *>(U + k * ldu + r);
+        u0[k] = v.x;
+        u1[k] = v.y;
+      } else {
+        u0[k] = U[k * ldu + r];
+      }
+    }
+  }
+  float o0[16], o1[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) o0[j] = o1[j] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    if (k < s) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const float cv = C[k * 32 + j];
+        o0[j] = fmaf(u0[k], cv, o0[j]);
+        o1[j] = fmaf(u1[k], cv, o1[j]);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j)
+    if (j < s) store_rows2(Out + j * ldo + r, o0[j], o1[j], two);
 }
 
 // LDS-staged variant: each 32-row chunk of P and U is transposed through LDS.
@@ -1505,6 +1627,18 @@ void zero_ints(int *p, int cnt, hipStream_t st) {
 template <typename TX>
 void spmm_upfront(const CsrDev &A, const TX *X, long long ldx, float *Y, long long ldy, int s,
                   int g, hipStream_t st) {
+  if (g >= 100) {
+    // LDS-staged nonzeros, rows of up to 7 entries unrolled.
+    g -= 100;
+    const dim3 grid(grid1(A.n), (unsigned)((s + g - 1) / g));
+    if (g == 16)
+      hipLaunchKernelGGL((k_spmm_lds<float, TX, float, float, 16, 7>), grid, dim3(kThreads), 0, st,
+                         A.n, A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
+    else
+      hipLaunchKernelGGL((k_spmm_lds<float, TX, float, float, 8, 7>), grid, dim3(kThreads), 0, st,
+                         A.n, A.rowPtr, A.colInd, A.val32, X, ldx, Y, ldy, s);
+    return;
+  }
   const dim3 grid(grid1(A.n), (unsigned)((s + g - 1) / g));
   if (g == 4)
     hipLaunchKernelGGL((k_spmm_w<float, TX, float, float, 4, 8>), grid, dim3(kThreads), 0, st, A.n,
@@ -1525,7 +1659,8 @@ void spmm32(const CsrDev &A, const bf16 *X, long long ldx, float *Y, long long l
 void spmm32(const CsrDev &A, const float *X, long long ldx, float *Y, long long ldy, int s,
             int cg, hipStream_t st) {
   if (cg >= 100) {
-    // Up-front index variant: cg - 100 columns per thread, rows of up to 8 entries unrolled.
+    // Up-front index variant: cg - 100 columns per thread, rows of up to 8 entries unrolled
+    // (cg >= 200: the LDS-staged one, up to 7).
     spmm_upfront<float>(A, X, ldx, Y, ldy, s, cg - 100, st);
     return;
   }
@@ -1739,6 +1874,26 @@ void gemm_nn(const bf16 *P, long long ldp, int p, const float *C, const float *I
   }
   gemm_nn_t<bf16, float>(P, ldp, p, C, In, ldi, a, b, Out, ldo, s, n, un, st);
 }
+template <typename TO>
+void scale_block_t(const float *U, long long ldu, const float *C, TO *Out, long long ldo, int s,
+                   long long n, hipStream_t st) {
+  const long long pairs = (n + 1) / 2;
+  hipLaunchKernelGGL(k_scale_block<TO>, dim3((unsigned)((pairs + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, st, U, ldu, C, Out, ldo, s, n);
+}
+bool scale_block(const float *U, long long ldu, const float *C, float *Out, long long ldo, int s,
+                 long long n, hipStream_t st) {
+  if (s > 16 || (ldu & 1) || (ldo & 1)) return false;
+  scale_block_t<float>(U, ldu, C, Out, ldo, s, n, st);
+  return true;
+}
+bool scale_block(const float *U, long long ldu, const float *C, bf16 *Out, long long ldo, int s,
+                 long long n, hipStream_t st) {
+  if (s > 16 || (ldu & 1) || (ldo & 1)) return false;
+  scale_block_t<bf16>(U, ldu, C, Out, ldo, s, n, st);
+  return true;
+}
+
 void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *In, long long ldi,
              float a, float b, bf16 *Out, long long ldo, int s, long long n, int un,
              hipStream_t st) {

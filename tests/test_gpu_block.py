@@ -383,3 +383,56 @@ def test_inner_stagnation_returns_to_refinement(torch_gpu):
     assert rc == gdmr.DMR_ERR_NOCONV
     assert st.cycles <= 4 * 26
     assert np.all(np.isfinite(rel))
+
+
+def _ragged_rows(N=24, seed=5):
+    """conv_diff_3d(N) with extra off-diagonal entries: rows [0, 4096) keep the 7-point stencil,
+    every 16th row of [4096, 8192) gets 3 more entries (long rows inside workgroups whose
+    nonzeros still fit the LDS stage), every row past 8192 gets 2 more (workgroups that do not
+    fit).  Extra values are +-0.005; columns are distinct within a row."""
+    A = gdmr.conv_diff_3d(N)
+    rng = np.random.default_rng(seed)
+    rows_c, rows_v = [], []
+    for i in range(A.n):
+        c = list(A.col[A.rowptr[i]:A.rowptr[i + 1]])
+        v = list(A.val[A.rowptr[i]:A.rowptr[i + 1]])
+        extra = 3 if (4096 <= i < 8192 and i % 16 == 0) else (2 if i >= 8192 else 0)
+        while extra:
+            j = int(rng.integers(A.n))
+            if j not in c:
+                c.append(j)
+                v.append(float(rng.choice([-0.005, 0.005])))
+                extra -= 1
+        rows_c.append(c)
+        rows_v.append(v)
+    rp = np.zeros(A.n + 1, dtype=np.int32)
+    rp[1:] = np.cumsum([len(c) for c in rows_c])
+    return gdmr.CsrMatrix(A.n, rp, np.concatenate(rows_c).astype(np.int32),
+                          np.concatenate(rows_v).astype(np.float64))
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_spmm_variants_identical_on_ragged_rows(torch_gpu, monkeypatch, prec):
+    """Every SpMM variant (one thread per row with 4, 8 or 16 columns; indices loaded up front;
+    indices staged in LDS) sums each row's entries in order, so block solves that differ only in
+    the pinned variant give identical bits, on rows of 7 to 10 entries that take every path of
+    the staged kernel.  Each solve meets 1e-10 in FP64."""
+    A = _ragged_rows()
+    n, s = A.n, 4
+    B = np.stack([gdmr.rand_normal(77, j, n) for j in range(s)])
+    monkeypatch.setenv("DMR_BLOCK_TUNE", "0")
+    codes = [104, 108, 116, 208, 216] + ([4, 8, 16] if prec == "fp32" else [])
+    ref = None
+    for code in codes:
+        monkeypatch.setenv("DMR_BLOCK_SPMM", str(code))
+        rc, rel, st, X = _block(torch_gpu, A, B, mb=4, k=4,
+                                prec=gdmr.PREC_BF16 if prec == "bf16" else None,
+                                inner_tol=3e-2 if prec == "bf16" else None)
+        worst = max(_true_res(A, B[j], X[j]) for j in range(s))
+        print(f"{prec} spmm {code}: rc {rc} cycles {st.cycles} worst true residual {worst:.2e}",
+              flush=True)
+        assert rc == 0 and worst <= 1e-10 * 1.001
+        if ref is None:
+            ref = X
+        else:
+            assert np.array_equal(X, ref), code

@@ -212,7 +212,11 @@ __global__ __launch_bounds__(kThreads) void k_spmm_lds(int n, const int *__restr
 }
 
 // R = B - A X in FP64.
-// Column c uses the values val + c * ldv (ldv = 0: one operator for every column).
+// Column c uses the values val + c * ldv (ldv = 0: one operator for every column).  blockIdx.y
+// selects a group of CG columns.  A row of up to W entries loads its column indices once, then
+// issues every value load and gather of the group at once; longer rows loop.  Each column's sum
+// runs over the row's entries in order from zero, then R = B - sum.
+template <int CG, int W>
 __global__ __launch_bounds__(kThreads) void k_resid_block(int n, const int *__restrict__ rowPtr,
                                                           const int *__restrict__ colInd,
                                                           const double *__restrict__ val,
@@ -222,25 +226,42 @@ __global__ __launch_bounds__(kThreads) void k_resid_block(int n, const int *__re
                                                           const double *__restrict__ B,
                                                           long long ldb, double *__restrict__ R,
                                                           long long ldr, int s) {
-  // blockIdx.y selects a group of four columns, so one pass touches the vectors of four columns.
   const int row = blockIdx.x * kThreads + threadIdx.x;
   if (row >= n) return;
   const int b = rowPtr[row], e = rowPtr[row + 1];
-  {
-    const int c0 = blockIdx.y * 4;
-    double acc[4];
+  const int c0 = blockIdx.y * CG;
+  double acc[CG];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[c] = 0.0;
+  for (int c = 0; c < CG; ++c) acc[c] = 0.0;
+  if (e - b <= W) {
+    int col[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) col[q] = b + q < e ? colInd[b + q] : row;
+    double v[CG][W], x[CG][W];
+#pragma unroll
+    for (int c = 0; c < CG; ++c)
+#pragma unroll
+      for (int q = 0; q < W; ++q) {
+        const bool in = b + q < e && c0 + c < s;
+        v[c][q] = in ? val[b + q + (c0 + c) * ldv] : 0.0;
+        x[c][q] = in ? X[col[q] + (c0 + c) * ldx] : 0.0;
+      }
+#pragma unroll
+    for (int c = 0; c < CG; ++c)
+#pragma unroll
+      for (int q = 0; q < W; ++q)
+        if (b + q < e) acc[c] = fma(v[c][q], x[c][q], acc[c]);
+  } else {
     for (int q = b; q < e; ++q) {
       const long long col = colInd[q];
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
+      for (int c = 0; c < CG; ++c)
         if (c0 + c < s) acc[c] = fma(val[q + (c0 + c) * ldv], X[col + (c0 + c) * ldx], acc[c]);
     }
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-      if (c0 + c < s) R[row + (c0 + c) * ldr] = B[row + (c0 + c) * ldb] - acc[c];
   }
+#pragma unroll
+  for (int c = 0; c < CG; ++c)
+    if (c0 + c < s) R[row + (c0 + c) * ldr] = B[row + (c0 + c) * ldb] - acc[c];
 }
 
 // out[q] = (float)(sum_c val[q + c * ldv] / count), the sum in column order in FP64: the entrywise
@@ -1014,29 +1035,45 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn_lds(const float *__restric
 }
 
 // C[i][j] = sum_g part_g[i][j] (FP64), for i < p, j < s: to C64 (column-major, element (i, j) at
-// j * ldc + i) and to C32 (row-major p x 32, zero beyond s).  One block per row i: thread
-// (j, q) = (t % 32, t / 32) sums partials g = q, q + 8, ... in order, then the eight sums are
-// added in order of q (a fixed order, independent of timing).
+// j * ldc + i) and to C32 (row-major p x 32, zero beyond s).  One block per row i; JW = 16 (s <= 16,
+// only the columns that hold data are read) or 32 columns: thread (j, q) = (t % JW, t / JW) sums
+// partials g = q, q + NQ, ... (NQ = 256 / JW) in order, then the NQ sums are added in order of q
+// (a fixed order, independent of timing).
+template <int JW>
 __global__ __launch_bounds__(kThreads) void k_gemm_tn_reduce(const float *__restrict__ part,
                                                              int G, int pPad, int p, int s,
                                                              double *__restrict__ C64, int ldc,
                                                              float *__restrict__ C32) {
+  constexpr int NQ = kThreads / JW;
   __shared__ double red[kThreads];
-  const int i = blockIdx.x, t = threadIdx.x, j = t & 31, q = t >> 5;
+  const int i = blockIdx.x, t = threadIdx.x, j = t % JW, q = t / JW;
   double a = 0.0;
   const float *src = part + (long long)i * 32 + j;
   const long long stride = (long long)pPad * 32;
 #pragma unroll 8
-  for (int g = q; g < G; g += 8) a = a + (double)src[g * stride];
+  for (int g = q; g < G; g += NQ) a = a + (double)src[g * stride];
   red[t] = a;
   __syncthreads();
-  if (q != 0) return;
+  if (t >= 32) return;
+  // Thread t < 32 writes column t (columns JW .. 31 are zero).
   double sum = 0.0;
+  if (t < JW) {
 #pragma unroll
-  for (int w = 0; w < 8; ++w) sum = sum + red[w * 32 + j];
-  if (j >= s) sum = 0.0;
-  if (C64 && j < s) C64[(long long)j * ldc + i] = sum;
-  if (C32) C32[i * 32 + j] = (float)sum;
+    for (int w = 0; w < NQ; ++w) sum = sum + red[w * JW + t];
+  }
+  if (t >= s) sum = 0.0;
+  if (C64 && t < s) C64[(long long)t * ldc + i] = sum;
+  if (C32) C32[i * 32 + t] = (float)sum;
+}
+
+void tn_reduce(const float *part, int G, int pPad, int p, int s, double *C64, int ldc, float *C32,
+               hipStream_t st) {
+  if (s <= 16)
+    hipLaunchKernelGGL(k_gemm_tn_reduce<16>, dim3(p), dim3(kThreads), 0, st, part, G, pPad, p, s,
+                       C64, ldc, C32);
+  else
+    hipLaunchKernelGGL(k_gemm_tn_reduce<32>, dim3(p), dim3(kThreads), 0, st, part, G, pPad, p, s,
+                       C64, ldc, C32);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1679,9 +1716,9 @@ void spmm32(const CsrDev &A, const float *X, long long ldx, float *Y, long long 
 void resid_block(const CsrDev &A, const double *val, long long ldv, const double *X,
                  long long ldx, const double *B, long long ldb, double *R, long long ldr, int s,
                  hipStream_t st) {
-  hipLaunchKernelGGL(k_resid_block, dim3(grid1(A.n), (s + 3) / 4), dim3(kThreads), 0, st, A.n,
-                     A.rowPtr, A.colInd, val ? val : A.val, val ? ldv : 0LL, X, ldx, B, ldb, R,
-                     ldr, s);
+  hipLaunchKernelGGL((k_resid_block<4, 7>), dim3(grid1(A.n), (s + 3) / 4), dim3(kThreads), 0, st,
+                     A.n, A.rowPtr, A.colInd, val ? val : A.val, val ? ldv : 0LL, X, ldx, B, ldb,
+                     R, ldr, s);
 }
 
 void mean_values(const double *val, long long ldv, int count, long long nnz, float *out,
@@ -1751,8 +1788,7 @@ void gemm_tn16(const TP *P, long long ldp, int p, int pPad, const float *U, long
 }
 
 void gram_reduce(const float *part, int G, int s, double *G64, hipStream_t st) {
-  hipLaunchKernelGGL(k_gemm_tn_reduce, dim3(s), dim3(kThreads), 0, st, part, G, 32, s, s, G64, s,
-                     (float *)nullptr);
+  tn_reduce(part, G, 32, s, s, G64, s, nullptr, st);
 }
 
 long long tn_rows(long long n, int G) {
@@ -1789,8 +1825,7 @@ void gemm_tn(const bf16 *P, long long ldp, int p, const float *U, long long ldu,
   } else {
     gemm_tn16<bf16>(P, ldp, p, pPad, U, ldu, s, n, G, rows, part, st);
   }
-  hipLaunchKernelGGL(k_gemm_tn_reduce, dim3(p), dim3(kThreads), 0, st, part, G, pPad, p, s, C64,
-                     ldc, C32);
+  tn_reduce(part, G, pPad, p, s, C64, ldc, C32, st);
 }
 
 void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu, int s,
@@ -1808,8 +1843,7 @@ void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu
     hipLaunchKernelGGL(k_gemm_tn, dim3(G), dim3(kThreads), 0, st, P, ldp, p, pPad, U, ldu, s, n,
                        rows, part);
   }
-  hipLaunchKernelGGL(k_gemm_tn_reduce, dim3(p), dim3(kThreads), 0, st, part, G, pPad, p, s, C64,
-                     ldc, C32);
+  tn_reduce(part, G, pPad, p, s, C64, ldc, C32, st);
 }
 
 template <typename TP, typename TO>

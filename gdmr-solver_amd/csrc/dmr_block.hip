@@ -1390,7 +1390,11 @@ __global__ __launch_bounds__(D * D) void k_chol_qr_el(const double *__restrict__
 // BWT: the bandwidth as a compile-time constant (16) for the unrolled reflector application, or
 // 0 for a runtime bandwidth.  Both compute the same bits: rows of the window past the matrix are
 // zero, so a reflector padded to the full bandwidth adds exact zeros.
-template <int BWT>
+// STG (BWT = 16 only): the register path described in the kernel, with the rows entering the
+// window fetched kLsqChunk steps at a time, one chunk ahead, through an LDS stage.
+constexpr int kLsqChunk = 8;
+
+template <int BWT, bool STG>
 __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H, int rows,
                                                     int cols, int bw,
                                                     const double *__restrict__ T, int s,
@@ -1400,6 +1404,8 @@ __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H
                                                     double *__restrict__ Y,
                                                     float *__restrict__ Yr32,
                                                     float *__restrict__ ZH32) {
+  // Right-hand-side columns that can be nonzero: s, which equals the bandwidth (16 when BWT = 16).
+  constexpr int SW = BWT > 0 && BWT <= 16 ? 16 : 32;
   extern __shared__ double sm[];
   __shared__ double s_tau, s_rmax;
   const int tid = threadIdx.x, bd = blockDim.x;
@@ -1408,90 +1414,167 @@ __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H
   auto src = [&](int r, int col) -> double {
     return col < cols ? H[(long long)col * rows + r] : T[(long long)(col - cols) * rows + r];
   };
-  for (int e = tid; e < W * nc; e += bd) {
-    const int r = e / nc, col = e % nc;
-    win[e] = r >= rows ? 0.0 : src(r, col);
-  }
-  __syncthreads();
-  for (int c = 0; c < cols; ++c) {
-    const int L = (c + bw < rows - 1 ? c + bw : rows - 1) - c;  // reflector tail length
-    double *rowc = win + (c % W) * nc;
-    // The row entering the window after this step, fetched now.
-    const int rn = c + W;
-    const double incoming = (tid < nc && rn < rows) ? src(rn, tid) : 0.0;
-    if (tid < 64) {
-      double x = 0.0;
-      if (tid >= 1 && tid <= L) x = win[((c + tid) % W) * nc + c];
-      double xn2 = x * x;
+  if constexpr (STG) {
+    // Register path (BWT = 16, cols + s <= blockDim.x): thread tid owns column tid and holds its
+    // window rows c .. c + 16 in registers (x[0] = row c).  Per step: the owner of column c
+    // forms the reflector from its registers and posts it (v and tau, double-buffered by step
+    // parity) in LDS; one barrier; every later column takes the reflector, writes its final
+    // row c to R or Qt and shifts its window up by one row, the new row coming from the stage.
+    double x[BWT + 1];
+    const int col = tid;
+    const bool own = col < nc;
 #pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) xn2 = xn2 + __shfl_xor(xn2, off, 64);
-      const double alpha = rowc[c];
-      double tau = 0.0, sc = 0.0, beta = alpha;
-      if (xn2 != 0.0) {
-        beta = -copysign(sqrt(fma(alpha, alpha, xn2)), alpha);
-        tau = (beta - alpha) / beta;
-        sc = 1.0 / (alpha - beta);
+    for (int l = 0; l <= BWT; ++l) x[l] = (own && l < rows) ? src(l, col) : 0.0;
+    double *stage = sm;                       // kLsqChunk x nc
+    double *vbuf = sm + kLsqChunk * nc;       // 2 x 32: v[0 .. BWT - 1], tau at [BWT]
+    double pre[kLsqChunk];
+#pragma unroll
+    for (int q = 0; q < kLsqChunk; ++q) pre[q] = (own && W + q < rows) ? src(W + q, col) : 0.0;
+    for (int cb = 0; cb < cols; cb += kLsqChunk) {
+      // Rows cb + W + q for the steps cb + q (each thread reads back only its own column, so no
+      // barrier guards the stage); then the next chunk's rows are fetched.
+#pragma unroll
+      for (int q = 0; q < kLsqChunk; ++q)
+        if (own) stage[q * nc + col] = pre[q];
+#pragma unroll
+      for (int q = 0; q < kLsqChunk; ++q) {
+        const int r = cb + kLsqChunk + W + q;
+        pre[q] = (own && r < rows) ? src(r, col) : 0.0;
       }
-      if (xn2 != 0.0 && tid >= 1 && tid <= L) win[((c + tid) % W) * nc + c] = x * sc;
-      if (tid == 0) {
-        rowc[c] = beta;
-        s_tau = tau;
-      }
-    }
-    __syncthreads();
-    const double tau = s_tau;
-    if (tau != 0.0) {
-      const int c0 = c % W;
-      for (int col = c + 1 + tid; col < nc; col += bd) {
-        if constexpr (BWT > 0) {
-          // All loads of the column's window entries issued before the sums.
-          double vr[BWT], xr[BWT];
+      const int ce = cb + kLsqChunk < cols ? cb + kLsqChunk : cols;
+      for (int c = cb; c < ce; ++c) {
+        double *vb = vbuf + (c & 1) * 32;
+        if (col == c) {
+          const int L = (c + bw < rows - 1 ? c + bw : rows - 1) - c;  // reflector tail length
+          double xn2 = 0.0;
 #pragma unroll
-          for (int l = 1; l <= BWT; ++l) {
-            const int sl = c0 + l < W ? c0 + l : c0 + l - W;
-            vr[l - 1] = win[sl * nc + c];
-            xr[l - 1] = win[sl * nc + col];
+          for (int l = 1; l <= BWT; ++l)
+            if (l <= L) xn2 = fma(x[l], x[l], xn2);
+          const double alpha = x[0];
+          double tau = 0.0, sc = 0.0, beta = alpha;
+          if (xn2 != 0.0) {
+            beta = -copysign(sqrt(fma(alpha, alpha, xn2)), alpha);
+            tau = (beta - alpha) / beta;
+            sc = 1.0 / (alpha - beta);
           }
-          double w = rowc[col];
 #pragma unroll
-          for (int l = 0; l < BWT; ++l) w = fma(vr[l], xr[l], w);
-          const double tw = tau * w;
-          rowc[col] = rowc[col] - tw;
+          for (int l = 1; l <= BWT; ++l) vb[l - 1] = (xn2 != 0.0 && l <= L) ? x[l] * sc : 0.0;
+          vb[BWT] = tau;
+          R[(long long)c * cols + c] = beta;
+        }
+        __syncthreads();
+        if (own && col > c) {
+          const double tau = vb[BWT];
+          if (tau != 0.0) {
+            double v[BWT];
 #pragma unroll
-          for (int l = 1; l <= BWT; ++l) {
-            const int sl = c0 + l < W ? c0 + l : c0 + l - W;
-            win[sl * nc + col] = fma(-tw, vr[l - 1], xr[l - 1]);
+            for (int l = 0; l < BWT; ++l) v[l] = vb[l];
+            double w = x[0];
+#pragma unroll
+            for (int l = 0; l < BWT; ++l) w = fma(v[l], x[l + 1], w);
+            const double tw = tau * w;
+            x[0] = x[0] - tw;
+#pragma unroll
+            for (int l = 0; l < BWT; ++l) x[l + 1] = fma(-tw, v[l], x[l + 1]);
           }
-        } else {
-          double w = rowc[col];
-          for (int l = 1; l <= L; ++l) {
-            const int sl = c0 + l < W ? c0 + l : c0 + l - W;
-            const double *rl = win + sl * nc;
-            w = fma(rl[c], rl[col], w);
-          }
-          const double tw = tau * w;
-          rowc[col] = rowc[col] - tw;
-          for (int l = 1; l <= L; ++l) {
-            const int sl = c0 + l < W ? c0 + l : c0 + l - W;
-            double *rl = win + sl * nc;
-            rl[col] = fma(-tw, rl[c], rl[col]);
-          }
+          if (col < cols)
+            R[(long long)col * cols + c] = x[0];
+          else
+            Qt[(long long)c * 32 + (col - cols)] = x[0];
+#pragma unroll
+          for (int l = 0; l < BWT; ++l) x[l] = x[l + 1];
+          x[BWT] = stage[(c - cb) * nc + col];
         }
       }
     }
     __syncthreads();
-    for (int col = tid; col < nc; col += bd) {
-      const double v = rowc[col];
-      if (col >= c && col < cols) R[(long long)col * cols + c] = v;
-      if (col >= cols) Qt[(long long)c * 32 + (col - cols)] = v;
-      rowc[col] = col == tid ? incoming : (rn < rows ? src(rn, col) : 0.0);
+  } else {
+    for (int e = tid; e < W * nc; e += bd) {
+      const int r = e / nc, col = e % nc;
+      win[e] = r >= rows ? 0.0 : src(r, col);
     }
     __syncthreads();
+    for (int c = 0; c < cols; ++c) {
+      const int L = (c + bw < rows - 1 ? c + bw : rows - 1) - c;  // reflector tail length
+      double *rowc = win + (c % W) * nc;
+      // The row entering the window after this step, fetched now.
+      const int rn = c + W;
+      const double incoming = (tid < nc && rn < rows) ? src(rn, tid) : 0.0;
+      if (tid < 64) {
+        double x = 0.0;
+        if (tid >= 1 && tid <= L) x = win[((c + tid) % W) * nc + c];
+        double xn2 = x * x;
+  #pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) xn2 = xn2 + __shfl_xor(xn2, off, 64);
+        const double alpha = rowc[c];
+        double tau = 0.0, sc = 0.0, beta = alpha;
+        if (xn2 != 0.0) {
+          beta = -copysign(sqrt(fma(alpha, alpha, xn2)), alpha);
+          tau = (beta - alpha) / beta;
+          sc = 1.0 / (alpha - beta);
+        }
+        if (xn2 != 0.0 && tid >= 1 && tid <= L) win[((c + tid) % W) * nc + c] = x * sc;
+        if (tid == 0) {
+          rowc[c] = beta;
+          s_tau = tau;
+        }
+      }
+      __syncthreads();
+      const double tau = s_tau;
+      if (tau != 0.0) {
+        const int c0 = c % W;
+        for (int col = c + 1 + tid; col < nc; col += bd) {
+          if constexpr (BWT > 0) {
+            // All loads of the column's window entries issued before the sums.
+            double vr[BWT], xr[BWT];
+  #pragma unroll
+            for (int l = 1; l <= BWT; ++l) {
+              const int sl = c0 + l < W ? c0 + l : c0 + l - W;
+              vr[l - 1] = win[sl * nc + c];
+              xr[l - 1] = win[sl * nc + col];
+            }
+            double w = rowc[col];
+  #pragma unroll
+            for (int l = 0; l < BWT; ++l) w = fma(vr[l], xr[l], w);
+            const double tw = tau * w;
+            rowc[col] = rowc[col] - tw;
+  #pragma unroll
+            for (int l = 1; l <= BWT; ++l) {
+              const int sl = c0 + l < W ? c0 + l : c0 + l - W;
+              win[sl * nc + col] = fma(-tw, vr[l - 1], xr[l - 1]);
+            }
+          } else {
+            double w = rowc[col];
+            for (int l = 1; l <= L; ++l) {
+              const int sl = c0 + l < W ? c0 + l : c0 + l - W;
+              const double *rl = win + sl * nc;
+              w = fma(rl[c], rl[col], w);
+            }
+            const double tw = tau * w;
+            rowc[col] = rowc[col] - tw;
+            for (int l = 1; l <= L; ++l) {
+              const int sl = c0 + l < W ? c0 + l : c0 + l - W;
+              double *rl = win + sl * nc;
+              rl[col] = fma(-tw, rl[c], rl[col]);
+            }
+          }
+        }
+      }
+      __syncthreads();
+      for (int col = tid; col < nc; col += bd) {
+        const double v = rowc[col];
+        if (col >= c && col < cols) R[(long long)col * cols + c] = v;
+        if (col >= cols) Qt[(long long)c * 32 + (col - cols)] = v;
+        rowc[col] = col == tid ? incoming : (rn < rows ? src(rn, col) : 0.0);
+      }
+      __syncthreads();
+    }
   }
   // Back substitution R Y = Qt in 32-row blocks; a negligible pivot leaves its unknown at zero.
   double *Tl = sm;              // cols x 32: right-hand sides, updated in place
   double *Dg = sm + cols * 32;  // 32 x 33: diagonal block of R (row-major, padded)
   __shared__ double Yb[32 * 32];
+  __shared__ double Rinv[32];
   for (int e = tid; e < cols * 32; e += bd) Tl[e] = (e & 31) < s ? Qt[e] : 0.0;
   if (tid < 64) {
     double m = 0.0;
@@ -1506,7 +1589,11 @@ __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H
     const int i0 = i1 - 32 > 0 ? i1 - 32 : 0, b = i1 - i0;
     for (int e = tid; e < b * b; e += bd) {
       const int r = e % b, l = e / b;  // R(i0 + r, i0 + l)
-      Dg[r * 33 + l] = l >= r ? R[(long long)(i0 + l) * cols + i0 + r] : 0.0;
+      const double v = l >= r ? R[(long long)(i0 + l) * cols + i0 + r] : 0.0;
+      Dg[r * 33 + l] = v;
+      // The pivots' reciprocals (used only where the pivot is not negligible), here so that the
+      // solve below multiplies instead of dividing on its dependency chain.
+      if (l == r) Rinv[r] = 1.0 / v;
     }
     __syncthreads();
     if (tid < 32) {
@@ -1523,34 +1610,36 @@ __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H
           for (int l = r + 1; l < 32; ++l)
             if (l < b) a = fma(-Dg[r * 33 + l], yv[l], a);
           const double d = Dg[r * 33 + r];
-          yv[r] = (jr < s && fabs(d) > piv) ? a / d : 0.0;
+          yv[r] = (jr < s && fabs(d) > piv) ? a * Rinv[r] : 0.0;
           Yb[r * 32 + jr] = yv[r];
           Y[(long long)(i0 + r) * 32 + jr] = yv[r];
         }
       }
     }
     __syncthreads();
-    // Rows above the block: Tl[r][:] -= sum_l R(r, i0 + l) Y(i0 + l, :), a thread per row (the
-    // loads of a column of R are contiguous across threads; Y is read by broadcast).
+    // Rows above the block: Tl[r][:] -= sum_l R(r, i0 + l) Y(i0 + l, :) over the SW columns that
+    // can be nonzero, a thread per row (the loads of a column of R are contiguous across threads;
+    // Y is read by broadcast).
     for (int r = tid; r < i0; r += bd) {
-      double a[32];
+      double a[SW];
 #pragma unroll
-      for (int jr = 0; jr < 32; ++jr) a[jr] = Tl[r * 32 + jr];
-      for (int l0 = 0; l0 < b; l0 += 8) {
-        double rv[8];  // eight loads of R in flight
+      for (int jr = 0; jr < SW; ++jr) a[jr] = Tl[r * 32 + jr];
+#pragma unroll 1
+      for (int l0 = 0; l0 < 32; l0 += 16) {
+        double rv[16];  // sixteen loads of R in flight
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
+        for (int q = 0; q < 16; ++q)
           rv[q] = l0 + q < b ? R[(long long)(i0 + l0 + q) * cols + r] : 0.0;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < 16; ++q) {
           if (l0 + q < b) {
 #pragma unroll
-            for (int jr = 0; jr < 32; ++jr) a[jr] = fma(-rv[q], Yb[(l0 + q) * 32 + jr], a[jr]);
+            for (int jr = 0; jr < SW; ++jr) a[jr] = fma(-rv[q], Yb[(l0 + q) * 32 + jr], a[jr]);
           }
         }
       }
 #pragma unroll
-      for (int jr = 0; jr < 32; ++jr) Tl[r * 32 + jr] = a[jr];
+      for (int jr = 0; jr < SW; ++jr) Tl[r * 32 + jr] = a[jr];
     }
     __syncthreads();
   }
@@ -1558,29 +1647,35 @@ __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H
     const int j = e & 31;
     Yr32[e] = j < s ? (float)(Y[e] * rho[j]) : 0.0f;
   }
-  // ZH = H Y with Y staged in LDS.
+  // ZH = H Y with Y staged in LDS: thread (r, g) forms columns 8g .. 8g + 7 of row r (SW / 8
+  // groups), sixteen loads of H in flight.
   double *Yl = sm;
   for (int e = tid; e < cols * 32; e += bd) Yl[e] = Y[e];
   __syncthreads();
-  for (int r = tid; r < rows; r += bd) {
-    double z[32];
+  constexpr int NG = SW / 8;
+  for (int e = tid; e < rows * NG; e += bd) {
+    const int r = e % rows, g = e / rows;
+    double z[8];
 #pragma unroll
-    for (int j = 0; j < 32; ++j) z[j] = 0.0;
-    for (int c0 = 0; c0 < cols; c0 += 8) {
-      double hv[8];  // eight loads of H in flight
+    for (int j = 0; j < 8; ++j) z[j] = 0.0;
+    for (int c0 = 0; c0 < cols; c0 += 16) {
+      double hv[16];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) hv[q] = c0 + q < cols ? H[(long long)(c0 + q) * rows + r] : 0.0;
+      for (int q = 0; q < 16; ++q)
+        hv[q] = c0 + q < cols ? H[(long long)(c0 + q) * rows + r] : 0.0;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
+      for (int q = 0; q < 16; ++q) {
         if (c0 + q < cols) {
 #pragma unroll
-          for (int j = 0; j < 32; ++j) z[j] = fma(hv[q], Yl[(c0 + q) * 32 + j], z[j]);
+          for (int j = 0; j < 8; ++j) z[j] = fma(hv[q], Yl[(c0 + q) * 32 + 8 * g + j], z[j]);
         }
       }
     }
 #pragma unroll
-    for (int j = 0; j < 32; ++j) ZH32[r * 32 + j] = j < s ? (float)z[j] : 0.0f;
+    for (int j = 0; j < 8; ++j) ZH32[r * 32 + 8 * g + j] = 8 * g + j < s ? (float)z[j] : 0.0f;
   }
+  for (int e = tid; e < rows * (32 - SW); e += bd)  // columns no group forms
+    ZH32[(e / (32 - SW)) * 32 + SW + e % (32 - SW)] = 0.0f;
 }
 
 // Converts the k selected right singular vectors C (cols x k, ld cols, FP64) to FP32 row-major
@@ -1645,8 +1740,9 @@ inline unsigned grid1(long long n) { return (unsigned)((n + kThreads - 1) / kThr
 void prepare() {
   // Once per process, before any stream capture: the LDS limits of the one-workgroup kernels.
   static const bool done = [] {
-    dmr::allow_max_lds(reinterpret_cast<const void *>(k_lsq_block<0>));
-    dmr::allow_max_lds(reinterpret_cast<const void *>(k_lsq_block<16>));
+    dmr::allow_max_lds(reinterpret_cast<const void *>(k_lsq_block<0, false>));
+    dmr::allow_max_lds(reinterpret_cast<const void *>(k_lsq_block<16, false>));
+    dmr::allow_max_lds(reinterpret_cast<const void *>(k_lsq_block<16, true>));
     dmr::prepare_dense();
     return true;
   }();
@@ -1944,22 +2040,37 @@ void chol_qr(const double *G, int s, double relTol, float *Sinv32, double *Sacc,
                        first, Hdst, ldh, brk);
 }
 
-size_t lsq_block_lds(int cols, int bw, int s) {
-  const size_t win = (size_t)(bw + 1) * (cols + s), back = (size_t)cols * 32 + 32 * 33;
+// LDS of the least squares on the register path (reg) or the window path.
+size_t lsq_lds(int cols, int bw, int s, bool reg) {
+  const size_t win = reg ? (size_t)kLsqChunk * (cols + s) + 64 : (size_t)(bw + 1) * (cols + s);
+  const size_t back = (size_t)cols * 32 + 32 * 33;
   return sizeof(double) * (win > back ? win : back);
+}
+// The register path: bandwidth 16, one thread per column of [H | T], and its LDS fits.
+bool lsq_staged(int cols, int bw, int s) {
+  return bw == 16 && s <= 16 && cols + s <= 1024 && lsq_lds(cols, bw, s, true) <= 140 * 1024;
+}
+
+size_t lsq_block_lds(int cols, int bw, int s) {
+  return lsq_lds(cols, bw, s, lsq_staged(cols, bw, s));
 }
 
 void lsq_block(const double *H, int rows, int cols, int bw, const double *T, int s,
                const double *rho, double *R, double *Qt, double *Y, float *Yr32, float *ZH32,
                hipStream_t st) {
   const size_t lds = lsq_block_lds(cols, bw, s);
+  const bool stg = lsq_staged(cols, bw, s);
   prepare();
-  if (bw == 16)
-    hipLaunchKernelGGL(k_lsq_block<16>, dim3(1), dim3(1024), lds, st, H, rows, cols, bw, T, s, rho,
-                       R, Qt, Y, Yr32, ZH32);
+#define DMR_LSQ(B, S)                                                                            \
+  hipLaunchKernelGGL((k_lsq_block<B, S>), dim3(1), dim3(1024), lds, st, H, rows, cols, bw, T, s, \
+                     rho, R, Qt, Y, Yr32, ZH32)
+  if (stg)
+    DMR_LSQ(16, true);
+  else if (bw == 16 && s <= 16)
+    DMR_LSQ(16, false);
   else
-    hipLaunchKernelGGL(k_lsq_block<0>, dim3(1), dim3(1024), lds, st, H, rows, cols, bw, T, s, rho,
-                       R, Qt, Y, Yr32, ZH32);
+    DMR_LSQ(0, false);
+#undef DMR_LSQ
 }
 
 void pack_coef(const double *C, int cols, int k, float *C32, hipStream_t st) {

@@ -73,6 +73,7 @@ struct dmrBlockContext {
   double *R64 = nullptr;
   double *H = nullptr, *T = nullptr, *Hw = nullptr, *Tw = nullptr, *Y = nullptr, *Hred = nullptr;
   float *C32 = nullptr, *Yr32 = nullptr, *ZH32 = nullptr, *Sinv32 = nullptr, *Cz32 = nullptr;
+  float *Cpair = nullptr;  // coefficients of the paired residual and solution update
   float *part = nullptr;
   size_t partFloats = 0;
   double *G64 = nullptr, *Sacc = nullptr, *Csvd = nullptr, *sig = nullptr;
@@ -199,7 +200,7 @@ Sizes sizes(long long n, int mb, int k, int sMax, int G, long long ld) {
                               (size_t)colsMax * 32 + (size_t)rRed * cRed + 32 * 32 * 2 +
                               (size_t)cRed * 32 + cRed + 2 + 8 * 64 +
                               (size_t)dmr::blk::colnorm_blocks(n) * 32) +
-            sizeof(float) * ((size_t)rowsMax * 32 * 2 + (size_t)colsMax * 32 + 32 * 32 +
+            sizeof(float) * ((size_t)rowsMax * 32 * 3 + (size_t)colsMax * 32 + 32 * 32 +
                              (size_t)cRed * 32 + (size_t)G * pPadMax * 32) +
             sizeof(int) * 64 + 64 * 32;
   return z;
@@ -280,6 +281,7 @@ int ctx_alloc(dmrBlockContext *c) {
   c->cpart = m.take<double>((size_t)dmr::blk::colnorm_blocks(c->n) * 32);
   c->C32 = m.take<float>((size_t)rowsMax * 32);
   c->ZH32 = m.take<float>((size_t)rowsMax * 32);
+  c->Cpair = m.take<float>((size_t)rowsMax * 32);
   c->Yr32 = m.take<float>((size_t)colsMax * 32);
   c->Sinv32 = m.take<float>(32 * 32);
   c->Cz32 = m.take<float>((size_t)cRed * 32);
@@ -596,18 +598,33 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
     dmr::blk::lsq_block(c->H, rows, cols, s, c->T, s, c->rho, c->Hw, c->Tw, c->Y, c->Yr32,
                         c->ZH32, st);
   });
-  timedf(c, st, DMR_PH_COMBO, pb * rows + vb * 2 * s, 2.0 * n * rows * s, [&] {
-    nn(c, P, rows, c->ZH32, c->R, 1.0f, -1.0f, c->R, s, st);
-  });
+  // R -= P (H Y) and Xa += [V_0 .. V_{mb-1}] Y: over a bf16 basis with the matrix-core update,
+  // one pass over P for both (gemm_nn_pair); else two.
+  bool paired = false;
+  if constexpr (std::is_same<TP, bf16>::value) paired = s <= 16 && c->tune.nnBasis >= 100;
+  if (paired) {
+    if constexpr (std::is_same<TP, bf16>::value) {
+      timedf(c, st, DMR_PH_COMBO, pb * rows + vb * 4 * s, 2.0 * n * (rows + cols - p0) * s, [&] {
+        dmr::blk::pack_pair(c->ZH32, c->Yr32, p0, cols, rows, s, c->Cpair, st);
+        dmr::blk::gemm_nn_pair(P, ld, rows, c->Cpair, c->R, c->Xa, ld, s, n, c->tune.nnBasis,
+                               st);
+      });
+    }
+  } else {
+    timedf(c, st, DMR_PH_COMBO, pb * rows + vb * 2 * s, 2.0 * n * rows * s, [&] {
+      nn(c, P, rows, c->ZH32, c->R, 1.0f, -1.0f, c->R, s, st);
+    });
+  }
   timed(c, st, DMR_PH_SCALE, vb * s,
         [&] { dmr::blk::colnorms<float>(c->R, ld, s, n, c->cpart, c->q, 1, st); });
   if (deflate)
     timedf(c, st, DMR_PH_ZGEMM, vb * (k + 2 * s), 2.0 * n * k * s, [&] {
       nn(c, c->Z, k, c->Yr32, c->Xa, 1.0f, 1.0f, c->Xa, s, st);
     });
-  timedf(c, st, DMR_PH_COMBO, pb * mb * s + vb * 2 * s, 2.0 * n * mb * s * s, [&] {
-    nn(c, V0, mb * s, c->Yr32 + (long long)p0 * 32, c->Xa, 1.0f, 1.0f, c->Xa, s, st);
-  });
+  if (!paired)
+    timedf(c, st, DMR_PH_COMBO, pb * mb * s + vb * 2 * s, 2.0 * n * mb * s * s, [&] {
+      nn(c, V0, mb * s, c->Yr32 + (long long)p0 * 32, c->Xa, 1.0f, 1.0f, c->Xa, s, st);
+    });
   // Subspace refresh, part 2: Z <- normalize([Z V_0 .. V_{kr-1}] C) once the SVD has joined.
   if (refreshed) {
     hipStreamWaitEvent(st, c->evJoin, 0);

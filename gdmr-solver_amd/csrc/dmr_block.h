@@ -70,6 +70,14 @@ void gemm_nn(const bf16 *P, long long ldp, int p, const float *C, const float *I
              float a, float b, float *Out, long long ldo, int s, long long n, int un,
              hipStream_t st, float *gram = nullptr);
 int gemm_nn_gram_blocks(long long n, int un);
+// One pass over a bf16 basis for two blocks of s <= 16 columns (ld ld): X1 -= P[:, 0:p] C[:, 0:16]
+// and X2 -= P[:, 0:p] C[:, 16:32] (C row-major p x 32; un >= 100 as for gemm_nn).
+void gemm_nn_pair(const bf16 *P, long long ldp, int p, const float *C, float *X1, float *X2,
+                  long long ld, int s, long long n, int un, hipStream_t st);
+// Cp (rows x 32) = [ZH32[:, 0:16] | -Yr32 on rows p0 .. cols - 1, zero elsewhere] for
+// gemm_nn_pair: the residual and solution updates of a cycle in one pass.
+void pack_pair(const float *ZH32, const float *Yr32, int p0, int cols, int rows, int s, float *Cp,
+               hipStream_t st);
 void gram_reduce(const float *part, int G, int s, double *G64, hipStream_t st);
 void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *In, long long ldi,
              float a, float b, bf16 *Out, long long ldo, int s, long long n, int un,

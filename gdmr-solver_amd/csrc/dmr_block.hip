@@ -744,7 +744,8 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nn_b(const bf16 *__restrict__
                                                         int p, const float *__restrict__ C,
                                                         const float *In, long long ldi, float a,
                                                         float b, TO *Out, long long ldo, int s,
-                                                        long long n, float *__restrict__ gram) {
+                                                        long long n, float *__restrict__ gram,
+                                                        const float *In2, TO *Out2) {
   extern __shared__ uint16_t Cl[];  // [hi | lo][16 NT output columns][KP]; then the epilogue
   const int KP = (p + 15) & ~15;
   uint16_t *Ch = Cl, *Clo = Cl + (size_t)16 * NT * KP;
@@ -852,25 +853,29 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nn_b(const bf16 *__restrict__
 #pragma unroll
       for (int cc = 0; cc < 8; ++cc) {
         const int jl = 2 * cc + h, j = 16 * t + jl;
-        if (j >= s) continue;
+        // With Out2 (NT = 2), output columns 16 .. 31 are columns 0 .. 15 of In2 / Out2, and
+        // each of the two blocks has s - 16 columns.
+        if (NT == 2 && Out2 ? jl >= s - 16 : j >= s) continue;
+        const float *inc = (NT == 2 && t == 1 && Out2) ? In2 + jl * ldi : In + j * ldi;
+        TO *outc = (NT == 2 && t == 1 && Out2) ? Out2 + jl * ldo : Out + j * ldo;
         const float4 pv = *reinterpret_cast<const float4 *>(tile + jl * stride + q4);
         const float pr[4] = {pv.x, pv.y, pv.z, pv.w};
         float in[4] = {0.f, 0.f, 0.f, 0.f};
         if (a != 0.0f) {
           if (cnt == 4) {
-            const float4 iv = *reinterpret_cast<const float4 *>(In + j * ldi + row);
+            const float4 iv = *reinterpret_cast<const float4 *>(inc + row);
             in[0] = iv.x;
             in[1] = iv.y;
             in[2] = iv.z;
             in[3] = iv.w;
           } else {
-            for (int e = 0; e < cnt; ++e) in[e] = In[j * ldi + row + e];
+            for (int e = 0; e < cnt; ++e) in[e] = inc[row + e];
           }
         }
         float o[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) o[e] = a != 0.0f ? a * in[e] + b * pr[e] : b * pr[e];
-        store_rows4(Out + j * ldo + row, o, cnt);
+        store_rows4(outc + row, o, cnt);
         if (NT == 1 && gram)  // the output as stored (rows past n stay zero) for the Gram matrix
           *reinterpret_cast<float4 *>(tile + jl * stride + q4) = make_float4(o[0], o[1], o[2], o[3]);
       }
@@ -1678,6 +1683,21 @@ __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H
     ZH32[(e / (32 - SW)) * 32 + SW + e % (32 - SW)] = 0.0f;
 }
 
+// Coefficients of the paired update (gemm_nn_pair), rows x 32 row-major: columns 0 .. 15 are
+// ZH32's, columns 16 .. 16 + s - 1 are -Yr32's on rows p0 .. cols - 1 and zero elsewhere.
+__global__ void k_pack_pair(const float *__restrict__ ZH32, const float *__restrict__ Yr32, int p0,
+                            int cols, int rows, int s, float *__restrict__ Cp) {
+  const int e = blockIdx.x * kThreads + threadIdx.x;
+  if (e >= rows * 32) return;
+  const int r = e >> 5, j = e & 31;
+  float v = 0.0f;
+  if (j < 16)
+    v = ZH32[e];
+  else if (j - 16 < s && r >= p0 && r < cols)
+    v = -Yr32[r * 32 + (j - 16)];
+  Cp[e] = v;
+}
+
 // Converts the k selected right singular vectors C (cols x k, ld cols, FP64) to FP32 row-major
 // cols x 32.
 __global__ void k_pack_coef(const double *__restrict__ C, int cols, int k, float *__restrict__ C32) {
@@ -1970,6 +1990,30 @@ void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *
              hipStream_t st) {
   gemm_nn_t<float, float>(P, ldp, p, C, In, ldi, a, b, Out, ldo, s, n, un, st);
 }
+void gemm_nn_pair(const bf16 *P, long long ldp, int p, const float *C, float *X1, float *X2,
+                  long long ld, int s, long long n, int un, hipStream_t st) {
+  const int RB = un >= 101 ? 2 : 1;
+  const long long waves = (n + 128 * RB - 1) / (128 * RB);
+  const unsigned grid = (unsigned)((waves + 3) / 4);
+  const int KP = (p + 15) & ~15;
+  const size_t epi = sizeof(float) * 4 * kNnbTile;
+  const size_t l2 = sizeof(uint16_t) * 2 * 2 * 16 * KP;
+  const size_t lds = l2 > epi ? l2 : epi;
+  const int sp = 16 + s;  // output columns: X1's s, then X2's from column 16
+  if (RB == 2)
+    hipLaunchKernelGGL((k_gemm_nn_b<2, float, 2>), dim3(grid), dim3(kThreads), lds, st, P, ldp, p,
+                       C, X1, ld, 1.0f, -1.0f, X1, ld, sp, n, nullptr, X2, X2);
+  else
+    hipLaunchKernelGGL((k_gemm_nn_b<2, float, 1>), dim3(grid), dim3(kThreads), lds, st, P, ldp, p,
+                       C, X1, ld, 1.0f, -1.0f, X1, ld, sp, n, nullptr, X2, X2);
+}
+
+void pack_pair(const float *ZH32, const float *Yr32, int p0, int cols, int rows, int s, float *Cp,
+               hipStream_t st) {
+  hipLaunchKernelGGL(k_pack_pair, dim3(grid1((long long)rows * 32)), dim3(kThreads), 0, st, ZH32,
+                     Yr32, p0, cols, rows, s, Cp);
+}
+
 int gemm_nn_gram_blocks(long long n, int un) {
   const int RB = un >= 101 ? 2 : 1;
   const long long waves = (n + 128 * RB - 1) / (128 * RB);
@@ -1990,16 +2034,20 @@ void gemm_nn(const bf16 *P, long long ldp, int p, const float *C, const float *I
     const size_t lds1 = l1 > epi ? l1 : epi, lds2 = l2 > epi ? l2 : epi;
     if (s <= 16 && RB == 2)
       hipLaunchKernelGGL((k_gemm_nn_b<1, float, 2>), dim3(grid), dim3(kThreads), lds1, st, P, ldp,
-                         p, C, In, ldi, a, b, Out, ldo, s, n, gram);
+                         p, C, In, ldi, a, b, Out, ldo, s, n, gram, nullptr,
+                         (float *)nullptr);
     else if (s <= 16)
       hipLaunchKernelGGL((k_gemm_nn_b<1, float, 1>), dim3(grid), dim3(kThreads), lds1, st, P, ldp,
-                         p, C, In, ldi, a, b, Out, ldo, s, n, gram);
+                         p, C, In, ldi, a, b, Out, ldo, s, n, gram, nullptr,
+                         (float *)nullptr);
     else if (RB == 2)
       hipLaunchKernelGGL((k_gemm_nn_b<2, float, 2>), dim3(grid), dim3(kThreads), lds2, st, P, ldp,
-                         p, C, In, ldi, a, b, Out, ldo, s, n, nullptr);
+                         p, C, In, ldi, a, b, Out, ldo, s, n, nullptr, nullptr,
+                         (float *)nullptr);
     else
       hipLaunchKernelGGL((k_gemm_nn_b<2, float, 1>), dim3(grid), dim3(kThreads), lds2, st, P, ldp,
-                         p, C, In, ldi, a, b, Out, ldo, s, n, nullptr);
+                         p, C, In, ldi, a, b, Out, ldo, s, n, nullptr, nullptr,
+                         (float *)nullptr);
     return;
   }
   gemm_nn_t<bf16, float>(P, ldp, p, C, In, ldi, a, b, Out, ldo, s, n, un, st);

@@ -87,6 +87,7 @@ struct dmrBlockContext {
   int cholPasses = 0;  // Cholesky QR passes per block normalisation; 0: by storage type
   bool fusedGram = true;  // first Gram matrix from the update's epilogue where possible
   int spmmForce = 0;      // SpMM variant pinned by DMR_BLOCK_SPMM (0: the tuned one)
+  bool foldT = true;      // R's projection in the last block step's pass where possible
   // Launch shapes chosen by the autotuner (per block width s), and whether it ran.
   struct Tuning {
     int tnRowsProj = 512, tnVarProj = 0;  // gemm_tn for projections (p > s)
@@ -510,6 +511,9 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
   const int rows = p0 + (mb + 1) * s, cols = p0 + mb * s;
   float *U = c->Ub;
   const int gp = gram_parts(c, s);  // Gram matrices from the update's epilogue, when possible
+  // R's projection folded into the last block step (bf16 basis, matrix-core projection, s = 16).
+  const bool foldT = std::is_same<TP, bf16>::value && s == 16 && mb >= 1 &&
+                     c->tune.tnVarProj == 3 && c->foldT;
   const double vb = 4.0 * (double)n;              // one FP32 column
   const double pb = (double)sizeof(TP) * (double)n;  // one basis column
   dmr::fill<double>(c->H, (long long)rows * cols, 0.0, st);
@@ -575,7 +579,18 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
     timedf(c, st, DMR_PH_SPMV, spmm_bytes(c, s) - (4.0 - sizeof(TP)) * n * s,
            2.0 * c->csr.nnz * s, [&] { spmm(c, Vj, U, s, st); });
     c->st.matvecs += s;
-    timedf(c, st, DMR_PH_GEMVT, pb * pj + vb * s, 2.0 * n * pj * s, [&] {
+    // The last block step also projects the cycle's residual block R on P[:, 0:pj] in the same
+    // pass (to T), leaving only the last block's projection for the end of the cycle.
+    const bool withT = foldT && j == mb - 1;
+    timedf(c, st, DMR_PH_GEMVT, pb * pj + vb * s * (withT ? 2 : 1),
+           2.0 * n * pj * s * (withT ? 2 : 1), [&] {
+      if constexpr (std::is_same<TP, bf16>::value) {
+        if (withT) {
+          dmr::blk::gemm_tn_pair(P, ld, pj, U, c->R, ld, n, tn_blocks(c, c->tune.tnRowsProj),
+                                 c->part, Hcol, rows, c->C32, c->T, rows, st);
+          return;
+        }
+      }
       tn(c, P, pj, U, s, Hcol, rows, c->C32, false, st);
     });
     timedf(c, st, DMR_PH_UPDATE, pb * pj + vb * 2 * s, 2.0 * n * pj * s, [&] {
@@ -590,10 +605,17 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
     cholqr2(c, U, s, Hcol + pj, rows, slot, st, gp);
     if (refreshed && j == kr - 1) fork_svd();
   }
-  // Full projection of the residual block, least squares, residual and solution updates.
-  timedf(c, st, DMR_PH_GEMVT, pb * rows + vb * s, 2.0 * n * rows * s, [&] {
-    tn(c, P, rows, c->R, s, c->T, rows, nullptr, false, st);
-  });
+  // Projection of the residual block on P (the part the last block step left), least squares,
+  // residual and solution updates.
+  if (foldT) {
+    timedf(c, st, DMR_PH_GEMVT, pb * s + vb * s, 2.0 * n * s * s, [&] {
+      tn(c, P + (long long)cols * ld, s, c->R, s, c->T + cols, rows, nullptr, false, st);
+    });
+  } else {
+    timedf(c, st, DMR_PH_GEMVT, pb * rows + vb * s, 2.0 * n * rows * s, [&] {
+      tn(c, P, rows, c->R, s, c->T, rows, nullptr, false, st);
+    });
+  }
   timed(c, st, DMR_PH_LSQ, 0.0, [&] {
     dmr::blk::lsq_block(c->H, rows, cols, s, c->T, s, c->rho, c->Hw, c->Tw, c->Y, c->Yr32,
                         c->ZH32, st);
@@ -832,6 +854,7 @@ int dmrBlockCreate(dmrBlockHandle *h, int device, int elements, int blockSteps, 
     if (v == 1 || v == 2) c->cholPasses = v;
   }
   if (const char *e = std::getenv("DMR_BLOCK_FUSED_GRAM")) c->fusedGram = e[0] != '0';
+  if (const char *e = std::getenv("DMR_BLOCK_FOLD_T")) c->foldT = e[0] != '0';
   if (const char *e = std::getenv("DMR_BLOCK_SPMM")) {
     const int v = std::atoi(e);
     for (int ok : {4, 8, 16, 104, 108, 116, 208, 216})

@@ -56,6 +56,12 @@ void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu
 void gemm_tn(const bf16 *P, long long ldp, int p, const float *U, long long ldu, int s,
              long long n, int G, int variant, float *part, double *C64, int ldc, float *C32,
              hipStream_t st);
+// Two 16-column blocks U and U2 (pitch ldu) against a bf16 P in one pass (the matrix-core
+// kernel): P^T U to C64a (ld ldca) and C32 (row-major p x 32, columns 16 .. 31 zero), P^T U2 to
+// C64b (ld ldcb).  part as for gemm_tn (pPad x 32 floats per row range).
+void gemm_tn_pair(const bf16 *P, long long ldp, int p, const float *U, const float *U2,
+                  long long ldu, long long n, int G, float *part, double *C64a, int ldca,
+                  float *C32, double *C64b, int ldcb, hipStream_t st);
 // Out = a In + b P[:, 0:p] C (C row-major p x 32; In unused when a == 0).  Out may alias In or P.
 // un (2, 4, 8 or 16): column pairs of P loaded ahead of their MFMAs.
 void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *In, long long ldi,

@@ -1406,12 +1406,13 @@ __global__ __launch_bounds__(D * D) void k_chol_qr_el(const double *__restrict__
 // BWT: the bandwidth as a compile-time constant (16) for the unrolled reflector application, or
 // 0 for a runtime bandwidth.  Both compute the same bits: rows of the window past the matrix are
 // zero, so a reflector padded to the full bandwidth adds exact zeros.
-// STG (BWT = 16 only): the register path described in the kernel, with the rows entering the
+// STG (BWT = 16 or 32): the register path described in the kernel, with the rows entering the
 // window fetched kLsqChunk steps at a time, one chunk ahead, through an LDS stage.
 constexpr int kLsqChunk = 8;
 
 template <int BWT, bool STG>
-__global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H, int rows,
+__global__ __launch_bounds__(BWT == 32 ? 512 : 1024) void k_lsq_block(const double *__restrict__ H,
+                                                                      int rows,
                                                     int cols, int bw,
                                                     const double *__restrict__ T, int s,
                                                     const double *__restrict__ rho,
@@ -1431,8 +1432,8 @@ __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H
     return col < cols ? H[(long long)col * rows + r] : T[(long long)(col - cols) * rows + r];
   };
   if constexpr (STG) {
-    // Register path (BWT = 16, cols + s <= blockDim.x): thread tid owns column tid and holds its
-    // window rows c .. c + 16 in registers (x[0] = row c).  Per step: the owner of column c
+    // Register path (BWT = 16 or 32, cols + s <= blockDim.x): thread tid owns column tid and
+    // holds its window rows c .. c + BWT in registers (x[0] = row c).  Per step: the owner of column c
     // forms the reflector from its registers and posts it (v and tau, double-buffered by step
     // parity) in LDS; one barrier; every later column takes the reflector, writes its final
     // row c to R or Qt and shifts its window up by one row, the new row coming from the stage.
@@ -1442,7 +1443,7 @@ __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H
 #pragma unroll
     for (int l = 0; l <= BWT; ++l) x[l] = (own && l < rows) ? src(l, col) : 0.0;
     double *stage = sm;                       // kLsqChunk x nc
-    double *vbuf = sm + kLsqChunk * nc;       // 2 x 32: v[0 .. BWT - 1], tau at [BWT]
+    double *vbuf = sm + kLsqChunk * nc;       // 2 x 64: v[0 .. BWT - 1], tau at [BWT]
     double pre[kLsqChunk];
 #pragma unroll
     for (int q = 0; q < kLsqChunk; ++q) pre[q] = (own && W + q < rows) ? src(W + q, col) : 0.0;
@@ -1459,7 +1460,7 @@ __global__ __launch_bounds__(1024) void k_lsq_block(const double *__restrict__ H
       }
       const int ce = cb + kLsqChunk < cols ? cb + kLsqChunk : cols;
       for (int c = cb; c < ce; ++c) {
-        double *vb = vbuf + (c & 1) * 32;
+        double *vb = vbuf + (c & 1) * 64;
         if (col == c) {
           const int L = (c + bw < rows - 1 ? c + bw : rows - 1) - c;  // reflector tail length
           double xn2 = 0.0;
@@ -1774,6 +1775,7 @@ void prepare() {
     dmr::allow_max_lds(reinterpret_cast<const void *>(k_lsq_block<0, false>));
     dmr::allow_max_lds(reinterpret_cast<const void *>(k_lsq_block<16, false>));
     dmr::allow_max_lds(reinterpret_cast<const void *>(k_lsq_block<16, true>));
+    dmr::allow_max_lds(reinterpret_cast<const void *>(k_lsq_block<32, true>));
     dmr::prepare_dense();
     return true;
   }();
@@ -2115,13 +2117,19 @@ void chol_qr(const double *G, int s, double relTol, float *Sinv32, double *Sacc,
 
 // LDS of the least squares on the register path (reg) or the window path.
 size_t lsq_lds(int cols, int bw, int s, bool reg) {
-  const size_t win = reg ? (size_t)kLsqChunk * (cols + s) + 64 : (size_t)(bw + 1) * (cols + s);
+  const size_t win = reg ? (size_t)kLsqChunk * (cols + s) + 128 : (size_t)(bw + 1) * (cols + s);
   const size_t back = (size_t)cols * 32 + 32 * 33;
   return sizeof(double) * (win > back ? win : back);
 }
 // The register path: bandwidth 16, one thread per column of [H | T], and its LDS fits.
+// Bandwidth 32: 512 threads (the window rows of a column take 66 registers).  DMR_LSQ_WINDOW=1
+// keeps every shape on the window path.
 bool lsq_staged(int cols, int bw, int s) {
-  return bw == 16 && s <= 16 && cols + s <= 1024 && lsq_lds(cols, bw, s, true) <= 140 * 1024;
+  if (const char *e = std::getenv("DMR_LSQ_WINDOW"))
+    if (e[0] == '1') return false;
+  const int threads = bw == 32 ? 512 : 1024;
+  return ((bw == 16 && s <= 16) || (bw == 32 && s <= 32)) && cols + s <= threads &&
+         lsq_lds(cols, bw, s, true) <= 140 * 1024;
 }
 
 size_t lsq_block_lds(int cols, int bw, int s) {
@@ -2135,9 +2143,11 @@ void lsq_block(const double *H, int rows, int cols, int bw, const double *T, int
   const bool stg = lsq_staged(cols, bw, s);
   prepare();
 #define DMR_LSQ(B, S)                                                                            \
-  hipLaunchKernelGGL((k_lsq_block<B, S>), dim3(1), dim3(1024), lds, st, H, rows, cols, bw, T, s, \
-                     rho, R, Qt, Y, Yr32, ZH32)
-  if (stg)
+  hipLaunchKernelGGL((k_lsq_block<B, S>), dim3(1), dim3(B == 32 ? 512 : 1024), lds, st, H, rows, \
+                     cols, bw, T, s, rho, R, Qt, Y, Yr32, ZH32)
+  if (stg && bw == 32)
+    DMR_LSQ(32, true);
+  else if (stg)
     DMR_LSQ(16, true);
   else if (bw == 16 && s <= 16)
     DMR_LSQ(16, false);

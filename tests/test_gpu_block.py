@@ -436,3 +436,25 @@ def test_spmm_variants_identical_on_ragged_rows(torch_gpu, monkeypatch, prec):
             ref = X
         else:
             assert np.array_equal(X, ref), code
+
+
+@pytest.mark.parametrize("s", [16, 32])
+def test_lsq_register_path_matches_window_path(torch_gpu, monkeypatch, s):
+    """The block least squares on its register path (one thread per column of [H | T], for
+    bandwidth 16 and 32) and on its LDS-window path (DMR_LSQ_WINDOW=1) solve the same systems:
+    both meet 1e-10 in FP64 and their solutions agree to the accuracy the residual allows."""
+    A = gdmr.conv_diff_3d(16, jitter=0.01, seed=12, stream=0)
+    n = A.n
+    B = np.stack([gdmr.rand_normal(91, j, n) for j in range(s)])
+    X = {}
+    for path in ("register", "window"):
+        monkeypatch.setenv("DMR_LSQ_WINDOW", "1" if path == "window" else "0")
+        rc, rel, st, X[path] = _block(torch_gpu, A, B, mb=6, k=8)
+        worst = max(_true_res(A, B[j], X[path][j]) for j in range(s))
+        print(f"s {s} {path}: rc {rc} cycles {st.cycles} worst true residual {worst:.2e}",
+              flush=True)
+        assert rc == 0 and worst <= 1e-10 * 1.001
+    err = max(np.linalg.norm(X["register"][j] - X["window"][j]) / np.linalg.norm(X["window"][j])
+              for j in range(s))
+    print(f"s {s}: max relative difference {err:.2e}", flush=True)
+    assert err <= XTOL

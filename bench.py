@@ -241,16 +241,13 @@ MFMA_BF16_PEAK = 2.5e15
 def roofline(prof, block=False, bf16=False):
     """The dominant streaming kernel of the headline leg; achieved = algorithmic bytes / kernel
     time (HIP events around each launch on the solve stream).  Single-vector legs: the multi-dot
-    (GEMV-T) or the Gram-Schmidt update, whichever took more time.  Block leg: the basis GEMM
-    Out = a In + b P C over the stored basis P (k_gemm_nn_b with a bf16 basis, k_gemm_nn with an
-    FP32 one): the block updates U -= P C and the residual and solution updates, every launch of
-    which is timed on its own; its matrix-core rate is reported beside the HBM one."""
+    (GEMV-T) or the Gram-Schmidt update, whichever took more time.  Block leg: the block update
+    U = U - P C over the stored basis P (k_gemm_nn_b<1, float, RB> with a bf16 basis, k_gemm_nn
+    with an FP32 one), every launch of which is timed on its own; its matrix-core rate is
+    reported beside the HBM one."""
     if block:
-        ms = prof["update"][0] + prof["combo"][0]
-        by = prof["update"][1] + prof["combo"][1]
-        cnt = prof["update"][2] + prof["combo"][2]
-        fl = prof["update"][3] + prof["combo"][3]
-        kernel = "k_gemm_nn_b" if bf16 else "k_gemm_nn"
+        ms, by, cnt, fl = prof["update"][:4]
+        kernel = "k_gemm_nn_b<1>" if bf16 else "k_gemm_nn"
     else:
         best = max(("gemvt", "update"), key=lambda p: prof[p][0])
         ms, by, cnt = prof[best][:3]

@@ -48,6 +48,16 @@ __device__ inline uint32_t f_to_bf(float f) {  // finite inputs
 }
 __device__ inline float ldf(const float *p, long long i) { return p[i]; }
 __device__ inline float ldf(const bf16 *p, long long i) { return bf_to_f(p[i].bits); }
+// Unsigned 32-bit offsets from a wavefront-uniform base: one global load with a scalar base and a
+// 32-bit vector offset, no 64-bit address arithmetic per element.  The byte offset must fit 32
+// bits: i < 2^30 for FP32 (a block basis of that many rows would not fit in device memory).
+__device__ inline float ldf(const float *p, unsigned i) {
+  return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(p) + (i << 2));
+}
+__device__ inline float ldf(const bf16 *p, unsigned i) {
+  return bf_to_f(
+      *reinterpret_cast<const uint16_t *>(reinterpret_cast<const char *>(p) + (i << 1)));
+}
 
 // Four consecutive outputs (row .. row + 3 of one column; full when all four are below n).
 __device__ inline void store_rows4(float *dst, const float (&w)[4], int cnt) {
@@ -113,12 +123,12 @@ __global__ __launch_bounds__(kThreads) void k_spmm_w(int n, const int *__restric
 #pragma unroll
   for (int c = 0; c < CG; ++c) acc[c] = (TA)0;
   if (e - b <= W) {
-    int col[W];
+    unsigned col[W];
     TA v[W];
 #pragma unroll
     for (int q = 0; q < W; ++q) {
       const bool in = b + q < e;
-      col[q] = in ? colInd[b + q] : row;
+      col[q] = (unsigned)(in ? colInd[b + q] : row);
       v[q] = in ? (TA)val[b + q] : (TA)0;
     }
 #pragma unroll
@@ -177,12 +187,12 @@ __global__ __launch_bounds__(kThreads) void k_spmm_lds(int n, const int *__restr
 #pragma unroll
   for (int c = 0; c < CG; ++c) acc[c] = (TA)0;
   if (staged && e - b <= W) {
-    int col[W];
+    unsigned col[W];
     TA v[W];
 #pragma unroll
     for (int q = 0; q < W; ++q) {
       const bool in = b + q < e;
-      col[q] = in ? sCol[b - gb + q] : row;
+      col[q] = (unsigned)(in ? sCol[b - gb + q] : row);
       v[q] = in ? (TA)sVal[b - gb + q] : (TA)0;
     }
 #pragma unroll

@@ -69,7 +69,6 @@ struct dmrBlockContext {
   // newest block in FP32 while it is orthogonalised (the product A V_j, or A Z, or the start
   // block), written into its slot of P by the last normalisation.
   float *P = nullptr, *Z = nullptr, *Zn = nullptr, *R = nullptr, *Xa = nullptr, *Ub = nullptr;
-  uint16_t *Wb = nullptr;  // bf16 output of a block update (s <= 16), before its normalisation
   bool bfBasis = false;
   double *R64 = nullptr;
   double *H = nullptr, *T = nullptr, *Hw = nullptr, *Tw = nullptr, *Y = nullptr, *Hred = nullptr;
@@ -89,7 +88,6 @@ struct dmrBlockContext {
   bool fusedGram = true;  // first Gram matrix from the update's epilogue where possible
   int spmmForce = 0;      // SpMM variant pinned by DMR_BLOCK_SPMM (0: the tuned one)
   bool foldT = true;      // R's projection in the last block step's pass where possible
-  bool wbf = true;        // bf16 update output before the normalisation where possible
   // Launch shapes chosen by the autotuner (per block width s), and whether it ran.
   struct Tuning {
     int tnRowsProj = 512, tnVarProj = 0;  // gemm_tn for projections (p > s)
@@ -198,7 +196,7 @@ Sizes sizes(long long n, int mb, int k, int sMax, int G, long long ld) {
   const int rRed = rowsMax, cRed = colsMax;
   Sizes z;
   z.big = sizeof(float) * (size_t)ld * (size_t)(rowsMax + 2 * k + 2 * sMax + std::max(sMax, k)) +
-          sizeof(double) * (size_t)ld * sMax + sizeof(uint16_t) * (size_t)ld * std::min(sMax, 16);
+          sizeof(double) * (size_t)ld * sMax;
   z.small = sizeof(double) * ((size_t)rowsMax * colsMax * 2 + (size_t)rowsMax * 32 * 2 +
                               (size_t)colsMax * 32 + (size_t)rRed * cRed + 32 * 32 * 2 +
                               (size_t)cRed * 32 + cRed + 2 + 8 * 64 +
@@ -260,7 +258,6 @@ int ctx_alloc(dmrBlockContext *c) {
   c->Xa = b.take<float>((size_t)ld * s);
   c->R64 = b.take<double>((size_t)ld * s);
   c->Ub = b.take<float>((size_t)ld * std::max(s, std::max(k, 1)));
-  c->Wb = b.take<uint16_t>((size_t)ld * std::min(s, 16));
   Carver m{(char *)c->small};
   const int rowsMax = c->rowsMax, colsMax = c->colsMax;
   const int pPadMax = (rowsMax + 31) & ~31;
@@ -468,10 +465,9 @@ void autotune(dmrBlockContext *c, int s, hipStream_t st) {
 // to Hdst (ld ldh) when given.
 // gramParts > 0: the first pass's Gram matrix is already in c->part as that many partials
 // (written by the update that produced U).
-// Wb (with gramParts, one pass): the block to normalise is Wb (bf16), not U.
 template <typename TO>
 void cholqr2(dmrBlockContext *c, float *U, int sc, double *Hdst, int ldh, TO *dst,
-             hipStream_t st, int gramParts = 0, const bf16 *Wb = nullptr) {
+             hipStream_t st, int gramParts = 0) {
   const long long n = c->n;
   // A bf16-stored block is orthogonal only to bf16 rounding (about 4e-3), so one Cholesky QR
   // pass, orthogonal to about kappa^2 * 6e-8, is enough there; FP32 storage keeps two.
@@ -489,18 +485,13 @@ void cholqr2(dmrBlockContext *c, float *U, int sc, double *Hdst, int ldh, TO *ds
       dmr::blk::chol_qr(c->G64, sc, kDepTol, c->Sinv32, c->Sacc, pass == 0,
                         pass == passes - 1 ? Hdst : nullptr, ldh, c->flags, st);
     });
-    timedf(c, st, DMR_PH_ORTH, (Wb ? 2.0 : 4.0) * n * sc + (double)sizeof(TO) * n * sc,
+    timedf(c, st, DMR_PH_ORTH, 4.0 * n * sc + (double)sizeof(TO) * n * sc,
            2.0 * n * sc * sc, [&] {
       if (pass < passes - 1) {
         if (!dmr::blk::scale_block(U, c->ld, c->Sinv32, U, c->ld, sc, n, st))
           nn(c, U, sc, c->Sinv32, nullptr, 0.0f, 1.0f, U, sc, st);
-      } else {
-        bool done = false;
-        if constexpr (std::is_same<TO, bf16>::value) {
-          if (Wb) done = dmr::blk::scale_block(Wb, c->ld, c->Sinv32, dst, c->ld, sc, n, st);
-        }
-        if (!done && !dmr::blk::scale_block(U, c->ld, c->Sinv32, dst, c->ld, sc, n, st))
-          nn(c, U, sc, c->Sinv32, nullptr, 0.0f, 1.0f, dst, sc, st);
+      } else if (!dmr::blk::scale_block(U, c->ld, c->Sinv32, dst, c->ld, sc, n, st)) {
+        nn(c, U, sc, c->Sinv32, nullptr, 0.0f, 1.0f, dst, sc, st);
       }
     });
   }
@@ -520,16 +511,11 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
   const int rows = p0 + (mb + 1) * s, cols = p0 + mb * s;
   float *U = c->Ub;
   const int gp = gram_parts(c, s);  // Gram matrices from the update's epilogue, when possible
-  // With those, and one Cholesky QR pass, the update writes its output in bf16 (Wb) and the
-  // normalisation reads it from there: U stays the SpMM's product, read twice and never rewritten.
-  const bool useWb = gp > 0 && c->wbf && c->cholPasses != 2;
-  bf16 *Wb = reinterpret_cast<bf16 *>(c->Wb);
   // R's projection folded into the last block step (bf16 basis, matrix-core projection, s = 16).
   const bool foldT = std::is_same<TP, bf16>::value && s == 16 && mb >= 1 &&
                      c->tune.tnVarProj == 3 && c->foldT;
   const double vb = 4.0 * (double)n;              // one FP32 column
   const double pb = (double)sizeof(TP) * (double)n;  // one basis column
-  const double wb = 2.0 * (double)n;                 // one bf16 column
   dmr::fill<double>(c->H, (long long)rows * cols, 0.0, st);
   dmr::blk::zero_ints(c->flags, 2, st);
   // The subspace refresh uses the k smallest right singular vectors of the leading block of H
@@ -572,13 +558,8 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
     timedf(c, st, DMR_PH_GEMVT, pb * k + vb * s, 2.0 * n * k * s, [&] {
       tn(c, P, k, c->R, s, nullptr, 0, c->C32, false, st);
     });
-    timedf(c, st, DMR_PH_UPDATE, pb * k + vb * s + (useWb ? wb : vb) * s, 2.0 * n * k * s, [&] {
+    timedf(c, st, DMR_PH_UPDATE, pb * k + vb * 2 * s, 2.0 * n * k * s, [&] {
       if constexpr (std::is_same<TP, bf16>::value) {
-        if (useWb) {
-          dmr::blk::gemm_nn_w(P, ld, k, c->C32, c->R, ld, 1.0f, -1.0f, Wb, ld, s, n,
-                              c->tune.nnBasis, st, c->part);
-          return;
-        }
         if (gp) {
           nn_gram(c, P, k, c->C32, c->R, 1.0f, -1.0f, U, s, st);
           return;
@@ -589,8 +570,7 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
   } else {
     hipMemcpyAsync(U, c->R, sizeof(float) * ld * s, hipMemcpyDeviceToDevice, st);
   }
-  cholqr2(c, U, s, (double *)nullptr, 0, V0, st, deflate ? gp : 0,
-          deflate && useWb ? Wb : nullptr);
+  cholqr2(c, U, s, (double *)nullptr, 0, V0, st, deflate ? gp : 0);
   for (int j = 0; j < mb; ++j) {
     const TP *Vj = P + (long long)(p0 + j * s) * ld;
     const int pj = p0 + (j + 1) * s;
@@ -613,14 +593,8 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
       }
       tn(c, P, pj, U, s, Hcol, rows, c->C32, false, st);
     });
-    timedf(c, st, DMR_PH_UPDATE, pb * pj + vb * s + (useWb ? wb : vb) * s, 2.0 * n * pj * s,
-           [&] {
+    timedf(c, st, DMR_PH_UPDATE, pb * pj + vb * 2 * s, 2.0 * n * pj * s, [&] {
       if constexpr (std::is_same<TP, bf16>::value) {
-        if (useWb) {
-          dmr::blk::gemm_nn_w(P, ld, pj, c->C32, U, ld, 1.0f, -1.0f, Wb, ld, s, n,
-                              c->tune.nnBasis, st, c->part);
-          return;
-        }
         if (gp) {
           nn_gram(c, P, pj, c->C32, U, 1.0f, -1.0f, U, s, st);
           return;
@@ -628,7 +602,7 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
       }
       nn(c, P, pj, c->C32, U, 1.0f, -1.0f, U, s, st);
     });
-    cholqr2(c, U, s, Hcol + pj, rows, slot, st, gp, useWb ? Wb : nullptr);
+    cholqr2(c, U, s, Hcol + pj, rows, slot, st, gp);
     if (refreshed && j == kr - 1) fork_svd();
   }
   // Projection of the residual block on P (the part the last block step left), least squares,
@@ -881,7 +855,6 @@ int dmrBlockCreate(dmrBlockHandle *h, int device, int elements, int blockSteps, 
   }
   if (const char *e = std::getenv("DMR_BLOCK_FUSED_GRAM")) c->fusedGram = e[0] != '0';
   if (const char *e = std::getenv("DMR_BLOCK_FOLD_T")) c->foldT = e[0] != '0';
-  if (const char *e = std::getenv("DMR_BLOCK_W_BF16")) c->wbf = e[0] != '0';
   if (const char *e = std::getenv("DMR_BLOCK_SPMM")) {
     const int v = std::atoi(e);
     for (int ok : {4, 8, 16, 104, 108, 116, 208, 216})

@@ -76,11 +76,6 @@ void gemm_nn(const bf16 *P, long long ldp, int p, const float *C, const float *I
              float a, float b, float *Out, long long ldo, int s, long long n, int un,
              hipStream_t st, float *gram = nullptr);
 int gemm_nn_gram_blocks(long long n, int un);
-// The matrix-core update (un >= 100, s <= 16) with a bf16 Out, which must not alias In; with gram,
-// the partial Gram matrices are those of Out as stored (rounded).
-void gemm_nn_w(const bf16 *P, long long ldp, int p, const float *C, const float *In,
-               long long ldi, float a, float b, bf16 *Out, long long ldo, int s, long long n,
-               int un, hipStream_t st, float *gram);
 // One pass over a bf16 basis for two blocks of s <= 16 columns (ld ld): X1 -= P[:, 0:p] C[:, 0:16]
 // and X2 -= P[:, 0:p] C[:, 16:32] (C row-major p x 32; un >= 100 as for gemm_nn).
 void gemm_nn_pair(const bf16 *P, long long ldp, int p, const float *C, float *X1, float *X2,
@@ -98,8 +93,6 @@ void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *
 bool scale_block(const float *U, long long ldu, const float *C, float *Out, long long ldo, int s,
                  long long n, hipStream_t st);
 bool scale_block(const float *U, long long ldu, const float *C, bf16 *Out, long long ldo, int s,
-                 long long n, hipStream_t st);
-bool scale_block(const bf16 *U, long long ldu, const float *C, bf16 *Out, long long ldo, int s,
                  long long n, hipStream_t st);
 // Cholesky QR step on the Gram matrix G (s x s, ld s): see k_chol_qr.
 void chol_qr(const double *G, int s, double relTol, float *Sinv32, double *Sacc, int first,

@@ -20,7 +20,6 @@
 
 #include <cmath>
 #include <cstdlib>
-#include <type_traits>
 
 #include "dmr_block.h"
 
@@ -890,13 +889,8 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nn_b(const bf16 *__restrict__
 #pragma unroll
         for (int e = 0; e < 4; ++e) o[e] = a != 0.0f ? a * in[e] + b * pr[e] : b * pr[e];
         store_rows4(outc + row, o, cnt);
-        if (NT == 1 && gram) {  // the output as stored (rows past n stay zero) for the Gram matrix
-          if constexpr (std::is_same<TO, bf16>::value) {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) o[e] = bf_to_f(f_to_bf(o[e]));
-          }
+        if (NT == 1 && gram)  // the output as stored (rows past n stay zero) for the Gram matrix
           *reinterpret_cast<float4 *>(tile + jl * stride + q4) = make_float4(o[0], o[1], o[2], o[3]);
-        }
       }
     }
     if (NT == 1 && gram) {
@@ -947,27 +941,8 @@ __device__ inline void store_rows2(bf16 *dst, float v0, float v1, bool two) {
     dst[0].bits = (uint16_t)f_to_bf(v0);
 }
 
-__device__ inline void load_rows2(const float *src, bool two, float &v0, float &v1) {
-  if (two) {
-    const float2 v = *reinterpret_cast<const float2 *>(src);
-    v0 = v.x;
-    v1 = v.y;
-  } else {
-    v0 = src[0];
-  }
-}
-__device__ inline void load_rows2(const bf16 *src, bool two, float &v0, float &v1) {
-  if (two) {
-    const uint32_t w = *reinterpret_cast<const uint32_t *>(src);
-    v0 = __uint_as_float(w << 16);
-    v1 = __uint_as_float(w & 0xffff0000u);
-  } else {
-    v0 = bf_to_f(src[0].bits);
-  }
-}
-
-template <typename TU, typename TO>
-__global__ __launch_bounds__(kThreads) void k_scale_block(const TU *U, long long ldu,
+template <typename TO>
+__global__ __launch_bounds__(kThreads) void k_scale_block(const float *U, long long ldu,
                                                           const float *__restrict__ C, TO *Out,
                                                           long long ldo, int s, long long n) {
   const long long r = 2 * ((long long)blockIdx.x * kThreads + threadIdx.x);
@@ -977,7 +952,15 @@ __global__ __launch_bounds__(kThreads) void k_scale_block(const TU *U, long long
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     u0[k] = u1[k] = 0.0f;
-    if (k < s) load_rows2(U + k * ldu + r, two, u0[k], u1[k]);
+    if (k < s) {
+      if (two) {
+        const float2 v = *reinterpret_cast<const float2 *>(U + k * ldu + r);
+        u0[k] = v.x;
+        u1[k] = v.y;
+      } else {
+        u0[k] = U[k * ldu + r];
+      }
+    }
   }
   float o0[16], o1[16];
 #pragma unroll
@@ -2074,24 +2057,6 @@ int gemm_nn_gram_blocks(long long n, int un) {
   return (int)((waves + 3) / 4);
 }
 
-void gemm_nn_w(const bf16 *P, long long ldp, int p, const float *C, const float *In,
-               long long ldi, float a, float b, bf16 *Out, long long ldo, int s, long long n,
-               int un, hipStream_t st, float *gram) {
-  const int RB = un >= 101 ? 2 : 1;
-  const long long waves = (n + 128 * RB - 1) / (128 * RB);
-  const unsigned grid = (unsigned)((waves + 3) / 4);
-  const int KP = (p + 15) & ~15;
-  const size_t epi = sizeof(float) * 4 * kNnbTile;
-  const size_t l1 = sizeof(uint16_t) * 2 * 16 * KP;
-  const size_t lds = l1 > epi ? l1 : epi;
-  if (RB == 2)
-    hipLaunchKernelGGL((k_gemm_nn_b<1, bf16, 2>), dim3(grid), dim3(kThreads), lds, st, P, ldp, p,
-                       C, In, ldi, a, b, Out, ldo, s, n, gram, nullptr, (bf16 *)nullptr);
-  else
-    hipLaunchKernelGGL((k_gemm_nn_b<1, bf16, 1>), dim3(grid), dim3(kThreads), lds, st, P, ldp, p,
-                       C, In, ldi, a, b, Out, ldo, s, n, gram, nullptr, (bf16 *)nullptr);
-}
-
 void gemm_nn(const bf16 *P, long long ldp, int p, const float *C, const float *In, long long ldi,
              float a, float b, float *Out, long long ldo, int s, long long n, int un,
              hipStream_t st, float *gram) {
@@ -2124,29 +2089,23 @@ void gemm_nn(const bf16 *P, long long ldp, int p, const float *C, const float *I
   }
   gemm_nn_t<bf16, float>(P, ldp, p, C, In, ldi, a, b, Out, ldo, s, n, un, st);
 }
-template <typename TU, typename TO>
-void scale_block_t(const TU *U, long long ldu, const float *C, TO *Out, long long ldo, int s,
+template <typename TO>
+void scale_block_t(const float *U, long long ldu, const float *C, TO *Out, long long ldo, int s,
                    long long n, hipStream_t st) {
   const long long pairs = (n + 1) / 2;
-  hipLaunchKernelGGL((k_scale_block<TU, TO>), dim3((unsigned)((pairs + kThreads - 1) / kThreads)),
+  hipLaunchKernelGGL(k_scale_block<TO>, dim3((unsigned)((pairs + kThreads - 1) / kThreads)),
                      dim3(kThreads), 0, st, U, ldu, C, Out, ldo, s, n);
-}
-bool scale_block(const bf16 *U, long long ldu, const float *C, bf16 *Out, long long ldo, int s,
-                 long long n, hipStream_t st) {
-  if (s > 16 || (ldu & 1) || (ldo & 1)) return false;
-  scale_block_t<bf16, bf16>(U, ldu, C, Out, ldo, s, n, st);
-  return true;
 }
 bool scale_block(const float *U, long long ldu, const float *C, float *Out, long long ldo, int s,
                  long long n, hipStream_t st) {
   if (s > 16 || (ldu & 1) || (ldo & 1)) return false;
-  scale_block_t<float, float>(U, ldu, C, Out, ldo, s, n, st);
+  scale_block_t<float>(U, ldu, C, Out, ldo, s, n, st);
   return true;
 }
 bool scale_block(const float *U, long long ldu, const float *C, bf16 *Out, long long ldo, int s,
                  long long n, hipStream_t st) {
   if (s > 16 || (ldu & 1) || (ldo & 1)) return false;
-  scale_block_t<float, bf16>(U, ldu, C, Out, ldo, s, n, st);
+  scale_block_t<bf16>(U, ldu, C, Out, ldo, s, n, st);
   return true;
 }
 

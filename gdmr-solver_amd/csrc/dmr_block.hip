@@ -1701,8 +1701,10 @@ __global__ __launch_bounds__(BWT == 32 ? 512 : 1024) void k_lsq_block(const doub
 #pragma unroll
     for (int j = 0; j < 8; ++j) ZH32[r * 32 + 8 * g + j] = 8 * g + j < s ? (float)z[j] : 0.0f;
   }
-  for (int e = tid; e < rows * (32 - SW); e += bd)  // columns no group forms
-    ZH32[(e / (32 - SW)) * 32 + SW + e % (32 - SW)] = 0.0f;
+  if constexpr (SW < 32) {
+    for (int e = tid; e < rows * (32 - SW); e += bd)  // columns no group forms
+      ZH32[(e / (32 - SW)) * 32 + SW + e % (32 - SW)] = 0.0f;
+  }
 }
 
 // Coefficients of the paired update (gemm_nn_pair), rows x 32 row-major: columns 0 .. 15 are

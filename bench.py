@@ -1,6 +1,6 @@
 """Benchmark: solves/sec at 1e-10 relative residual (FP64-checked) on the 128^3 sequence.
 
-Workload (the 3-D configuration of the DMR north star): 3-D convection-diffusion, 7-point, 128^3
+Workload: 3-D convection-diffusion, 7-point, 128^3
 (n = 2,097,152, nnz = 14,581,760), CSR FP64, a sequence of 16 systems whose per-cell diffusion
 is jittered U(-1%, +1%) per step (seeded), N(0,1) right-hand sides, m = 50, k = 20, tol 1e-10,
 deflation subspace recycled across the sequence (mode = 0 for the first system, 1 after).

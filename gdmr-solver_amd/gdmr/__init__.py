@@ -25,8 +25,7 @@ LIB_PATH = os.path.join(LIB_DIR, "libgDMR.so")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
-        f"libgDMR.so not found at {LIB_PATH}; build it with `make -C gdmr-solver_amd` "
-        "(or __graft_entry__.build())")
+        f"libgDMR.so not found at {LIB_PATH}; build it with `make -C gdmr-solver_amd`")
 
 lib = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
 

@@ -1,6 +1,6 @@
 """Batched multi-right-hand-side benchmark: block DMR against sequential FP64 DMR.
 
-Workload (batched configuration of the DMR north star): 3-D anisotropic diffusion, 7-point,
+Workload (batched): 3-D anisotropic diffusion, 7-point,
 96^3 (n = 884,736), coefficients (1, 0.1, 0.01); 32 simultaneous N(0,1) right-hand sides sharing
 one deflation subspace of k = 32; tol 1e-10 checked in FP64 per column.
 

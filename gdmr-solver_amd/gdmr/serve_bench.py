@@ -1,6 +1,6 @@
 """Serving load generator over the gDMRServe.h loop: Poisson arrivals, p50/p99 latency.
 
-Workload (serving configuration of the DMR north star): three operator families
+Workload (serving): three operator families
   * 2-D convection-diffusion, 5-point, 512^2  (n = 262,144)
   * 3-D convection-diffusion, 7-point, 64^3   (n = 262,144)
   * 3-D convection-diffusion, 7-point, 160^3  (n = 4,096,000)

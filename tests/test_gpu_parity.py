@@ -1,6 +1,6 @@
 """GPU parity of the FP64 DMR solver against the FP64 host oracle.
 
-Bar (north star): iteration count within +/-1 and residual history within 1e-12 relative to the
+Bar: iteration count within +/-1 and residual history within 1e-12 relative to the
 oracle; solutions are compared too.  The oracle sums in the device's reduction order, so the
 histories are expected to agree bit for bit; the assertions use the stated tolerance.
 """

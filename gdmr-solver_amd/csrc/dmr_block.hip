@@ -1633,9 +1633,16 @@ __global__ __launch_bounds__(BWT == 32 ? 512 : 1024) void k_lsq_block(const doub
         yv[r] = 0.0;
         if (r < b) {
           double a = Tl[(i0 + r) * 32 + jr];
+          // The row's entries in two halves, each loaded at once before its multiply-adds.
 #pragma unroll
-          for (int l = r + 1; l < 32; ++l)
-            if (l < b) a = fma(-Dg[r * 33 + l], yv[l], a);
+          for (int h0 = r + 1; h0 < 32; h0 += 16) {
+            double dr[16];
+#pragma unroll
+            for (int l = h0; l < h0 + 16 && l < 32; ++l) dr[l - h0] = Dg[r * 33 + l];
+#pragma unroll
+            for (int l = h0; l < h0 + 16 && l < 32; ++l)
+              if (l < b) a = fma(-dr[l - h0], yv[l], a);
+          }
           const double d = Dg[r * 33 + r];
           yv[r] = (jr < s && fabs(d) > piv) ? a * Rinv[r] : 0.0;
           Yb[r * 32 + jr] = yv[r];

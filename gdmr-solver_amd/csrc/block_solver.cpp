@@ -720,6 +720,10 @@ int block_solve(dmrBlockContext *c, int mode, int s, double *X, long long ldx, c
   bool useZ = mode != 0 && c->hasZ;
   bool cycled = false;
   double bmax = 0.0;
+  // maxCycles bounds the restart cycles of the whole solve, summed over the refinement steps (as
+  // the single-vector mixed solve bounds them by matRank), so a slowly converging solve returns
+  // DMR_ERR_NOCONV with its best iterate instead of running maxOuter inner solves of maxCycles.
+  int cyclesLeft = maxCycles;
   for (int outer = 0;; ++outer) {
     const double valBytes = 8.0 * c->csr.nnz * (c->colVal ? s : 1);
     timed(c, st, DMR_PH_SPMV, valBytes + 4.0 * c->csr.nnz + 4.0 * (n + 1) + 24.0 * n * s, [&] {
@@ -742,7 +746,7 @@ int block_solve(dmrBlockContext *c, int mode, int s, double *X, long long ldx, c
     c->st.beta = bmax;
     c->st.rho = worst;
     if (done) return DMR_OK;
-    if (outer >= c->maxOuter) return DMR_ERR_NOCONV;
+    if (outer >= c->maxOuter || cyclesLeft <= 0) return DMR_ERR_NOCONV;
     if (!std::isfinite(worst)) return DMR_ERR_RUNAWAY;
     // Inner tolerances per column, and the scale of each residual column to unit norm.
     std::vector<double> tolIn(s);
@@ -760,7 +764,8 @@ int block_solve(dmrBlockContext *c, int mode, int s, double *X, long long ldx, c
     // restarts (bounded by maxOuter), instead of running to maxCycles.
     double bestEst = HUGE_VAL;
     int stall = 0;
-    for (int cyc = 0; cyc < maxCycles; ++cyc) {
+    while (cyclesLeft > 0) {
+      --cyclesLeft;
       const bool deflate = c->k > 0 && c->hasZ && (useZ || cycled);
       bool refreshed = false;
       if (c->graphs && !c->prof.on) {

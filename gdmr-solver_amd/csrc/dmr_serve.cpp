@@ -25,6 +25,9 @@
 
 namespace {
 
+// Largest restart-cycle budget a request may carry (its matRank).
+constexpr int kMaxRequestCycles = 100000;
+
 struct Family {
   int n = 0, nnz = 0, m = 0, k = 0, prec = DMR_PREC_FP64;
   // Block family: block handle, block steps, widest block, staging of a batched launch.
@@ -236,7 +239,9 @@ void run_block(Family *F, std::vector<Request> &batch, hipStream_t stream,
   const bool have = dmrBlockHasZ(F->bh) != 0;
   std::vector<double> rel(total, 0.0);
   if (rc == DMR_OK) {
-    const int maxCycles = std::max(1, std::min(batch[0].matRank, 100000));
+    // The launch's cycle budget: the first request's (at most kMaxRequestCycles), summed over
+    // the refinement steps.
+    const int maxCycles = std::min(batch[0].matRank, kMaxRequestCycles);
     rc = dmrBlockSolve(F->bh, have ? 1 : 0, total, X, F->n, B, F->n, stream, batch[0].tol,
                        maxCycles, rel.data());
   }
@@ -450,7 +455,7 @@ int dmrServerSetBlockPrecision(dmrServerHandle s, int family, int precision) {
 int dmrServerSubmit(dmrServerHandle s, int family, const double *val, int nrhs,
                     const double *const *b, double *const *x, double solTol, int matRank,
                     long long *id) {
-  if (!s || !val || nrhs < 1 || !b || !x || matRank < 0 || !(solTol >= 0.0)) return DMR_ERR_ARG;
+  if (!s || !val || nrhs < 1 || !b || !x || matRank < 1 || !(solTol >= 0.0)) return DMR_ERR_ARG;
   for (int i = 0; i < nrhs; ++i)
     if (!b[i] || !x[i]) return DMR_ERR_ARG;
   Request r;

@@ -100,7 +100,7 @@ class Load:
         torch.cuda.current_stream().synchronize()
         rid = self.srv.submit(self.fam_id[sp["fam"]], f["vals"][sp["draw"]].data_ptr(),
                               [f["b"][i].data_ptr() for i in sp["rhs"]],
-                              [x.data_ptr() for x in xs], self.args.tol, 100000)
+                              [x.data_ptr() for x in xs], self.args.tol, self.args.max_cycles)
         self.live[rid] = (sp, xs)
         return rid
 
@@ -108,6 +108,9 @@ class Load:
         for res in results:
             sp, xs = self.live.pop(res.id)
             self.done.append((res, sp))
+            _log(f"request {res.id}: family {self.fams[sp['fam']]['name']}, {res.nrhs} RHS in a "
+                 f"launch of {res.batch}, {res.cycles} cycles, status {res.status}, latency "
+                 f"{res.endTime - res.submitTime:.2f}s")
             fcount = sum(1 for c in self.checked if c[0] == sp["fam"])
             if check_budget and fcount < check_budget:
                 f = self.fams[sp["fam"]]
@@ -142,6 +145,9 @@ def main(argv=None):
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--prec", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--max-cycles", type=int, default=4000,
+                    help="restart-cycle budget of one launch (the request's matRank): a launch "
+                         "that spends it returns DMR_ERR_NOCONV")
     ap.add_argument("--draws", type=int, default=4)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--check", type=int, default=2, help="requests per family checked in FP64")
@@ -206,6 +212,7 @@ def main(argv=None):
                                   cycles_per_solve=(sum(r.cycles for r in rs) /
                                                     max(1, sum(r.nrhs for r in rs))))
     status_ok = all(r.status == 0 for r in res)
+    not_converged = sum(1 for r in res if r.status != 0)
     worst = max((c[1] for c in L.checked), default=None)
     line = {
         "metric": "serving requests/s at 1e-10 (FP64-checked sample), p50/p99 latency",
@@ -221,6 +228,8 @@ def main(argv=None):
         "service_p50_ms": _pct(svc, 50), "service_p99_ms": _pct(svc, 99),
         "families": per_fam,
         "all_status_ok": status_ok,
+        "requests_not_converged": not_converged,
+        "max_cycles_per_launch": args.max_cycles,
         "mean_rhs_per_launch": mean_batch,
         "checked_requests": len(L.checked),
         "max_true_rel_residual": worst,

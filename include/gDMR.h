@@ -203,9 +203,10 @@ int dmrBlockSetPrecision(dmrBlockHandle h, int precision);
 int dmrBlockSetRefinement(dmrBlockHandle h, double innerTol, int maxOuter);
 /* Solve A X = B for nrhs <= maxRhs columns.  X: device, column-major n x nrhs (leading dimension
  * ldx), x0 in / solution out.  B: device, ldb.  mode != 0 starts from the stored subspace (else
- * the first cycle is undeflated).  maxCycles caps the restart cycles of each inner solve.
- * relRes (host, nrhs): the FP64-checked ||b - A x|| / ||b|| of each column.  Returns 0 when every
- * column meets solTol, else a DMR_ERR_* code. */
+ * the first cycle is undeflated).  maxCycles caps the restart cycles of the whole solve, summed
+ * over its refinement steps: once they are spent the solve returns DMR_ERR_NOCONV with X holding
+ * the last refined iterate.  relRes (host, nrhs): the FP64-checked ||b - A x|| / ||b|| of each
+ * column.  Returns 0 when every column meets solTol, else a DMR_ERR_* code. */
 int dmrBlockSolve(dmrBlockHandle h, int mode, int nrhs, double *X, long long ldx, const double *B,
                   long long ldb, hipStream_t stream, double solTol, int maxCycles,
                   double *relRes);

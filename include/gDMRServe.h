@@ -69,7 +69,11 @@ int dmrServerSetBlockPrecision(dmrServerHandle s, int family, int precision);
 /* Refinement settings of a mixed-precision family (dmrSetRefinement), before its first request. */
 int dmrServerSetRefinement(dmrServerHandle s, int family, double innerTol, int maxOuter);
 /* val: device values (nnz doubles) of this request's operator; b[i], x[i] (i < nrhs): device
- * vectors of n doubles (x[i] is x0 on entry, the solution on completion). */
+ * vectors of n doubles (x[i] is x0 on entry, the solution on completion).  matRank (>= 1)
+ * bounds the restart cycles: of each right-hand side's solve for a plain family (the matRank of
+ * dmrSolve), of the whole launch for a block family (the maxCycles of dmrBlockSolve, summed over
+ * its refinement steps, at most 100000).  A request that spends it completes with
+ * DMR_ERR_NOCONV. */
 int dmrServerSubmit(dmrServerHandle s, int family, const double *val, int nrhs,
                     const double *const *b, double *const *x, double solTol, int matRank,
                     long long *id);

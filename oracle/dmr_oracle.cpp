@@ -355,19 +355,33 @@ struct OracleCtx {
   bool keepLast = false;
   std::vector<double> lastP, lastH, lastZ, lastW;
   int lastD = 0;
+  // With keepLast: the start choice of every cycle of the last solve (1 = from the residual,
+  // 0 = from the old Krylov vector (m-k)/2), and, for the most recent cycle that restarted from
+  // the old vector, the basis before its setup (stagPrevP) and after its Arnoldi process (stagP).
+  std::vector<int> startLog;
+  std::vector<double> stagPrevP, stagP;
+  int stagD = -1;
+  // The stagnation restart rule of arnoldiSetupDMR; switched off only by tests that show the
+  // rule changes the residual history.
+  bool stagRule = true;
 };
 
 namespace {
 
 // One restart cycle's Arnoldi process (arnoldiSetupDMR + arnoldiDMR).  Returns the breakdown
-// step (0 if none).
-int arnoldi(OracleCtx &c, int d, int matRank, double tol) {
+// step (0 if none); *fromResidualOut: whether the start vector was the residual.
+int arnoldi(OracleCtx &c, int d, int matRank, double tol, bool *fromResidualOut) {
   Vecs &v = c.v;
   const long long n = v.n, ld = v.ld;
   const int m = v.m, k = v.k, G = c.G;
   const int o = d > 0 ? k : 0;
+  // arnoldiSetupDMR (gDMR.cu:2600-2621): restart from the old Krylov vector (m-k)/2 while the
+  // stagnation flag is set, except every period-th flagged cycle.  A zero period (the reference
+  // divides by it) means "from the residual".
   const int period = (m - k) > 0 ? (2 * matRank) / (m - k) : 0;
-  const bool fromResidual = c.kryLoopFlag == 0 || period == 0 || (c.kryFlagCount % period) == 0;
+  const bool fromResidual = !c.stagRule || c.kryLoopFlag == 0 || period == 0 ||
+                            (c.kryFlagCount % period) == 0;
+  *fromResidualOut = fromResidual;
   double *Po = v.col(v.P, o);
   if (fromResidual)
     std::copy(v.r.begin(), v.r.end(), Po);
@@ -440,14 +454,24 @@ int run_cycles(OracleCtx &c, int startDef, double tol, int matRank, double beta,
     if (rho / beta > 1.0) return 7;
     const int d = (grdItr == 0 && startDef == 0) ? 0 : k;
     const int o = d > 0 ? k : 0;
-    std::vector<double> Zused;
-    if (c.keepLast) Zused = v.Z;
-    const int brk = arnoldi(c, d, matRank, tol);
+    std::vector<double> Zused, Pbefore;
+    if (c.keepLast) {
+      Zused = v.Z;
+      Pbefore = v.P;
+    }
+    bool fromResidual = true;
+    const int brk = arnoldi(c, d, matRank, tol, &fromResidual);
     if (c.keepLast) {
       c.lastP = v.P;
       c.lastH = c.H;
       c.lastZ = Zused;
       c.lastD = d;
+      c.startLog.push_back(fromResidual ? 1 : 0);
+      if (!fromResidual) {
+        c.stagPrevP = Pbefore;
+        c.stagP = v.P;
+        c.stagD = d;
+      }
     }
     double *Po = v.col(v.P, o);
     // updateAppResDMR
@@ -542,6 +566,8 @@ double oracle_solve(OracleCtx *c, int mode, const double *b, double *x, double t
   const long long n = v.n;
   const int k = v.k, G = c->G;
   c->hist.clear();
+  c->startLog.clear();
+  c->stagD = -1;
   c->cycles = c->matvecs = c->breakdowns = 0;
   *numIts = matRank * matRank;
   // mode != 0 without a stored subspace runs the first cycle undeflated, as mode 0.
@@ -627,6 +653,35 @@ int oracle_last_cycle(OracleCtx *c, double *P, double *H, double *Z) {
               Z + (long long)j * v.n);
   return c->lastD;
 }
+
+// Start choice of each cycle of the last solve (keep_last): 1 from the residual, 0 from the old
+// Krylov vector.  Returns the cycle count.
+int oracle_start_log(OracleCtx *c, int *log, int cap) {
+  const int cnt = (int)c->startLog.size();
+  for (int i = 0; i < std::min(cap, cnt); ++i) log[i] = c->startLog[i];
+  return cnt;
+}
+
+// The most recent cycle of the last solve that restarted from the old Krylov vector (keep_last):
+// the basis before its setup (prevP) and after its Arnoldi process (P), n x (m+1) each.  Returns
+// its d, or -1 when no cycle restarted that way.
+int oracle_stagnation_cycle(OracleCtx *c, double *prevP, double *P) {
+  const Vecs &v = c->v;
+  if (c->stagD < 0) return -1;
+  for (int j = 0; j <= v.m; ++j) {
+    std::copy(c->stagPrevP.begin() + (long long)j * v.ld,
+              c->stagPrevP.begin() + (long long)j * v.ld + v.n, prevP + (long long)j * v.n);
+    std::copy(c->stagP.begin() + (long long)j * v.ld, c->stagP.begin() + (long long)j * v.ld + v.n,
+              P + (long long)j * v.n);
+  }
+  return c->stagD;
+}
+
+// Stagnation restart rule on (default) or off; off only to show in tests that the rule matters.
+void oracle_set_stagnation_rule(OracleCtx *c, int on) { c->stagRule = on != 0; }
+
+// Persistent stagnation counter (kryFlagCountDMR, gDMR.cu:890).
+int oracle_get_stagnation_count(OracleCtx *c) { return c->kryFlagCount; }
 
 void oracle_spmv(int n, const int *rp, const int *ci, const double *va, const double *x,
                  double *y) {

@@ -37,6 +37,10 @@ _sig("oracle_has_z", C.c_int, [_vp])
 _sig("oracle_get_z", None, [_vp, _dp])
 _sig("oracle_set_z", None, [_vp, _dp])
 _sig("oracle_last_cycle", C.c_int, [_vp, _dp, _dp, _dp])
+_sig("oracle_start_log", C.c_int, [_vp, _ip, C.c_int])
+_sig("oracle_stagnation_cycle", C.c_int, [_vp, _dp, _dp])
+_sig("oracle_set_stagnation_rule", None, [_vp, C.c_int])
+_sig("oracle_get_stagnation_count", C.c_int, [_vp])
 _sig("oracle_lsq", None, [_dp, C.c_int, _dp, _dp, _dp])
 _sig("oracle_svd", C.c_int, [_dp, C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp])
 _sig("oracle_spmv", None, [C.c_int, _ip, _ip, _dp, _dp, _dp])
@@ -149,3 +153,28 @@ class Oracle:
         d = lib.oracle_last_cycle(self.h, _d(P), _d(H), _d(Z))
         return (P.reshape(self.m + 1, self.n), H.reshape(self.m, self.m + 1).T.copy(),
                 Z[: self.k * self.n].reshape(self.k, self.n), d)
+
+    def start_log(self) -> np.ndarray:
+        """Per cycle of the last solve (keep_last=True): 1 if the start vector was the residual,
+        0 if it was the old Krylov vector (m-k)/2 (the stagnation restart)."""
+        cnt = lib.oracle_start_log(self.h, None, 0)
+        out = np.zeros(max(cnt, 1), dtype=np.int32)
+        lib.oracle_start_log(self.h, _i(out), cnt)
+        return out[:cnt]
+
+    def stagnation_cycle(self):
+        """(prevP, P, d) of the most recent stagnation restart of the last solve (keep_last=True):
+        the basis before that cycle's setup and after its Arnoldi process, (m+1, n) each; None
+        when no cycle restarted from the old vector."""
+        prev = np.zeros((self.m + 1) * self.n)
+        P = np.zeros((self.m + 1) * self.n)
+        d = lib.oracle_stagnation_cycle(self.h, _d(prev), _d(P))
+        if d < 0:
+            return None
+        return prev.reshape(self.m + 1, self.n), P.reshape(self.m + 1, self.n), d
+
+    def set_stagnation_rule(self, on: bool):
+        lib.oracle_set_stagnation_rule(self.h, 1 if on else 0)
+
+    def stagnation_count(self) -> int:
+        return lib.oracle_get_stagnation_count(self.h)

@@ -458,3 +458,42 @@ def test_lsq_register_path_matches_window_path(torch_gpu, monkeypatch, s):
               for j in range(s))
     print(f"s {s}: max relative difference {err:.2e}", flush=True)
     assert err <= XTOL
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_cycle_budget_bounds_the_whole_solve(torch_gpu, prec):
+    """maxCycles bounds the restart cycles of the whole block solve, summed over its refinement
+    steps: a solve that needs more cycles than its budget (2-D convection-diffusion 32^2, 4 block
+    steps, 1e-10) returns DMR_ERR_NOCONV after at most the budget, with X holding the last refined
+    iterate (finite residuals below the right-hand sides'), and the same handle then solves to
+    the tolerance with a large budget."""
+    torch = torch_gpu
+    A = gdmr.conv_diff_2d(32, jitter=0.01, seed=71)
+    n, s, budget = A.n, 4, 3
+    B = np.stack([gdmr.rand_normal(72, j, n) for j in range(s)])
+    solver = gdmr.BlockSolver(n, 4, 8, s)
+    rp, ci, va = _dev(torch, A.rowptr), _dev(torch, A.col), _dev(torch, A.val)
+    solver.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A.nnz, keep=(rp, ci, va))
+    if prec == "bf16":
+        solver.set_precision(gdmr.PREC_BF16)
+        solver.set_refinement(3e-2, 60)
+    else:
+        solver.set_refinement(1e-2, 60)
+    Bd = _dev(torch, B)
+    Xd = torch.zeros(s, n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    rc, rel = solver.solve(Xd.data_ptr(), n, Bd.data_ptr(), n, s, 1e-10, max_cycles=budget)
+    st, hist = solver.stats()
+    print(f"{prec}: budget {budget}: rc {rc} cycles {st.cycles} rel {rel.max():.2e}", flush=True)
+    assert rc == gdmr.DMR_ERR_NOCONV
+    assert st.cycles <= budget
+    assert np.all(np.isfinite(rel)) and rel.max() < 1.0
+    X = Xd.cpu().numpy()
+    for j in range(s):
+        assert abs(_true_res(A, B[j], X[j]) - rel[j]) <= 1e-6 * rel[j]
+    rc, rel = solver.solve(Xd.data_ptr(), n, Bd.data_ptr(), n, s, 1e-10, max_cycles=5000,
+                           mode=1)
+    st, hist = solver.stats()
+    print(f"{prec}: large budget: rc {rc} cycles {st.cycles} rel {rel.max():.2e}", flush=True)
+    assert rc == 0 and rel.max() <= 1e-10
+    solver.close()

@@ -9,6 +9,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
+import cases
 import gdmr
 import oracle
 
@@ -277,3 +278,34 @@ def test_matrix_market_operator(tmp_path):
     assert abs(its_mtx - itso) <= 1 and abs(res_mtx - reso) <= HIST_RTOL * reso
     tr = _true_res(A, b, x_mtx)
     assert tr <= tol * 1.001, tr
+
+
+@pytest.mark.parametrize("n,alpha,m,k,mat_rank", cases.STAG_CASES)
+def test_stagnation_restart_matches_oracle(torch_gpu, n, alpha, m, k, mat_rank):
+    """A nearly stalling operator with deflation: several cycles restart from the old Krylov
+    vector (m-k)/2 (arnoldiSetupDMR, gDMR.cu:2586-2632), the stagnation counter persists across
+    solves (gDMR.cu:890), and the HIP path's histories, cycle-limit exits and subspace follow the
+    oracle through two solves of a sequence."""
+    torch = torch_gpu
+    A = cases.shift_plus_diag(n, alpha)
+    s = gdmr.Solver(n, m, k)
+    rp, ci, va = _dev(torch, A.rowptr), _dev(torch, A.col), _dev(torch, A.val)
+    s.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A.nnz, keep=(rp, ci, va))
+    ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n), keep_last=True)
+    for t in range(2):
+        b = gdmr.rand_normal(3, t, n)
+        xd = torch.zeros(n, dtype=torch.float64, device="cuda")
+        bd = _dev(torch, b)
+        res, its = s.solve(xd.data_ptr(), bd.data_ptr(), 1e-10, mat_rank, mode=t)
+        st, hist = s.stats()
+        xo, reso, itso, sto = ora.solve(b, tol=1e-10, mat_rank=mat_rank, mode=t)
+        restarts = int((ora.start_log() == 0).sum())
+        print(f"stagnation n={n} t={t}: restarts from the old vector {restarts}, status "
+              f"{st.status} vs {sto}, stagnation count {gdmr.lib.dmrGetStagnationCount(s.h)} vs "
+              f"{ora.stagnation_count()}", flush=True)
+        assert restarts >= 1
+        _compare(f"stag[{t}]", its, hist, xd.cpu().numpy(), itso, ora.history(), xo)
+        assert st.status == sto == gdmr.DMR_ERR_NOCONV and len(hist) == len(ora.history())
+        assert gdmr.lib.dmrGetStagnationCount(s.h) == ora.stagnation_count()
+        assert np.max(np.abs(s.export_z() - ora.get_z())) <= 1e-9
+    s.close()

@@ -250,3 +250,38 @@ def test_block_family_batches_within_tolerance(torch_gpu, basis):
                   flush=True)
             assert err <= XTOL_BATCH, (i, err)
     assert batched > 0
+
+
+def test_block_family_request_budget(torch_gpu):
+    """A request's matRank is the cycle budget of its block launch: a slowly converging family
+    with a small budget completes with DMR_ERR_NOCONV after at most that many cycles (the server
+    stays live), and the family's next request with a large budget converges."""
+    torch = torch_gpu
+    A = gdmr.conv_diff_2d(32, jitter=0.01, seed=81)
+    rp, ci, va = _dev(torch, A.rowptr), _dev(torch, A.col), _dev(torch, A.val)
+    B = [gdmr.rand_normal(82, j, A.n) for j in range(3)]
+    bd = [_dev(torch, b) for b in B]
+    srv = gdmr.Server(slots=2, hbm_reserve=1 << 28)
+    try:
+        f = srv.add_block_family(A.n, A.nnz, rp.data_ptr(), ci.data_ptr(), 4, 8, 8)
+        srv.set_block_precision(f, gdmr.PREC_BF16)
+        srv.set_refinement(f, 3e-2, 60)
+        out = []
+        for budget in (3, 5000):
+            xs = [torch.zeros(A.n, dtype=torch.float64, device="cuda") for _ in B]
+            torch.cuda.synchronize()
+            srv.submit(f, va.data_ptr(), [b.data_ptr() for b in bd], [x.data_ptr() for x in xs],
+                       1e-10, budget)
+            srv.drain()
+            res = srv.wait(timeout=0.0, cap=4)
+            assert len(res) == 1
+            out.append((res[0], [x.cpu().numpy() for x in xs]))
+    finally:
+        srv.close()
+    (r0, x0), (r1, x1) = out
+    print(f"budget 3: status {r0.status} cycles {r0.cycles}; budget 5000: status {r1.status} "
+          f"cycles {r1.cycles} max rel {r1.maxRelRes:.2e}", flush=True)
+    assert r0.status == gdmr.DMR_ERR_NOCONV and r0.cycles <= 3
+    assert r1.status == 0
+    for b, x in zip(B, x1):
+        assert np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b) <= 1e-10 * 1.05

@@ -3,14 +3,16 @@
 Each check is chosen so a plausible mistake (a dropped term, a wrong sign or index, a transposed
 operand, a wrong column picked from the SVD) fails it: independent dense assembly of the PDE
 stencils, LAPACK (numpy) least squares and SVD, an independent textbook GMRES(m) for k = 0, the
-Arnoldi relation and basis orthogonality of a deflated cycle, breakdown on an operator with few
-distinct eigenvalues, exact deflation, and FP64-checked true residuals.
+Arnoldi relation and basis orthogonality of a deflated cycle, the subspace refresh recomputed
+from LAPACK's SVD, the stagnation restart's start vector and rule with deflation, breakdown on an
+operator with few distinct eigenvalues, exact deflation, and FP64-checked true residuals.
 """
 import math
 
 import numpy as np
 import pytest
 
+import cases
 import gdmr
 import oracle
 
@@ -302,3 +304,113 @@ def test_early_exit_semantics():
     x0, res0, its0, st0 = fresh0.solve(b, tol=1e-8, mode=0)
     assert st1 == 0 and its1 == its0 and np.array_equal(h1, fresh0.history())
     assert np.array_equal(x1, x0)
+
+
+# ------------------------------------------------------------------------------------------------
+# Subspace refresh (genDeflatDMR, gDMR.cu:1583-1704) recomputed independently.
+# ------------------------------------------------------------------------------------------------
+def _refresh_from_numpy(P, H, Z, d, k, m):
+    """Z_new recomputed from a cycle's basis with numpy.linalg.svd (LAPACK): the source basis is
+    [Z | V_0 .. V_{m-d-1}] for a deflated cycle (the Krylov slots start at o = k) and
+    [V_0 .. V_{m-1}] for an undeflated one; column i uses the right singular vector of the i-th of
+    the k smallest singular values of H, in descending order.  Returns the unnormalised
+    combinations (k, n), the singular values and the source basis."""
+    _, S, Vt = np.linalg.svd(H)
+    V = Vt.T
+    src = np.vstack([Z, P[k:m]]) if d > 0 else P[:m]
+    zt = np.array([src.T @ V[:, m - k + i] for i in range(k)])
+    return zt, S, src, V
+
+
+def _sign_free_err(a, b):
+    return min(np.linalg.norm(a - b), np.linalg.norm(a + b))
+
+
+@pytest.mark.parametrize("N,m,k,seed", [(24, 20, 6, 2), (32, 30, 10, 0)])
+def test_refresh_matches_lapack_svd(N, m, k, seed):
+    """The oracle's refreshed subspace equals normalize([Z V] Vs[:, m-k:]) computed with LAPACK's
+    SVD of the cycle's H, column by column up to sign, for an undeflated first cycle and for a
+    deflated cycle.  ||A z~_i|| equals sigma_{m-k+i} for the unnormalised combination (the
+    Arnoldi relation carries it), and the check fails for W in place of Z or for the Krylov
+    coefficients taken without the offset d."""
+    A = gdmr.conv_diff_2d(N, jitter=0.01, seed=seed)
+    n = A.n
+    As = A.to_scipy()
+    # Undeflated: one cycle only (mat_rank = 0 stops before the second).
+    ora0 = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n), keep_last=True)
+    ora0.solve(gdmr.rand_normal(1, 0, n), tol=1e-14, mat_rank=0, mode=0)
+    P, H, Z, d = ora0.last_cycle()
+    assert d == 0 and ora0.counts()[0] == 1
+    zt, S, _, _ = _refresh_from_numpy(P, H, Z, d, k, m)
+    Znew = ora0.get_z()
+    for i in range(k):
+        assert _sign_free_err(zt[i] / np.linalg.norm(zt[i]), Znew[i]) < 1e-11
+    # Deflated: the last cycle of the second solve of a sequence.
+    ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n), keep_last=True)
+    for mode in (0, 1):
+        ora.solve(gdmr.rand_normal(1, mode, n), tol=1e-6, mode=mode)
+    P, H, Z, d = ora.last_cycle()
+    assert d == k
+    zt, S, src, V = _refresh_from_numpy(P, H, Z, d, k, m)
+    Znew = ora.get_z()
+    for i in range(k):
+        assert _sign_free_err(zt[i] / np.linalg.norm(zt[i]), Znew[i]) < 1e-11
+        sig = S[m - k + i]
+        assert abs(np.linalg.norm(As @ zt[i]) - sig) <= 1e-9 * sig
+    # Planted mistakes in the recomputation: each misses the oracle's subspace by far.
+    zW = P[:m].T @ V[:, m - k]  # W (slots 0..k-1) in place of Z
+    assert _sign_free_err(zW / np.linalg.norm(zW), Znew[0]) > 1e-2
+    v = V[:, m - k]
+    zD = Z.T @ v[:k] + P[k:m].T @ v[:m - k]  # Krylov coefficients without the offset d
+    assert _sign_free_err(zD / np.linalg.norm(zD), Znew[0]) > 1e-2
+    zS = src.T @ V[:, m - k - 1]  # one singular vector too far (the k+1-th smallest)
+    assert _sign_free_err(zS / np.linalg.norm(zS), Znew[0]) > 1e-2
+
+
+# ------------------------------------------------------------------------------------------------
+# Stagnation restart with deflation (arnoldiSetupDMR, gDMR.cu:2586-2632; flag gDMR.cu:2506-2514).
+# ------------------------------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("n,alpha,m,k,mat_rank", cases.STAG_CASES)
+def test_stagnation_restart_with_deflation(n, alpha, m, k, mat_rank):
+    A = cases.shift_plus_diag(n, alpha)
+    b = gdmr.rand_normal(3, 0, n)
+    ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n), keep_last=True)
+    x, res, its, st = ora.solve(b, tol=1e-10, mat_rank=mat_rank, mode=0)
+    h = ora.history()
+    log = ora.start_log()
+    assert st == 6 and len(log) == len(h) == mat_rank + 1
+    # The start choice of every cycle follows the reference's rule, replayed from the residual
+    # history: the flag is q = rho_c / rho_{c-1} > 0.995 (gDMR.cu:2506-2514), the counter
+    # persists, and a flagged cycle restarts from the old vector unless count % period == 0.
+    period = 2 * mat_rank // (m - k)
+    q = h / np.concatenate([[1.0], h[:-1]])
+    count, flag = 0, 0
+    for c in range(len(h)):
+        from_res = flag == 0 or period == 0 or count % period == 0
+        assert log[c] == (1 if from_res else 0), c
+        if q[c] > 0.995:
+            flag, count = 1, count + 1
+        else:
+            flag = 0
+    assert count == ora.stagnation_count() and (log == 0).sum() >= 2
+    # The restarted cycle's start vector is the old Krylov vector (m-k)/2 of the slots o = k,
+    # orthogonalised against the new W and normalised (deflatorDMR(0, 0)).
+    prevP, P, d = ora.stagnation_cycle()
+    assert d == k
+    W = P[:k]
+    slot = k + (m - k) // 2
+
+    def start_from(v):
+        v = v - W.T @ (W @ v)
+        return v / np.linalg.norm(v)
+
+    assert np.linalg.norm(start_from(prevP[slot]) - P[k]) < 1e-12
+    for wrong in (slot - 1, slot + 1, k):  # neighbouring slots and the old residual slot
+        assert np.linalg.norm(start_from(prevP[wrong]) - P[k]) > 1e-3
+    # The rule changes the residual history (so a wrong slot would show in any comparison).
+    off = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n))
+    off.set_stagnation_rule(False)
+    off.solve(b, tol=1e-10, mat_rank=mat_rank, mode=0)
+    assert np.max(np.abs(off.history() - h) / h) > 1e-2

@@ -9,12 +9,14 @@ Three legs run on the same sequence, each in a fresh solver context:
   * fp64     -- the plain FP64 DMR, one system after the other (the parity-checked path);
   * mixed    -- FP32 Krylov basis and SpMV with FP32-accumulated reductions inside FP64 iterative
                 refinement; Hessenberg LSQ / SVD in FP64; one system after the other;
-  * seqblock -- mixed + batched: the 16 systems as one block solve.  Column t is refined in FP64
-                against its own operator A_t; the inner block DMR (block Gram-Schmidt as
-                tall-skinny GEMMs on the matrix cores, 16 block steps, recycled subspace k = 20,
-                basis stored as bf16 and multiplied on the bf16 matrix cores with FP32 operands
-                split into hi + lo parts, FP32 accumulation) shares the entrywise mean of the A_t.
-                Every system's latency is the block's.
+  * seqblock -- mixed + batched: the sequence as block solves of at most 16 consecutive systems
+                (balanced sizes; 16 systems: one block, 20: two of 10), one after the other on
+                one handle that recycles the deflation subspace from block to block.  Column t is
+                refined in FP64 against its own operator A_t; the inner block DMR (block
+                Gram-Schmidt as tall-skinny GEMMs on the matrix cores, 16 block steps, recycled
+                subspace k = 20, basis stored as bf16 and multiplied on the bf16 matrix cores
+                with FP32 operands split into hi + lo parts, FP32 accumulation) shares the
+                entrywise mean of its block's A_t.  Every system's latency is its block's.
 A step is one solve of one system.  `value` is the solves/sec of the fastest leg whose every
 system reaches 1e-10 checked in FP64.  Every solve's true residual ||b - A x|| / ||b|| (with its
 own operator) is recomputed on the host in FP64 after the timed region.
@@ -165,60 +167,95 @@ def run_block1_leg(mats, rhs, dev_vals, rp, ci, m, k, tol, warmup, inner_tol=1e-
                 x=[x.cpu().numpy() for x in xs])
 
 
+def seq_groups(count, block_max):
+    """Consecutive systems split into ceil(count / block_max) blocks of balanced size."""
+    nb = -(-count // block_max)
+    base, extra = divmod(count, nb)
+    groups, t = [], 0
+    for g in range(nb):
+        size = base + (1 if g < extra else 0)
+        groups.append((t, t + size))
+        t += size
+    return groups
+
+
 def run_seqblock_leg(mats, rhs, dev_vals, rp, ci, mb, k, tol, warmup, inner_tol, profile,
-                     prec="bf16"):
-    """The whole sequence as one block solve: column t is system t with its own FP64 operator
-    A_t in the refinement residual, while the inner FP32 block DMR (block Gram-Schmidt on the
-    matrix cores) shares the entrywise mean of the A_t (dmrBlockSetColumnOperators).  Every
-    column's latency is the block's."""
+                     prec="bf16", block_max=16):
+    """The sequence as block solves of at most block_max systems each (consecutive systems,
+    balanced sizes): column t is system t with its own FP64 operator A_t in the refinement
+    residual, while the inner FP32 block DMR (block Gram-Schmidt on the matrix cores) shares the
+    entrywise mean of its block's A_t (dmrBlockSetColumnOperators).  One handle solves the blocks
+    one after the other and recycles the deflation subspace from block to block.  Every system's
+    latency is its block's solve time."""
     import torch
 
     import gdmr
-    n, nnz, s = mats[0].n, mats[0].nnz, len(mats)
-    vals = torch.stack(dev_vals).contiguous()  # s x nnz, column t's operator at row t
+    n, nnz = mats[0].n, mats[0].nnz
+    groups = seq_groups(len(mats), block_max)
+    smax = max(g1 - g0 for g0, g1 in groups)
+    vals = torch.stack(dev_vals).contiguous()  # count x nnz, system t's operator at row t
     B = torch.stack([torch.from_numpy(b) for b in rhs]).to("cuda").contiguous()
 
     def make():
-        h = gdmr.BlockSolver(n, mb, k, s)
+        h = gdmr.BlockSolver(n, mb, k, smax)
         h.set_csr(rp.data_ptr(), ci.data_ptr(), vals[0].data_ptr(), nnz)
-        h.set_column_operators(vals.data_ptr(), nnz, s, inner_mean=True)
         h.set_refinement(inner_tol, 60)
         h.set_precision(gdmr.PREC_BF16 if prec == "bf16" else gdmr.PREC_FP32)
         return h
 
+    def solve_all(h, X, recycled):
+        out = []
+        for gi, (g0, g1) in enumerate(groups):
+            h.set_column_operators(vals[g0].data_ptr(), nnz, g1 - g0, inner_mean=True)
+            ts = time.perf_counter()
+            rc, rel = h.solve(X[g0].data_ptr(), n, B[g0].data_ptr(), n, g1 - g0, tol,
+                              max_cycles=100000, mode=1 if (gi or recycled) else 0)
+            dt = time.perf_counter() - ts
+            st, _ = h.stats()
+            out.append(dict(seconds=dt, rc=rc, rel=rel, cycles=st.cycles))
+        return out
+
     for w in range(warmup):
         h = make()
-        X = torch.zeros(s, n, dtype=torch.float64, device="cuda")
+        X = torch.zeros(len(mats), n, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
-        h.solve(X.data_ptr(), n, B.data_ptr(), n, s, tol, max_cycles=100000)
+        solve_all(h, X, False)
         h.close()
         log(f"[seqblock] warm-up {w} done")
     h = make()
-    X = torch.zeros(s, n, dtype=torch.float64, device="cuda")
+    X = torch.zeros(len(mats), n, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    rc, rel = h.solve(X.data_ptr(), n, B.data_ptr(), n, s, tol, max_cycles=100000)
+    blocks = solve_all(h, X, False)
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
-    st, hist = h.stats()
-    log(f"[seqblock] {s} systems as one block: {total*1e3:.1f} ms, rc {rc}, {st.cycles} cycles, "
-        f"max residual {rel.max():.3e}")
+    for (g0, g1), b in zip(groups, blocks):
+        log(f"[seqblock] systems {g0}..{g1 - 1} as one block: {b['seconds'] * 1e3:.1f} ms, rc "
+            f"{b['rc']}, {b['cycles']} cycles, max residual {b['rel'].max():.3e}")
+    tuning = h.tuning()
     prof = None
     if profile:
         # Phase profile (HIP events around every launch, eager launches instead of graph
-        # replays) on a re-solve from the recycled subspace; not part of the timed numbers.
+        # replays) on a re-solve of the same blocks from the recycled subspace; not part of the
+        # timed numbers.
         h.set_profiling(True)
-        Xp = torch.zeros(s, n, dtype=torch.float64, device="cuda")
+        Xp = torch.zeros(len(mats), n, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
-        h.solve(Xp.data_ptr(), n, B.data_ptr(), n, s, tol, max_cycles=100000, mode=1)
+        solve_all(h, Xp, True)
         prof = h.profile()
         h.set_profiling(False)
     h.close()
     Xh = X.cpu().numpy()
     true = [float(np.linalg.norm(rhs[t] - A.matvec(Xh[t])) / np.linalg.norm(rhs[t]))
             for t, A in enumerate(mats)]
-    return dict(total=total, lat=[total] * s, cycles=[st.cycles] * s, reported=list(rel),
-                true=true, prof=prof, x=[Xh[t] for t in range(s)], block=True)
+    lat, cycles, reported = [], [], []
+    for (g0, g1), b in zip(groups, blocks):
+        lat += [b["seconds"]] * (g1 - g0)
+        cycles += [b["cycles"]] * (g1 - g0)
+        reported += list(b["rel"])
+    return dict(total=total, lat=lat, cycles=cycles, reported=reported, true=true, prof=prof,
+                x=[Xh[t] for t in range(len(mats))], block=True, tuning=tuning,
+                blocks=[g1 - g0 for g0, g1 in groups])
 
 
 def summarize(leg, tol):
@@ -238,39 +275,54 @@ MFMA_FP32_PEAK = 157.3e12
 MFMA_BF16_PEAK = 2.5e15
 
 
-def roofline(prof, block=False, bf16=False):
-    """The dominant streaming kernel of the headline leg; achieved = algorithmic bytes / kernel
-    time (HIP events around each launch on the solve stream).  Single-vector legs: the multi-dot
-    (GEMV-T) or the Gram-Schmidt update, whichever took more time.  Block leg: the block update
-    U = U - P C over the stored basis P (k_gemm_nn_b<1, float, RB> with a bf16 basis, k_gemm_nn
-    with an FP32 one), every launch of which is timed on its own; its matrix-core rate is
-    reported beside the HBM one."""
-    if block:
-        ms, by, cnt, fl = prof["update"][:4]
-        kernel = "k_gemm_nn_b<1>" if bf16 else "k_gemm_nn"
-    else:
-        best = max(("gemvt", "update"), key=lambda p: prof[p][0])
-        ms, by, cnt = prof[best][:3]
-        fl = None
-        kernel = "k_multidot" if best == "gemvt" else "k_combo(SUB)"
+def kernel_of(phase, leg, tuning):
+    """The rocprof name of the one kernel a single-kernel phase launches in a leg, or None."""
+    if leg == "seqblock":
+        if phase == "update":
+            return "k_gemm_nn_b<1, float, %d>" % (2 if tuning["nn_unroll"] >= 101 else 1)
+        if phase == "gemvt":
+            return "k_gemm_tn16b"
+        if phase == "spmv":
+            return "k_spmm_lds" if tuning["spmm_cols"] >= 200 else "k_spmm_w"
+        return None
+    return {"spmv": "k_spmv", "gemvt": "k_multidot", "update": "k_combo"}.get(phase)
+
+
+def roofline(prof, leg, tuning=None, bf16=False, n=0, s=0):
+    """The dominant streaming kernel of the headline leg: of the phases that launch one kernel
+    each (SpMV or SpMM, projection, update), the one with the most measured time.
+    achieved = its algorithmic bytes / its time (HIP events around each launch on the solve
+    stream, in the profiled re-solve)."""
+    cands = [p for p in ("spmv", "gemvt", "update")
+             if prof.get(p) and prof[p][2] > 0 and kernel_of(p, leg, tuning)]
+    best = max(cands, key=lambda p: prof[p][0])
+    ms, by, cnt = prof[best][:3]
+    fl = prof[best][3] if len(prof[best]) > 3 else None
+    kernel = kernel_of(best, leg, tuning)
     ach = by / (ms * 1e-3) if ms > 0 else 0.0
-    out = {"bound": "hbm", "kernel": kernel,
+    out = {"bound": "hbm", "kernel": kernel, "phase": best,
            "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
            "frac": ach / HBM_PEAK, "traffic": None, "launches": cnt,
            "avg_launch_us": ms * 1e3 / max(cnt, 1),
-           "bytes_per_launch": by / max(cnt, 1)}
+           "bytes_per_launch": by / max(cnt, 1),
+           "phase_ms": {p: round(prof[p][0], 3) for p in cands}}
     # HBM traffic per launch of the same kernel from the calibrated FETCH_SIZE / WRITE_SIZE
-    # passes committed under profiles/ (rocprofv3 --pmc cannot run inside this process).
+    # passes committed under profiles/ (rocprofv3 --pmc cannot run inside this process).  Used
+    # only when it was measured on this kernel at this shape (n, block width, algorithmic bytes
+    # per launch within 2%) and is not below the algorithmic bytes.
     tf_path = os.path.join(ROOT, "profiles", "roofline_traffic.json")
     if os.path.exists(tf_path):
         rec = json.load(open(tf_path)).get(kernel)
-        if rec:
+        bpl = out["bytes_per_launch"]
+        if (rec and rec.get("n") == n and rec.get("s") == s and
+                abs(rec["bytes_per_launch"] - bpl) <= 0.02 * bpl and
+                rec["traffic_bytes_per_launch"] >= 0.99 * rec["bytes_per_launch"]):
             out["traffic"] = rec["traffic_bytes_per_launch"]
             out["traffic_source"] = rec["source"]
+            out["traffic_over_algorithmic"] = rec["traffic_bytes_per_launch"] / bpl
     if fl:
-        # The useful flops of the contraction.  The bf16 kernel issues each product twice (hi
-        # and lo parts of the FP32 operand) on the bf16 matrix cores; the FP32 one on the FP32
-        # cores.
+        # The useful flops of the contraction.  The bf16 kernels issue each product twice (hi
+        # and lo parts of the FP32 operand) on the bf16 matrix cores.
         tf = fl / (ms * 1e-3)
         peak = MFMA_BF16_PEAK if bf16 else MFMA_FP32_PEAK
         issued = 2.0 * tf if bf16 else tf
@@ -280,26 +332,52 @@ def roofline(prof, block=False, bf16=False):
     return out
 
 
-def cpu_baseline(mats, rhs, m, k, tol, gpu_cycles_per_solve, sample_cycles):
-    """The FP64 host oracle on system 0 of the same sequence, capped at sample_cycles restart
-    cycles; solves/sec is extrapolated with the GPU FP64 leg's mean cycles per solve."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(mats, rhs, m, k, tol, gpu_cycles_per_solve, sample_cycles, sample_cycles_1t):
+    """The FP64 host oracle on system 0 of the same sequence, capped at a few restart cycles,
+    at the host's core count and at one thread; solves/sec is extrapolated with the GPU FP64
+    leg's mean cycles per solve.  The host's DRAM bandwidth (triad) is measured beside it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import gdmr
     import oracle
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = int(os.environ.get("OMP_NUM_THREADS", cores))
     A = mats[0]
-    ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(A.n))
-    t0 = time.perf_counter()
-    ora.solve(rhs[0], tol=tol, mat_rank=sample_cycles - 1, mode=0)
-    dt = time.perf_counter() - t0
-    cyc = ora.counts()[0]
-    per_cycle = dt / max(cyc, 1)
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": 1.0 / (per_cycle * gpu_cycles_per_solve), "unit": "solves/s", "cores": cores,
-            "kind": "port",
-            "sample": f"FP64 oracle, system 0 of the 128^3 sequence, first {cyc} restart cycles "
-                      f"in {dt:.1f} s ({per_cycle:.2f} s/cycle, OpenMP {cores} threads); "
-                      f"solves/s = 1/(s per cycle x {gpu_cycles_per_solve:.1f} mean cycles per "
-                      f"solve of the GPU FP64 leg)"}
+    runs = {}
+    for threads, cyc_cap in ((cores, sample_cycles), (1, sample_cycles_1t)):
+        oracle.set_threads(threads)
+        bw = oracle.stream_triad(1 << 26, 3)
+        ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(A.n))
+        t0 = time.perf_counter()
+        ora.solve(rhs[0], tol=tol, mat_rank=cyc_cap - 1, mode=0)
+        dt = time.perf_counter() - t0
+        cyc = ora.counts()[0]
+        ora.close()
+        per_cycle = dt / max(cyc, 1)
+        runs[threads] = dict(seconds=dt, cycles=cyc, per_cycle=per_cycle, dram_GBps=bw / 1e9,
+                             solves_per_s=1.0 / (per_cycle * gpu_cycles_per_solve))
+        log(f"[cpu] {threads} threads: {cyc} cycles in {dt:.1f} s ({per_cycle:.2f} s/cycle), "
+            f"DRAM triad {bw / 1e9:.1f} GB/s")
+    oracle.set_threads(cores)
+    r, r1 = runs[cores], runs[1]
+    return {"value": r["solves_per_s"], "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle, system 0 of the 128^3 sequence, first {r['cycles']} restart "
+                      f"cycles in {r['seconds']:.1f} s ({r['per_cycle']:.2f} s/cycle, OpenMP "
+                      f"{cores} threads); solves/s = 1/(s per cycle x {gpu_cycles_per_solve:.1f} "
+                      f"mean cycles per solve of the GPU FP64 leg)",
+            "one_thread": {"value": r1["solves_per_s"], "cycles": r1["cycles"],
+                           "seconds": r1["seconds"], "s_per_cycle": r1["per_cycle"]},
+            "cpu_model": cpu_model(),
+            "dram_triad_GBps": {"cores": r["dram_GBps"], "one_thread": r1["dram_GBps"]}}
 
 
 def main():
@@ -323,7 +401,13 @@ def main():
                     help="relative tolerance of each inner block solve of the seqblock leg")
     ap.add_argument("--seq-prec", default="bf16", choices=("bf16", "fp32"),
                     help="basis storage of the seqblock leg")
-    ap.add_argument("--cpu-cycles", type=int, default=3)
+    ap.add_argument("--seq-block-max", type=int, default=16,
+                    help="most systems per block of the seqblock leg (the sequence is split into "
+                         "balanced blocks of at most this many)")
+    ap.add_argument("--cpu-cycles", type=int, default=5,
+                    help="restart cycles of the CPU baseline sample at the host's core count")
+    ap.add_argument("--cpu-cycles-1t", type=int, default=2,
+                    help="restart cycles of the one-thread CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--inner-tol", type=float, default=1e-5,
@@ -370,7 +454,8 @@ def main():
         if name == "seqblock":
             legs[name] = run_seqblock_leg(mats, rhs, dev_vals, rp, ci, args.seq_block_steps,
                                           args.seq_k, tol, args.warmup, args.seq_inner_tol,
-                                          not args.no_profile, args.seq_prec)
+                                          not args.no_profile, args.seq_prec,
+                                          args.seq_block_max)
             continue
         if name == "block1":
             legs[name] = run_block1_leg(mats, rhs, dev_vals, rp, ci, m, k, tol, args.warmup,
@@ -417,9 +502,17 @@ def main():
         for nm in [nm for nm in out_legs if nm != "fp64"]:
             line[f"speedup_{nm}_vs_fp64"] = (out_legs[nm]["solves_per_s"] /
                                              out_legs["fp64"]["solves_per_s"])
+    if "seqblock" in legs:
+        line["config"]["seqblock"] = {"blocks": legs["seqblock"]["blocks"],
+                                      "block_steps": args.seq_block_steps, "k": args.seq_k,
+                                      "inner_tol": args.seq_inner_tol, "basis": args.seq_prec,
+                                      "tuning": legs["seqblock"]["tuning"]}
+    line["latency_ms"] = {nm: {"p50": out_legs[nm]["p50_ms"], "p99": out_legs[nm]["p99_ms"]}
+                          for nm in out_legs}
     if legs[head]["prof"] is not None:
-        line["roofline"] = roofline(legs[head]["prof"], legs[head].get("block", False),
-                                    head == "seqblock" and args.seq_prec == "bf16")
+        line["roofline"] = roofline(legs[head]["prof"], head, legs[head].get("tuning"),
+                                    head == "seqblock" and args.seq_prec == "bf16",
+                                    mats[0].n, max(legs[head].get("blocks", [1])))
         line["phase_ms"] = {nm: {p: round(v[0], 3) for p, v in lg["prof"].items()}
                             for nm, lg in legs.items() if lg["prof"]}
         line["phase_GBps"] = {nm: {p: round(v[1] / (v[0] * 1e-3) / 1e9, 1)
@@ -448,7 +541,8 @@ def main():
     if not args.no_cpu:
         ref = out_legs.get("fp64", hs)
         cyc = float(np.mean(ref["cycles"]))
-        line["cpu_baseline"] = cpu_baseline(mats, rhs, m, k, tol, cyc, args.cpu_cycles)
+        line["cpu_baseline"] = cpu_baseline(mats, rhs, m, k, tol, cyc, args.cpu_cycles,
+                                            args.cpu_cycles_1t)
     print(json.dumps(line), flush=True)
 
 

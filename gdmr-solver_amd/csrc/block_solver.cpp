@@ -298,14 +298,22 @@ int tn_blocks(const dmrBlockContext *c, int rowsTarget) {
 
 using dmr::bf16;
 
-// C = P^T U with the tuned shape of its class (Gram matrix or projection).
+// Bytes the fixed-order sum of G partials of a p-row product reads (16 or 32 columns each).
+double reduce_bytes(int G, int p, int s) { return 4.0 * G * p * (s <= 16 ? 16 : 32); }
+
+// C = P^T U with the tuned shape of its class (Gram matrix or projection): the partials timed
+// under `phase` (bytes and flops of the product), their fixed-order sum under DMR_PH_REDUCE.
 template <typename TP>
-void tn(dmrBlockContext *c, const TP *P, int p, const float *U, int s, double *C64, int ldc,
-        float *C32, bool gram, hipStream_t st) {
+void tn(dmrBlockContext *c, int phase, double bytes, double flops, const TP *P, int p,
+        const float *U, int s, double *C64, int ldc, float *C32, bool gram, hipStream_t st) {
   const auto &t = c->tune;
-  dmr::blk::gemm_tn(P, c->ld, p, U, c->ld, s, c->n,
-                    tn_blocks(c, gram ? t.tnRowsGram : t.tnRowsProj),
-                    gram ? t.tnVarGram : t.tnVarProj, c->part, C64, ldc, C32, st);
+  const int G = tn_blocks(c, gram ? t.tnRowsGram : t.tnRowsProj);
+  timedf(c, st, phase, bytes, flops, [&] {
+    dmr::blk::gemm_tn(P, c->ld, p, U, c->ld, s, c->n, G, gram ? t.tnVarGram : t.tnVarProj,
+                      c->part, C64, ldc, C32, st, false);
+  });
+  timed(c, st, DMR_PH_REDUCE, reduce_bytes(G, p, s),
+        [&] { dmr::blk::gemm_tn_reduce(c->part, G, p, s, C64, ldc, C32, st); });
 }
 
 template <typename TP, typename TO>
@@ -474,12 +482,11 @@ void cholqr2(dmrBlockContext *c, float *U, int sc, double *Hdst, int ldh, TO *ds
   const int passes = c->cholPasses ? c->cholPasses : (c->bfBasis ? 1 : 2);
   for (int pass = 0; pass < passes; ++pass) {
     if (pass == 0 && gramParts > 0) {
-      timed(c, st, DMR_PH_ORTH, 0.0,
+      timed(c, st, DMR_PH_REDUCE, reduce_bytes(gramParts, sc, sc),
             [&] { dmr::blk::gram_reduce(c->part, gramParts, sc, c->G64, st); });
     } else {
-      timedf(c, st, DMR_PH_ORTH, 4.0 * n * sc, 2.0 * n * sc * sc, [&] {
-        tn(c, U, sc, U, sc, c->G64, sc, nullptr, true, st);
-      });
+      tn(c, DMR_PH_ORTH, 4.0 * n * sc, 2.0 * n * sc * sc, U, sc, U, sc, c->G64, sc, nullptr, true,
+         st);
     }
     timed(c, st, DMR_PH_ORTH, 0.0, [&] {
       dmr::blk::chol_qr(c->G64, sc, kDepTol, c->Sinv32, c->Sacc, pass == 0,
@@ -511,8 +518,9 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
   const int rows = p0 + (mb + 1) * s, cols = p0 + mb * s;
   float *U = c->Ub;
   const int gp = gram_parts(c, s);  // Gram matrices from the update's epilogue, when possible
-  // R's projection folded into the last block step (bf16 basis, matrix-core projection, s = 16).
-  const bool foldT = std::is_same<TP, bf16>::value && s == 16 && mb >= 1 &&
+  // R's projection folded into the last block step (bf16 basis, matrix-core projection,
+  // s <= 16: U and R as the two 16-column tiles of one pass).
+  const bool foldT = std::is_same<TP, bf16>::value && s <= 16 && mb >= 1 &&
                      c->tune.tnVarProj == 3 && c->foldT;
   const double vb = 4.0 * (double)n;              // one FP32 column
   const double pb = (double)sizeof(TP) * (double)n;  // one basis column
@@ -555,9 +563,8 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
   }
   TP *V0 = P + (long long)p0 * ld;
   if (deflate) {
-    timedf(c, st, DMR_PH_GEMVT, pb * k + vb * s, 2.0 * n * k * s, [&] {
-      tn(c, P, k, c->R, s, nullptr, 0, c->C32, false, st);
-    });
+    tn(c, DMR_PH_GEMVT, pb * k + vb * s, 2.0 * n * k * s, P, k, c->R, s, nullptr, 0, c->C32,
+       false, st);
     timedf(c, st, DMR_PH_UPDATE, pb * k + vb * 2 * s, 2.0 * n * k * s, [&] {
       if constexpr (std::is_same<TP, bf16>::value) {
         if (gp) {
@@ -582,17 +589,23 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
     // The last block step also projects the cycle's residual block R on P[:, 0:pj] in the same
     // pass (to T), leaving only the last block's projection for the end of the cycle.
     const bool withT = foldT && j == mb - 1;
-    timedf(c, st, DMR_PH_GEMVT, pb * pj + vb * s * (withT ? 2 : 1),
-           2.0 * n * pj * s * (withT ? 2 : 1), [&] {
-      if constexpr (std::is_same<TP, bf16>::value) {
-        if (withT) {
-          dmr::blk::gemm_tn_pair(P, ld, pj, U, c->R, ld, n, tn_blocks(c, c->tune.tnRowsProj),
-                                 c->part, Hcol, rows, c->C32, c->T, rows, st);
-          return;
-        }
+    bool paired = false;
+    if constexpr (std::is_same<TP, bf16>::value) {
+      if (withT) {
+        paired = true;
+        const int G = tn_blocks(c, c->tune.tnRowsProj);
+        timedf(c, st, DMR_PH_GEMVT, pb * pj + vb * s * 2, 2.0 * n * pj * s * 2, [&] {
+          dmr::blk::gemm_tn_pair(P, ld, pj, U, c->R, ld, s, n, G, c->part, Hcol, rows, c->C32,
+                                 c->T, rows, st, false);
+        });
+        timed(c, st, DMR_PH_REDUCE, reduce_bytes(G, pj, 32), [&] {
+          dmr::blk::gemm_tn_reduce(c->part, G, pj, s, Hcol, rows, c->C32, st, true, c->T, rows);
+        });
       }
-      tn(c, P, pj, U, s, Hcol, rows, c->C32, false, st);
-    });
+    }
+    if (!paired)
+      tn(c, DMR_PH_GEMVT, pb * pj + vb * s, 2.0 * n * pj * s, P, pj, U, s, Hcol, rows, c->C32,
+         false, st);
     timedf(c, st, DMR_PH_UPDATE, pb * pj + vb * 2 * s, 2.0 * n * pj * s, [&] {
       if constexpr (std::is_same<TP, bf16>::value) {
         if (gp) {
@@ -608,13 +621,11 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
   // Projection of the residual block on P (the part the last block step left), least squares,
   // residual and solution updates.
   if (foldT) {
-    timedf(c, st, DMR_PH_GEMVT, pb * s + vb * s, 2.0 * n * s * s, [&] {
-      tn(c, P + (long long)cols * ld, s, c->R, s, c->T + cols, rows, nullptr, false, st);
-    });
+    tn(c, DMR_PH_GEMVT, pb * s + vb * s, 2.0 * n * s * s, P + (long long)cols * ld, s, c->R, s,
+       c->T + cols, rows, nullptr, false, st);
   } else {
-    timedf(c, st, DMR_PH_GEMVT, pb * rows + vb * s, 2.0 * n * rows * s, [&] {
-      tn(c, P, rows, c->R, s, c->T, rows, nullptr, false, st);
-    });
+    tn(c, DMR_PH_GEMVT, pb * rows + vb * s, 2.0 * n * rows * s, P, rows, c->R, s, c->T, rows,
+       nullptr, false, st);
   }
   timed(c, st, DMR_PH_LSQ, 0.0, [&] {
     dmr::blk::lsq_block(c->H, rows, cols, s, c->T, s, c->rho, c->Hw, c->Tw, c->Y, c->Yr32,
@@ -726,7 +737,7 @@ int block_solve(dmrBlockContext *c, int mode, int s, double *X, long long ldx, c
   int cyclesLeft = maxCycles;
   for (int outer = 0;; ++outer) {
     const double valBytes = 8.0 * c->csr.nnz * (c->colVal ? s : 1);
-    timed(c, st, DMR_PH_SPMV, valBytes + 4.0 * c->csr.nnz + 4.0 * (n + 1) + 24.0 * n * s, [&] {
+    timed(c, st, DMR_PH_RESID, valBytes + 4.0 * c->csr.nnz + 4.0 * (n + 1) + 24.0 * n * s, [&] {
       dmr::blk::resid_block(c->csr, c->colVal, c->colLd, X, ldx, B, ldb, c->R64, ld, s, st);
     });
     c->st.matvecs += s;

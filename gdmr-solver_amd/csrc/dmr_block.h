@@ -48,20 +48,26 @@ void colnorms(const T *X, long long ld, int s, long long n, double *partial, dou
 int gemm_tn_blocks(long long n, int cap, int rowsTarget);
 size_t gemm_tn_lds(int pPad);
 // variant 0: register-direct 32x32x2 MFMA; 1: LDS-staged; 2: register-direct 16x16x4 MFMA.
+// reduce = false launches the partials only; gemm_tn_reduce then sums them (the same two launches
+// as reduce = true, for callers that time them apart).
 void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu, int s,
              long long n, int G, int variant, float *part, double *C64, int ldc, float *C32,
-             hipStream_t st);
+             hipStream_t st, bool reduce = true);
 // bf16 P (basis storage), FP32 U: variant 3 uses the bf16 matrix cores (U split into hi + lo
 // bf16 parts), any other the FP32 ones on unpacked values.
 void gemm_tn(const bf16 *P, long long ldp, int p, const float *U, long long ldu, int s,
              long long n, int G, int variant, float *part, double *C64, int ldc, float *C32,
-             hipStream_t st);
-// Two 16-column blocks U and U2 (pitch ldu) against a bf16 P in one pass (the matrix-core
-// kernel): P^T U to C64a (ld ldca) and C32 (row-major p x 32, columns 16 .. 31 zero), P^T U2 to
-// C64b (ld ldcb).  part as for gemm_tn (pPad x 32 floats per row range).
+             hipStream_t st, bool reduce = true);
+// The fixed-order sum of gemm_tn's G partials into C64 / C32 (as gemm_tn describes them).  pair:
+// the two-block layout of gemm_tn_pair (s columns of each block, the second block to C64b).
+void gemm_tn_reduce(const float *part, int G, int p, int s, double *C64, int ldc, float *C32,
+                    hipStream_t st, bool pair = false, double *C64b = nullptr, int ldcb = 0);
+// Two blocks U and U2 of s <= 16 columns each (pitch ldu) against a bf16 P in one pass (the
+// matrix-core kernel): P^T U to C64a (ld ldca) and C32 (row-major p x 32, columns s .. 31 zero),
+// P^T U2 to C64b (ld ldcb).  part as for gemm_tn (pPad x 32 floats per row range).
 void gemm_tn_pair(const bf16 *P, long long ldp, int p, const float *U, const float *U2,
-                  long long ldu, long long n, int G, float *part, double *C64a, int ldca,
-                  float *C32, double *C64b, int ldcb, hipStream_t st);
+                  long long ldu, int s, long long n, int G, float *part, double *C64a, int ldca,
+                  float *C32, double *C64b, int ldcb, hipStream_t st, bool reduce = true);
 // Out = a In + b P[:, 0:p] C (C row-major p x 32; In unused when a == 0).  Out may alias In or P.
 // un (2, 4, 8 or 16): column pairs of P loaded ahead of their MFMAs.
 void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *In, long long ldi,

@@ -617,7 +617,8 @@ __device__ inline void split4(float x0, float x1, float x2, float x3, s16x4 &hi,
 // column, so a column's four lanes read 128 contiguous bytes), and MFMA group g (g = 0..3) takes
 // rows r0 + 16kk + 4g .. + 3 as K indices 4kk .. 4kk + 3, once with the hi and once with the lo
 // part of U.
-// U2 (NU = 2): columns 16 .. 31 of U are columns 0 .. 15 of U2 (pitch ldu).
+// U2 (NU = 2): columns 16 .. 15 + s2 of the product are columns 0 .. s2 - 1 of U2 (pitch ldu),
+// and U has s <= 16 columns.
 template <int TPW, int NU>
 __global__ __launch_bounds__(kThreads) void k_gemm_tn16b(const bf16 *__restrict__ P, long long ldp,
                                                          int p, int pPad,
@@ -625,7 +626,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn16b(const bf16 *__restrict_
                                                          long long ldu, int s, long long n,
                                                          long long rowsPerBlock,
                                                          float *__restrict__ part,
-                                                         const float *__restrict__ U2) {
+                                                         const float *__restrict__ U2, int s2) {
   __shared__ float red[4 * TPW * NU * 256];
   const int nT = (p + 15) / 16;
   const int nTG = (nT + TPW - 1) / TPW;
@@ -648,8 +649,12 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn16b(const bf16 *__restrict_
   }
   const float *uc[NU];
 #pragma unroll
-  for (int u = 0; u < NU; ++u)
-    uc[u] = u * 16 + m >= s ? nullptr : (u == 1 && U2 ? U2 + m * ldu : U + (u * 16 + m) * ldu);
+  for (int u = 0; u < NU; ++u) {
+    if (u == 1 && U2)
+      uc[u] = m < s2 ? U2 + m * ldu : nullptr;
+    else
+      uc[u] = u * 16 + m < s ? U + (u * 16 + m) * ldu : nullptr;
+  }
   if (rg < nRG) {
     const long long step = 64LL * nRG;
     long long r0 = rBeg + 64 * rg;
@@ -1056,14 +1061,15 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn_lds(const float *__restric
 // j * ldc + i) and to C32 (row-major p x 32, zero beyond s).  One block per row i; JW = 16 (s <= 16,
 // only the columns that hold data are read) or 32 columns: thread (j, q) = (t % JW, t / JW) sums
 // partials g = q, q + NQ, ... (NQ = 256 / JW) in order, then the NQ sums are added in order of q
-// (a fixed order, independent of timing).
+// (a fixed order, independent of timing).  split > 0 (JW = 32): columns 0 .. s - 1 (s <= split)
+// go to C64 and C32, columns split .. split + s2 - 1 to C64b (its columns 0 .. s2 - 1).
 template <int JW>
 __global__ __launch_bounds__(kThreads) void k_gemm_tn_reduce(const float *__restrict__ part,
                                                              int G, int pPad, int p, int s,
                                                              double *__restrict__ C64, int ldc,
                                                              float *__restrict__ C32,
                                                              double *__restrict__ C64b, int ldcb,
-                                                             int split) {
+                                                             int split, int s2) {
   constexpr int NQ = kThreads / JW;
   __shared__ double red[kThreads];
   const int i = blockIdx.x, t = threadIdx.x, j = t % JW, q = t / JW;
@@ -1081,25 +1087,27 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn_reduce(const float *__rest
 #pragma unroll
     for (int w = 0; w < NQ; ++w) sum = sum + red[w * JW + t];
   }
-  if (t >= s) sum = 0.0;
   if (split > 0 && t >= split) {
-    // Columns split .. s - 1 go to C64b (its columns 0 ..), and are zero in C32.
-    if (C64b && t < s) C64b[(long long)(t - split) * ldcb + i] = sum;
+    // Columns split .. split + s2 - 1 go to C64b (its columns 0 ..), and are zero in C32.
+    if (C64b && t - split < s2) C64b[(long long)(t - split) * ldcb + i] = sum;
     if (C32) C32[i * 32 + t] = 0.0f;
-  } else {
+    return;
+  }
+  if (t >= s) sum = 0.0;
+  {
     if (C64 && t < s) C64[(long long)t * ldc + i] = sum;
     if (C32) C32[i * 32 + t] = (float)sum;
   }
 }
 
 void tn_reduce(const float *part, int G, int pPad, int p, int s, double *C64, int ldc, float *C32,
-               hipStream_t st, double *C64b = nullptr, int ldcb = 0, int split = 0) {
-  if (s <= 16)
+               hipStream_t st, double *C64b = nullptr, int ldcb = 0, int split = 0, int s2 = 0) {
+  if (s <= 16 && split == 0)
     hipLaunchKernelGGL(k_gemm_tn_reduce<16>, dim3(p), dim3(kThreads), 0, st, part, G, pPad, p, s,
-                       C64, ldc, C32, C64b, ldcb, split);
+                       C64, ldc, C32, C64b, ldcb, split, s2);
   else
     hipLaunchKernelGGL(k_gemm_tn_reduce<32>, dim3(p), dim3(kThreads), 0, st, part, G, pPad, p, s,
-                       C64, ldc, C32, C64b, ldcb, split);
+                       C64, ldc, C32, C64b, ldcb, split, s2);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1945,25 +1953,25 @@ long long tn_rows(long long n, int G) {
 }
 
 // bf16 P with the matrix-core kernel (variant 3) or the FP32 one (else); U2 as for k_gemm_tn16b
-// (variant 3, s = 32), the result's columns split .. 31 then going to C64b.
+// (variant 3, s <= 16 columns of U and s2 of U2), the product of U2 then going to C64b.
 void gemm_tn_b(const bf16 *P, long long ldp, int p, const float *U, const float *U2,
                long long ldu, int s, long long n, int G, int variant, float *part, double *C64,
-               int ldc, float *C32, double *C64b, int ldcb, int split, hipStream_t st) {
+               int ldc, float *C32, double *C64b, int ldcb, int s2, hipStream_t st, bool reduce) {
   const int pPad = (p + 31) & ~31;
   const long long rows = tn_rows(n, G);
   if (variant == 3) {
     const int nT = (p + 15) / 16;
     const int nTG = nT <= 6 ? 1 : (nT <= 12 ? 2 : 4);
     const int tpw = (nT + nTG - 1) / nTG;
-    const int nu = s <= 16 ? 1 : 2;
+    const int nu = (s <= 16 && !U2) ? 1 : 2;
 #define DMR_TN16B(T)                                                                             \
   if (tpw == T) {                                                                                \
     if (nu == 1)                                                                                 \
       hipLaunchKernelGGL((k_gemm_tn16b<T, 1>), dim3(G), dim3(kThreads), 0, st, P, ldp, p, pPad,  \
-                         U, ldu, s, n, rows, part, U2);                                          \
+                         U, ldu, s, n, rows, part, U2, s2);                                      \
     else                                                                                         \
       hipLaunchKernelGGL((k_gemm_tn16b<T, 2>), dim3(G), dim3(kThreads), 0, st, P, ldp, p, pPad,  \
-                         U, ldu, s, n, rows, part, U2);                                          \
+                         U, ldu, s, n, rows, part, U2, s2);                                      \
   }
     DMR_TN16B(1)
     DMR_TN16B(2)
@@ -1975,24 +1983,31 @@ void gemm_tn_b(const bf16 *P, long long ldp, int p, const float *U, const float 
   } else {
     gemm_tn16<bf16>(P, ldp, p, pPad, U, ldu, s, n, G, rows, part, st);
   }
-  tn_reduce(part, G, pPad, p, s, C64, ldc, C32, st, C64b, ldcb, split);
+  if (reduce) tn_reduce(part, G, pPad, p, s, C64, ldc, C32, st, C64b, ldcb, U2 ? 16 : 0, s2);
 }
 
 void gemm_tn(const bf16 *P, long long ldp, int p, const float *U, long long ldu, int s,
              long long n, int G, int variant, float *part, double *C64, int ldc, float *C32,
-             hipStream_t st) {
-  gemm_tn_b(P, ldp, p, U, nullptr, ldu, s, n, G, variant, part, C64, ldc, C32, nullptr, 0, 0, st);
+             hipStream_t st, bool reduce) {
+  gemm_tn_b(P, ldp, p, U, nullptr, ldu, s, n, G, variant, part, C64, ldc, C32, nullptr, 0, 0, st,
+            reduce);
 }
 
 void gemm_tn_pair(const bf16 *P, long long ldp, int p, const float *U, const float *U2,
-                  long long ldu, long long n, int G, float *part, double *C64a, int ldca,
-                  float *C32, double *C64b, int ldcb, hipStream_t st) {
-  gemm_tn_b(P, ldp, p, U, U2, ldu, 32, n, G, 3, part, C64a, ldca, C32, C64b, ldcb, 16, st);
+                  long long ldu, int s, long long n, int G, float *part, double *C64a, int ldca,
+                  float *C32, double *C64b, int ldcb, hipStream_t st, bool reduce) {
+  gemm_tn_b(P, ldp, p, U, U2, ldu, s, n, G, 3, part, C64a, ldca, C32, C64b, ldcb, s, st, reduce);
+}
+
+void gemm_tn_reduce(const float *part, int G, int p, int s, double *C64, int ldc, float *C32,
+                    hipStream_t st, bool pair, double *C64b, int ldcb) {
+  tn_reduce(part, G, (p + 31) & ~31, p, s, C64, ldc, C32, st, C64b, ldcb, pair ? 16 : 0,
+            pair ? s : 0);
 }
 
 void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu, int s,
              long long n, int G, int variant, float *part, double *C64, int ldc, float *C32,
-             hipStream_t st) {
+             hipStream_t st, bool reduce) {
   const int pPad = (p + 31) & ~31;
   const long long rows = tn_rows(n, G);
   if (variant == 2) {
@@ -2005,7 +2020,7 @@ void gemm_tn(const float *P, long long ldp, int p, const float *U, long long ldu
     hipLaunchKernelGGL(k_gemm_tn, dim3(G), dim3(kThreads), 0, st, P, ldp, p, pPad, U, ldu, s, n,
                        rows, part);
   }
-  tn_reduce(part, G, pPad, p, s, C64, ldc, C32, st);
+  if (reduce) tn_reduce(part, G, pPad, p, s, C64, ldc, C32, st);
 }
 
 template <typename TP, typename TO>

@@ -125,7 +125,8 @@ _sig("dmrBlockSetOptions", C.c_int, [_vp, C.c_int, C.c_int])
 _sig("dmrBlockGetTuning", C.c_int, [_vp, _ip])
 _sig("dmrSetProfiling", C.c_int, [_vp, C.c_int])
 _sig("dmrGetProfile", C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_longlong)])
-PHASES = ("spmv", "gemvt", "update", "scale", "combo", "zgemm", "lsq", "svd", "orth")
+PHASES = ("spmv", "gemvt", "update", "scale", "combo", "zgemm", "lsq", "svd", "orth", "reduce",
+          "resid")
 # Problems and random numbers.
 _sig("dmrGenConvDiff2D", C.c_longlong, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.c_ulonglong, C.c_ulonglong, _ip, _ip, _dp])

@@ -151,7 +151,9 @@ enum {
   DMR_PH_LSQ = 6,     /* Hessenberg least squares                               */
   DMR_PH_SVD = 7,     /* Hessenberg Jacobi SVD                                  */
   DMR_PH_ORTH = 8,    /* fused Gram-Schmidt step (projections, updates, norms)  */
-  DMR_NPHASES = 9
+  DMR_PH_REDUCE = 9,  /* fixed-order sums of block projection partials          */
+  DMR_PH_RESID = 10,  /* FP64 refinement residuals b - A x                      */
+  DMR_NPHASES = 11
 };
 int dmrSetProfiling(dmrHandle h, int on);
 /* Accumulated milliseconds, algorithmic bytes and launch counts per phase since profiling was

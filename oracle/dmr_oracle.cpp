@@ -26,10 +26,15 @@
 // independent textbook GMRES(m), Arnoldi relation and orthogonality, breakdown on a diagonal
 // operator with few distinct eigenvalues, and FP64-checked true residuals.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace {
 
@@ -691,6 +696,49 @@ void oracle_spmv(int n, const int *rp, const int *ci, const double *va, const do
   A.ci.assign(ci, ci + rp[n]);
   A.va.assign(va, va + rp[n]);
   spmv(A, x, y);
+}
+
+// OpenMP threads of the oracle's parallel loops (the CPU baseline times it at 1 thread and at
+// the host's core count).
+void oracle_set_threads(int threads) {
+#ifdef _OPENMP
+  omp_set_num_threads(threads > 0 ? threads : 1);
+#else
+  (void)threads;
+#endif
+}
+
+int oracle_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+// Host DRAM bandwidth: a[i] = b[i] + 3 c[i] over three arrays of n doubles with the current
+// thread count, best of `reps` passes after a first-touch pass; returns bytes per second
+// (24 n bytes moved per pass).
+double oracle_stream_triad(long long n, int reps) {
+  std::vector<double> a(n), b(n), c(n);
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < n; ++i) {
+    a[i] = 0.0;
+    b[i] = 1.0;
+    c[i] = 2.0;
+  }
+  double best = 0.0;
+  for (int r = 0; r < reps; ++r) {
+    const auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n; ++i) a[i] = b[i] + 3.0 * c[i];
+    const double dt =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (dt > 0.0) best = std::max(best, 24.0 * (double)n / dt);
+  }
+  volatile double sink = a[n / 2];
+  (void)sink;
+  return best;
 }
 
 // Dot product in the device reduction order with G blocks.

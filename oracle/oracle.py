@@ -45,6 +45,24 @@ _sig("oracle_lsq", None, [_dp, C.c_int, _dp, _dp, _dp])
 _sig("oracle_svd", C.c_int, [_dp, C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp])
 _sig("oracle_spmv", None, [C.c_int, _ip, _ip, _dp, _dp, _dp])
 _sig("oracle_dot", C.c_double, [_dp, _dp, C.c_longlong, C.c_int])
+_sig("oracle_set_threads", None, [C.c_int])
+_sig("oracle_get_threads", C.c_int, [])
+_sig("oracle_stream_triad", C.c_double, [C.c_longlong, C.c_int])
+
+
+def set_threads(threads: int):
+    """OpenMP threads of the oracle's loops."""
+    lib.oracle_set_threads(threads)
+
+
+def get_threads() -> int:
+    return lib.oracle_get_threads()
+
+
+def stream_triad(n: int = 1 << 26, reps: int = 5) -> float:
+    """Host DRAM bandwidth in bytes/s (a = b + 3c over three arrays of n doubles, best of reps)
+    with the current thread count."""
+    return lib.oracle_stream_triad(n, reps)
 
 
 def _d(a):

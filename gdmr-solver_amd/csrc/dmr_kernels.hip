@@ -20,8 +20,11 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
+#include <string>
 
 #include "dmr_internal.h"
+#include "gDMR.h"
 
 namespace dmr {
 namespace {
@@ -884,6 +887,42 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
   }
 }
 
+// The same product without a precomputed row partition: workgroup g takes the fixed rows
+// [256 g, 256 g + 256) of n.  A row of at most kSpmvNnzCap entries is summed by its thread over
+// its products in column order (the sum k_spmv forms from LDS, so the bits are the same); a
+// longer row is reduced by the whole workgroup with k_spmv's strided order and block_sum.
+__global__ __launch_bounds__(kBlock) void k_spmv_rows(int n, const int *__restrict__ rp,
+                                                      const int *__restrict__ ci,
+                                                      const double *__restrict__ va,
+                                                      const double *__restrict__ x,
+                                                      double *__restrict__ y) {
+  const int r0 = blockIdx.x * kBlock;
+  const int r = r0 + (int)threadIdx.x;
+  if (r < n) {
+    const int a = rp[r], b = rp[r + 1];
+    if (b - a <= kSpmvNnzCap) {
+      double s = 0.0;
+      for (int e = a; e < b; ++e) {
+        const double t = va[e] * x[ci[e]];
+        s = s + t;
+      }
+      y[r] = s;
+    }
+  }
+  const int rEnd = r0 + kBlock < n ? r0 + kBlock : n;
+  for (int q = r0; q < rEnd; ++q) {  // uniform over the workgroup
+    const int e0 = rp[q], e1 = rp[q + 1];
+    if (e1 - e0 <= kSpmvNnzCap) continue;
+    double s = 0.0;
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlock) {
+      const double t = va[e] * x[ci[e]];
+      s = s + t;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) y[q] = s;
+  }
+}
+
 // r = b - A x in FP64 with a fixed-order sum of squares (the refinement residual).  Block
 // partials are per SpMV row block.
 __global__ __launch_bounds__(kBlock) void k_resid64(const int *__restrict__ rp,
@@ -1185,3 +1224,57 @@ template void convert<bf16, double>(bf16 *, const double *, long long, hipStream
 template void convert<double, bf16>(double *, const bf16 *, long long, hipStream_t);
 
 }  // namespace dmr
+
+// =============================================================================================
+// In-place CSR product without allocation (dmrCsrApplyStatic in gDMRC.h).  The product is staged
+// in a static device array, `scratch`, of kStaticRows doubles, then copied over x.  Host code
+// uses the array through the address hipGetSymbolAddress returns for the current device (the
+// array's name is a host-side stand-in), looked up once per device on its first call.
+// =============================================================================================
+namespace {
+constexpr long long kStaticRows = 1LL << 22;  // 32 MiB per GPU
+constexpr int kStaticDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) double scratch[kStaticRows];
+
+namespace {
+std::mutex g_scratchMu;
+double *g_scratchPtr[kStaticDevices] = {};
+}  // namespace
+
+extern "C" int dmrCsrApplyStatic(int n, int nnz, const int *rowPtr, const int *colInd,
+                                 const double *val, double *x, hipStream_t stream) {
+  dmr::clear_error();
+  if (n < 1 || nnz < 0 || !rowPtr || !colInd || !val || !x)
+    return dmr::set_error(DMR_ERR_ARG, "dmrCsrApplyStatic: invalid arguments");
+  if ((long long)n > kStaticRows)
+    return dmr::set_error(DMR_ERR_ARG,
+                          "dmrCsrApplyStatic: n exceeds the static staging array (4194304 rows)");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kStaticDevices)
+    return dmr::set_error(DMR_ERR_HIP, "dmrCsrApplyStatic: no current device");
+  double *y = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (!g_scratchPtr[dev]) {
+      void *ptr = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !ptr)
+        return dmr::set_error(DMR_ERR_HIP, (std::string("dmrCsrApplyStatic: hipGetSymbolAddress: ") +
+                                            hipGetErrorString(e)).c_str());
+      g_scratchPtr[dev] = static_cast<double *>(ptr);
+    }
+    y = g_scratchPtr[dev];
+  }
+  const unsigned grid = (unsigned)((n + dmr::kBlock - 1) / dmr::kBlock);
+  hipLaunchKernelGGL(dmr::k_spmv_rows, dim3(grid), dim3(dmr::kBlock), 0, stream, n, rowPtr,
+                     colInd, val, (const double *)x, y);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(x, y, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, stream);
+  if (e != hipSuccess)
+    return dmr::set_error(DMR_ERR_HIP, (std::string("dmrCsrApplyStatic: ") +
+                                        hipGetErrorString(e)).c_str());
+  return DMR_OK;
+}

@@ -125,6 +125,7 @@ _sig("dmrBlockSetOptions", C.c_int, [_vp, C.c_int, C.c_int])
 _sig("dmrBlockGetTuning", C.c_int, [_vp, _ip])
 _sig("dmrSetProfiling", C.c_int, [_vp, C.c_int])
 _sig("dmrGetProfile", C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_longlong)])
+_sig("dmrCsrApplyStatic", C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp])
 PHASES = ("spmv", "gemvt", "update", "scale", "combo", "zgemm", "lsq", "svd", "orth", "reduce",
           "resid")
 # Problems and random numbers.
@@ -151,7 +152,7 @@ HEADER_SYMBOLS = (
     "dmrImportZ dmrExportZ "
     "dmrHasZ dmrGetStatsH dmrSetRefinement dmrGlobalHandle dmrFootprint dmrSetProfiling "
     "dmrGetProfile dmrSubspaceBytes dmrSubspaceToDevice dmrSubspaceFromDevice "
-    "dmrGetStagnationCount dmrSetStagnationCount "
+    "dmrGetStagnationCount dmrSetStagnationCount dmrCsrApplyStatic "
     "dmrBlockCreate dmrBlockDestroy dmrBlockSetOperatorCsr dmrBlockSetOperatorValues "
     "dmrBlockSetRefinement dmrBlockSetColumnOperators dmrBlockSetPrecision dmrBlockSolve dmrBlockGetStats dmrBlockHasZ dmrBlockImportZ "
     "dmrBlockExportZ dmrBlockSetProfiling dmrBlockGetProfile dmrBlockFootprint "

@@ -159,6 +159,18 @@ int dmrSetProfiling(dmrHandle h, int on);
 /* Accumulated milliseconds, algorithmic bytes and launch counts per phase since profiling was
  * switched on; each array has DMR_NPHASES entries (any may be NULL).  Returns DMR_NPHASES. */
 int dmrGetProfile(dmrHandle h, double *ms, double *bytes, long long *launches);
+/* x <- A x in place, allocating nothing: the contract of linOptLS (x <- A x on the device source
+ * vector, testLS.cu:97-128) for a CSR operator and vector passed in as device pointers (rowPtr
+ * n+1 and colInd nnz int32, val nnz doubles, x n doubles), enqueued on stream.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The product is staged in a
+ * static device array of 4,194,304 doubles (32 MiB, held on every GPU the library runs on until
+ * the process ends) and copied over x; n above that is rejected with DMR_ERR_ARG.  Every call on
+ * one GPU stages through the same array, so calls that overlap on one GPU race.  Each row sums its
+ * products in the order linOptLS does, so the results are the same bits.  Returns 0 or a
+ * DMR_ERR_* code (also through dmrGetLastError). */
+int dmrCsrApplyStatic(int n, int nnz, const int *rowPtr, const int *colInd, const double *val,
+                      double *x, hipStream_t stream);
 /* Bytes of HBM a handle of this shape allocates (for admission by free HBM). */
 size_t dmrFootprint(int elements, int basisDim, int deflatDim, int precision);
 

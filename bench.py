@@ -13,10 +13,11 @@ Three legs run on the same sequence, each in a fresh solver context:
                 (balanced sizes; 16 systems: one block, 20: two of 10), one after the other on
                 one handle that recycles the deflation subspace from block to block.  Column t is
                 refined in FP64 against its own operator A_t; the inner block DMR (block
-                Gram-Schmidt as tall-skinny GEMMs on the matrix cores, 16 block steps, recycled
-                subspace k = 20, basis stored as bf16 and multiplied on the bf16 matrix cores
-                with FP32 operands split into hi + lo parts, FP32 accumulation) shares the
-                entrywise mean of its block's A_t.  Every system's latency is its block's.
+                Gram-Schmidt as tall-skinny GEMMs on the matrix cores, round(16 sqrt(16 / s))
+                block steps for blocks of s systems, recycled subspace k = 20, basis stored as
+                bf16 and multiplied on the bf16 matrix cores with FP32 operands split into
+                hi + lo parts, FP32 accumulation) shares the entrywise mean of its block's A_t.
+                Every system's latency is its block's.
 A step is one solve of one system.  `value` is the solves/sec of the fastest leg whose every
 system reaches 1e-10 checked in FP64.  Every solve's true residual ||b - A x|| / ||b|| (with its
 own operator) is recomputed on the host in FP64 after the timed region.
@@ -179,20 +180,35 @@ def seq_groups(count, block_max):
     return groups
 
 
+def seq_block_steps(s, k, max_cols=384):
+    """Block steps per cycle for blocks of s systems: round(16 sqrt(16 / s)), the measured best
+    on the 128^3 sequence (16 steps for blocks of 16, 20 for blocks of 10), capped so that the
+    basis k + (steps + 1) s fits max_cols columns."""
+    steps = int(round(16.0 * (16.0 / s) ** 0.5))
+    return max(1, min(steps, (max_cols - k) // s - 1))
+
+
 def run_seqblock_leg(mats, rhs, dev_vals, rp, ci, mb, k, tol, warmup, inner_tol, profile,
-                     prec="bf16", block_max=16):
+                     prec="bf16", block_max=16, concurrent=1):
     """The sequence as block solves of at most block_max systems each (consecutive systems,
     balanced sizes): column t is system t with its own FP64 operator A_t in the refinement
     residual, while the inner FP32 block DMR (block Gram-Schmidt on the matrix cores) shares the
-    entrywise mean of its block's A_t (dmrBlockSetColumnOperators).  One handle solves the blocks
-    one after the other and recycles the deflation subspace from block to block.  Every system's
-    latency is its block's solve time."""
+    entrywise mean of its block's A_t (dmrBlockSetColumnOperators).  concurrent = 1: one handle
+    solves the blocks one after the other and recycles the deflation subspace from block to
+    block.  concurrent = c > 1: the blocks are dealt to c handles, each with its own stream and
+    host thread, which run at the same time (each recycling its subspace over its own blocks), so
+    one block's latency-bound steps (least squares, Cholesky factors, partial sums, the host's
+    per-cycle decision) overlap another's streaming kernels.  Every system's latency is its
+    block's solve time."""
+    import threading
+
     import torch
 
     import gdmr
     n, nnz = mats[0].n, mats[0].nnz
     groups = seq_groups(len(mats), block_max)
     smax = max(g1 - g0 for g0, g1 in groups)
+    concurrent = max(1, min(concurrent, len(groups)))
     vals = torch.stack(dev_vals).contiguous()  # count x nnz, system t's operator at row t
     B = torch.stack([torch.from_numpy(b) for b in rhs]).to("cuda").contiguous()
 
@@ -203,32 +219,63 @@ def run_seqblock_leg(mats, rhs, dev_vals, rp, ci, mb, k, tol, warmup, inner_tol,
         h.set_precision(gdmr.PREC_BF16 if prec == "bf16" else gdmr.PREC_FP32)
         return h
 
-    def solve_all(h, X, recycled):
-        out = []
-        for gi, (g0, g1) in enumerate(groups):
+    def solve_groups(h, X, gids, recycled, stream=0):
+        out = {}
+        for i, gi in enumerate(gids):
+            g0, g1 = groups[gi]
             h.set_column_operators(vals[g0].data_ptr(), nnz, g1 - g0, inner_mean=True)
             ts = time.perf_counter()
             rc, rel = h.solve(X[g0].data_ptr(), n, B[g0].data_ptr(), n, g1 - g0, tol,
-                              max_cycles=100000, mode=1 if (gi or recycled) else 0)
+                              max_cycles=100000, mode=1 if (i or recycled) else 0, stream=stream)
             dt = time.perf_counter() - ts
             st, _ = h.stats()
-            out.append(dict(seconds=dt, rc=rc, rel=rel, cycles=st.cycles))
+            out[gi] = dict(seconds=dt, rc=rc, rel=rel, cycles=st.cycles)
         return out
 
+    def solve_all(hs, X, recycled, streams=None):
+        """Blocks dealt round-robin to the handles; with streams, one thread per handle."""
+        deal = [list(range(j, len(groups), len(hs))) for j in range(len(hs))]
+        if streams is None:
+            out = {}
+            for h, gids in zip(hs, deal):
+                out.update(solve_groups(h, X, gids, recycled))
+            return [out[gi] for gi in range(len(groups))]
+        res = [None] * len(hs)
+
+        def work(j):
+            res[j] = solve_groups(hs[j], X, deal[j], recycled, streams[j].cuda_stream)
+            streams[j].synchronize()
+
+        th = [threading.Thread(target=work, args=(j,)) for j in range(len(hs))]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        out = {}
+        for r in res:
+            out.update(r)
+        return [out[gi] for gi in range(len(groups))]
+
+    # Warm-up on throwaway handles, one after the other (it also tunes the launch shapes, which
+    # are cached per shape for the process), then fresh handles for the timed sequence.
     for w in range(warmup):
         h = make()
         X = torch.zeros(len(mats), n, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
-        solve_all(h, X, False)
+        solve_all([h], X, False)
         h.close()
         log(f"[seqblock] warm-up {w} done")
-    h = make()
+    hs = [make() for _ in range(concurrent)]
+    streams = [torch.cuda.Stream() for _ in hs] if concurrent > 1 else None
     X = torch.zeros(len(mats), n, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    blocks = solve_all(h, X, False)
+    blocks = solve_all(hs, X, False, streams)
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
+    for extra in hs[1:]:
+        extra.close()
+    h = hs[0]
     for (g0, g1), b in zip(groups, blocks):
         log(f"[seqblock] systems {g0}..{g1 - 1} as one block: {b['seconds'] * 1e3:.1f} ms, rc "
             f"{b['rc']}, {b['cycles']} cycles, max residual {b['rel'].max():.3e}")
@@ -241,7 +288,7 @@ def run_seqblock_leg(mats, rhs, dev_vals, rp, ci, mb, k, tol, warmup, inner_tol,
         h.set_profiling(True)
         Xp = torch.zeros(len(mats), n, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
-        solve_all(h, Xp, True)
+        solve_all([h], Xp, True)
         prof = h.profile()
         h.set_profiling(False)
     h.close()
@@ -255,7 +302,7 @@ def run_seqblock_leg(mats, rhs, dev_vals, rp, ci, mb, k, tol, warmup, inner_tol,
         reported += list(b["rel"])
     return dict(total=total, lat=lat, cycles=cycles, reported=reported, true=true, prof=prof,
                 x=[Xh[t] for t in range(len(mats))], block=True, tuning=tuning,
-                blocks=[g1 - g0 for g0, g1 in groups])
+                blocks=[g1 - g0 for g0, g1 in groups], concurrent=concurrent)
 
 
 def summarize(leg, tol):
@@ -394,8 +441,9 @@ def main():
                     help="comma list of fp64, mixed (FP32 basis), seqblock (the sequence as one "
                          "block), bf16 (bf16 basis storage) and block1 (the block solver with one "
                          "right-hand side)")
-    ap.add_argument("--seq-block-steps", type=int, default=16,
-                    help="block steps per cycle of the seqblock leg")
+    ap.add_argument("--seq-block-steps", type=int, default=0,
+                    help="block steps per cycle of the seqblock leg (0: by block width, "
+                         "round(16 sqrt(16 / s)): 16 for blocks of 16, 20 for blocks of 10)")
     ap.add_argument("--seq-k", type=int, default=20, help="recycled subspace of the seqblock leg")
     ap.add_argument("--seq-inner-tol", type=float, default=3e-2,
                     help="relative tolerance of each inner block solve of the seqblock leg")
@@ -404,6 +452,9 @@ def main():
     ap.add_argument("--seq-block-max", type=int, default=16,
                     help="most systems per block of the seqblock leg (the sequence is split into "
                          "balanced blocks of at most this many)")
+    ap.add_argument("--seq-concurrent", type=int, default=1,
+                    help="blocks of the seqblock leg solved at the same time, each on its own "
+                         "handle, stream and host thread")
     ap.add_argument("--cpu-cycles", type=int, default=5,
                     help="restart cycles of the CPU baseline sample at the host's core count")
     ap.add_argument("--cpu-cycles-1t", type=int, default=2,
@@ -452,10 +503,13 @@ def main():
     legs = {}
     for name in args.legs.split(","):
         if name == "seqblock":
+            if args.seq_block_steps <= 0:
+                smax = max(g1 - g0 for g0, g1 in seq_groups(len(mats), args.seq_block_max))
+                args.seq_block_steps = seq_block_steps(smax, args.seq_k)
             legs[name] = run_seqblock_leg(mats, rhs, dev_vals, rp, ci, args.seq_block_steps,
                                           args.seq_k, tol, args.warmup, args.seq_inner_tol,
                                           not args.no_profile, args.seq_prec,
-                                          args.seq_block_max)
+                                          args.seq_block_max, args.seq_concurrent)
             continue
         if name == "block1":
             legs[name] = run_block1_leg(mats, rhs, dev_vals, rp, ci, m, k, tol, args.warmup,
@@ -504,6 +558,7 @@ def main():
                                              out_legs["fp64"]["solves_per_s"])
     if "seqblock" in legs:
         line["config"]["seqblock"] = {"blocks": legs["seqblock"]["blocks"],
+                                      "concurrent": legs["seqblock"]["concurrent"],
                                       "block_steps": args.seq_block_steps, "k": args.seq_k,
                                       "inner_tol": args.seq_inner_tol, "basis": args.seq_prec,
                                       "tuning": legs["seqblock"]["tuning"]}

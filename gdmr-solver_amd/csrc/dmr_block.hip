@@ -40,6 +40,13 @@ __device__ inline f32x16 mfma32(float a, float b, f32x16 c) {
 // Row i of the 32 x 32 MFMA result held in accumulator v by lane l (column l % 32).
 __device__ inline int mfma_row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
 
+// Per-workgroup partial products P^T U are stored row by row: element (row, col) of workgroup g
+// of a G-workgroup launch at (row * G + g) * 32 + col, so the G partials of one row are one
+// contiguous run that the fixed-order reduction streams with 16-byte loads.
+__device__ inline long long part_index(int row, int col) {
+  return ((long long)row * gridDim.x + blockIdx.x) * 32 + col;
+}
+
 // bf16 storage: the upper 16 bits of a binary32, rounded to nearest even on store.
 __device__ inline float bf_to_f(uint32_t bits16) { return __uint_as_float(bits16 << 16); }
 __device__ inline uint32_t f_to_bf(float f) {  // finite inputs
@@ -415,7 +422,6 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn(const float *__restrict__ 
   }
   __syncthreads();
   if (rg != 0) return;
-  float *out = part + (long long)blockIdx.x * pPad * 32;
 #pragma unroll
   for (int t = 0; t < kTPW; ++t) {
     const int tile = tg * kTPW + t;
@@ -424,7 +430,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn(const float *__restrict__ 
       for (int v = 0; v < 16; ++v) {
         float a = 0.0f;
         for (int g = 0; g < nRG; ++g) a = a + red[((g * nTG + tg) * kTPW + t) * 1024 + v * 64 + lane];
-        out[(tile * 32 + mfma_row(v, lane)) * 32 + i] = a;
+        part[part_index(tile * 32 + mfma_row(v, lane), i)] = a;
       }
     }
   }
@@ -565,7 +571,6 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn16(const TP *__restrict__ P
   }
   __syncthreads();
   if (rg != 0) return;
-  float *out = part + (long long)blockIdx.x * pPad * 32;
 #pragma unroll
   for (int t = 0; t < TPW; ++t) {
     const int tile = tg * TPW + t;
@@ -577,7 +582,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn16(const TP *__restrict__ P
           float a = 0.0f;
           for (int g = 0; g < nRG; ++g)
             a = a + red[(((g * nTG + tg) * TPW + t) * NU + u) * 256 + v * 64 + lane];
-          out[(tile * 16 + 4 * kk + v) * 32 + u * 16 + m] = a;
+          part[part_index(tile * 16 + 4 * kk + v, u * 16 + m)] = a;
         }
     }
   }
@@ -725,7 +730,6 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn16b(const bf16 *__restrict_
   }
   __syncthreads();
   if (rg != 0) return;
-  float *out = part + (long long)blockIdx.x * pPad * 32;
 #pragma unroll
   for (int t = 0; t < TPW; ++t) {
     const int tile = tg * TPW + t;
@@ -737,7 +741,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn16b(const bf16 *__restrict_
           float a = 0.0f;
           for (int g = 0; g < nRG; ++g)
             a = a + red[(((g * nTG + tg) * TPW + t) * NU + u) * 256 + v * 64 + lane];
-          out[(tile * 16 + 4 * kk + v) * 32 + u * 16 + m] = a;
+          part[part_index(tile * 16 + 4 * kk + v, u * 16 + m)] = a;
         }
     }
   }
@@ -755,8 +759,8 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn16b(const bf16 *__restrict_
 constexpr int kNnbTile = 128 * 16 + 16 * 4;  // floats per wavefront: 16 columns of 128 (+4 pad)
 
 // With gram (NT = 1 only), the workgroup also writes the partial Gram matrix Out^T Out of its
-// rows, gram[blockIdx.x * 1024 + i * 32 + j] for i, j < 16 (FP32 16x16x4 MFMA over the output
-// tile in LDS, the wavefronts added in order): the layout of gemm_tn's partials (pPad = 32).
+// rows, element (i, j) at gram[part_index(i, j)] for i, j < 16 (FP32 16x16x4 MFMA over the output
+// tile in LDS, the wavefronts added in order): the layout of gemm_tn's partials.
 template <int NT, typename TO, int RB>
 __global__ __launch_bounds__(kThreads) void k_gemm_nn_b(const bf16 *__restrict__ P, long long ldp,
                                                         int p, const float *__restrict__ C,
@@ -915,14 +919,14 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nn_b(const bf16 *__restrict__
     for (int v = 0; v < 4; ++v) red[v * 64 + lane] = gacc[v];
     __syncthreads();
     if (wave == 0) {
-      float *outp = gram + (long long)blockIdx.x * 1024;
+
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         float g = 0.0f;
 #pragma unroll
         for (int w = 0; w < 4; ++w)
           g = g + reinterpret_cast<const float *>(Cl)[w * kNnbTile + v * 64 + lane];
-        outp[(4 * kg + v) * 32 + m] = g;
+        gram[part_index(4 * kg + v, m)] = g;
       }
     }
   }
@@ -1045,47 +1049,80 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn_lds(const float *__restric
     }
     __syncthreads();
   }
-  float *out = part + (long long)blockIdx.x * pPad * 32;
 #pragma unroll
   for (int t = 0; t < kMaxTiles / 4; ++t) {
     const int tile = wave + 4 * t;
     if (tile < tiles) {
 #pragma unroll
       for (int v = 0; v < 16; ++v)
-        out[(tile * 32 + mfma_row(v, lane)) * 32 + (lane & 31)] = acc[t][v];
+        part[part_index(tile * 32 + mfma_row(v, lane), lane & 31)] = acc[t][v];
     }
   }
 }
 
 // C[i][j] = sum_g part_g[i][j] (FP64), for i < p, j < s: to C64 (column-major, element (i, j) at
-// j * ldc + i) and to C32 (row-major p x 32, zero beyond s).  One block per row i; JW = 16 (s <= 16,
-// only the columns that hold data are read) or 32 columns: thread (j, q) = (t % JW, t / JW) sums
-// partials g = q, q + NQ, ... (NQ = 256 / JW) in order, then the NQ sums are added in order of q
-// (a fixed order, independent of timing).  split > 0 (JW = 32): columns 0 .. s - 1 (s <= split)
-// go to C64 and C32, columns split .. split + s2 - 1 to C64b (its columns 0 .. s2 - 1).
+// j * ldc + i) and to C32 (row-major p x 32, zero beyond s).  One block of kRedThreads threads
+// per row i, whose G partials are contiguous (part_index); JW = 16 (s <= 16, only the columns
+// that hold data are read) or 32 columns, read four at a time: thread (c4, q) = (t % (JW / 4),
+// t / (JW / 4)) adds partials g = q, q + NQ, ... (NQ = kRedThreads / (JW / 4)) in order, all of a
+// batch's loads issued before its adds; then per column the NQ sums are added in order of q, in
+// kRedSeg consecutive segments and the segments in order (a fixed order, independent of timing).
+// split > 0 (JW = 32): columns 0 .. s - 1 (s <= split) go to C64 and C32, columns
+// split .. split + s2 - 1 to C64b (its columns 0 .. s2 - 1).
+constexpr int kRedThreads = 512;
+constexpr int kRedLoads = 8;  // partials loaded per batch and thread
+constexpr int kRedSeg = 8;
+
 template <int JW>
-__global__ __launch_bounds__(kThreads) void k_gemm_tn_reduce(const float *__restrict__ part,
-                                                             int G, int pPad, int p, int s,
-                                                             double *__restrict__ C64, int ldc,
-                                                             float *__restrict__ C32,
-                                                             double *__restrict__ C64b, int ldcb,
-                                                             int split, int s2) {
-  constexpr int NQ = kThreads / JW;
-  __shared__ double red[kThreads];
-  const int i = blockIdx.x, t = threadIdx.x, j = t % JW, q = t / JW;
-  double a = 0.0;
-  const float *src = part + (long long)i * 32 + j;
-  const long long stride = (long long)pPad * 32;
-#pragma unroll 8
-  for (int g = q; g < G; g += NQ) a = a + (double)src[g * stride];
-  red[t] = a;
+__global__ __launch_bounds__(kRedThreads) void k_gemm_tn_reduce(const float *__restrict__ part,
+                                                                int G, int p, int s,
+                                                                double *__restrict__ C64, int ldc,
+                                                                float *__restrict__ C32,
+                                                                double *__restrict__ C64b,
+                                                                int ldcb, int split, int s2) {
+  constexpr int TPG = JW / 4;          // threads per partial (four columns each)
+  constexpr int NQ = kRedThreads / TPG;
+  constexpr int QPS = NQ / kRedSeg;
+  __shared__ double red[NQ * JW];
+  __shared__ double seg[kRedSeg * JW];
+  const int i = blockIdx.x, t = threadIdx.x, c4 = t % TPG, q = t / TPG;
+  const float4 *src = reinterpret_cast<const float4 *>(part + (long long)i * G * 32) + c4;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (int g0 = q; g0 < G; g0 += NQ * kRedLoads) {
+    float4 v[kRedLoads];
+#pragma unroll
+    for (int u = 0; u < kRedLoads; ++u) {
+      const int g = g0 + u * NQ;
+      v[u] = g < G ? src[(long long)g * 8] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < kRedLoads; ++u) {
+      if (g0 + u * NQ < G) {
+        a0 = a0 + (double)v[u].x;
+        a1 = a1 + (double)v[u].y;
+        a2 = a2 + (double)v[u].z;
+        a3 = a3 + (double)v[u].w;
+      }
+    }
+  }
+  red[q * JW + 4 * c4 + 0] = a0;
+  red[q * JW + 4 * c4 + 1] = a1;
+  red[q * JW + 4 * c4 + 2] = a2;
+  red[q * JW + 4 * c4 + 3] = a3;
+  __syncthreads();
+  if (t < kRedSeg * JW) {
+    const int c = t % JW, z = t / JW;
+    double b = 0.0;
+    for (int qq = z * QPS; qq < (z + 1) * QPS; ++qq) b = b + red[qq * JW + c];
+    seg[z * JW + c] = b;
+  }
   __syncthreads();
   if (t >= 32) return;
   // Thread t < 32 writes column t (columns JW .. 31 are zero).
   double sum = 0.0;
   if (t < JW) {
 #pragma unroll
-    for (int w = 0; w < NQ; ++w) sum = sum + red[w * JW + t];
+    for (int z = 0; z < kRedSeg; ++z) sum = sum + seg[z * JW + t];
   }
   if (split > 0 && t >= split) {
     // Columns split .. split + s2 - 1 go to C64b (its columns 0 ..), and are zero in C32.
@@ -1094,19 +1131,18 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tn_reduce(const float *__rest
     return;
   }
   if (t >= s) sum = 0.0;
-  {
-    if (C64 && t < s) C64[(long long)t * ldc + i] = sum;
-    if (C32) C32[i * 32 + t] = (float)sum;
-  }
+  if (C64 && t < s) C64[(long long)t * ldc + i] = sum;
+  if (C32) C32[i * 32 + t] = (float)sum;
 }
 
 void tn_reduce(const float *part, int G, int pPad, int p, int s, double *C64, int ldc, float *C32,
                hipStream_t st, double *C64b = nullptr, int ldcb = 0, int split = 0, int s2 = 0) {
+  (void)pPad;
   if (s <= 16 && split == 0)
-    hipLaunchKernelGGL(k_gemm_tn_reduce<16>, dim3(p), dim3(kThreads), 0, st, part, G, pPad, p, s,
+    hipLaunchKernelGGL(k_gemm_tn_reduce<16>, dim3(p), dim3(kRedThreads), 0, st, part, G, p, s,
                        C64, ldc, C32, C64b, ldcb, split, s2);
   else
-    hipLaunchKernelGGL(k_gemm_tn_reduce<32>, dim3(p), dim3(kThreads), 0, st, part, G, pPad, p, s,
+    hipLaunchKernelGGL(k_gemm_tn_reduce<32>, dim3(p), dim3(kRedThreads), 0, st, part, G, p, s,
                        C64, ldc, C32, C64b, ldcb, split, s2);
 }
 

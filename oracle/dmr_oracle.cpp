@@ -562,6 +562,12 @@ int oracle_set_csr(OracleCtx *c, const int *rp, const int *ci, const double *va)
   return 0;
 }
 
+// New values for the operator's sparsity pattern (nnz of them), keeping the recycled subspace
+// and the stagnation counter: the oracle side of dmrSetOperatorValues on a sequence.
+void oracle_set_values(OracleCtx *c, const double *va) {
+  std::copy(va, va + c->A.va.size(), c->A.va.begin());
+}
+
 // Same contract as dmrSolve / solverDMR (FP64).  x: in x0, out solution.  Returns the value the
 // solver returns; *numIts as the solver sets it; status: 0 converged, 6 cycle limit, 7 runaway,
 // -1 early exit.

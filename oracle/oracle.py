@@ -30,6 +30,7 @@ _sig("oracle_create", _vp, [C.c_int, C.c_int, C.c_int, C.c_int])
 _sig("oracle_destroy", None, [_vp])
 _sig("oracle_keep_last", None, [_vp, C.c_int])
 _sig("oracle_set_csr", C.c_int, [_vp, _ip, _ip, _dp])
+_sig("oracle_set_values", None, [_vp, _dp])
 _sig("oracle_solve", C.c_double, [_vp, C.c_int, _dp, _dp, C.c_double, C.c_int, _ip, _ip])
 _sig("oracle_history", C.c_int, [_vp, _dp, C.c_int])
 _sig("oracle_counts", None, [_vp, _ip, _ip, _ip])
@@ -118,6 +119,13 @@ class Oracle:
         lib.oracle_set_csr(self.h, _i(A.rowptr), _i(A.col), _d(A.val))
         if keep_last:
             lib.oracle_keep_last(self.h, 1)
+
+    def set_values(self, A):
+        """New operator values on the same sparsity pattern; the recycled subspace and the
+        stagnation counter are kept (a sequence of systems, as dmrSetOperatorValues)."""
+        assert A.nnz == self.A.nnz and np.array_equal(A.col, self.A.col)
+        self.A = A
+        lib.oracle_set_values(self.h, _d(A.val))
 
     def close(self):
         if self.h:

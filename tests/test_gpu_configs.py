@@ -1,0 +1,223 @@
+"""The benchmark's workloads at their full sizes on the GPU, each against the FP64 oracle.
+
+* B: 3-D convection-diffusion 128^3 (n = 2,097,152), a sequence of 16 systems with U(+-1%)
+  per-cell diffusion jitter, N(0,1) right-hand sides, FP64 DMR m = 50, k = 20, tol 1e-10, the
+  subspace recycled across the sequence (mode 0, then 1, as progDMR.c:166-174).  The first two
+  systems are compared with the oracle (histories and solutions, bar: iterations +-1, history
+  1e-12 relative); all 16 meet 1e-10 in a host FP64 residual.
+* C: the B sequence with FP32 and with bf16 basis storage inside FP64 refinement: every system
+  meets 1e-10 in FP64, and the error of systems 0 and 1 against the oracle's solution is bounded
+  by XTOL_MIXED.
+* D: 3-D anisotropic diffusion 96^3 (1, 0.1, 0.01), 32 right-hand sides as one block (block DMR,
+  10 block steps, k = 32, bf16 basis inside FP64 refinement): every column meets 1e-10, and two
+  columns agree with the oracle's solutions to XTOL_BLOCK.
+* E: the serving families at full size (2-D 512^2, 3-D 64^3 and 160^3 convection-diffusion,
+  m = 40, k = 20, FP64, per-request coefficient jitter): served alone and in a mix the results
+  are the same bits, every solve meets 1e-10, and the first request of each family equals the
+  oracle's solve.
+"""
+import numpy as np
+import pytest
+
+import gdmr
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+HIST_RTOL = 1e-12
+XTOL_MIXED = 1e-7   # ||x - x_oracle|| / ||x_oracle||: both meet 1e-10 on these systems
+XTOL_BLOCK = 1e-7
+TOL = 1e-10
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def _true(A, b, x):
+    return float(np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b))
+
+
+def _hist_rel(h, ho):
+    L = min(len(h), len(ho))
+    return float(np.max(np.abs(h[:L] - ho[:L]) / ho[:L])) if L else np.inf
+
+
+@pytest.fixture(scope="module")
+def seq128():
+    """The 128^3 sequence of the benchmark (seed 2024) and the oracle's FP64 solutions of its
+    first two systems, the second with the first's recycled subspace."""
+    mats = [gdmr.conv_diff_3d(128, nu0=0.1, beta=(1.0, 0.5, 0.25), jitter=0.01, seed=2024,
+                              stream=t) for t in range(16)]
+    rhs = [gdmr.rand_normal(2024, 1000 + t, mats[0].n) for t in range(16)]
+    m, k = 50, 20
+    ora = oracle.Oracle(mats[0], m, k, gdmr.reduction_blocks(mats[0].n))
+    ref = []
+    for t in range(2):
+        if t:
+            ora.set_values(mats[t])
+        x, res, its, st = ora.solve(rhs[t], tol=TOL, mode=t)
+        ref.append(dict(x=x, res=res, its=its, status=st, hist=ora.history()))
+        print(f"oracle 128^3 system {t}: {its} iterations, residual {res:.3e}", flush=True)
+    ora.close()
+    return mats, rhs, ref
+
+
+def _run_sequence(torch, mats, rhs, prec, inner_tol=None):
+    A0 = mats[0]
+    n = A0.n
+    s = gdmr.Solver(n, 50, 20, precision=prec)
+    rp, ci = _dev(torch, A0.rowptr), _dev(torch, A0.col)
+    vals = [_dev(torch, A.val) for A in mats]
+    s.set_csr(rp.data_ptr(), ci.data_ptr(), vals[0].data_ptr(), A0.nnz, keep=(rp, ci))
+    if inner_tol is not None:
+        s.set_refinement(inner_tol, 60)
+    out = []
+    for t, A in enumerate(mats):
+        s.set_values(vals[t].data_ptr())
+        xd = torch.zeros(n, dtype=torch.float64, device="cuda")
+        bd = _dev(torch, rhs[t])
+        res, its = s.solve(xd.data_ptr(), bd.data_ptr(), TOL, 100000, mode=1 if t else 0)
+        st, hist = s.stats()
+        out.append(dict(x=xd.cpu().numpy(), res=res, its=its, status=st.status, hist=hist,
+                        cycles=st.cycles))
+    s.close()
+    return out
+
+
+def test_config_b_fp64_sequence_128(torch_gpu, seq128):
+    mats, rhs, ref = seq128
+    out = _run_sequence(torch_gpu, mats, rhs, gdmr.PREC_FP64)
+    for t in range(2):
+        g, o = out[t], ref[t]
+        rel = _hist_rel(g["hist"], o["hist"])
+        bitwise = np.array_equal(g["hist"], o["hist"]) and np.array_equal(g["x"], o["x"])
+        print(f"B system {t}: its {g['its']} vs {o['its']}, history rel diff {rel:.2e}, "
+              f"bitwise {bitwise}", flush=True)
+        assert abs(g["its"] - o["its"]) <= 1 and rel <= HIST_RTOL
+        assert np.linalg.norm(g["x"] - o["x"]) <= 1e-9 * np.linalg.norm(o["x"])
+    for t, (A, b, g) in enumerate(zip(mats, rhs, out)):
+        tr = _true(A, b, g["x"])
+        print(f"B system {t}: {g['cycles']} cycles, true residual {tr:.2e}", flush=True)
+        assert g["status"] == 0 and tr <= TOL * 1.01, (t, tr)
+
+
+@pytest.mark.parametrize("prec", [gdmr.PREC_FP32, gdmr.PREC_BF16], ids=["fp32", "bf16"])
+def test_config_c_mixed_sequence_128(torch_gpu, seq128, prec):
+    mats, rhs, ref = seq128
+    inner = 1e-5 if prec == gdmr.PREC_FP32 else 1e-2
+    out = _run_sequence(torch_gpu, mats, rhs, prec, inner_tol=inner)
+    for t, (A, b, g) in enumerate(zip(mats, rhs, out)):
+        tr = _true(A, b, g["x"])
+        line = f"C[{prec}] system {t}: {g['cycles']} cycles, true residual {tr:.2e}"
+        if t < 2:
+            err = np.linalg.norm(g["x"] - ref[t]["x"]) / np.linalg.norm(ref[t]["x"])
+            line += f", error vs oracle {err:.2e}"
+            assert err <= XTOL_MIXED, (t, err)
+        print(line, flush=True)
+        assert g["status"] == 0 and tr <= TOL * 1.01, (t, tr)
+        assert abs(g["res"] - tr) <= 1e-3 * tr + 1e-16
+
+
+def test_config_d_block_32_rhs_96(torch_gpu):
+    torch = torch_gpu
+    A = gdmr.aniso_3d(96, coef=(1.0, 0.1, 0.01))
+    n, s = A.n, 32
+    B = np.stack([gdmr.rand_normal(96, j, n) for j in range(s)])
+    bs = gdmr.BlockSolver(n, 10, 32, s)
+    rp, ci, va = _dev(torch, A.rowptr), _dev(torch, A.col), _dev(torch, A.val)
+    bs.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A.nnz, keep=(rp, ci, va))
+    bs.set_precision(gdmr.PREC_BF16)
+    bs.set_refinement(3e-2, 60)
+    Bd = _dev(torch, B)
+    Xd = torch.zeros(s, n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    rc, rel = bs.solve(Xd.data_ptr(), n, Bd.data_ptr(), n, s, TOL, max_cycles=100000)
+    st, _ = bs.stats()
+    bs.close()
+    X = Xd.cpu().numpy()
+    print(f"D: rc {rc}, {st.cycles} block cycles, max relRes {rel.max():.2e}", flush=True)
+    assert rc == 0
+    for j in range(s):
+        tr = _true(A, B[j], X[j])
+        assert tr <= TOL * 1.01 and abs(tr - rel[j]) <= 1e-3 * tr, (j, tr, rel[j])
+    # Two columns against the oracle's FP64 solution (plain DMR, m = 80, k = 20).
+    ora = oracle.Oracle(A, 80, 20, gdmr.reduction_blocks(n))
+    for j in (0, s - 1):
+        xo, reso, itso, sto = ora.solve(B[j], tol=1e-11, mode=0)
+        err = np.linalg.norm(X[j] - xo) / np.linalg.norm(xo)
+        print(f"D column {j}: oracle {itso} iterations (status {sto}), error {err:.2e}",
+              flush=True)
+        assert sto == 0 and err <= XTOL_BLOCK, (j, err)
+    ora.close()
+
+
+def _serving_families():
+    """(name, generator(draw) -> CsrMatrix): the serving workload's three families."""
+    return [("cd2d-512", lambda t: gdmr.conv_diff_2d(512, jitter=0.01, seed=41, stream=t)),
+            ("cd3d-64", lambda t: gdmr.conv_diff_3d(64, jitter=0.01, seed=42, stream=t)),
+            ("cd3d-160", lambda t: gdmr.conv_diff_3d(160, jitter=0.01, seed=43, stream=t))]
+
+
+def test_config_e_serving_families_full_size(torch_gpu):
+    torch = torch_gpu
+    m, k = 40, 20
+    fams = []
+    for fi, (name, gen) in enumerate(_serving_families()):
+        mats = [gen(d) for d in range(2)]
+        reqs = []
+        for q, nr in enumerate((2, 1)):
+            reqs.append(dict(draw=q, B=[gdmr.rand_normal(500 + fi, 10 * q + j, mats[0].n)
+                                        for j in range(nr)]))
+        fams.append(dict(name=name, mats=mats, reqs=reqs, rp=_dev(torch, mats[0].rowptr),
+                         ci=_dev(torch, mats[0].col), vals=[_dev(torch, A.val) for A in mats]))
+    for f in fams:
+        for r in f["reqs"]:
+            r["b"] = [_dev(torch, b) for b in r["B"]]
+
+    def serve(slots, order):
+        srv = gdmr.Server(slots=slots, hbm_reserve=1 << 30)
+        try:
+            ids = [srv.add_family(f["mats"][0].n, f["mats"][0].nnz, f["rp"].data_ptr(),
+                                  f["ci"].data_ptr(), m, k, gdmr.PREC_FP64) for f in fams]
+            xs, rid = {}, {}
+            for fi, q in order:
+                f, r = fams[fi], fams[fi]["reqs"][q]
+                xs[(fi, q)] = [torch.zeros(f["mats"][0].n, dtype=torch.float64, device="cuda")
+                               for _ in r["b"]]
+                torch.cuda.synchronize()
+                i = srv.submit(ids[fi], f["vals"][r["draw"]].data_ptr(),
+                               [b.data_ptr() for b in r["b"]],
+                               [x.data_ptr() for x in xs[(fi, q)]], TOL, 100000)
+                rid[i] = (fi, q)
+            srv.drain()
+            res = srv.wait(timeout=0.0, cap=64)
+        finally:
+            srv.close()
+        assert len(res) == len(order)
+        return {rid[r.id]: (r, [x.cpu().numpy() for x in xs[rid[r.id]]]) for r in res}
+
+    alone = {}
+    for fi in range(len(fams)):
+        alone.update(serve(1, [(fi, q) for q in range(2)]))
+    mix = serve(3, [(fi, q) for q in range(2) for fi in range(len(fams))])
+    for key, (ra, xa) in alone.items():
+        rm, xm = mix[key]
+        f, r = fams[key[0]], fams[key[0]]["reqs"][key[1]]
+        A = f["mats"][r["draw"]]
+        assert ra.status == 0 and rm.status == 0
+        for b, a, mm in zip(r["B"], xa, xm):
+            assert np.array_equal(a, mm), key
+            assert _true(A, b, a) <= TOL * 1.01
+        print(f"E {f['name']} request {key[1]}: alone == mix bitwise, {ra.cycles} cycles",
+              flush=True)
+    for fi, f in enumerate(fams):
+        # The family's first solve (first right-hand side of its first request, fresh subspace).
+        A, b = f["mats"][0], f["reqs"][0]["B"][0]
+        ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(A.n))
+        xo, reso, itso, sto = ora.solve(b, tol=TOL, mat_rank=100000, mode=0)
+        ora.close()
+        x = alone[(fi, 0)][1][0]
+        err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+        print(f"E {f['name']}: first solve vs oracle {err:.2e} ({itso} iterations)", flush=True)
+        assert sto == 0 and np.array_equal(x, xo), (f["name"], err)

@@ -414,3 +414,24 @@ def test_stagnation_restart_with_deflation(n, alpha, m, k, mat_rank):
     off.set_stagnation_rule(False)
     off.solve(b, tol=1e-10, mat_rank=mat_rank, mode=0)
     assert np.max(np.abs(off.history() - h) / h) > 1e-2
+
+
+def test_set_values_keeps_the_sequence_state():
+    """Changing the operator values (a jittered sequence) keeps the subspace and the stagnation
+    counter: solving system 1 after set_values equals a fresh oracle of system 1 given system 0's
+    subspace, and differs from solving system 0's operator again."""
+    A0 = gdmr.conv_diff_2d(20, jitter=0.01, seed=8, stream=0)
+    A1 = gdmr.conv_diff_2d(20, jitter=0.01, seed=8, stream=1)
+    n, m, k = A0.n, 16, 5
+    b0, b1 = gdmr.rand_normal(8, 0, n), gdmr.rand_normal(8, 1, n)
+    seq = oracle.Oracle(A0, m, k, gdmr.reduction_blocks(n))
+    seq.solve(b0, tol=1e-10, mode=0)
+    Z0 = seq.get_z()
+    seq.set_values(A1)
+    x1, _, _, _ = seq.solve(b1, tol=1e-10, mode=1)
+    fresh = oracle.Oracle(A1, m, k, gdmr.reduction_blocks(n))
+    fresh.set_z(Z0)
+    xf, _, _, _ = fresh.solve(b1, tol=1e-10, mode=1)
+    assert np.array_equal(x1, xf) and np.array_equal(seq.history(), fresh.history())
+    assert np.linalg.norm(b1 - A1.matvec(x1)) / np.linalg.norm(b1) <= 1e-10
+    assert np.linalg.norm(b1 - A0.matvec(x1)) / np.linalg.norm(b1) > 1e-6

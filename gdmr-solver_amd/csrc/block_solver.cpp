@@ -88,6 +88,10 @@ struct dmrBlockContext {
   bool fusedGram = true;  // first Gram matrix from the update's epilogue where possible
   int spmmForce = 0;      // SpMM variant pinned by DMR_BLOCK_SPMM (0: the tuned one)
   bool foldT = true;      // R's projection in the last block step's pass where possible
+  // Launch choices pinned over the tuned ones (DMR_BLOCK_NN_BASIS: 100 or 101, the bf16
+  // matrix-core update; DMR_BLOCK_TN_VAR: the projection variant, 2 or 3 for a bf16 basis); 0:
+  // as tuned.
+  int nnPin = 0, tnVarPin = 0;
   // Launch shapes chosen by the autotuner (per block width s), and whether it ran.
   struct Tuning {
     int tnRowsProj = 512, tnVarProj = 0;  // gemm_tn for projections (p > s)
@@ -726,6 +730,9 @@ int block_solve(dmrBlockContext *c, int mode, int s, double *X, long long ldx, c
     c->val32Stale = false;
   }
   if (c->autotune && c->tune.s != s) autotune(c, s, st);
+  if (c->bfBasis && c->nnPin) c->tune.nnBasis = c->nnPin;
+  if (c->tnVarPin && (c->bfBasis ? c->tnVarPin >= 2 : c->tnVarPin <= 2))
+    c->tune.tnVarProj = c->tnVarPin;
   dmr::blk::colnorms<double>(B, ldb, s, n, c->cpart, c->bn, 1, st);
   BCHK(hipMemcpyAsync(hb, c->bn, sizeof(double) * s, hipMemcpyDeviceToHost, st));
   bool useZ = mode != 0 && c->hasZ;
@@ -871,6 +878,14 @@ int dmrBlockCreate(dmrBlockHandle *h, int device, int elements, int blockSteps, 
   }
   if (const char *e = std::getenv("DMR_BLOCK_FUSED_GRAM")) c->fusedGram = e[0] != '0';
   if (const char *e = std::getenv("DMR_BLOCK_FOLD_T")) c->foldT = e[0] != '0';
+  if (const char *e = std::getenv("DMR_BLOCK_NN_BASIS")) {
+    const int v = std::atoi(e);
+    if (v == 100 || v == 101) c->nnPin = v;
+  }
+  if (const char *e = std::getenv("DMR_BLOCK_TN_VAR")) {
+    const int v = std::atoi(e);
+    if (v >= 0 && v <= 3) c->tnVarPin = v;
+  }
   if (const char *e = std::getenv("DMR_BLOCK_SPMM")) {
     const int v = std::atoi(e);
     for (int ok : {4, 8, 16, 104, 108, 116, 208, 216})

@@ -126,6 +126,26 @@ _sig("dmrBlockGetTuning", C.c_int, [_vp, _ip])
 _sig("dmrSetProfiling", C.c_int, [_vp, C.c_int])
 _sig("dmrGetProfile", C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_longlong)])
 _sig("dmrCsrApplyStatic", C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp])
+# Kernel-level entry points of the block solver (gDMRKernels.h).
+_ll, _i32, _f32, _f64 = C.c_longlong, C.c_int, C.c_float, C.c_double
+KBASIS_FP32, KBASIS_BF16 = 0, 1
+_sig("dmrKernelTnWorkspace", C.c_size_t, [_ll, _i32, _i32])
+_sig("dmrKernelGemmTn", C.c_int, [_i32, _vp, _ll, _i32, _vp, _ll, _i32, _ll, _i32, _i32, _vp, _vp,
+                                  _i32, _vp, _vp])
+_sig("dmrKernelGemmTnPair", C.c_int, [_vp, _ll, _i32, _vp, _vp, _ll, _i32, _ll, _i32, _vp, _vp,
+                                      _i32, _vp, _vp, _i32, _vp])
+_sig("dmrKernelGemmNn", C.c_int, [_i32, _vp, _ll, _i32, _vp, _vp, _ll, _f32, _f32, _vp, _ll, _i32,
+                                  _ll, _i32, _vp, _vp])
+_sig("dmrKernelGramBlocks", C.c_int, [_ll, _i32])
+_sig("dmrKernelGramReduce", C.c_int, [_vp, _i32, _i32, _vp, _vp])
+_sig("dmrKernelGemmNnPair", C.c_int, [_vp, _ll, _i32, _vp, _vp, _vp, _ll, _i32, _ll, _i32, _vp])
+_sig("dmrKernelSpmm", C.c_int, [_i32, _i32, _i32, _vp, _vp, _vp, _vp, _ll, _vp, _ll, _i32, _i32,
+                                _vp])
+_sig("dmrKernelCholQr", C.c_int, [_vp, _i32, _f64, _vp, _vp, _i32, _vp, _i32, _vp, _vp])
+_sig("dmrKernelScaleBlock", C.c_int, [_vp, _ll, _vp, _vp, _ll, _i32, _i32, _ll, _vp])
+_sig("dmrKernelLsqBlock", C.c_int, [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp,
+                                    _vp, _vp])
+_sig("dmrKernelLsqFits", C.c_int, [_i32, _i32, _i32])
 PHASES = ("spmv", "gemvt", "update", "scale", "combo", "zgemm", "lsq", "svd", "orth", "reduce",
           "resid")
 # Problems and random numbers.
@@ -163,7 +183,10 @@ HEADER_SYMBOLS = (
     "dmrServerWait dmrServerDrain dmrServerNow dmrServerStats dmrServerDestroy "
     "initLS freeLS impImgLS expImgLS impSrcLS expSrcLS impOptLS expOptLS solveLS linOptLS "
     "impCsrLS nnzLS dmrReadMatrixMarket dmrWriteMatrixMarket impMtxLS dmrGenConvDiff2D dmrGenConvDiff3D dmrGenAniso3D dmrRandNormal "
-    "dmrRandUniform dmrSplitMix64").split()
+    "dmrRandUniform dmrSplitMix64 "
+    "dmrKernelTnWorkspace dmrKernelGemmTn dmrKernelGemmTnPair dmrKernelGemmNn dmrKernelGramBlocks "
+    "dmrKernelGramReduce dmrKernelGemmNnPair dmrKernelSpmm dmrKernelCholQr dmrKernelScaleBlock "
+    "dmrKernelLsqBlock dmrKernelLsqFits").split()
 
 
 def _ptr(a: np.ndarray, ctype):

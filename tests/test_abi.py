@@ -18,7 +18,8 @@ def _declared(header):
 
 
 def test_headers_fully_exported():
-    declared = _declared("gDMR.h") | _declared("gDMRC.h") | _declared("gDMRServe.h")
+    declared = (_declared("gDMR.h") | _declared("gDMRC.h") | _declared("gDMRServe.h") |
+                _declared("gDMRKernels.h"))
     assert {"initDMR", "freeDMR", "solverDMR", "impSVDMR", "expSVDMR", "initLS", "solveLS",
             "linOptLS", "impCsrLS", "dmrServerSubmit", "dmrServerWait"} <= declared
     out = subprocess.run(["nm", "-D", "--defined-only", gdmr.LIB_PATH], capture_output=True,
@@ -73,3 +74,28 @@ def test_server_rejects_bad_arguments_without_device():
     assert gdmr.lib.dmrServerSubmit(None, 0, None, 1, None, None, 1e-8, 10,
                                     gdmr.C.byref(rid)) == gdmr.DMR_ERR_ARG
     assert gdmr.lib.dmrServerDrain(None) == gdmr.DMR_ERR_ARG
+
+
+def test_kernel_entry_points_check_shapes_without_device():
+    """The kernel-level entry points reject shapes the kernels do not take before any launch."""
+    lib = gdmr.lib
+    assert lib.dmrKernelTnWorkspace(1000, 20, 512) == 2 * 32 * 32
+    assert lib.dmrKernelTnWorkspace(1000, 20, 100) == 0  # rows per workgroup: multiple of 64
+    assert lib.dmrKernelGemmTn(0, None, 10, 4, None, 10, 4, 10, 64, 0, None, None, 4, None,
+                               None) == gdmr.DMR_ERR_ARG
+    one = gdmr.C.c_void_p(8)  # never dereferenced: every call below fails its shape check
+    assert lib.dmrKernelGemmTn(0, one, 10, 400, one, 10, 4, 10, 64, 0, one, one, 400, None,
+                               None) == gdmr.DMR_ERR_ARG  # p > 384
+    assert lib.dmrKernelGemmTn(0, one, 10, 4, one, 10, 4, 10, 64, 3, one, one, 4, None,
+                               None) == gdmr.DMR_ERR_ARG  # variant 3 needs a bf16 basis
+    assert lib.dmrKernelGemmNn(1, one, 10, 4, one, None, 10, 0.0, 1.0, one, 10, 20, 10, 100, one,
+                               None) == gdmr.DMR_ERR_ARG  # Gram partials need s <= 16
+    assert lib.dmrKernelGemmNnPair(one, 10, 4, one, one, one, 10, 17, 10, 100,
+                                   None) == gdmr.DMR_ERR_ARG
+    assert lib.dmrKernelSpmm(1, 10, 30, one, one, one, one, 10, one, 10, 4, 8,
+                             None) == gdmr.DMR_ERR_ARG  # bf16 X takes 104-216
+    assert lib.dmrKernelCholQr(one, 33, 1e-12, one, one, 1, None, 0, one,
+                               None) == gdmr.DMR_ERR_ARG
+    assert lib.dmrKernelLsqBlock(one, 30, 20, 8, one, 4, one, one, one, one, one, one,
+                                 None) == gdmr.DMR_ERR_ARG  # rows != cols + bw
+    assert lib.dmrKernelLsqFits(100, 16, 16) == 1 and lib.dmrKernelLsqFits(100, 40, 16) == 0

@@ -497,3 +497,36 @@ def test_cycle_budget_bounds_the_whole_solve(torch_gpu, prec):
     print(f"{prec}: large budget: rc {rc} cycles {st.cycles} rel {rel.max():.2e}", flush=True)
     assert rc == 0 and rel.max() <= 1e-10
     solver.close()
+
+
+@pytest.mark.parametrize("s", [16, 20])
+def test_bf16_block_fast_paths_on_and_off(torch_gpu, monkeypatch, s):
+    """A bf16 block of 16 or 20 columns with the bf16 matrix-core update and projection pinned
+    (DMR_BLOCK_NN_BASIS, DMR_BLOCK_TN_VAR), once with the folded residual projection and the Gram
+    matrix from the update's epilogue (DMR_BLOCK_FOLD_T, DMR_BLOCK_FUSED_GRAM) and once without,
+    and with the bf16 update at 128 and 256 rows per wavefront: every column meets 1e-10 in FP64
+    and the solutions agree to XTOL with the plain configuration's.  At s = 20 the folded paths do
+    not apply (they need s <= 16): the two-tile kernels and the window least squares run instead."""
+    torch = torch_gpu
+    A = gdmr.conv_diff_3d(24, jitter=0.01, seed=21)
+    n = A.n
+    B = np.stack([gdmr.rand_normal(22, j, n) for j in range(s)])
+    sols = {}
+    for nn_basis in ("100", "101"):
+        for fused in ("0", "1"):
+            monkeypatch.setenv("DMR_BLOCK_NN_BASIS", nn_basis)
+            monkeypatch.setenv("DMR_BLOCK_TN_VAR", "3")
+            monkeypatch.setenv("DMR_BLOCK_FOLD_T", fused)
+            monkeypatch.setenv("DMR_BLOCK_FUSED_GRAM", fused)
+            rc, rel, st, X = _block(torch, A, B, mb=5, k=16, prec=gdmr.PREC_BF16, inner_tol=3e-2)
+            print(f"s {s} nn {nn_basis} fused {fused}: rc {rc} cycles {st.cycles} "
+                  f"max rel {rel.max():.2e}", flush=True)
+            assert rc == 0
+            for j in range(s):
+                assert _true_res(A, B[j], X[j]) <= 1e-10 * 1.001, j
+            sols[(nn_basis, fused)] = X
+    base = sols[("100", "0")]
+    for key, X in sols.items():
+        err = np.max(np.linalg.norm(X - base, axis=1) / np.linalg.norm(base, axis=1))
+        print(f"  {key} vs plain: {err:.2e}", flush=True)
+        assert err <= XTOL, key

@@ -291,6 +291,11 @@ def run_seqblock_leg(mats, rhs, dev_vals, rp, ci, mb, k, tol, warmup, inner_tol,
         solve_all([h], Xp, True)
         prof = h.profile()
         h.set_profiling(False)
+        tot = sum(v[0] for v in prof.values())
+        log(f"[seqblock] profiled re-solve: {tot:.1f} ms of event time; per phase (ms, launches, "
+            "algorithmic TB/s): " + ", ".join(
+                f"{p} {v[0]:.1f}/{v[2]}/{v[1] / (v[0] * 1e9):.2f}" if v[1] > 0 else
+                f"{p} {v[0]:.1f}/{v[2]}" for p, v in prof.items() if v[2] > 0))
     h.close()
     Xh = X.cpu().numpy()
     true = [float(np.linalg.norm(rhs[t] - A.matvec(Xh[t])) / np.linalg.norm(rhs[t]))

@@ -71,7 +71,7 @@ struct dmrBlockContext {
   float *P = nullptr, *Z = nullptr, *Zn = nullptr, *R = nullptr, *Xa = nullptr, *Ub = nullptr;
   bool bfBasis = false;
   double *R64 = nullptr;
-  double *H = nullptr, *T = nullptr, *Hw = nullptr, *Tw = nullptr, *Y = nullptr, *Hred = nullptr;
+  double *H = nullptr, *T = nullptr, *Hw = nullptr, *Tw = nullptr, *Y = nullptr;
   float *C32 = nullptr, *Yr32 = nullptr, *ZH32 = nullptr, *Sinv32 = nullptr, *Cz32 = nullptr;
   float *Cpair = nullptr;  // coefficients of the paired residual and solution update
   float *part = nullptr;
@@ -88,9 +88,10 @@ struct dmrBlockContext {
   bool fusedGram = true;  // first Gram matrix from the update's epilogue where possible
   int spmmForce = 0;      // SpMM variant pinned by DMR_BLOCK_SPMM (0: the tuned one)
   bool foldT = true;      // R's projection in the last block step's pass where possible
-  // Launch choices pinned over the tuned ones (DMR_BLOCK_NN_BASIS: 100 or 101, the bf16
-  // matrix-core update; DMR_BLOCK_TN_VAR: the projection variant, 2 or 3 for a bf16 basis); 0:
-  // as tuned.
+  bool svdSide = true;    // the refresh's SVD on the side stream (DMR_BLOCK_SVD_SIDE=0: inline)
+  // Launch choices pinned over the tuned ones (DMR_BLOCK_NN_BASIS: the update over a bf16 basis,
+  // 4, 8 or 16 for the FP32 matrix cores, 100 or 101 for the bf16 ones; DMR_BLOCK_TN_VAR: the
+  // projection variant, 2 or 3 for a bf16 basis); 0: as tuned.
   int nnPin = 0, tnVarPin = 0;
   // Launch shapes chosen by the autotuner (per block width s), and whether it ran.
   struct Tuning {
@@ -109,6 +110,8 @@ struct dmrBlockContext {
   // residual and solution updates.
   hipStream_t side = nullptr;
   hipEvent_t evFork = nullptr, evJoin = nullptr;
+  // Hand-off between the caller's stream and gst, on which every captured cycle is replayed.
+  hipEvent_t evIn = nullptr, evOut = nullptr;
   struct Cap {
     int s, deflate, parity;
     hipGraphExec_t exec;
@@ -197,12 +200,12 @@ struct Sizes {
 Sizes sizes(long long n, int mb, int k, int sMax, int G, long long ld) {
   const int rowsMax = k + (mb + 1) * sMax, colsMax = k + mb * sMax;
   const int pPadMax = (rowsMax + 31) & ~31;
-  const int rRed = rowsMax, cRed = colsMax;
+  const int cRed = colsMax;
   Sizes z;
   z.big = sizeof(float) * (size_t)ld * (size_t)(rowsMax + 2 * k + 2 * sMax + std::max(sMax, k)) +
           sizeof(double) * (size_t)ld * sMax;
   z.small = sizeof(double) * ((size_t)rowsMax * colsMax * 2 + (size_t)rowsMax * 32 * 2 +
-                              (size_t)colsMax * 32 + (size_t)rRed * cRed + 32 * 32 * 2 +
+                              (size_t)colsMax * 32 + 32 * 32 * 2 +
                               (size_t)cRed * 32 + cRed + 2 + 8 * 64 +
                               (size_t)dmr::blk::colnorm_blocks(n) * 32) +
             sizeof(float) * ((size_t)rowsMax * 32 * 3 + (size_t)colsMax * 32 + 32 * 32 +
@@ -225,6 +228,8 @@ void ctx_release(dmrBlockContext *c) {
   if (c->side) hipStreamDestroy(c->side);
   if (c->evFork) hipEventDestroy(c->evFork);
   if (c->evJoin) hipEventDestroy(c->evJoin);
+  if (c->evIn) hipEventDestroy(c->evIn);
+  if (c->evOut) hipEventDestroy(c->evOut);
   if (c->big) hipFree(c->big);
   if (c->small) hipFree(c->small);
   if (c->ownedVal32) hipFree(c->ownedVal32);
@@ -265,13 +270,12 @@ int ctx_alloc(dmrBlockContext *c) {
   Carver m{(char *)c->small};
   const int rowsMax = c->rowsMax, colsMax = c->colsMax;
   const int pPadMax = (rowsMax + 31) & ~31;
-  const int rRed = rowsMax, cRed = colsMax;
+  const int cRed = colsMax;
   c->H = m.take<double>((size_t)rowsMax * colsMax);
   c->Hw = m.take<double>((size_t)rowsMax * colsMax);
   c->T = m.take<double>((size_t)rowsMax * 32);
   c->Tw = m.take<double>((size_t)rowsMax * 32);
   c->Y = m.take<double>((size_t)colsMax * 32);
-  c->Hred = m.take<double>((size_t)rRed * cRed);
   c->G64 = m.take<double>(32 * 32);
   c->Sacc = m.take<double>(32 * 32);
   c->Csvd = m.take<double>((size_t)cRed * 32);
@@ -547,17 +551,20 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
       refreshed = cR >= k && dmr::svd_lds(rR, cR) <= 160 * 1024;
     }
   }
+  hipStream_t ss = c->svdSide ? c->side : st;
   auto fork_svd = [&] {
-    hipEventRecord(c->evFork, st);
-    hipStreamWaitEvent(c->side, c->evFork, 0);
-    hipMemcpy2DAsync(c->Hred, sizeof(double) * rR, c->H, sizeof(double) * rows,
-                     sizeof(double) * rR, cR, hipMemcpyDeviceToDevice, c->side);
-    timed(c, c->side, DMR_PH_SVD, 0.0, [&] {
-      dmr::svd_smallest_rect(c->Hred, rR, rR, cR, k, c->svdTol, kSvdMaxSweeps, 1, c->Csvd,
-                             c->sig, c->flags + 1, c->side);
-      dmr::blk::pack_coef(c->Csvd, cR, k, c->Cz32, c->side);
+    if (c->svdSide) {
+      hipEventRecord(c->evFork, st);
+      hipStreamWaitEvent(c->side, c->evFork, 0);
+    }
+    // The SVD reads the leading rR x cR block of H in place: the steps still running on the
+    // main stream write only columns cR and beyond.
+    timed(c, ss, DMR_PH_SVD, 0.0, [&] {
+      dmr::svd_smallest_rect(c->H, rR, rows, cR, k, c->svdTol, kSvdMaxSweeps, 1, c->Csvd,
+                             c->sig, c->flags + 1, ss);
+      dmr::blk::pack_coef(c->Csvd, cR, k, c->Cz32, ss);
     });
-    hipEventRecord(c->evJoin, c->side);
+    if (c->svdSide) hipEventRecord(c->evJoin, c->side);
   };
   if (deflate) {
     timedf(c, st, DMR_PH_SPMV, spmm_bytes(c, k), 2.0 * c->csr.nnz * k,
@@ -664,7 +671,7 @@ bool cycle_t(dmrBlockContext *c, TP *P, int s, bool deflate, hipStream_t st) {
     });
   // Subspace refresh, part 2: Z <- normalize([Z V_0 .. V_{kr-1}] C) once the SVD has joined.
   if (refreshed) {
-    hipStreamWaitEvent(st, c->evJoin, 0);
+    if (c->svdSide) hipStreamWaitEvent(st, c->evJoin, 0);
     const int kv = cR - p0;  // kr * s Krylov columns
     timed(c, st, DMR_PH_ZGEMM, vb * (p0 + k + 2 * k) + pb * kv, [&] {
       if (deflate) {
@@ -691,8 +698,10 @@ bool cycle(dmrBlockContext *c, int s, bool deflate, hipStream_t st) {
 
 // One restart cycle as a replay of its captured graph.  The cycle's launch sequence depends only
 // on (s, deflated, which of the two subspace buffers holds Z) once the shapes are tuned, so each
-// such variant is captured once (on the handle's own stream, in thread-local capture mode) and
-// replayed on the caller's stream.
+// such variant is captured once (on the handle's own stream gst, in thread-local capture mode).
+// Replays also run on gst, ordered after the caller's stream by an event and the caller's stream
+// after them, so a handle that callers drive from several streams and threads (the serving
+// loop's slots) always replays its graphs on the stream they were captured on.
 int cycle_graph(dmrBlockContext *c, int s, bool deflate, hipStream_t st, bool *refreshed) {
   const int parity = c->Z == c->zbuf0 ? 0 : 1;
   const dmrBlockContext::Cap *cap = nullptr;
@@ -712,7 +721,11 @@ int cycle_graph(dmrBlockContext *c, int s, bool deflate, hipStream_t st, bool *r
     c->st.matvecs = mv0;
     cap = &c->caps.back();
   }
-  BCHK(hipGraphLaunch(cap->exec, st));
+  BCHK(hipEventRecord(c->evIn, st));
+  BCHK(hipStreamWaitEvent(c->gst, c->evIn, 0));
+  BCHK(hipGraphLaunch(cap->exec, c->gst));
+  BCHK(hipEventRecord(c->evOut, c->gst));
+  BCHK(hipStreamWaitEvent(st, c->evOut, 0));
   c->st.matvecs += cap->matvecs;
   *refreshed = cap->refreshed;
   return DMR_OK;
@@ -878,9 +891,10 @@ int dmrBlockCreate(dmrBlockHandle *h, int device, int elements, int blockSteps, 
   }
   if (const char *e = std::getenv("DMR_BLOCK_FUSED_GRAM")) c->fusedGram = e[0] != '0';
   if (const char *e = std::getenv("DMR_BLOCK_FOLD_T")) c->foldT = e[0] != '0';
+  if (const char *e = std::getenv("DMR_BLOCK_SVD_SIDE")) c->svdSide = e[0] != '0';
   if (const char *e = std::getenv("DMR_BLOCK_NN_BASIS")) {
     const int v = std::atoi(e);
-    if (v == 100 || v == 101) c->nnPin = v;
+    if (v == 4 || v == 8 || v == 16 || v == 100 || v == 101) c->nnPin = v;
   }
   if (const char *e = std::getenv("DMR_BLOCK_TN_VAR")) {
     const int v = std::atoi(e);
@@ -903,7 +917,9 @@ int dmrBlockCreate(dmrBlockHandle *h, int device, int elements, int blockSteps, 
   if (rc == DMR_OK && (hipStreamCreateWithFlags(&c->gst, hipStreamNonBlocking) != hipSuccess ||
                        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
                        hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess ||
-                       hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming) != hipSuccess))
+                       hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming) != hipSuccess ||
+                       hipEventCreateWithFlags(&c->evIn, hipEventDisableTiming) != hipSuccess ||
+                       hipEventCreateWithFlags(&c->evOut, hipEventDisableTiming) != hipSuccess))
     rc = dmr::set_error(DMR_ERR_HIP, "stream creation failed");
   if (rc != DMR_OK) {
     ctx_release(c);

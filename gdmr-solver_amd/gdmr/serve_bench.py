@@ -49,7 +49,11 @@ class Load:
         self.torch, self.args = torch, args
         self.fams = []
         t0 = time.time()
-        for fi, (name, gen) in enumerate(_families(args.small)):
+        fams = list(enumerate(_families(args.small)))
+        if args.families:
+            keep = args.families.split(",")
+            fams = [f for f in fams if f[1][0] in keep]
+        for fi, (name, gen) in fams:
             draws = [gen(j) for j in range(args.draws)]
             A0 = draws[0]
             dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
@@ -109,8 +113,8 @@ class Load:
             sp, xs = self.live.pop(res.id)
             self.done.append((res, sp))
             _log(f"request {res.id}: family {self.fams[sp['fam']]['name']}, {res.nrhs} RHS in a "
-                 f"launch of {res.batch}, {res.cycles} cycles, status {res.status}, latency "
-                 f"{res.endTime - res.submitTime:.2f}s")
+                 f"launch of {res.batch}, {res.cycles} cycles, status {res.status}, worst "
+                 f"residual {res.maxRelRes:.2e}, latency {res.endTime - res.submitTime:.2f}s")
             fcount = sum(1 for c in self.checked if c[0] == sp["fam"])
             if check_budget and fcount < check_budget:
                 f = self.fams[sp["fam"]]
@@ -152,6 +156,8 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--check", type=int, default=2, help="requests per family checked in FP64")
     ap.add_argument("--small", action="store_true", help="small families (quick runs)")
+    ap.add_argument("--families", default="",
+                    help="comma list of the families to serve (default: all three)")
     ap.add_argument("--block", action="store_true",
                     help="block families: multi-RHS requests as one block solve, and queued "
                          "requests with the same operator batched into one launch")

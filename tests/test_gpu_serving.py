@@ -285,3 +285,177 @@ def test_block_family_request_budget(torch_gpu):
     assert r1.status == 0
     for b, x in zip(B, x1):
         assert np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b) <= 1e-10 * 1.05
+
+
+def test_block_family_slots_match_one_thread_replay(torch_gpu):
+    """A bf16 block family served from four slots (each launch on whichever slot's thread and
+    stream picks it up) gives, launch by launch, the cycle counts and the solutions bit for bit of
+    the same launches replayed in order on one thread and one stream through a fresh block handle
+    (the server batches requests by timing, so the replay takes each launch's composition from the
+    served results)."""
+    torch = torch_gpu
+    draws = [gdmr.conv_diff_3d(40, jitter=0.01, seed=71, stream=t) for t in range(2)]
+    A0 = draws[0]
+    rp, ci = _dev(torch, A0.rowptr), _dev(torch, A0.col)
+    vals = [_dev(torch, A.val) for A in draws]
+    pool = [_dev(torch, gdmr.rand_normal(72, j, A0.n)) for j in range(6)]
+    rng = np.random.default_rng(73)
+    specs = [(int(w // 3) % 2, [int(i) for i in rng.permutation(6)[:int(rng.integers(1, 6))]])
+             for w in range(15)]
+    mb, k, smax, tol, budget = 6, 16, 16, 1e-10, 3000
+    srv = gdmr.Server(slots=4, hbm_reserve=1 << 28)
+    served = {}
+    try:
+        f = srv.add_block_family(A0.n, A0.nnz, rp.data_ptr(), ci.data_ptr(), mb, k, smax)
+        srv.set_block_precision(f, gdmr.PREC_BF16)
+        srv.set_refinement(f, 3e-2, 60)
+        ids = {}
+        # Three waves of five requests: queued together, so later requests join launches.
+        for wave in range(3):
+            for q in range(5 * wave, 5 * wave + 5):
+                d, cols = specs[q]
+                xs = [torch.zeros(A0.n, dtype=torch.float64, device="cuda") for _ in cols]
+                torch.cuda.synchronize()
+                rid = srv.submit(f, vals[d].data_ptr(), [pool[j].data_ptr() for j in cols],
+                                 [x.data_ptr() for x in xs], tol, budget)
+                ids[rid] = (q, xs)
+            srv.drain()
+        for r in srv.wait(timeout=0.0, cap=64):
+            q, xs = ids[r.id]
+            served[q] = (r, [x.cpu().numpy() for x in xs])
+    finally:
+        srv.close()
+    assert len(served) == len(specs)
+    launches = {}
+    for q, (r, _) in served.items():
+        assert r.status == 0, (q, r.status)
+        launches.setdefault((r.slot, r.startTime), []).append(q)
+    order = sorted(launches.items(), key=lambda kv: kv[0][1])
+    print(f"{len(order)} launches on slots {[key[0] for key, _ in order]}, widths "
+          f"{[sum(len(specs[q][1]) for q in qs) for _, qs in order]}", flush=True)
+    bs = gdmr.BlockSolver(A0.n, mb, k, smax)
+    try:
+        first = min(order[0][1])
+        bs.set_csr(rp.data_ptr(), ci.data_ptr(), vals[specs[first][0]].data_ptr(), A0.nnz)
+        bs.set_precision(gdmr.PREC_BF16)
+        bs.set_refinement(3e-2, 60)
+        stream = torch.cuda.Stream()
+        for (slot, _), qs in order:
+            qs = sorted(qs)  # the server batches in submission order
+            cols = [j for q in qs for j in specs[q][1]]
+            B = torch.stack([pool[j] for j in cols]).contiguous()  # column c at c * n
+            X = torch.zeros_like(B)
+            torch.cuda.synchronize()
+            bs.set_values(vals[specs[qs[0]][0]].data_ptr())
+            rc, _ = bs.solve(X.data_ptr(), A0.n, B.data_ptr(), A0.n, len(cols), tol, budget,
+                             1 if bs.has_z() else 0, stream=stream.cuda_stream)
+            stream.synchronize()
+            assert rc == 0
+            st, _ = bs.stats()
+            Xh = X.cpu().numpy()
+            print(f"launch of {qs} on slot {slot}: served {served[qs[0]][0].cycles} cycles, "
+                  f"replay {st.cycles}", flush=True)
+            c = 0
+            for q in qs:
+                r, xs = served[q]
+                assert r.cycles == st.cycles and r.batch == len(cols)
+                for x in xs:
+                    assert np.array_equal(x, Xh[c]), f"request {q} differs from the replay"
+                    c += 1
+    finally:
+        bs.close()
+    for q, (r, xs) in served.items():
+        A = draws[specs[q][0]]
+        for j, x in zip(specs[q][1], xs):
+            b = pool[j].cpu().numpy()
+            assert np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b) <= tol * 1.05
+
+
+def test_block_handle_driven_from_several_threads(torch_gpu):
+    """One bf16 block handle driven in turn by four long-lived threads, each with its own stream
+    (the serving loop's slot pattern: captured cycle graphs recorded in one thread and replayed
+    from the others), gives the cycle counts and the solutions bit for bit of the same launches
+    run on one thread and one stream through a second handle."""
+    import queue
+    import threading
+
+    torch = torch_gpu
+    draws = [gdmr.conv_diff_3d(40, jitter=0.01, seed=74, stream=t) for t in range(2)]
+    A0 = draws[0]
+    rp, ci = _dev(torch, A0.rowptr), _dev(torch, A0.col)
+    vals = [_dev(torch, A.val) for A in draws]
+    pool = [_dev(torch, gdmr.rand_normal(75, j, A0.n)) for j in range(8)]
+    rng = np.random.default_rng(76)
+    launches = [(int(L // 2) % 2, [int(i) for i in rng.permutation(8)[:int(rng.integers(1, 9))]])
+                for L in range(12)]
+    mb, k, smax, tol, budget = 6, 16, 16, 1e-10, 3000
+
+    def handle():
+        bs = gdmr.BlockSolver(A0.n, mb, k, smax)
+        bs.set_csr(rp.data_ptr(), ci.data_ptr(), vals[launches[0][0]].data_ptr(), A0.nnz)
+        bs.set_precision(gdmr.PREC_BF16)
+        bs.set_refinement(3e-2, 60)
+        return bs
+
+    def launch(bs, L, stream):
+        d, cols = launches[L]
+        B = torch.stack([pool[j] for j in cols]).contiguous()  # column c at c * n
+        X = torch.zeros_like(B)
+        torch.cuda.synchronize()
+        bs.set_values(vals[d].data_ptr())
+        rc, _ = bs.solve(X.data_ptr(), A0.n, B.data_ptr(), A0.n, len(cols), tol, budget,
+                         1 if bs.has_z() else 0, stream=stream.cuda_stream)
+        stream.synchronize()
+        st, _ = bs.stats()
+        return rc, st.cycles, X.cpu().numpy()
+
+    bs = handle()
+    threaded = {}
+    try:
+        jobs = [queue.Queue() for _ in range(4)]
+        done = queue.Queue()
+
+        def worker(i):
+            torch.cuda.set_device(0)
+            stream = torch.cuda.Stream()
+            while True:
+                L = jobs[i].get()
+                if L is None:
+                    return
+                try:
+                    done.put((L, launch(bs, L, stream), threading.get_ident()))
+                except Exception as e:  # reported by the main thread
+                    done.put((L, e, threading.get_ident()))
+
+        threads = [threading.Thread(target=worker, args=(i,)) for i in range(4)]
+        for t in threads:
+            t.start()
+        try:
+            for L in range(len(launches)):
+                jobs[L % 4].put(L)  # one launch at a time, each from the next thread
+                L2, out, tid = done.get(timeout=120)
+                assert L2 == L
+                if isinstance(out, Exception):
+                    raise out
+                threaded[L] = (out, tid)
+        finally:
+            for q in jobs:
+                q.put(None)
+            for t in threads:
+                t.join(timeout=120)
+    finally:
+        bs.close()
+    assert len({tid for _, tid in threaded.values()}) == 4
+    ref = handle()
+    try:
+        stream = torch.cuda.Stream()
+        for L in range(len(launches)):
+            rc, cyc, X = launch(ref, L, stream)
+            (rc_t, cyc_t, X_t), _ = threaded[L]
+            print(f"launch {L} (width {len(launches[L][1])}): threads {cyc_t} cycles, one "
+                  f"thread {cyc}", flush=True)
+            assert rc == rc_t == 0
+            assert cyc == cyc_t
+            assert np.array_equal(X, X_t), f"launch {L} differs across threads"
+    finally:
+        ref.close()

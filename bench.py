@@ -342,12 +342,16 @@ def kernel_of(phase, leg, tuning):
 
 def roofline(prof, leg, tuning=None, bf16=False, n=0, s=0):
     """The dominant streaming kernel of the headline leg: of the phases that launch one kernel
-    each (SpMV or SpMM, projection, update), the one with the most measured time.
-    achieved = its algorithmic bytes / its time (HIP events around each launch on the solve
-    stream, in the profiled re-solve)."""
+    each (SpMV or SpMM, projection, update), the one with the most measured time; in the seqblock
+    leg always the block update (one kernel instance per process; the projection, a family of
+    instances by column count, takes within a few percent of its time and is reported in
+    phase_GBps).  achieved = its algorithmic bytes / its time (HIP events around each launch on
+    the solve stream, in the profiled re-solve)."""
     cands = [p for p in ("spmv", "gemvt", "update")
              if prof.get(p) and prof[p][2] > 0 and kernel_of(p, leg, tuning)]
     best = max(cands, key=lambda p: prof[p][0])
+    if leg == "seqblock" and "update" in cands:
+        best = "update"
     ms, by, cnt = prof[best][:3]
     fl = prof[best][3] if len(prof[best]) > 3 else None
     kernel = kernel_of(best, leg, tuning)
@@ -578,6 +582,9 @@ def main():
         line["phase_GBps"] = {nm: {p: round(v[1] / (v[0] * 1e-3) / 1e9, 1)
                                    for p, v in lg["prof"].items() if v[0] > 0 and v[1] > 0}
                               for nm, lg in legs.items() if lg["prof"]}
+        line["phase_bytes_per_launch"] = {
+            nm: {p: round(v[1] / v[2]) for p, v in lg["prof"].items() if v[1] > 0 and v[2] > 0}
+            for nm, lg in legs.items() if lg["prof"]}
     if not args.no_batched:
         # The batched form of the metric: 32 right-hand sides of one operator as one block solve
         # (block Gram-Schmidt on the matrix cores, FP32 inside FP64 refinement) against the plain

@@ -613,8 +613,9 @@ double oracle_solve(OracleCtx *c, int mode, const double *b, double *x, double t
   c->rho = rho;
   *status = rc;
   if (rc) {
+    // The cycle-limit and runaway exits return before the solution export (gDMR.cu:2816-2829
+    // against 2885): x is left as passed in.
     *numIts = grdItr;
-    std::copy(v.xa.begin(), v.xa.begin() + n, x);
     return rho / beta;
   }
   *numIts = grdItr - 1;

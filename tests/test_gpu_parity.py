@@ -306,6 +306,9 @@ def test_stagnation_restart_matches_oracle(torch_gpu, n, alpha, m, k, mat_rank):
         assert restarts >= 1
         _compare(f"stag[{t}]", its, hist, xd.cpu().numpy(), itso, ora.history(), xo)
         assert st.status == sto == gdmr.DMR_ERR_NOCONV and len(hist) == len(ora.history())
+        # The cycle-limit exit returns before the solution export (gDMR.cu:2816-2829): x stays
+        # the initial guess on both sides.
+        assert np.array_equal(xd.cpu().numpy(), xo) and not xo.any()
         assert gdmr.lib.dmrGetStagnationCount(s.h) == ora.stagnation_count()
         assert np.max(np.abs(s.export_z() - ora.get_z())) <= 1e-9
     s.close()

@@ -855,6 +855,32 @@ __device__ inline void stv(double *p, long long i, double v) { p[i] = v; }
 __device__ inline void stv(float *p, long long i, float v) { p[i] = v; }
 __device__ inline void stv(bf16 *p, long long i, float v) { p[i].bits = (uint16_t)bf_bits(v); }
 
+// prod[e] = va[e] * x[ci[e]] for e < ne <= kSpmvNnzCap, entry e on thread e % kBlock: all index
+// and value loads of the thread first, then all of its gathers, then the products.  Slots past
+// ne load entry ne - 1 again (a cache hit), so no load waits on a branch.
+constexpr int kSpmvEpt = kSpmvNnzCap / kBlock;
+template <typename TV, typename TX, typename TA>
+__device__ inline void spmv_products(const int *__restrict__ ci, const TV *__restrict__ va, int ne,
+                                     const TX *__restrict__ x, TA *prod) {
+  if (ne <= 0) return;
+  int idx[kSpmvEpt];
+  TV v[kSpmvEpt];
+#pragma unroll
+  for (int i = 0; i < kSpmvEpt; ++i) {
+    const int e = min((int)threadIdx.x + kBlock * i, ne - 1);
+    idx[i] = ci[e];
+    v[i] = va[e];
+  }
+  TA xv[kSpmvEpt];
+#pragma unroll
+  for (int i = 0; i < kSpmvEpt; ++i) xv[i] = (TA)ldv(x, idx[i]);
+#pragma unroll
+  for (int i = 0; i < kSpmvEpt; ++i) {
+    const int e = (int)threadIdx.x + kBlock * i;
+    if (e < ne) prod[e] = (TA)v[i] * xv[i];
+  }
+}
+
 template <typename TV, typename TX, typename TA>
 __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
                                                  const int *__restrict__ ci,
@@ -875,15 +901,20 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
     if (threadIdx.x == 0) stv(y, rs, s);
     return;
   }
-  for (int e = threadIdx.x; e < ne; e += kBlock)
-    prod[e] = (TA)va[e0 + e] * (TA)ldv(x, ci[e0 + e]);
+  // The thread's row bounds, then its <= kSpmvEpt entries' indices and values, then their
+  // gathers: every load of a phase in flight at once.
+  const bool hasRow = (int)threadIdx.x < nr;
+  int ra = 0, rb = 0;
+  if (hasRow) {
+    ra = rp[rs + threadIdx.x] - e0;
+    rb = rp[rs + threadIdx.x + 1] - e0;
+  }
+  spmv_products<TV, TX, TA>(ci + e0, va + e0, ne, x, prod);
   __syncthreads();
-  if ((int)threadIdx.x < nr) {
-    const int r = rs + threadIdx.x;
-    const int a = rp[r] - e0, b = rp[r + 1] - e0;
+  if (hasRow) {
     TA s = 0;
-    for (int e = a; e < b; ++e) s = s + prod[e];
-    stv(y, r, s);
+    for (int e = ra; e < rb; ++e) s = s + prod[e];
+    stv(y, rs + threadIdx.x, s);
   }
 }
 
@@ -951,14 +982,21 @@ __global__ __launch_bounds__(kBlock) void k_resid64(const int *__restrict__ rp,
       ss = v * v;
     }
   } else {
-    for (int e = threadIdx.x; e < ne; e += kBlock) prod[e] = va[e0 + e] * x[ci[e0 + e]];
+    const bool hasRow = (int)threadIdx.x < nr;
+    int a = 0, bb = 0;
+    double bv = 0.0;
+    if (hasRow) {
+      a = rp[rs + threadIdx.x] - e0;
+      bb = rp[rs + threadIdx.x + 1] - e0;
+      bv = b[rs + threadIdx.x];
+    }
+    spmv_products<double, double, double>(ci + e0, va + e0, ne, x, prod);
     __syncthreads();
-    if ((int)threadIdx.x < nr) {
+    if (hasRow) {
       const int row = rs + threadIdx.x;
-      const int a = rp[row] - e0, bb = rp[row + 1] - e0;
       double s = 0.0;
       for (int e = a; e < bb; ++e) s = s + prod[e];
-      const double v = b[row] - s;
+      const double v = bv - s;
       r[row] = v;
       ss = v * v;
     }

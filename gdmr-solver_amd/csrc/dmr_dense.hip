@@ -7,10 +7,12 @@
 //   (gDMR.cu:591-633, 690-839).  The implicit application of the reflectors to t gives the same
 //   y as forming Q explicitly.
 // * svd_smallest: right singular vectors by one-sided Jacobi (Hestenes 1958; Demmel & Veselic
-//   1992) with the round-robin ("chess tournament") pair ordering, one 64-wide wavefront per
-//   column pair, m/2 disjoint pairs per round.  Replaces cusolverDnCgesvdj (gDMR.cu:634-689).
-//   Singular values are sorted descending; the k vectors of the smallest ones are returned, each
-//   with its largest-magnitude component made positive.
+//   1992) with the round-robin ("chess tournament") pair ordering, one 32-lane half wavefront
+//   per column pair, m/2 disjoint pairs per round.  Replaces cusolverDnCgesvdj
+//   (gDMR.cu:634-689).  Singular values are sorted descending; the k vectors of the smallest ones
+//   are returned, each with its largest-magnitude component made positive.
+// * image_qr / image_fill: the image A Z of a refreshed deflation subspace from the cycle's
+//   Arnoldi relation (QR of H C by modified Gram-Schmidt), for the FP32 inner solves.
 //
 // Every operation order here (sequential per-thread sums, wavefront xor butterflies, explicit
 // fma) is fixed, so the FP64 oracle can reproduce the results bit for bit.
@@ -217,7 +219,101 @@ __global__ __launch_bounds__(kSvdThreads) void k_svd(const double *__restrict__ 
   if (tid == 0) *sweepsOut = sweep;
 }
 
+// Sum over a 64-lane wavefront (xor butterfly); every lane gets the result.
+__device__ inline double wave_sum64(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+  return v;
+}
+
+// The image of a refreshed deflation subspace from the Arnoldi relation of the cycle that
+// produced it.  The cycle satisfies A [Z V] = P H with P = [W V_0 .. V_{m-d}] (m + 1 orthonormal
+// basis columns) and H its (m+1) x m Hessenberg matrix, and the refresh takes Z~ = [Z V] C
+// (C: m x k, ld m).  So A Z~ = P (H C), and the QR factorisation H C = Q R gives
+// A Z = (P Q)(R D^-1), D = diag(||z~_i||): an orthonormal W = P Q and an upper triangular factor,
+// in exact arithmetic the same pair as the QR of the recomputed image A Z (k operator
+// applications and a Gram-Schmidt QR, gDMR.cu:1823-2031).  Modified Gram-Schmidt in FP64, one
+// wavefront per remaining column.  Q: (m+1) x k (ld m+1); R: k x k (ld k), zero below the
+// diagonal.  *ok = 0 when a column is numerically dependent on the previous ones.
+__global__ __launch_bounds__(256) void k_image_qr(const double *__restrict__ H, int m,
+                                                  const double *__restrict__ C, int k,
+                                                  double *__restrict__ Q, double *__restrict__ R,
+                                                  int *__restrict__ ok) {
+  extern __shared__ double sm[];
+  const int rows = m + 1;
+  double *M = sm;  // rows x k, ld rows
+  __shared__ double s_nrm;
+  __shared__ int s_ok;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  for (int idx = tid; idx < rows * k; idx += blockDim.x) {
+    const int i = idx % rows, j = idx / rows;
+    double a = 0.0;
+    for (int l = 0; l < m; ++l) a = fma(H[i + (long long)rows * l], C[l + (long long)m * j], a);
+    M[idx] = a;
+  }
+  for (int idx = tid; idx < k * k; idx += blockDim.x) R[idx] = 0.0;
+  if (tid == 0) s_ok = 1;
+  __syncthreads();
+  double first = 0.0;
+  for (int j = 0; j < k; ++j) {
+    double *mj = M + (long long)rows * j;
+    if (wv == 0) {
+      double s = 0.0;
+      for (int r = lane; r < rows; r += 64) s = fma(mj[r], mj[r], s);
+      s = wave_sum64(s);
+      if (lane == 0) s_nrm = sqrt(s);
+    }
+    __syncthreads();
+    double nrm = s_nrm;
+    if (j == 0) first = nrm;
+    if (!(nrm > 1e-12 * first)) {
+      if (tid == 0) s_ok = 0;
+      nrm = 1.0;
+    }
+    for (int r = tid; r < rows; r += blockDim.x) {
+      const double q = mj[r] / nrm;
+      mj[r] = q;
+      Q[r + (long long)rows * j] = q;
+    }
+    if (tid == 0) R[j + (long long)k * j] = nrm;
+    __syncthreads();
+    for (int l = j + 1 + wv; l < k; l += nw) {
+      double *ml = M + (long long)rows * l;
+      double s = 0.0;
+      for (int r = lane; r < rows; r += 64) s = fma(mj[r], ml[r], s);
+      s = wave_sum64(s);
+      for (int r = lane; r < rows; r += 64) ml[r] = fma(-s, mj[r], ml[r]);
+      if (lane == 0) R[j + (long long)k * l] = s;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) *ok = s_ok;
+}
+
+// H[r, i] = R[r, i] / ||z~_i|| (scalZ[i] = 1 / ||z~_i||) for r <= i < k: the deflation columns
+// of a cycle whose image comes from k_image_qr (H zeroed before).
+__global__ void k_image_fill(double *__restrict__ H, int m, int k, const double *__restrict__ R,
+                             const double *__restrict__ scalZ) {
+  for (int idx = threadIdx.x; idx < k * k; idx += blockDim.x) {
+    const int r = idx % k, i = idx / k;
+    if (r <= i) H[r + (long long)(m + 1) * i] = R[r + (long long)k * i] * scalZ[i];
+  }
+}
+
 }  // namespace
+
+size_t image_qr_lds(int m, int k) { return sizeof(double) * (size_t)(m + 1) * (size_t)k; }
+
+void image_qr(const double *H, int m, const double *C, int k, double *Q, double *R, int *ok,
+              hipStream_t s) {
+  const size_t lds = image_qr_lds(m, k);
+  if (lds > 64 * 1024) allow_max_lds(reinterpret_cast<const void *>(k_image_qr));
+  hipLaunchKernelGGL(k_image_qr, dim3(1), dim3(256), lds, s, H, m, C, k, Q, R, ok);
+}
+
+void image_fill(double *H, int m, int k, const double *R, const double *scalZ, hipStream_t s) {
+  hipLaunchKernelGGL(k_image_fill, dim3(1), dim3(256), 0, s, H, m, k, R, scalZ);
+}
 
 void lsq(const double *H, int m, const double *t, double *y, double *z, hipStream_t s) {
   const size_t lds = sizeof(double) * ((size_t)(m + 1) * m + (m + 1) + m);

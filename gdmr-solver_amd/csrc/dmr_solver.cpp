@@ -24,6 +24,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "dmr_internal.h"
@@ -58,6 +59,9 @@ void clear_err() {
 // (gDMR.cu:89).  The stagnation threshold 0.995 is gDMR.cu:2506.
 constexpr double kSvdTol = 1e-14;
 constexpr int kSvdMaxSweeps = 128;
+// The FP32 and bf16 inner solves of the mixed path use the reference's own FP32 tolerance
+// (svTolDMR = 2e-7, gDMR.cu:88): their basis carries FP32 rounding, so sweeps past it add nothing.
+constexpr double kSvdTolLow = 2e-7;
 constexpr double kStagnation = 0.995;
 
 long long pad_ld(long long n) { return dmr::basis_ld(n); }
@@ -65,6 +69,7 @@ long long pad_ld(long long n) { return dmr::basis_ld(n); }
 template <typename TS> struct Work {
   TS *P = nullptr, *Z = nullptr, *Zn = nullptr, *r = nullptr, *xa = nullptr, *g = nullptr,
      *u = nullptr;
+  TS *Wn = nullptr;  // image of the refreshed subspace (FP32 inner solves)
   bool hasZ = false;
 };
 
@@ -138,6 +143,14 @@ struct dmrContext {
   // Deflated Krylov steps project against [W V] in one Gram-Schmidt stage instead of the
   // reference's deflate-then-CGS; default for mixed precision (DMR_ONE_STAGE overrides).
   bool oneStage = false;
+  // FP32 inner solves take the image A Z of a refreshed subspace from the Arnoldi relation of the
+  // cycle that refreshed it (W = P Q, R from the QR of H C) instead of k operator applications
+  // and a Gram-Schmidt QR (DMR_IMPLICIT_IMAGE=0 turns it off).  imgValid: Wn, imgQ / imgR and
+  // scalZ hold the image of the current Z under the current operator values.
+  bool implicitImage = false;
+  bool imgValid = false;
+  double *imgQ = nullptr, *imgR = nullptr, *imgScal = nullptr;
+  int *imgOk = nullptr;
   dmr::RedWork red{};
   dmr::RedWork redOp{};  // sized to the SpMV block count of the operator
   void *redOpPool = nullptr;
@@ -169,10 +182,11 @@ size_t footprint(int n, int m, int k, int prec) {
   const long long ld = pad_ld(n);
   size_t big = sizeof(double) * (size_t)ld * (size_t)(m + 1 + 2 * k + 4);
   if (prec != DMR_PREC_FP64) {
-    big += (prec == DMR_PREC_BF16 ? 2 : 4) * (size_t)ld * (size_t)(m + 1 + 2 * k + 4);
+    big += (prec == DMR_PREC_BF16 ? 2 : 4) * (size_t)ld * (size_t)(m + 1 + 3 * k + 4);
     big += sizeof(double) * (size_t)ld * 3;
   }
-  const size_t small = sizeof(double) * ((size_t)(m + 1) * m * 2 + 8 * (m + 2) + 64 + 2 * k) +
+  const size_t small = sizeof(double) * ((size_t)(m + 1) * m * 2 + 8 * (m + 2) + 64 + 2 * k +
+                                         (size_t)(m + 1 + k) * k + k) +
                        sizeof(double) * dmr::kMaxPartials + sizeof(unsigned) * dmr::kCounters +
                        4096;
   return big + small;
@@ -186,7 +200,7 @@ int ctx_alloc(dmrContext *c) {
   size_t big32 = 0;
   const size_t es = c->prec == DMR_PREC_BF16 ? 2 : 4;  // inner basis element size
   if (c->prec != DMR_PREC_FP64) {
-    big32 = es * (size_t)ld * (size_t)(m + 1 + 2 * k + 4) + sizeof(double) * ld * 3;
+    big32 = es * (size_t)ld * (size_t)(m + 1 + 3 * k + 4) + sizeof(double) * ld * 3;
     big = 0;  // the FP64 Krylov basis is not needed by a mixed-precision context
   }
   const size_t total = big + big32;
@@ -231,6 +245,8 @@ int ctx_alloc(dmrContext *c) {
       w.g = f;
       f += ld;
       w.u = f;
+      f += ld;
+      w.Wn = f;
     };
     if (c->prec == DMR_PREC_BF16)
       carve(c->w16, (dmr::bf16 *)d);
@@ -242,7 +258,8 @@ int ctx_alloc(dmrContext *c) {
                         + (m + 2) * 4         // t, y, z, cv
                         + (size_t)m * (k + 1) // C
                         + (m + 2)             // sig
-                        + 16 + 16 + 64 + 8;   // scal, scal2, scalZ, one
+                        + 16 + 16 + 64 + 8    // scal, scal2, scalZ, one
+                        + (size_t)(m + 1 + k) * k + k;  // imgQ, imgR, imgScal
   const size_t smallBytes = sizeof(double) * (nsmall + dmr::kMaxPartials) +
                             sizeof(unsigned) * dmr::kCounters + sizeof(int) * 64;
   HIPCHK(hipMalloc(&c->smallPool, smallBytes));
@@ -270,6 +287,12 @@ int ctx_alloc(dmrContext *c) {
   s += 64;
   c->one = s;
   s += 8;
+  c->imgQ = s;
+  s += (size_t)(m + 1) * k;
+  c->imgR = s;
+  s += (size_t)k * k;
+  c->imgScal = s;
+  s += k;
   c->red.partials = s;
   s += dmr::kMaxPartials;
   c->red.counters = (unsigned *)s;
@@ -278,10 +301,14 @@ int ctx_alloc(dmrContext *c) {
   c->sweeps = ip + 1;
   c->bar = reinterpret_cast<unsigned *>(ip + 2);
   c->devErr = ip + 4;
+  c->imgOk = ip + 5;
   // The fused cooperative step kernel is bitwise equivalent but measured slower than the
   // separate kernels at 128^3 (2.2 vs ~4 TB/s), so it is opt-in.
   c->fused = dmr::env_flag("DMR_FUSED", false);
   c->oneStage = dmr::env_flag("DMR_ONE_STAGE", c->prec != DMR_PREC_FP64);
+  c->implicitImage = c->prec == DMR_PREC_FP32 && k > 0 &&
+                     dmr::image_qr_lds(m, k) <= 160 * 1024 &&
+                     dmr::env_flag("DMR_IMPLICIT_IMAGE", true);
   const double one = 1.0;
   HIPCHK(hipMemcpy(c->one, &one, sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipHostMalloc((void **)&c->hostStatus, 64 * sizeof(double), hipHostMallocDefault));
@@ -503,8 +530,21 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
         return 0;
       });
     };
-    // Deflation image: A Z = W R by CGS (gDMR.cu:1823-2031).
-    for (int i = 0; i < d; ++i) {
+    // Deflation image: A Z = W R by CGS (gDMR.cu:1823-2031), or, in an FP32 inner solve whose Z
+    // was refreshed by the previous cycle under the same operator, from that cycle's Arnoldi
+    // relation: W = P Q (in Wn) and R diag(1 / ||z~||) (image_qr at the previous cycle's end).
+    const bool implicitImg = d > 0 && c->implicitImage && c->imgValid;
+    if (implicitImg) {
+      const int rcImg = timed(c, s, DMR_PH_COMBO, vb * 2.0 * k, [&] {
+        if (hipMemcpyAsync(w.P, w.Wn, sizeof(TS) * (size_t)ld * k, hipMemcpyDeviceToDevice, s) !=
+            hipSuccess)
+          return set_err(DMR_ERR_HIP, "image copy failed");
+        dmr::image_fill(c->H, m, k, c->imgR, c->scalZ, s);
+        return 0;
+      });
+      if (rcImg) return rcImg;
+    }
+    for (int i = 0; i < (implicitImg ? 0 : d); ++i) {
       int rc = timed(c, s, DMR_PH_SPMV, spmvBytes, [&] { return op(w.Z + (long long)i * ld, w.g); });
       if (rc) return rc;
       ++c->st.matvecs;
@@ -543,8 +583,10 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
       if (hipEventRecord(c->evH, s) != hipSuccess ||
           hipStreamWaitEvent(c->side, c->evH, 0) != hipSuccess)
         return set_err(DMR_ERR_HIP, "side stream fork failed");
+      const double svdTol = std::is_same<TS, double>::value ? kSvdTol : kSvdTolLow;
       timed(c, c->side, DMR_PH_SVD, 0.0, [&] {
-        dmr::svd_smallest(c->H, m, k, kSvdTol, kSvdMaxSweeps, c->C, c->sig, c->sweeps, c->side);
+        dmr::svd_smallest(c->H, m, k, svdTol, kSvdMaxSweeps, c->C, c->sig, c->sweeps, c->side);
+        if (c->implicitImage) dmr::image_qr(c->H, m, c->C, k, c->imgQ, c->imgR, c->imgOk, c->side);
         return 0;
       });
       if (hipEventRecord(c->evSvd, c->side) != hipSuccess)
@@ -589,6 +631,12 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
         dmr::scale_cols<TS>(w.Zn, k, ld, n, c->scalZ, s);
         return 0;
       });
+      if (c->implicitImage)
+        timed(c, s, DMR_PH_ZGEMM, vb * (m + 1 + k), [&] {
+          dmr::combo_multi<TS>(w.Wn, k, w.P, m + 1, c->imgQ, nullptr, 0, nullptr, m + 1, ld, n,
+                               c->imgScal, red, s);
+          return 0;
+        });
     }
     if (hipMemcpyAsync(c->hostStatus, c->scal2, 4 * sizeof(double), hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
@@ -597,7 +645,10 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
         hipMemcpyAsync(c->hostStatus + 9, c->devErr, sizeof(int), hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
         hipMemcpyAsync(c->hostStatus + 10, c->sweeps, sizeof(int), hipMemcpyDeviceToHost, s) !=
-            hipSuccess)
+            hipSuccess ||
+        (c->implicitImage && refresh &&
+         hipMemcpyAsync(c->hostStatus + 11, c->imgOk, sizeof(int), hipMemcpyDeviceToHost, s) !=
+             hipSuccess))
       return set_err(DMR_ERR_HIP, "status readback failed");
     if (hipStreamSynchronize(s) != hipSuccess) return set_err(DMR_ERR_HIP, "stream sync failed");
     if (c->prof.on) prof_collect(c);
@@ -612,6 +663,9 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
     // breakdown step is zero and H is padded with identity columns, so singular vectors from the
     // padding combine zero vectors (the reference then divides by sqrt(0)); such a refresh is
     // discarded and the previous subspace is kept.
+    int imgOk = 0;
+    if (c->implicitImage && refresh) std::memcpy(&imgOk, c->hostStatus + 11, sizeof(int));
+    c->imgValid = c->implicitImage && refresh && bflag == 0 && imgOk == 1;
     if (refresh && bflag == 0) {
       std::swap(w.Z, w.Zn);
       w.hasZ = true;
@@ -765,6 +819,9 @@ double solve_mixed(dmrContext *c, int mode, double *x, const double *b, hipStrea
   int totalCycles = 0;
   double relres = 1.0;
   c->kryLoopFlag = 0;
+  // The operator values may have changed since the last solve: the first deflated cycle
+  // recomputes the image of Z.
+  c->imgValid = false;
   bool useZ = mode != 0 && w.hasZ;
   for (int outer = 0;; ++outer) {
     dmr::residual64(c->csr, c->x64, c->b64, c->r64, c->redOp, c->scal, s);

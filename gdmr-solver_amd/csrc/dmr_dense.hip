@@ -229,13 +229,14 @@ __device__ inline double wave_sum64(double v) {
 // The image of a refreshed deflation subspace from the Arnoldi relation of the cycle that
 // produced it.  The cycle satisfies A [Z V] = P H with P = [W V_0 .. V_{m-d}] (m + 1 orthonormal
 // basis columns) and H its (m+1) x m Hessenberg matrix, and the refresh takes Z~ = [Z V] C
-// (C: m x k, ld m).  So A Z~ = P (H C), and the QR factorisation H C = Q R gives
+// (C: m x k, ld m; H with leading dimension ldH).  So A Z~ = P (H C), and the QR factorisation
+// H C = Q R gives
 // A Z = (P Q)(R D^-1), D = diag(||z~_i||): an orthonormal W = P Q and an upper triangular factor,
 // in exact arithmetic the same pair as the QR of the recomputed image A Z (k operator
 // applications and a Gram-Schmidt QR, gDMR.cu:1823-2031).  Modified Gram-Schmidt in FP64, one
 // wavefront per remaining column.  Q: (m+1) x k (ld m+1); R: k x k (ld k), zero below the
 // diagonal.  *ok = 0 when a column is numerically dependent on the previous ones.
-__global__ __launch_bounds__(256) void k_image_qr(const double *__restrict__ H, int m,
+__global__ __launch_bounds__(256) void k_image_qr(const double *__restrict__ H, int ldH, int m,
                                                   const double *__restrict__ C, int k,
                                                   double *__restrict__ Q, double *__restrict__ R,
                                                   int *__restrict__ ok) {
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(256) void k_image_qr(const double *__restrict__ H, 
   for (int idx = tid; idx < rows * k; idx += blockDim.x) {
     const int i = idx % rows, j = idx / rows;
     double a = 0.0;
-    for (int l = 0; l < m; ++l) a = fma(H[i + (long long)rows * l], C[l + (long long)m * j], a);
+    for (int l = 0; l < m; ++l) a = fma(H[i + (long long)ldH * l], C[l + (long long)m * j], a);
     M[idx] = a;
   }
   for (int idx = tid; idx < k * k; idx += blockDim.x) R[idx] = 0.0;
@@ -304,11 +305,11 @@ __global__ void k_image_fill(double *__restrict__ H, int m, int k, const double 
 
 size_t image_qr_lds(int m, int k) { return sizeof(double) * (size_t)(m + 1) * (size_t)k; }
 
-void image_qr(const double *H, int m, const double *C, int k, double *Q, double *R, int *ok,
-              hipStream_t s) {
+void image_qr(const double *H, int ldH, int m, const double *C, int k, double *Q, double *R,
+              int *ok, hipStream_t s) {
   const size_t lds = image_qr_lds(m, k);
   if (lds > 64 * 1024) allow_max_lds(reinterpret_cast<const void *>(k_image_qr));
-  hipLaunchKernelGGL(k_image_qr, dim3(1), dim3(256), lds, s, H, m, C, k, Q, R, ok);
+  hipLaunchKernelGGL(k_image_qr, dim3(1), dim3(256), lds, s, H, ldH, m, C, k, Q, R, ok);
 }
 
 void image_fill(double *H, int m, int k, const double *R, const double *scalZ, hipStream_t s) {

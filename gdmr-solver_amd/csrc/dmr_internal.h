@@ -196,13 +196,14 @@ void svd_smallest_rect(const double *H, int rows, int ldH, int m, int k, double 
                        int zeroLast, double *C, double *sig, int *sweeps, hipStream_t s);
 
 // The image A Z = W R_Z of the subspace refreshed from a cycle, from that cycle's Arnoldi
-// relation instead of k operator applications: Q R = H C (C the m x k refresh coefficients, Q
+// relation instead of k operator applications: Q R = H C (H: the leading (m+1) x m block of the
+// cycle's Hessenberg matrix, leading dimension ldH; C the m x k refresh coefficients, ld m; Q
 // (m+1) x k, R k x k upper triangular, *ok = 0 for a dependent column), W = P Q by combo_multi,
 // and image_fill puts R diag(scalZ) into the first k columns of the next cycle's H.  One
 // workgroup each; image_qr needs image_qr_lds(m, k) <= 160 KiB of LDS.
 size_t image_qr_lds(int m, int k);
-void image_qr(const double *H, int m, const double *C, int k, double *Q, double *R, int *ok,
-              hipStream_t s);
+void image_qr(const double *H, int ldH, int m, const double *C, int k, double *Q, double *R,
+              int *ok, hipStream_t s);
 void image_fill(double *H, int m, int k, const double *R, const double *scalZ, hipStream_t s);
 
 }  // namespace dmr

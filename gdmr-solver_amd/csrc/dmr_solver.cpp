@@ -149,6 +149,10 @@ struct dmrContext {
   // scalZ hold the image of the current Z under the current operator values.
   bool implicitImage = false;
   bool imgValid = false;
+  // Krylov steps at the end of a cycle left out of its subspace refresh, so that the refresh SVD
+  // overlaps them: 0 in the FP64 path (the reference's refresh), DMR_REFRESH_TAIL (default 2)
+  // in the FP32 and bf16 inner solves.
+  int refreshTail = 0;
   double *imgQ = nullptr, *imgR = nullptr, *imgScal = nullptr;
   int *imgOk = nullptr;
   dmr::RedWork red{};
@@ -306,6 +310,9 @@ int ctx_alloc(dmrContext *c) {
   // separate kernels at 128^3 (2.2 vs ~4 TB/s), so it is opt-in.
   c->fused = dmr::env_flag("DMR_FUSED", false);
   c->oneStage = dmr::env_flag("DMR_ONE_STAGE", c->prec != DMR_PREC_FP64);
+  c->refreshTail = c->prec == DMR_PREC_FP64
+                       ? 0
+                       : (int)dmr::env_ll("DMR_REFRESH_TAIL", 2, 0, m);
   c->implicitImage = c->prec == DMR_PREC_FP32 && k > 0 &&
                      dmr::image_qr_lds(m, k) <= 160 * 1024 &&
                      dmr::env_flag("DMR_IMPLICIT_IMAGE", true);
@@ -558,6 +565,35 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
       dmr::Fin fin{dmr::FIN_NORM, c->scal, nullptr, 0.0, 0, nullptr};
       orthogonalize(Po, Po, nullptr, 0, nullptr, w.P, k, c->cv, 1, fin, nullptr);
     }
+    // ---- genDeflatDMR, first half (gDMR.cu:1583-1704): the SVD of the Hessenberg matrix runs on
+    // the side stream, overlapped with the work that does not need it.  It takes the leading
+    // (cols + 1) x cols block, cols = d + lead: the whole matrix (lead = m - d) in the FP64 path;
+    // in the FP32 and bf16 inner solves.the block final refreshTail Krylov steps before the end
+    // of the cycle, so that the SVD runs beside those steps (the refreshed subspace then combines
+    // Z and the first lead Krylov vectors).
+    const bool refresh = k > 0;
+    const int lead = (m - d - c->refreshTail >= 1 && d + m - d - c->refreshTail >= k)
+                         ? m - d - c->refreshTail
+                         : m - d;
+    const int cols = d + lead;
+    bool forked = false;
+    auto fork_refresh = [&]() -> int {
+      forked = true;
+      if (hipEventRecord(c->evH, s) != hipSuccess ||
+          hipStreamWaitEvent(c->side, c->evH, 0) != hipSuccess)
+        return set_err(DMR_ERR_HIP, "side stream fork failed");
+      const double svdTol = std::is_same<TS, double>::value ? kSvdTol : kSvdTolLow;
+      timed(c, c->side, DMR_PH_SVD, 0.0, [&] {
+        dmr::svd_smallest_rect(c->H, cols + 1, m + 1, cols, k, svdTol, kSvdMaxSweeps, 0, c->C,
+                               c->sig, c->sweeps, c->side);
+        if (c->implicitImage)
+          dmr::image_qr(c->H, m + 1, cols, c->C, k, c->imgQ, c->imgR, c->imgOk, c->side);
+        return 0;
+      });
+      if (hipEventRecord(c->evSvd, c->side) != hipSuccess)
+        return set_err(DMR_ERR_HIP, "side stream join failed");
+      return 0;
+    };
     // Krylov steps (gDMR.cu:2069-2328): deflatorDMR(1, j) puts W^T g into H[0:k, d+j-1], then
     // CGS against V_0..V_{j-1} puts h into H[d:d+j, d+j-1]; H[d+j, d+j-1] = v_j . g.
     for (int j = 1; j <= m - d; ++j) {
@@ -574,23 +610,15 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
         orthogonalize(w.g, vj, nullptr, 0, nullptr, w.P, k + j, hcol, 3, fin, c->flag);
       else
         orthogonalize(w.g, vj, w.P, d > 0 ? k : 0, hcol, Po, j, hcol + d, 3, fin, c->flag);
+      if (refresh && j == lead && lead < m - d) {
+        rc = fork_refresh();
+        if (rc) return rc;
+      }
     }
     dmr::breakdown_fixup<TS>(w.P, ld, n, o, d, m, c->H, c->flag, s);
-    // ---- genDeflatDMR, first half (gDMR.cu:1583-1704): the SVD of H needs H only, so it runs on
-    // the side stream, overlapped with the least squares and the updates below.
-    const bool refresh = k > 0;
-    if (refresh) {
-      if (hipEventRecord(c->evH, s) != hipSuccess ||
-          hipStreamWaitEvent(c->side, c->evH, 0) != hipSuccess)
-        return set_err(DMR_ERR_HIP, "side stream fork failed");
-      const double svdTol = std::is_same<TS, double>::value ? kSvdTol : kSvdTolLow;
-      timed(c, c->side, DMR_PH_SVD, 0.0, [&] {
-        dmr::svd_smallest(c->H, m, k, svdTol, kSvdMaxSweeps, c->C, c->sig, c->sweeps, c->side);
-        if (c->implicitImage) dmr::image_qr(c->H, m, c->C, k, c->imgQ, c->imgR, c->imgOk, c->side);
-        return 0;
-      });
-      if (hipEventRecord(c->evSvd, c->side) != hipSuccess)
-        return set_err(DMR_ERR_HIP, "side stream join failed");
+    if (refresh && !forked) {
+      const int rc = fork_refresh();
+      if (rc) return rc;
     }
     // ---- updateAppResDMR (gDMR.cu:2411-2582)
     timed(c, s, DMR_PH_GEMVT, vb * (m + 2), [&] {
@@ -625,16 +653,16 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
     if (refresh) {
       if (hipStreamWaitEvent(s, c->evSvd, 0) != hipSuccess)
         return set_err(DMR_ERR_HIP, "side stream join failed");
-      timed(c, s, DMR_PH_ZGEMM, vb * (m + k) + vb * 2 * k, [&] {
-        dmr::combo_multi<TS>(w.Zn, k, Po, m - d, c->C + d, w.Z, d, c->C, m, ld, n, c->scalZ, red,
-                             s);
+      timed(c, s, DMR_PH_ZGEMM, vb * (cols + k) + vb * 2 * k, [&] {
+        dmr::combo_multi<TS>(w.Zn, k, Po, lead, c->C + d, w.Z, d, c->C, cols, ld, n, c->scalZ,
+                             red, s);
         dmr::scale_cols<TS>(w.Zn, k, ld, n, c->scalZ, s);
         return 0;
       });
       if (c->implicitImage)
-        timed(c, s, DMR_PH_ZGEMM, vb * (m + 1 + k), [&] {
-          dmr::combo_multi<TS>(w.Wn, k, w.P, m + 1, c->imgQ, nullptr, 0, nullptr, m + 1, ld, n,
-                               c->imgScal, red, s);
+        timed(c, s, DMR_PH_ZGEMM, vb * (cols + 1 + k), [&] {
+          dmr::combo_multi<TS>(w.Wn, k, w.P, cols + 1, c->imgQ, nullptr, 0, nullptr, cols + 1, ld,
+                               n, c->imgScal, red, s);
           return 0;
         });
     }

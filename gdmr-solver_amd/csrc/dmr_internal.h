@@ -70,7 +70,13 @@ struct RedWork {
   unsigned *counters;    // kCounters counters
 };
 constexpr int kMaxPartials = 128 * kMaxRedBlocks;
-constexpr int kCounters = 64;
+// Arrival counters of a grid reduction: one top counter and kArrivalFan sub-counters per column
+// group (blocks arrive on sub-counter b % kArrivalFan, the last of each sub-group on the top
+// counter), each on its own 128-byte line, for up to kCounterGroups column groups per launch.
+constexpr int kArrivalFan = 16;
+constexpr int kCounterStride = 32;
+constexpr int kCounterGroups = 8;
+constexpr int kCounters = kCounterGroups * (1 + kArrivalFan) * kCounterStride;
 
 // How the last block of a combination kernel finalises its reductions.
 enum FinMode : int {

@@ -18,6 +18,7 @@
 //    multiply-add is an explicit fma().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <mutex>
@@ -132,17 +133,38 @@ __device__ inline float get_partial(const float *p) {
                                            __HIP_MEMORY_SCOPE_AGENT));
 }
 
-// Arrival of a block at the end of a grid-wide reduction.  True in exactly one block, the last
-// to arrive.  Every block writes its partials from its first wavefront; lane 0 of that wavefront
-// then counts the arrival with one agent-scope acquire-release atomic, which orders those stores
-// before the count (one L2 write-back per block, not one per wavefront as a block-wide
-// __threadfence would cost) and, in the last block, orders the partial loads after it.
-__device__ inline bool arrive_last(unsigned *counter, unsigned total) {
+// Arrival of block b (of total) at the end of a grid-wide reduction.  True in exactly one block,
+// the last to arrive.  Every block writes its partials; thread 0 then counts the arrival with
+// agent-scope acquire-release atomics, which order those stores before the count (one L2
+// write-back per block, not one per wavefront as a block-wide __threadfence would cost) and, in
+// the last block, order the partial loads after it.  The count runs in two levels: block b
+// arrives on sub-counter b % kArrivalFan of its group, the last arrival of a sub-group on the
+// group's top counter.  Atomics on one address are serialised by the memory system (measured:
+// about 20 ns each, so 512 blocks on one counter cost 10 us per launch); here at most
+// total / kArrivalFan + 1 of them meet on any one counter.  The chain of acquire-release pairs
+// makes every block's partials visible to the last block.  Each counter is returned to zero by
+// the block that completes it.
+__device__ inline unsigned *group_counters(unsigned *base, int group) {
+  return base + (long long)group * (1 + kArrivalFan) * kCounterStride;
+}
+__device__ inline bool arrive_last(unsigned *ctr, unsigned b, unsigned total) {
   __shared__ int s_last;
   __syncthreads();
-  if (threadIdx.x == 0)
-    s_last = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
-             total - 1u;
+  if (threadIdx.x == 0) {
+    const unsigned fan = total < (unsigned)kArrivalFan ? total : (unsigned)kArrivalFan;
+    const unsigned sub = b % fan;
+    const unsigned subTotal = total / fan + (sub < total % fan ? 1u : 0u);
+    unsigned *sc = ctr + (1 + sub) * kCounterStride;
+    bool last = false;
+    if (__hip_atomic_fetch_add(sc, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+        subTotal - 1u) {
+      __hip_atomic_store(sc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      last = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+             fan - 1u;
+      if (last) __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_last = last ? 1 : 0;
+  }
   __syncthreads();
   return s_last != 0;
 }
@@ -226,25 +248,41 @@ __device__ inline void finalize(const Fin &fin, double ss, double sg) {
   }
 }
 
-// o += sum_l cs[l] * B_l[u] for l < p, in column order.  Columns are loaded eight at a time,
-// all eight loads issued before the fused multiply-adds; indices past p alias column p-1 (a
-// cache hit) and are not used.
+// o += sum_l cs[l] * B_l[u] for the NC columns of one chunk: all NC loads issued before the
+// fused multiply-adds, which run in column order.
+template <int NC, typename TS, typename TA, int N>
+__device__ inline void fma_chunk(const TS *__restrict__ B, long long ld, long long u, const TA *cs,
+                                 TA (&o)[N]) {
+  TA b[NC][N];
+#pragma unroll
+  for (int i = 0; i < NC; ++i) ldu(B + (long long)i * ld, u, b[i]);
+#pragma unroll
+  for (int i = 0; i < NC; ++i) {
+    const TA c = cs[i];
+#pragma unroll
+    for (int e = 0; e < N; ++e) o[e] = fmac(c, b[i][e], o[e]);
+  }
+}
+// o += sum_l cs[l] * B_l[u] for l < p, in column order: chunks of 16 columns' loads in flight,
+// then one chunk of 8, 4 and 1 to 3 for the rest, so no load reads a column past p.
 template <typename TS, typename TA, int N>
 __device__ inline void fma_cols(const TS *__restrict__ B, int p, long long ld, long long u,
                                 const TA *cs, TA (&o)[N]) {
-  // Sixteen columns' loads in flight at a time; the FMAs stay in column order.
-  for (int l0 = 0; l0 < p; l0 += 16) {
-    TA b[16][N];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) ldu(B + (long long)min(l0 + i, p - 1) * ld, u, b[i]);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      if (l0 + i < p) {
-        const TA c = cs[l0 + i];
-#pragma unroll
-        for (int e = 0; e < N; ++e) o[e] = fmac(c, b[i][e], o[e]);
-      }
-    }
+  int l0 = 0;
+  for (; l0 + 16 <= p; l0 += 16) fma_chunk<16>(B + (long long)l0 * ld, ld, u, cs + l0, o);
+  if (p - l0 >= 8) {
+    fma_chunk<8>(B + (long long)l0 * ld, ld, u, cs + l0, o);
+    l0 += 8;
+  }
+  if (p - l0 >= 4) {
+    fma_chunk<4>(B + (long long)l0 * ld, ld, u, cs + l0, o);
+    l0 += 4;
+  }
+  switch (p - l0) {
+    case 3: fma_chunk<3>(B + (long long)l0 * ld, ld, u, cs + l0, o); break;
+    case 2: fma_chunk<2>(B + (long long)l0 * ld, ld, u, cs + l0, o); break;
+    case 1: fma_chunk<1>(B + (long long)l0 * ld, ld, u, cs + l0, o); break;
+    default: break;
   }
 }
 
@@ -337,10 +375,9 @@ __global__ __launch_bounds__(kBlock) void k_multidot(const TS *__restrict__ B, l
   block_sum_cols<TA, CB>(acc, res);
   if (threadIdx.x < pc) put_partial(partials + (long long)(c0 + threadIdx.x) * G + blockIdx.x,
                                     res[threadIdx.x]);
-  if (!arrive_last(&counters[blockIdx.y], (unsigned)G)) return;
+  if (!arrive_last(group_counters(counters, blockIdx.y), blockIdx.x, (unsigned)G)) return;
   sum_partials_cols<TA>(partials, G, c0, pc, res);
   if (threadIdx.x < pc) out[(long long)(c0 + threadIdx.x) * ostride] = (double)res[threadIdx.x];
-  if (threadIdx.x == 0) counters[blockIdx.y] = 0u;
 }
 
 // =============================================================================================
@@ -418,12 +455,9 @@ __global__ __launch_bounds__(kBlock) void k_combo(const TS *in, TS *out, const T
   block_sum_cols<TA, NR>(v, res);
   if (threadIdx.x < NR) put_partial(partials + (long long)threadIdx.x * G + blockIdx.x,
                                     res[threadIdx.x]);
-  if (!arrive_last(counter, (unsigned)G)) return;
+  if (!arrive_last(counter, blockIdx.x, (unsigned)G)) return;
   sum_partials_cols<TA>(partials, G, 0, NR, res);
-  if (threadIdx.x == 0) {
-    finalize(fin, (double)res[0], NR == 2 ? (double)res[1] : 0.0);
-    *counter = 0u;
-  }
+  if (threadIdx.x == 0) finalize(fin, (double)res[0], NR == 2 ? (double)res[1] : 0.0);
 }
 
 // =============================================================================================
@@ -459,7 +493,27 @@ __global__ __launch_bounds__(kBlock) void k_combo_multi(
     for (int j = 0; j < JB; ++j)
 #pragma unroll
       for (int e = 0; e < N; ++e) a[j][e] = 0;
-    for (int l = 0; l < p1 + p2; ++l) {
+    // Eight columns' loads in flight, then their multiply-adds in column order; the last
+    // columns one at a time.
+    const int pt = p1 + p2;
+    int l = 0;
+    for (; l + 8 <= pt; l += 8) {
+      TA b[8][N];
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        ldu(l + i < p1 ? B1 + (long long)(l + i) * ld : B2 + (long long)(l + i - p1) * ld, u, b[i]);
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < JB; ++j) {
+          if (j < Jg) {
+            const TA c = cs[(l + i) * JB + j];
+#pragma unroll
+            for (int e = 0; e < N; ++e) a[j][e] = fmac(c, b[i][e], a[j][e]);
+          }
+        }
+    }
+    for (; l < pt; ++l) {
       TA b[N];
       ldu(l < p1 ? B1 + (long long)l * ld : B2 + (long long)(l - p1) * ld, u, b);
 #pragma unroll
@@ -487,10 +541,9 @@ __global__ __launch_bounds__(kBlock) void k_combo_multi(
   block_sum_cols<TA, JB>(ss, res);
   if (threadIdx.x < Jg)
     put_partial(partials + (long long)(j0 + threadIdx.x) * G + blockIdx.x, res[threadIdx.x]);
-  if (!arrive_last(&counters[blockIdx.y], (unsigned)G)) return;
+  if (!arrive_last(group_counters(counters, blockIdx.y), blockIdx.x, (unsigned)G)) return;
   sum_partials_cols<TA>(partials, G, j0, Jg, res);
   if (threadIdx.x < Jg) scal[j0 + threadIdx.x] = 1.0 / sqrt((double)res[threadIdx.x]);
-  if (threadIdx.x == 0) counters[blockIdx.y] = 0u;
 }
 
 // =============================================================================================
@@ -1003,14 +1056,13 @@ __global__ __launch_bounds__(kBlock) void k_resid64(const int *__restrict__ rp,
   }
   const double bs = block_sum(ss);
   if (threadIdx.x == 0) put_partial(partials + blockIdx.x, bs);
-  if (!arrive_last(counter, (unsigned)nb)) return;
+  if (!arrive_last(counter, blockIdx.x, (unsigned)nb)) return;
   if (threadIdx.x >= 64) return;
   const double tot = wave_sum_partials(partials, nb);
   if (threadIdx.x == 0) {
     scal[0] = tot;
     scal[1] = 1.0 / sqrt(tot);
     scal[2] = sqrt(tot);
-    *counter = 0u;
   }
 }
 
@@ -1087,12 +1139,16 @@ void multidot(const TS *B, long long ld, int p, const TS *x, long long n, double
   const int G = red_blocks<TS>(n);
   const long long nv = units<TS>(n);
   // Column groups of at most kDotCols columns (56 for FP64, whose 64-wide accumulators would
-  // leave one wavefront per SIMD), balanced; a group's width is rounded up to one of 1, 2, 4 or
-  // a multiple of 8 (the kernel template), so x is read once per group.
+  // leave one wavefront per SIMD), balanced; a group's width is rounded up to one of 1, 2, 4,
+  // 8, 12, 16, 20, 24 or a multiple of 8 (the kernel template), so x is read once per group and
+  // at most three loads per unit alias a column already read.
   const int widest = sizeof(TS) == 8 ? kDotCols - 8 : kDotCols;
   const int groups = (p + widest - 1) / widest;
   const int per = (p + groups - 1) / groups;
-  const int cb = per <= 4 ? (per <= 1 ? 1 : per <= 2 ? 2 : 4) : (per + 7) / 8 * 8;
+  const int cb = per <= 2    ? per
+                 : per <= 4  ? 4
+                 : per <= 24 ? std::max(8, (per + 3) / 4 * 4)
+                             : (per + 7) / 8 * 8;
   TA *part = (TA *)w.partials;
 #define DMR_MDOT(CBV)                                                                      \
   hipLaunchKernelGGL((k_multidot<TS, CBV>), dim3(G, groups), dim3(kBlock), 0, s, B, ld, p, x, \
@@ -1102,7 +1158,9 @@ void multidot(const TS *B, long long ld, int p, const TS *x, long long n, double
     case 2: DMR_MDOT(2); break;
     case 4: DMR_MDOT(4); break;
     case 8: DMR_MDOT(8); break;
+    case 12: DMR_MDOT(12); break;
     case 16: DMR_MDOT(16); break;
+    case 20: DMR_MDOT(20); break;
     case 24: DMR_MDOT(24); break;
     case 32: DMR_MDOT(32); break;
     case 40: DMR_MDOT(40); break;

@@ -972,7 +972,7 @@ int dmrSetOperatorCsr(dmrHandle h, int nnz, const int *rowPtr, const int *colInd
   HIPCHK(hipMemcpy(h->ownedBlockRows, br.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
   if (h->redOpPool) hipFree(h->redOpPool);
   h->redOpPool = nullptr;
-  const size_t rbytes = sizeof(double) * (size_t)std::max(nb, 1) + 64;
+  const size_t rbytes = sizeof(double) * (size_t)std::max(nb, 1) + sizeof(unsigned) * dmr::kCounters;
   HIPCHK(hipMalloc(&h->redOpPool, rbytes));
   HIPCHK(hipMemset(h->redOpPool, 0, rbytes));
   h->redOp.partials = h->redOpPool;

@@ -577,6 +577,23 @@ def main():
         line["roofline"] = roofline(legs[head]["prof"], head, legs[head].get("tuning"),
                                     head == "seqblock" and args.seq_prec == "bf16",
                                     mats[0].n, max(legs[head].get("blocks", [1])))
+        # The box's practical HBM read ceiling, measured here: a flat 16-byte streaming read of
+        # 1 GiB (four times the memory-side cache), best of 10.
+        import torch
+
+        import gdmr
+        probe = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        ceil = gdmr.probe_read_bandwidth(probe.data_ptr(), probe.numel(), 10)
+        del probe
+        line["roofline"]["ceiling"] = {
+            "read_GBps": ceil, "frac_of_peak": ceil * 1e9 / HBM_PEAK,
+            "probe": "flat 16-byte streaming read of 1 GiB, 8 loads in flight per thread, "
+                     "best of 10 (dmrProbeReadBandwidth)"}
+        line["roofline"]["frac_of_ceiling"] = line["roofline"]["achieved"] / ceil
+        log(f"[roofline] {line['roofline']['kernel']}: {line['roofline']['achieved']:.0f} GB/s, "
+            f"{100 * line['roofline']['frac']:.1f}% of 8 TB/s, "
+            f"{100 * line['roofline']['frac_of_ceiling']:.1f}% of the measured read ceiling "
+            f"{ceil:.0f} GB/s")
         line["phase_ms"] = {nm: {p: round(v[0], 3) for p, v in lg["prof"].items()}
                             for nm, lg in legs.items() if lg["prof"]}
         line["phase_GBps"] = {nm: {p: round(v[1] / (v[0] * 1e-3) / 1e9, 1)

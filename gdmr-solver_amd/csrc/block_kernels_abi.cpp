@@ -28,6 +28,41 @@ bool rows_ok(int rowsPerBlock) { return rowsPerBlock >= 64 && rowsPerBlock % 64 
 
 extern "C" {
 
+int dmrProbeReadBandwidth(const void *buf, size_t bytes, int reps, hipStream_t stream,
+                          double *gbps) {
+  dmr::clear_error();
+  if (!buf || !gbps || bytes < ((size_t)64 << 20) || bytes % 16 != 0 || reps < 1)
+    return bad("dmrProbeReadBandwidth: invalid arguments");
+  unsigned *sink = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipMalloc(&sink, sizeof(unsigned)) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    if (sink) hipFree(sink);
+    if (e0) hipEventDestroy(e0);
+    return dmr::set_error(DMR_ERR_HIP, "dmrProbeReadBandwidth: setup failed");
+  }
+  const long long units = (long long)(bytes / 16);
+  float best = 1e30f;
+  int rc = DMR_OK;
+  for (int r = 0; r < reps + 2 && rc == DMR_OK; ++r) {
+    hipEventRecord(e0, stream);
+    dmr::probe_read(buf, units, sink, stream);
+    hipEventRecord(e1, stream);
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      rc = dmr::set_error(DMR_ERR_HIP, "dmrProbeReadBandwidth: launch failed");
+      break;
+    }
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (r >= 2 && ms > 0.f && ms < best) best = ms;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(sink);
+  if (rc == DMR_OK) *gbps = (double)bytes / (best * 1e-3) / 1e9;
+  return rc;
+}
+
 size_t dmrKernelTnWorkspace(long long n, int p, int rowsPerBlock) {
   if (n < 1 || p < 1 || !rows_ok(rowsPerBlock)) return 0;
   const int G = blk::gemm_tn_blocks(n, blk::kMaxTnBlocks, rowsPerBlock);

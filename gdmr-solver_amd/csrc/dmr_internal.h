@@ -201,6 +201,10 @@ void allow_max_lds(const void *fn);
 void svd_smallest_rect(const double *H, int rows, int ldH, int m, int k, double tol, int maxSweeps,
                        int zeroLast, double *C, double *sig, int *sweeps, hipStream_t s);
 
+// Streaming read of units16 16-byte units of buf (the HBM read ceiling probe); sink is written
+// only in the practically impossible case that the folded words equal a fixed constant.
+void probe_read(const void *buf, long long units16, unsigned *sink, hipStream_t s);
+
 // The image A Z = W R_Z of the subspace refreshed from a cycle, from that cycle's Arnoldi
 // relation instead of k operator applications: Q R = H C (H: the leading (m+1) x m block of the
 // cycle's Hessenberg matrix, leading dimension ldH; C the m x k refresh coefficients, ld m; Q

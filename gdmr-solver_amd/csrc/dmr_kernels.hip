@@ -1073,11 +1073,35 @@ inline int grid_for(long long nv) {
   return (int)g;
 }
 
+// Streaming read of nu 16-byte units: each thread keeps eight loads in flight over a grid-stride
+// walk.  The loaded words are folded into one value that is stored only if it equals a constant,
+// which keeps every load without writing anything in practice.
+__global__ __launch_bounds__(kBlock) void k_probe_read(const uint4 *__restrict__ a, long long nu,
+                                                       unsigned *sink) {
+  constexpr int U = 8;
+  unsigned x = 0u;
+  const long long stride = (long long)gridDim.x * kBlock * U;
+  for (long long u = (long long)blockIdx.x * kBlock * U + threadIdx.x; u < nu; u += stride) {
+    uint4 v[U];
+#pragma unroll
+    for (int i = 0; i < U; ++i)
+      v[i] = u + (long long)i * kBlock < nu ? a[u + (long long)i * kBlock] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < U; ++i) x ^= v[i].x ^ v[i].y ^ v[i].z ^ v[i].w;
+  }
+  if (x == 0x9e3779b9u) sink[0] = x;
+}
+
 }  // namespace
 
 // =============================================================================================
 // Launchers
 // =============================================================================================
+void probe_read(const void *buf, long long units16, unsigned *sink, hipStream_t s) {
+  hipLaunchKernelGGL(k_probe_read, dim3(2048), dim3(kBlock), 0, s,
+                     static_cast<const uint4 *>(buf), units16, sink);
+}
+
 int spmv_partition(const int *rp, int n, int *br) {
   int nb = 0;
   int r = 0;

@@ -146,6 +146,7 @@ _sig("dmrKernelScaleBlock", C.c_int, [_vp, _ll, _vp, _vp, _ll, _i32, _i32, _ll, 
 _sig("dmrKernelLsqBlock", C.c_int, [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp,
                                     _vp, _vp])
 _sig("dmrKernelLsqFits", C.c_int, [_i32, _i32, _i32])
+_sig("dmrProbeReadBandwidth", C.c_int, [_vp, C.c_size_t, _i32, _vp, _dp])
 PHASES = ("spmv", "gemvt", "update", "scale", "combo", "zgemm", "lsq", "svd", "orth", "reduce",
           "resid")
 # Problems and random numbers.
@@ -186,11 +187,21 @@ HEADER_SYMBOLS = (
     "dmrRandUniform dmrSplitMix64 "
     "dmrKernelTnWorkspace dmrKernelGemmTn dmrKernelGemmTnPair dmrKernelGemmNn dmrKernelGramBlocks "
     "dmrKernelGramReduce dmrKernelGemmNnPair dmrKernelSpmm dmrKernelCholQr dmrKernelScaleBlock "
-    "dmrKernelLsqBlock dmrKernelLsqFits").split()
+    "dmrKernelLsqBlock dmrKernelLsqFits dmrProbeReadBandwidth").split()
 
 
 def _ptr(a: np.ndarray, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def probe_read_bandwidth(buf_ptr: int, nbytes: int, reps: int = 10) -> float:
+    """The device's measured HBM read bandwidth in GB/s (best of `reps` streaming reads of
+    `nbytes` of device memory at buf_ptr; see dmrProbeReadBandwidth)."""
+    out = C.c_double(0.0)
+    rc = lib.dmrProbeReadBandwidth(C.c_void_p(buf_ptr), nbytes, reps, None, C.byref(out))
+    if rc != 0:
+        raise RuntimeError("dmrProbeReadBandwidth: %s" % last_error()[1])
+    return out.value
 
 
 def last_error() -> tuple[int, str]:

@@ -84,6 +84,15 @@ int dmrKernelLsqBlock(const double *H, int rows, int cols, int bw, const double 
                       float *ZH32, hipStream_t stream);
 int dmrKernelLsqFits(int cols, int bw, int s);
 
+/* Measured HBM read bandwidth of the device: `bytes` (a multiple of 16, at least 64 MiB) of
+ * `buf` streamed by 16-byte loads, eight in flight per thread, 2048 workgroups; two warm-up
+ * launches, then `reps` timed launches (HIP events on `stream`), *gbps = the best.  A buffer
+ * several times the 256 MiB memory-side cache measures HBM.  The stream is synchronised.  No
+ * counterpart in the reference: it gives the practical ceiling that kernel bandwidths are
+ * compared with beside the nominal peak. */
+int dmrProbeReadBandwidth(const void *buf, size_t bytes, int reps, hipStream_t stream,
+                          double *gbps);
+
 #ifdef __cplusplus
 }
 #endif

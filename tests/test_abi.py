@@ -99,3 +99,8 @@ def test_kernel_entry_points_check_shapes_without_device():
     assert lib.dmrKernelLsqBlock(one, 30, 20, 8, one, 4, one, one, one, one, one, one,
                                  None) == gdmr.DMR_ERR_ARG  # rows != cols + bw
     assert lib.dmrKernelLsqFits(100, 16, 16) == 1 and lib.dmrKernelLsqFits(100, 40, 16) == 0
+    gbps = gdmr.C.c_double(0.0)
+    assert lib.dmrProbeReadBandwidth(one, 1 << 20, 4, None,
+                                     gdmr.C.byref(gbps)) == gdmr.DMR_ERR_ARG  # < 64 MiB
+    assert lib.dmrProbeReadBandwidth(one, (64 << 20) + 8, 4, None,
+                                     gdmr.C.byref(gbps)) == gdmr.DMR_ERR_ARG  # not 16-byte units

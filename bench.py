@@ -578,16 +578,17 @@ def main():
                                     head == "seqblock" and args.seq_prec == "bf16",
                                     mats[0].n, max(legs[head].get("blocks", [1])))
         # The box's practical HBM read ceiling, measured here: a flat 16-byte streaming read of
-        # 1 GiB (four times the memory-side cache), best of 10.
+        # 4 GiB (sixteen times the memory-side cache; one launch of about 0.7 ms, so the launch
+        # overhead is small), best of 10.
         import torch
 
         import gdmr
-        probe = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        probe = torch.empty(4 << 30, dtype=torch.uint8, device="cuda")
         ceil = gdmr.probe_read_bandwidth(probe.data_ptr(), probe.numel(), 10)
         del probe
         line["roofline"]["ceiling"] = {
             "read_GBps": ceil, "frac_of_peak": ceil * 1e9 / HBM_PEAK,
-            "probe": "flat 16-byte streaming read of 1 GiB, 8 loads in flight per thread, "
+            "probe": "flat 16-byte streaming read of 4 GiB, 8 loads in flight per thread, "
                      "best of 10 (dmrProbeReadBandwidth)"}
         line["roofline"]["frac_of_ceiling"] = line["roofline"]["achieved"] / ceil
         log(f"[roofline] {line['roofline']['kernel']}: {line['roofline']['achieved']:.0f} GB/s, "

@@ -251,7 +251,48 @@ def test_mixed_precision_reaches_fp64_tolerance(torch_gpu, prec):
     s.close()
 
 
-def test_matrix_market_operator(tmp_path):
+def test_mixed_implicit_image_and_early_refresh(torch_gpu, monkeypatch):
+    """The FP32 inner solve's deflation image taken from the refreshing cycle's Arnoldi relation
+    (and its refresh SVD forked two Krylov steps early) against the reference's recomputation
+    (k SpMVs and a Gram-Schmidt QR every cycle, refresh at the cycle end): on a jittered
+    sequence with recycling, both reach 1e-10 FP64-checked on every system, take cycle counts
+    within two of each other, and their solutions agree to the problem's conditioning.  A
+    solve after new operator values recomputes the image (the operator changed)."""
+    torch = torch_gpu
+    N, m, k, tol = 20, 30, 10, 1e-10
+    mats = [gdmr.conv_diff_3d(N, jitter=0.01, seed=5, stream=t) for t in range(4)]
+    n = mats[0].n
+    rp, ci = _dev(torch, mats[0].rowptr), _dev(torch, mats[0].col)
+    vals = [_dev(torch, A.val) for A in mats]
+    runs = {}
+    for label, image, tail in (("recomputed", "0", "0"), ("implicit", "1", "2")):
+        monkeypatch.setenv("DMR_IMPLICIT_IMAGE", image)
+        monkeypatch.setenv("DMR_REFRESH_TAIL", tail)
+        s = gdmr.Solver(n, m, k, precision=gdmr.PREC_FP32)
+        s.set_csr(rp.data_ptr(), ci.data_ptr(), vals[0].data_ptr(), mats[0].nnz,
+                  keep=(rp, ci, vals[0]))
+        xs, cyc = [], []
+        for t, A in enumerate(mats):
+            s.set_values(vals[t].data_ptr())
+            b = gdmr.rand_normal(5, 100 + t, n)
+            xd = torch.zeros(n, dtype=torch.float64, device="cuda")
+            res, its = s.solve(xd.data_ptr(), _dev(torch, b).data_ptr(), tol, 10000,
+                               mode=1 if t else 0)
+            x = xd.cpu().numpy()
+            assert _true_res(A, b, x) <= tol
+            st, _ = s.stats()
+            xs.append(x)
+            cyc.append(st.cycles)
+        s.close()
+        runs[label] = (xs, cyc)
+        print(f"{label}: cycles {cyc}", flush=True)
+    (xr, cr), (xi, ci_) = runs["recomputed"], runs["implicit"]
+    for t in range(len(mats)):
+        assert abs(cr[t] - ci_[t]) <= 2
+        assert np.linalg.norm(xi[t] - xr[t]) / np.linalg.norm(xr[t]) < 1e-7
+
+
+def test_matrix_market_operator(tmp_path):def test_matrix_market_operator(tmp_path):
     """The on-disk operator path: a Matrix Market file imported with impMtxLS gives the solve of
     the same matrix imported with impCsrLS, bit for bit, and matches the oracle."""
     A = gdmr.conv_diff_2d(40, jitter=0.01, seed=9, stream=2)

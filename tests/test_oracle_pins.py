@@ -369,6 +369,60 @@ def test_refresh_matches_lapack_svd(N, m, k, seed):
     assert _sign_free_err(zS / np.linalg.norm(zS), Znew[0]) > 1e-2
 
 
+def _pos_qr(M):
+    """QR with a positive diagonal of R (the factorisation a Gram-Schmidt QR produces)."""
+    Q, R = np.linalg.qr(M)
+    sg = np.sign(np.diag(R))
+    sg[sg == 0] = 1.0
+    return Q * sg, (R.T * sg).T
+
+
+@pytest.mark.parametrize("N,m,k,seed", [(24, 20, 6, 2), (32, 30, 10, 0)])
+def test_deflation_image_from_arnoldi_relation(N, m, k, seed):
+    """The FP32 inner solves take the image A Z = W R of a refreshed subspace from the refreshing
+    cycle's Arnoldi relation instead of k operator applications and a Gram-Schmidt QR
+    (gDMR.cu:1823-2031): with Z~ = [Z V] C and A [Z V] = P H, the QR H C = Q R gives W = P Q and
+    R diag(1 / ||z~_i||).  On the FP64 oracle's own cycle (a deflated one) this pair equals the
+    positive-diagonal QR of the recomputed A Z to FP64 accuracy, and a wrong coefficient block
+    or a basis shifted by one column misses it by far."""
+    A = gdmr.conv_diff_2d(N, jitter=0.01, seed=seed)
+    n = A.n
+    As = A.to_scipy()
+    ora = oracle.Oracle(A, m, k, gdmr.reduction_blocks(n), keep_last=True)
+    for mode in (0, 1):
+        ora.solve(gdmr.rand_normal(1, mode, n), tol=1e-6, mode=mode)
+    P, H, Z, d = ora.last_cycle()
+    assert d == k
+    Znew = ora.get_z()
+    zt, S, src, V = _refresh_from_numpy(P, H, Z, d, k, m)
+    # Refresh coefficients with the oracle's signs.
+    C = np.empty((m, k))
+    for i in range(k):
+        c = V[:, m - k + i]
+        zi = src.T @ c
+        if np.linalg.norm(zi / np.linalg.norm(zi) - Znew[i]) > 1e-6:
+            c = -c
+        C[:, i] = c
+    ztn = np.linalg.norm(src.T @ C, axis=0)
+    # Explicit image: the positive-diagonal QR of A Z_new.
+    We, Re = _pos_qr(np.column_stack([As @ Znew[i] for i in range(k)]))
+    # Implicit image from the relation.
+    Q, R = _pos_qr(H @ C)
+    Wi = P.T @ Q
+    Ri = R / ztn[None, :]
+    scale = np.abs(Re).max()
+    assert np.abs(Wi - We).max() < 1e-9
+    assert np.abs(Ri - Re).max() < 1e-9 * scale
+    assert np.abs(np.column_stack([As @ Znew[i] for i in range(k)]) - Wi @ Ri).max() < 1e-10 * scale
+    # Planted mistakes: the coefficient block one singular vector off, the basis shifted.
+    C2 = C.copy()
+    C2[:, 0] = V[:, m - k - 1]
+    Q2, _ = _pos_qr(H @ C2)
+    assert np.abs(P.T @ Q2 - We).max() > 1e-2
+    Qs, _ = _pos_qr(H[:m] @ C)
+    assert np.abs(P[1:].T @ Qs - We).max() > 1e-2
+
+
 # ------------------------------------------------------------------------------------------------
 # Stagnation restart with deflation (arnoldiSetupDMR, gDMR.cu:2586-2632; flag gDMR.cu:2506-2514).
 # ------------------------------------------------------------------------------------------------

@@ -292,7 +292,7 @@ def test_mixed_implicit_image_and_early_refresh(torch_gpu, monkeypatch):
         assert np.linalg.norm(xi[t] - xr[t]) / np.linalg.norm(xr[t]) < 1e-7
 
 
-def test_matrix_market_operator(tmp_path):def test_matrix_market_operator(tmp_path):
+def test_matrix_market_operator(tmp_path):
     """The on-disk operator path: a Matrix Market file imported with impMtxLS gives the solve of
     the same matrix imported with impCsrLS, bit for bit, and matches the oracle."""
     A = gdmr.conv_diff_2d(40, jitter=0.01, seed=9, stream=2)

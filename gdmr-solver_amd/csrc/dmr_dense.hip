@@ -301,9 +301,36 @@ __global__ void k_image_fill(double *__restrict__ H, int m, int k, const double 
   }
 }
 
+// Check of an implicit image on its last column (the smallest singular value, where the
+// relation's error weighs most).  Prep: col[r] = R[r, k-1] / ||z~_{k-1}||, the coefficients of
+// the expected image W col of the normalised z_{k-1}.  Final (after e = A z_{k-1} - W col with
+// ss = ||e||^2 in chk[0]): *ok = 0 when ||e|| > relTol ||col|| (W orthonormal: ||W col|| = ||col||)
+// or the check is not finite.
+__global__ void k_image_check_prep(const double *__restrict__ R, int k,
+                                   const double *__restrict__ scalZ, double *__restrict__ col) {
+  for (int r = threadIdx.x; r < k; r += blockDim.x)
+    col[r] = R[r + (long long)k * (k - 1)] * scalZ[k - 1];
+}
+__global__ void k_image_check_final(const double *__restrict__ col, int k,
+                                    const double *__restrict__ chk, double relTol, int *ok) {
+  if (threadIdx.x != 0) return;
+  double cc = 0.0;
+  for (int r = 0; r < k; ++r) cc = fma(col[r], col[r], cc);
+  const double e = sqrt(chk[0]);
+  if (!(e <= relTol * sqrt(cc))) *ok = 0;
+}
+
 }  // namespace
 
 size_t image_qr_lds(int m, int k) { return sizeof(double) * (size_t)(m + 1) * (size_t)k; }
+
+void image_check_prep(const double *R, int k, const double *scalZ, double *col, hipStream_t s) {
+  hipLaunchKernelGGL(k_image_check_prep, dim3(1), dim3(64), 0, s, R, k, scalZ, col);
+}
+void image_check_final(const double *col, int k, const double *chk, double relTol, int *ok,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_image_check_final, dim3(1), dim3(64), 0, s, col, k, chk, relTol, ok);
+}
 
 void image_qr(const double *H, int ldH, int m, const double *C, int k, double *Q, double *R,
               int *ok, hipStream_t s) {

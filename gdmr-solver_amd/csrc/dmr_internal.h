@@ -215,5 +215,12 @@ size_t image_qr_lds(int m, int k);
 void image_qr(const double *H, int ldH, int m, const double *C, int k, double *Q, double *R,
               int *ok, hipStream_t s);
 void image_fill(double *H, int m, int k, const double *R, const double *scalZ, hipStream_t s);
+// Check of an implicit image on column k - 1: image_check_prep writes the coefficients
+// col = R[:, k-1] / ||z~_{k-1}|| of its expected image W col; after the caller forms
+// e = A z_{k-1} - W col with ||e||^2 in chk[0], image_check_final clears *ok when
+// ||e|| > relTol ||col||.
+void image_check_prep(const double *R, int k, const double *scalZ, double *col, hipStream_t s);
+void image_check_final(const double *col, int k, const double *chk, double relTol, int *ok,
+                       hipStream_t s);
 
 }  // namespace dmr

@@ -62,6 +62,20 @@ constexpr int kSvdMaxSweeps = 128;
 // The FP32 and bf16 inner solves of the mixed path use the reference's own FP32 tolerance
 // (svTolDMR = 2e-7, gDMR.cu:88): their basis carries FP32 rounding, so sweeps past it add nothing.
 constexpr double kSvdTolLow = 2e-7;
+// Largest relative error of an implicit deflation image (checked on its last column) that the
+// next cycle accepts instead of recomputing the image (DMR_IMAGE_TOL overrides).  The check's
+// own FP32 product of a smallest-singular direction cancels to about 1e-7 times the operator's
+// conditioning, so a bound near the FP32 rounding rejects sound images (1e-4 rejected nearly all
+// at 128^3); 1e-2 keeps them and still catches a chain whose error grows (measured on the 2-D
+// 512^2 operator at m = 80, which diverges without the check).
+double image_tol() {
+  static const double v = [] {
+    const char *e = std::getenv("DMR_IMAGE_TOL");
+    const double x = e ? std::atof(e) : 0.0;
+    return x > 0.0 ? x : 1e-2;
+  }();
+  return v;
+}
 constexpr double kStagnation = 0.995;
 
 long long pad_ld(long long n) { return dmr::basis_ld(n); }
@@ -153,7 +167,8 @@ struct dmrContext {
   // overlaps them: 0 in the FP64 path (the reference's refresh), DMR_REFRESH_TAIL (default 2)
   // in the FP32 and bf16 inner solves.
   int refreshTail = 0;
-  double *imgQ = nullptr, *imgR = nullptr, *imgScal = nullptr;
+  double *imgQ = nullptr, *imgR = nullptr, *imgScal = nullptr, *imgCol = nullptr,
+         *imgChk = nullptr;
   int *imgOk = nullptr;
   dmr::RedWork red{};
   dmr::RedWork redOp{};  // sized to the SpMV block count of the operator
@@ -190,7 +205,7 @@ size_t footprint(int n, int m, int k, int prec) {
     big += sizeof(double) * (size_t)ld * 3;
   }
   const size_t small = sizeof(double) * ((size_t)(m + 1) * m * 2 + 8 * (m + 2) + 64 + 2 * k +
-                                         (size_t)(m + 1 + k) * k + k) +
+                                         (size_t)(m + 1 + k) * k + 2 * k + 8) +
                        sizeof(double) * dmr::kMaxPartials + sizeof(unsigned) * dmr::kCounters +
                        4096;
   return big + small;
@@ -263,7 +278,8 @@ int ctx_alloc(dmrContext *c) {
                         + (size_t)m * (k + 1) // C
                         + (m + 2)             // sig
                         + 16 + 16 + 64 + 8    // scal, scal2, scalZ, one
-                        + (size_t)(m + 1 + k) * k + k;  // imgQ, imgR, imgScal
+                        + (size_t)(m + 1 + k) * k + 2 * k + 8;  // imgQ, imgR, imgScal,
+                                                                // imgCol, imgChk
   const size_t smallBytes = sizeof(double) * (nsmall + dmr::kMaxPartials) +
                             sizeof(unsigned) * dmr::kCounters + sizeof(int) * 64;
   HIPCHK(hipMalloc(&c->smallPool, smallBytes));
@@ -297,6 +313,10 @@ int ctx_alloc(dmrContext *c) {
   s += (size_t)k * k;
   c->imgScal = s;
   s += k;
+  c->imgCol = s;
+  s += k;
+  c->imgChk = s;
+  s += 8;
   c->red.partials = s;
   s += dmr::kMaxPartials;
   c->red.counters = (unsigned *)s;
@@ -659,12 +679,32 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
         dmr::scale_cols<TS>(w.Zn, k, ld, n, c->scalZ, s);
         return 0;
       });
-      if (c->implicitImage)
+      if (c->implicitImage) {
         timed(c, s, DMR_PH_ZGEMM, vb * (cols + 1 + k), [&] {
           dmr::combo_multi<TS>(w.Wn, k, w.P, cols + 1, c->imgQ, nullptr, 0, nullptr, cols + 1, ld,
                                n, c->imgScal, red, s);
           return 0;
         });
+        // The relation carries the rounding of the cycle's basis, and in a chain of implicit
+        // images the error of each one feeds the next (through the Z rows of C, divided by
+        // ||z~||): on an ill-conditioned operator it can grow from cycle to cycle.  So every
+        // implicit image is checked on its last column, whose singular value is the smallest:
+        // e = A z_{k-1} - W R[:, k-1] / ||z~_{k-1}||, and a relative error above image_tol()
+        // makes the next cycle recompute the image (one SpMV and one k-column update here,
+        // against k SpMVs and a Gram-Schmidt QR saved per cycle).
+        int rcChk = timed(c, s, DMR_PH_SPMV, spmvBytes,
+                          [&] { return op(w.Zn + (long long)(k - 1) * ld, w.g); });
+        if (rcChk) return rcChk;
+        ++c->st.matvecs;
+        dmr::image_check_prep(c->imgR, k, c->scalZ, c->imgCol, s);
+        dmr::Fin finChk{dmr::FIN_STORE, c->imgChk, nullptr, 0.0, 0, nullptr};
+        timed(c, s, DMR_PH_COMBO, vb * (k + 2), [&] {
+          dmr::combo<TS>(dmr::COMB_SUB, w.g, w.u, w.Wn, k, c->imgCol, nullptr, 0, nullptr, ld, n,
+                         0.0, 1, nullptr, finChk, red, nullptr, s);
+          return 0;
+        });
+        dmr::image_check_final(c->imgCol, k, c->imgChk, image_tol(), c->imgOk, s);
+      }
     }
     if (hipMemcpyAsync(c->hostStatus, c->scal2, 4 * sizeof(double), hipMemcpyDeviceToHost, s) !=
             hipSuccess ||

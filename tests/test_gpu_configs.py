@@ -221,3 +221,29 @@ def test_config_e_serving_families_full_size(torch_gpu):
         err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
         print(f"E {f['name']}: first solve vs oracle {err:.2e} ({itso} iterations)", flush=True)
         assert sto == 0 and np.array_equal(x, xo), (f["name"], err)
+
+
+@pytest.mark.parametrize("m", [80, 96])
+def test_mixed_long_cycles_on_the_2d_serving_family(torch_gpu, m):
+    """The 2-D 512^2 serving family with FP32 inner solves and long restart cycles (m = 80, 96,
+    k = 20), three right-hand sides with the subspace recycled: every solve meets 1e-10 in FP64.
+    This operator is ill-conditioned enough that a chain of deflation images taken from the
+    Arnoldi relation without a check lost the relation within a few cycles and diverged at
+    m = 80; each image is now checked on its last column and recomputed when it has drifted."""
+    torch = torch_gpu
+    A = gdmr.conv_diff_2d(512, jitter=0.01, seed=41, stream=0)
+    n, k = A.n, 20
+    s = gdmr.Solver(n, m, k, precision=gdmr.PREC_FP32)
+    rp, ci, va = _dev(torch, A.rowptr), _dev(torch, A.col), _dev(torch, A.val)
+    s.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A.nnz, keep=(rp, ci, va))
+    s.set_refinement(1e-5)
+    for t in range(3):
+        b = gdmr.rand_normal(500, t, n)
+        xd = torch.zeros(n, dtype=torch.float64, device="cuda")
+        res, its = s.solve(xd.data_ptr(), _dev(torch, b).data_ptr(), TOL, 3000, mode=1 if t else 0)
+        st, _ = s.stats()
+        tr = _true(A, b, xd.cpu().numpy())
+        print(f"m {m} system {t}: cycles {st.cycles} matvecs {st.matvecs} residual {res:.2e} "
+              f"(true {tr:.2e})", flush=True)
+        assert st.status == 0 and np.isfinite(res) and tr <= TOL and st.cycles <= 200
+    s.close()

@@ -1486,11 +1486,15 @@ __global__ __launch_bounds__(BWT == 32 ? 512 : 1024) void k_lsq_block(const doub
     return col < cols ? H[(long long)col * rows + r] : T[(long long)(col - cols) * rows + r];
   };
   if constexpr (STG) {
-    // Register path (BWT = 16 or 32, cols + s <= blockDim.x): thread tid owns column tid and
-    // holds its window rows c .. c + BWT in registers (x[0] = row c).  Per step: the owner of column c
-    // forms the reflector from its registers and posts it (v and tau, double-buffered by step
-    // parity) in LDS; one barrier; every later column takes the reflector, writes its final
-    // row c to R or Qt and shifts its window up by one row, the new row coming from the stage.
+    // Register path (BWT = 16 or 32, bandwidth bw <= BWT, cols + s <= blockDim.x): thread tid
+    // owns column tid and holds its window rows c .. c + BWT in registers (x[0] = row c).  Per
+    // step: the owner of column c forms the reflector from its registers and posts it (v and tau,
+    // double-buffered by step parity) in LDS; one barrier; every later column takes the
+    // reflector, writes its final row c to R or Qt and shifts its window up by one row, the new
+    // row c + BWT + 1 coming from the stage.  A bandwidth below BWT gives reflectors whose tail
+    // past row c + bw is zero: those entries add exact zeros, and the window rows they would
+    // touch keep their values.
+    constexpr int WS = BWT + 1;  // rows held per column
     double x[BWT + 1];
     const int col = tid;
     const bool own = col < nc;
@@ -1500,16 +1504,16 @@ __global__ __launch_bounds__(BWT == 32 ? 512 : 1024) void k_lsq_block(const doub
     double *vbuf = sm + kLsqChunk * nc;       // 2 x 64: v[0 .. BWT - 1], tau at [BWT]
     double pre[kLsqChunk];
 #pragma unroll
-    for (int q = 0; q < kLsqChunk; ++q) pre[q] = (own && W + q < rows) ? src(W + q, col) : 0.0;
+    for (int q = 0; q < kLsqChunk; ++q) pre[q] = (own && WS + q < rows) ? src(WS + q, col) : 0.0;
     for (int cb = 0; cb < cols; cb += kLsqChunk) {
-      // Rows cb + W + q for the steps cb + q (each thread reads back only its own column, so no
+      // Rows cb + WS + q for the steps cb + q (each thread reads back only its own column, so no
       // barrier guards the stage); then the next chunk's rows are fetched.
 #pragma unroll
       for (int q = 0; q < kLsqChunk; ++q)
         if (own) stage[q * nc + col] = pre[q];
 #pragma unroll
       for (int q = 0; q < kLsqChunk; ++q) {
-        const int r = cb + kLsqChunk + W + q;
+        const int r = cb + kLsqChunk + WS + q;
         pre[q] = (own && r < rows) ? src(r, col) : 0.0;
       }
       const int ce = cb + kLsqChunk < cols ? cb + kLsqChunk : cols;
@@ -2191,14 +2195,14 @@ size_t lsq_lds(int cols, int bw, int s, bool reg) {
   const size_t back = (size_t)cols * 32 + 32 * 33;
   return sizeof(double) * (win > back ? win : back);
 }
-// The register path: bandwidth 16, one thread per column of [H | T], and its LDS fits.
-// Bandwidth 32: 512 threads (the window rows of a column take 66 registers).  DMR_LSQ_WINDOW=1
-// keeps every shape on the window path.
+// The register path: one thread per column of [H | T], its LDS fits; bandwidth up to 16 on the
+// 16-row window (1024 threads), up to 32 on the 32-row window (512 threads: the window rows of
+// a column take 66 registers).  DMR_LSQ_WINDOW=1 keeps every shape on the window path.
 bool lsq_staged(int cols, int bw, int s) {
   if (const char *e = std::getenv("DMR_LSQ_WINDOW"))
     if (e[0] == '1') return false;
-  const int threads = bw == 32 ? 512 : 1024;
-  return ((bw == 16 && s <= 16) || (bw == 32 && s <= 32)) && cols + s <= threads &&
+  const int threads = bw > 16 ? 512 : 1024;
+  return ((bw <= 16 && s <= 16) || (bw <= 32 && s <= 32)) && bw >= 1 && cols + s <= threads &&
          lsq_lds(cols, bw, s, true) <= 140 * 1024;
 }
 
@@ -2215,7 +2219,7 @@ void lsq_block(const double *H, int rows, int cols, int bw, const double *T, int
 #define DMR_LSQ(B, S)                                                                            \
   hipLaunchKernelGGL((k_lsq_block<B, S>), dim3(1), dim3(B == 32 ? 512 : 1024), lds, st, H, rows, \
                      cols, bw, T, s, rho, R, Qt, Y, Yr32, ZH32)
-  if (stg && bw == 32)
+  if (stg && bw > 16)
     DMR_LSQ(32, true);
   else if (stg)
     DMR_LSQ(16, true);

@@ -438,10 +438,11 @@ def test_spmm_variants_identical_on_ragged_rows(torch_gpu, monkeypatch, prec):
             assert np.array_equal(X, ref), code
 
 
-@pytest.mark.parametrize("s", [16, 32])
+@pytest.mark.parametrize("s", [10, 16, 20, 32])
 def test_lsq_register_path_matches_window_path(torch_gpu, monkeypatch, s):
-    """The block least squares on its register path (one thread per column of [H | T], for
-    bandwidth 16 and 32) and on its LDS-window path (DMR_LSQ_WINDOW=1) solve the same systems:
+    """The block least squares on its register path (one thread per column of [H | T], on a
+    16-row window for bandwidths up to 16 and a 32-row one up to 32) and on its LDS-window path
+    (DMR_LSQ_WINDOW=1) solve the same systems:
     both meet 1e-10 in FP64 and their solutions agree to the accuracy the residual allows."""
     A = gdmr.conv_diff_3d(16, jitter=0.01, seed=12, stream=0)
     n = A.n

@@ -334,10 +334,12 @@ def test_scale_block(torch_gpu, outbasis):
     _check(got, ref, absref, bound, f"scale_block out {outbasis}")
 
 
-@pytest.mark.parametrize("k,s,steps", [(20, 10, 16), (20, 16, 16), (32, 32, 10)])
+@pytest.mark.parametrize("k,s,steps", [(20, 10, 16), (20, 16, 16), (32, 32, 10), (20, 6, 20),
+                                        (20, 20, 10)])
 def test_lsq_block(torch_gpu, k, s, steps):
     """Banded block-Hessenberg least squares (lower bandwidth s): Y per right-hand side against
-    numpy.linalg.lstsq (LAPACK), Yr32 = Y diag(rho), ZH32 = H Y."""
+    numpy.linalg.lstsq (LAPACK), Yr32 = Y diag(rho), ZH32 = H Y.  Bandwidths 6, 10 and 16 run on
+    the register path's 16-row window, 20 and 32 on its 32-row window."""
     torch = torch_gpu
     cols, bw = k + steps * s, s
     rows = cols + bw

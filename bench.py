@@ -469,6 +469,10 @@ def main():
     ap.add_argument("--cpu-cycles-1t", type=int, default=2,
                     help="restart cycles of the one-thread CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune-file", default=os.path.join(ROOT, "gdmr-solver_amd", "tuning",
+                                                        "gfx950.txt"),
+                    help="persisted launch shapes of the block autotuner (read, and appended to "
+                         "for shapes it lacks); 'none': tune in this process only")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--inner-tol", type=float, default=1e-5,
                     help="relative tolerance of each inner FP32 solve of the mixed leg")
@@ -500,6 +504,11 @@ def main():
 
     import gdmr
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    tune_file = None if args.tune_file == "none" else args.tune_file
+    if tune_file:
+        os.makedirs(os.path.dirname(tune_file), exist_ok=True)
+    gdmr.set_tuning_file(tune_file)
+    log(f"block launch shapes: {tune_file or 'tuned in this process'}")
     N, m, k, tol = args.grid, args.m, args.k, args.tol
     t0 = time.time()
     mats, rhs = build_sequence(N, args.steps, args.seed)
@@ -556,7 +565,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"3-D conv-diff 7-pt {N}^3 sequence of {args.steps} jittered "
                                f"systems, m={m} k={k} tol {tol:g}, Z recycled",
-                   "n": mats[0].n, "nnz": mats[0].nnz, "headline_leg": head},
+                   "n": mats[0].n, "nnz": mats[0].nnz, "headline_leg": head,
+                   "tuning_file": (os.path.relpath(tune_file, ROOT) if tune_file else None)},
         "p50_ms": hs["p50_ms"],
         "p99_ms": hs["p99_ms"],
         "legs": out_legs,

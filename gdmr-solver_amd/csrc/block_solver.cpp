@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <mutex>
 #include <string>
@@ -363,6 +364,72 @@ struct TuneKey {
 std::mutex g_tuneMu;
 std::vector<std::pair<TuneKey, dmrBlockContext::Tuning>> g_tuned;
 
+// Persisted shapes: a text file (dmrBlockTuningFile, else DMR_TUNE_FILE) of one line per tuned
+// shape, "arch n nnz s k mb bf  tnRowsProj tnVarProj tnRowsGram tnVarGram nnUnroll nnBasis
+// spmmCols", read into the table before the first tuning of the process (lines of other device
+// architectures are skipped) and appended to after each new tuning.  Every process that uses
+// the file launches the same kernels for a shape, so its solves are bit-reproducible across
+// processes as well.
+std::string g_tuneFile;
+bool g_tuneFileSet = false;
+bool g_tuneLoaded = false;
+
+std::string device_arch() {
+  int d = 0;
+  hipDeviceProp_t p{};
+  if (hipGetDevice(&d) != hipSuccess || hipGetDeviceProperties(&p, d) != hipSuccess) return "";
+  std::string a = p.gcnArchName;
+  return a.substr(0, a.find(':'));
+}
+
+const std::string &tune_file() {
+  if (!g_tuneFileSet) {
+    g_tuneFileSet = true;
+    if (const char *e = std::getenv("DMR_TUNE_FILE")) g_tuneFile = e;
+  }
+  return g_tuneFile;
+}
+
+// Caller holds g_tuneMu.
+void tune_file_load_locked() {
+  if (g_tuneLoaded) return;
+  g_tuneLoaded = true;
+  const std::string &path = tune_file();
+  if (path.empty()) return;
+  FILE *f = std::fopen(path.c_str(), "r");
+  if (!f) return;
+  const std::string arch = device_arch();
+  char line[512];
+  while (std::fgets(line, sizeof line, f)) {
+    if (line[0] == '#') continue;
+    char a[64];
+    TuneKey k{};
+    dmrBlockContext::Tuning t;
+    if (std::sscanf(line, "%63s %lld %lld %d %d %d %d %d %d %d %d %d %d %d", a, &k.n, &k.nnz, &k.s,
+                    &k.k, &k.mb, &k.bf, &t.tnRowsProj, &t.tnVarProj, &t.tnRowsGram,
+                    &t.tnVarGram, &t.nnUnroll, &t.nnBasis, &t.spmmCols) != 14 ||
+        arch != a)
+      continue;
+    t.s = k.s;
+    bool have = false;
+    for (auto &e : g_tuned) have = have || e.first == k;
+    if (!have) g_tuned.emplace_back(k, t);
+  }
+  std::fclose(f);
+}
+
+// Caller holds g_tuneMu.
+void tune_file_append_locked(const TuneKey &k, const dmrBlockContext::Tuning &t) {
+  const std::string &path = tune_file();
+  if (path.empty()) return;
+  FILE *f = std::fopen(path.c_str(), "a");
+  if (!f) return;
+  std::fprintf(f, "%s %lld %lld %d %d %d %d %d %d %d %d %d %d %d\n", device_arch().c_str(), k.n,
+               k.nnz, k.s, k.k, k.mb, k.bf, t.tnRowsProj, t.tnVarProj, t.tnRowsGram, t.tnVarGram,
+               t.nnUnroll, t.nnBasis, t.spmmCols);
+  std::fclose(f);
+}
+
 // Times each candidate launch shape on the handle's own buffers (HIP events, best of 3 after a
 // warm-up launch) and keeps the fastest: gemm_tn rows per workgroup and variant, separately for
 // projections over the full basis and for s x s Gram matrices; the gemm_nn load-ahead depth; the
@@ -375,6 +442,7 @@ void autotune_t(dmrBlockContext *c, TP *P, int s, hipStream_t st) {
   const TuneKey key{c->n, c->csr.nnz, s, c->k, c->mb, bf ? 1 : 0};
   {
     std::lock_guard<std::mutex> lk(g_tuneMu);
+    tune_file_load_locked();
     for (auto &e : g_tuned)
       if (e.first == key) {
         c->tune = e.second;
@@ -468,6 +536,7 @@ void autotune_t(dmrBlockContext *c, TP *P, int s, hipStream_t st) {
   c->tune = t;
   std::lock_guard<std::mutex> lk(g_tuneMu);
   g_tuned.emplace_back(key, t);
+  tune_file_append_locked(key, t);
 }
 
 void autotune(dmrBlockContext *c, int s, hipStream_t st) {
@@ -1081,6 +1150,14 @@ int dmrBlockSetOptions(dmrBlockHandle h, int graphs, int autotune) {
     h->graphs = graphs != 0;
   }
   if (autotune >= 0) h->autotune = autotune != 0;
+  return DMR_OK;
+}
+
+int dmrBlockTuningFile(const char *path) {
+  std::lock_guard<std::mutex> lk(g_tuneMu);
+  g_tuneFile = path ? path : "";
+  g_tuneFileSet = true;
+  g_tuneLoaded = false;
   return DMR_OK;
 }
 

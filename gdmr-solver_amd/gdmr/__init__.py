@@ -123,6 +123,7 @@ _sig("dmrBlockGetProfile", C.c_int, [_vp, _dp, _dp, _dp, C.POINTER(C.c_longlong)
 _sig("dmrBlockFootprint", C.c_size_t, [C.c_int, C.c_int, C.c_int, C.c_int])
 _sig("dmrBlockSetOptions", C.c_int, [_vp, C.c_int, C.c_int])
 _sig("dmrBlockGetTuning", C.c_int, [_vp, _ip])
+_sig("dmrBlockTuningFile", C.c_int, [C.c_char_p])
 _sig("dmrSetProfiling", C.c_int, [_vp, C.c_int])
 _sig("dmrGetProfile", C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_longlong)])
 _sig("dmrCsrApplyStatic", C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp])
@@ -177,7 +178,7 @@ HEADER_SYMBOLS = (
     "dmrBlockCreate dmrBlockDestroy dmrBlockSetOperatorCsr dmrBlockSetOperatorValues "
     "dmrBlockSetRefinement dmrBlockSetColumnOperators dmrBlockSetPrecision dmrBlockSolve dmrBlockGetStats dmrBlockHasZ dmrBlockImportZ "
     "dmrBlockExportZ dmrBlockSetProfiling dmrBlockGetProfile dmrBlockFootprint "
-    "dmrBlockSetOptions dmrBlockGetTuning "
+    "dmrBlockSetOptions dmrBlockGetTuning dmrBlockTuningFile "
     "dmrServerCreate dmrServerAddFamily dmrServerAddBlockFamily dmrServerSetRefinement "
     "dmrServerSetBlockPrecision "
     "dmrServerSubmit dmrServerPoll "
@@ -192,6 +193,12 @@ HEADER_SYMBOLS = (
 
 def _ptr(a: np.ndarray, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def set_tuning_file(path: str | None) -> None:
+    """Persist the block autotuner's launch shapes in `path` (read before the first tuning of
+    the process, appended to after each new one); None turns persistence off."""
+    _check(lib.dmrBlockTuningFile(path.encode() if path else None), "dmrBlockTuningFile")
 
 
 def probe_read_bandwidth(buf_ptr: int, nbytes: int, reps: int = 10) -> float:

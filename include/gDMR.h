@@ -237,6 +237,13 @@ int dmrBlockSetOptions(dmrBlockHandle h, int graphs, int autotune);
 /* out[7]: gemm_tn rows per workgroup and variant for projections, the same for Gram matrices,
  * gemm_nn load-ahead depth, SpMM columns per thread, number of captured cycle graphs. */
 int dmrBlockGetTuning(dmrBlockHandle h, int *out);
+/* File of persisted launch shapes (no counterpart in the reference): one text line per tuned
+ * problem shape, keyed by the device architecture and (n, nnz, block width, deflatDim, block
+ * steps, basis storage).  Read before the process's first tuning, appended to after each new
+ * one; NULL or "" turns persistence off (the default unless DMR_TUNE_FILE names a file).
+ * Processes that share the file launch the same kernels for a shape, so their block solves give
+ * the same bits. */
+int dmrBlockTuningFile(const char *path);
 int dmrBlockImportZ(dmrBlockHandle h, const double *hostZ);
 int dmrBlockExportZ(dmrBlockHandle h, double *hostZ);
 /* Phase profile as dmrSetProfiling / dmrGetProfile: SPMV = SpMM and FP64 residuals, GEMVT =

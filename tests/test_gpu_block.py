@@ -531,3 +531,40 @@ def test_bf16_block_fast_paths_on_and_off(torch_gpu, monkeypatch, s):
         err = np.max(np.linalg.norm(X - base, axis=1) / np.linalg.norm(base, axis=1))
         print(f"  {key} vs plain: {err:.2e}", flush=True)
         assert err <= XTOL, key
+
+
+def test_tuning_file_persists_launch_shapes(torch_gpu, tmp_path):
+    """Launch shapes from a tuning file are used as they stand (no tuning run), and a shape the
+    file lacks is tuned and appended to it; lines of another device architecture are skipped."""
+    torch = torch_gpu
+    arch = torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
+    A = gdmr.conv_diff_3d(13, jitter=0.01, seed=31, stream=0)  # a shape no other test tunes
+    n, k, mb = A.n, 6, 5
+    path = tmp_path / "tune.txt"
+    # FP32 basis, s = 4: projection rows 256 / variant 2, Gram rows 512 / variant 0, 4 KiB of
+    # update loads in flight, SpMM 8 columns per thread (valid but not what tuning would pick
+    # on every run); the same shape under another architecture with different numbers.
+    path.write_text(f"# test shapes\nnot-{arch} {n} {A.nnz} 4 {k} {mb} 0 1024 1 1024 1 16 16 16\n"
+                    f"{arch} {n} {A.nnz} 4 {k} {mb} 0 256 2 512 0 4 4 8\n")
+    gdmr.set_tuning_file(str(path))
+    try:
+        B = np.stack([gdmr.rand_normal(61, j, n) for j in range(4)])
+        solver = gdmr.BlockSolver(n, mb, k, 4)
+        rp, ci, va = _dev(torch, A.rowptr), _dev(torch, A.col), _dev(torch, A.val)
+        solver.set_csr(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), A.nnz, keep=(rp, ci, va))
+        rc, rel, st, X = _block(torch, A, B, mb, k, solver=solver)
+        t = solver.tuning()
+        print(f"tuning from file: {t}", flush=True)
+        assert rc == 0 and max(_true_res(A, B[j], X[j]) for j in range(4)) <= 1e-10 * 1.001
+        assert (t["tn_rows_proj"], t["tn_lds_proj"], t["tn_rows_gram"], t["tn_lds_gram"],
+                t["nn_unroll"], t["spmm_cols"]) == (256, 2, 512, 0, 4, 8)
+        # s = 3 is not in the file: tuned, then appended.
+        rc, rel, st, X = _block(torch, A, B[:3], mb, k, solver=solver)
+        assert rc == 0
+        solver.close()
+        lines = [ln.split() for ln in path.read_text().splitlines() if not ln.startswith("#")]
+        new = [ln for ln in lines if ln[0] == arch and ln[3] == "3"]
+        assert len(lines) == 3 and len(new) == 1
+        assert new[0][1:7] == [str(n), str(A.nnz), "3", str(k), str(mb), "0"]
+    finally:
+        gdmr.set_tuning_file(None)

@@ -45,7 +45,7 @@ def parse(argv=None):
                     help="inner block-solve tolerance (default 1e-5 for fp32, 3e-2 for bf16)")
     ap.add_argument("--prec", default="fp32", choices=("fp32", "bf16"),
                     help="block basis storage")
-    ap.add_argument("--fp64-sample", type=int, default=4)
+    ap.add_argument("--fp64-sample", type=int, default=12)
     ap.add_argument("--seed", type=int, default=96)
     ap.add_argument("--no-profile", action="store_true")
     return ap.parse_args(argv)

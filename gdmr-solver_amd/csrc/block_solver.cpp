@@ -469,7 +469,9 @@ void autotune_t(dmrBlockContext *c, TP *P, int s, hipStream_t st) {
     return best;
   };
   const int pFull = c->k + (c->mb + 1) * s;
-  const int rowsCand[] = {256, 512, 1024, 2048};
+  // Rows per workgroup; a larger range means fewer partials for the fixed-order reduce timed
+  // with each candidate (at n = 2^21 the targets up to 2048 all hit the 1024-workgroup cap).
+  const int rowsCand[] = {256, 512, 1024, 2048, 4096, 8192};
   float bp = 1e30f, bg = 1e30f;
   for (int rows : rowsCand)
     for (int var = 0; var < 4; ++var) {

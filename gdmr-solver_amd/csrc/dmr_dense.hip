@@ -335,7 +335,7 @@ void image_check_final(const double *col, int k, const double *chk, double relTo
 void image_qr(const double *H, int ldH, int m, const double *C, int k, double *Q, double *R,
               int *ok, hipStream_t s) {
   const size_t lds = image_qr_lds(m, k);
-  if (lds > 64 * 1024) allow_max_lds(reinterpret_cast<const void *>(k_image_qr));
+  if (lds > 64 * 1024) prepare_dense();
   hipLaunchKernelGGL(k_image_qr, dim3(1), dim3(256), lds, s, H, ldH, m, C, k, Q, R, ok);
 }
 
@@ -345,9 +345,7 @@ void image_fill(double *H, int m, int k, const double *R, const double *scalZ, h
 
 void lsq(const double *H, int m, const double *t, double *y, double *z, hipStream_t s) {
   const size_t lds = sizeof(double) * ((size_t)(m + 1) * m + (m + 1) + m);
-  if (lds > 64 * 1024)
-    hipFuncSetAttribute(reinterpret_cast<const void *>(k_lsq),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (lds > 64 * 1024) prepare_dense();
   hipLaunchKernelGGL(k_lsq, dim3(1), dim3(kLsqThreads), lds, s, H, m, t, y, z);
 }
 
@@ -367,10 +365,12 @@ void allow_max_lds(const void *fn) {
 
 void prepare_dense() {
   // Once per process: the largest dynamic LDS (160 KiB less the static LDS) of the
-  // one-workgroup kernels.
+  // one-workgroup kernels.  Solver contexts call it at creation, so no launch needs it while
+  // its stream is being captured.
   static const bool done = [] {
     allow_max_lds(reinterpret_cast<const void *>(k_svd));
     allow_max_lds(reinterpret_cast<const void *>(k_lsq));
+    allow_max_lds(reinterpret_cast<const void *>(k_image_qr));
     return true;
   }();
   (void)done;

@@ -135,11 +135,14 @@ void multidot(const TS *B, long long ld, int p, const TS *x, long long n, double
 //   mode SUB: out = in - sum_l c1[l] B1_l - sum_l c2[l] B2_l   (sequential, one column at a time)
 //   mode ACC: out = in + beta * (sum_l c1[l] B1_l + sum_l c2[l] B2_l)   (in may equal out)
 // red: bit 0 accumulates ss = out.out, bit 1 sg = out.g; fin says what the last block does.
+// betaDev (optional): beta read from device memory at run time instead of the argument (a
+// value that changes between replays of a captured launch).
 enum CombMode : int { COMB_SUB = 0, COMB_ACC = 1 };
 template <typename TS>
 void combo(int mode, const TS *in, TS *out, const TS *B1, int p1, const double *c1, const TS *B2,
            int p2, const double *c2, long long ld, long long n, double beta, int red,
-           const TS *g, const Fin &fin, RedWork w, const int *flag, hipStream_t s);
+           const TS *g, const Fin &fin, RedWork w, const int *flag, hipStream_t s,
+           const double *betaDev = nullptr);
 
 // out_j = sum_l C1[l + ldc*j] B1_l + sum_l C2[l + ldc*j] B2_l, j < J, with ss_j; scal[j] =
 // 1/sqrt(ss_j).  Columns of out are at stride ld.

@@ -393,7 +393,8 @@ __global__ __launch_bounds__(kBlock) void k_combo(const TS *in, TS *out, const T
                                                   long long nv, double beta,
                                                   const TS *__restrict__ g, int G,
                                                   typename Acc<TS>::type *partials,
-                                                  unsigned *counter, Fin fin, const int *flag) {
+                                                  unsigned *counter, Fin fin, const int *flag,
+                                                  const double *betaDev) {
   using TA = typename Acc<TS>::type;
   constexpr int N = Unit<TS>::N;
   if (flag && *flag) return;
@@ -403,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void k_combo(const TS *in, TS *out, const T
     cs[l] = (TA)(MODE == COMB_SUB ? -c : c);
   }
   __syncthreads();
-  const TA tb = (TA)beta;
+  const TA tb = (TA)(betaDev ? *betaDev : beta);
   TA ss = 0, sg = 0;
   const long long stride = (long long)G * kBlock;
   for (long long u = (long long)blockIdx.x * kBlock + threadIdx.x; u < nv; u += stride) {
@@ -1199,7 +1200,7 @@ void multidot(const TS *B, long long ld, int p, const TS *x, long long n, double
 template <typename TS>
 void combo(int mode, const TS *in, TS *out, const TS *B1, int p1, const double *c1, const TS *B2,
            int p2, const double *c2, long long ld, long long n, double beta, int red, const TS *g,
-           const Fin &fin, RedWork w, const int *flag, hipStream_t s) {
+           const Fin &fin, RedWork w, const int *flag, hipStream_t s, const double *betaDev) {
   using TA = typename Acc<TS>::type;
   const int G = red_blocks<TS>(n);
   const long long nv = units<TS>(n);
@@ -1207,7 +1208,7 @@ void combo(int mode, const TS *in, TS *out, const TS *B1, int p1, const double *
   unsigned *cnt = w.counters;
 #define DMR_COMBO(M, R)                                                                         \
   hipLaunchKernelGGL((k_combo<TS, M, R>), dim3(G), dim3(kBlock), 0, s, in, out, B1, p1, c1, B2, \
-                     p2, c2, ld, nv, beta, g, G, part, cnt, fin, flag)
+                     p2, c2, ld, nv, beta, g, G, part, cnt, fin, flag, betaDev)
   if (mode == COMB_SUB) {
     switch (red) {
       case 0: DMR_COMBO(COMB_SUB, 0); break;
@@ -1322,7 +1323,7 @@ void breakdown_fixup(TS *P, long long ld, long long n, int o, int d, int m, doub
                              RedWork, const int *, hipStream_t);                                  \
   template void combo<TS>(int, const TS *, TS *, const TS *, int, const double *, const TS *, int, \
                           const double *, long long, long long, double, int, const TS *,          \
-                          const Fin &, RedWork, const int *, hipStream_t);                        \
+                          const Fin &, RedWork, const int *, hipStream_t, const double *);        \
   template void combo_multi<TS>(TS *, int, const TS *, int, const double *, const TS *, int,      \
                                 const double *, int, long long, long long, double *, RedWork,     \
                                 hipStream_t);                                                     \

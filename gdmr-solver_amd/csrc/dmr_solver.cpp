@@ -62,6 +62,7 @@ constexpr int kSvdMaxSweeps = 128;
 // The FP32 and bf16 inner solves of the mixed path use the reference's own FP32 tolerance
 // (svTolDMR = 2e-7, gDMR.cu:88): their basis carries FP32 rounding, so sweeps past it add nothing.
 constexpr double kSvdTolLow = 2e-7;
+constexpr int kMaxCycleGraphs = 16;
 // Largest relative error of an implicit deflation image (checked on its last column) that the
 // next cycle accepts instead of recomputing the image (DMR_IMAGE_TOL overrides).  The check's
 // own FP32 product of a smallest-singular direction cancels to about 1e-7 times the operator's
@@ -143,6 +144,23 @@ struct dmrContext {
   // stream does the least squares and the residual / solution updates.
   hipStream_t side = nullptr;
   hipEvent_t evH = nullptr, evSvd = nullptr;
+  // Captured restart cycles (CSR operators, profiling off; DMR_GRAPHS=1): each variant of a
+  // cycle's launch sequence is captured once on the context's own stream cs and replayed there,
+  // after an event on the caller's stream; the caller's stream then waits on an event after the
+  // replay.  At most kMaxCycleGraphs are kept (oldest dropped); all are dropped when the CSR
+  // pattern changes.
+  struct CycleGraph {
+    int d, fromResidual, implicitImg, zParity;
+    double tol;
+    const void *val, *val32;  // the operator values the capture read
+    hipGraphExec_t exec;
+    long long matvecs;
+  };
+  bool graphs = false;
+  hipStream_t cs = nullptr;
+  hipEvent_t evIn = nullptr, evOut = nullptr;
+  std::vector<CycleGraph> cycleGraphs;
+  double *rhoDev = nullptr;  // rho of the current cycle (read by the solution update)
   Work<double> w64;
   Work<float> w32;
   Work<dmr::bf16> w16;  // bf16 basis storage (DMR_PREC_BF16)
@@ -205,7 +223,7 @@ size_t footprint(int n, int m, int k, int prec) {
     big += sizeof(double) * (size_t)ld * 3;
   }
   const size_t small = sizeof(double) * ((size_t)(m + 1) * m * 2 + 8 * (m + 2) + 64 + 2 * k +
-                                         (size_t)(m + 1 + k) * k + 2 * k + 8) +
+                                         (size_t)(m + 1 + k) * k + 2 * k + 16) +
                        sizeof(double) * dmr::kMaxPartials + sizeof(unsigned) * dmr::kCounters +
                        4096;
   return big + small;
@@ -278,8 +296,9 @@ int ctx_alloc(dmrContext *c) {
                         + (size_t)m * (k + 1) // C
                         + (m + 2)             // sig
                         + 16 + 16 + 64 + 8    // scal, scal2, scalZ, one
-                        + (size_t)(m + 1 + k) * k + 2 * k + 8;  // imgQ, imgR, imgScal,
-                                                                // imgCol, imgChk
+                        + (size_t)(m + 1 + k) * k + 2 * k + 8  // imgQ, imgR, imgScal,
+                                                               // imgCol, imgChk
+                        + 8;                                   // rhoDev
   const size_t smallBytes = sizeof(double) * (nsmall + dmr::kMaxPartials) +
                             sizeof(unsigned) * dmr::kCounters + sizeof(int) * 64;
   HIPCHK(hipMalloc(&c->smallPool, smallBytes));
@@ -317,6 +336,8 @@ int ctx_alloc(dmrContext *c) {
   s += k;
   c->imgChk = s;
   s += 8;
+  c->rhoDev = s;
+  s += 8;
   c->red.partials = s;
   s += dmr::kMaxPartials;
   c->red.counters = (unsigned *)s;
@@ -342,6 +363,12 @@ int ctx_alloc(dmrContext *c) {
   HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&c->evH, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&c->evSvd, hipEventDisableTiming));
+  HIPCHK(hipStreamCreateWithFlags(&c->cs, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&c->evIn, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&c->evOut, hipEventDisableTiming));
+  c->graphs = dmr::env_flag("DMR_GRAPHS", false);
+  // LDS limits of the one-workgroup kernels, set before any cycle can be captured.
+  dmr::prepare_dense();
   return DMR_OK;
 }
 
@@ -359,7 +386,17 @@ void ctx_free(dmrContext *c) {
   if (c->evH) hipEventDestroy(c->evH);
   if (c->evSvd) hipEventDestroy(c->evSvd);
   if (c->side) hipStreamDestroy(c->side);
+  for (auto &g : c->cycleGraphs) hipGraphExecDestroy(g.exec);
+  if (c->evIn) hipEventDestroy(c->evIn);
+  if (c->evOut) hipEventDestroy(c->evOut);
+  if (c->cs) hipStreamDestroy(c->cs);
   delete c;
+}
+
+// Captured cycles hold the operator's device pointers: dropped when they change.
+void drop_cycle_graphs(dmrContext *c) {
+  for (auto &g : c->cycleGraphs) hipGraphExecDestroy(g.exec);
+  c->cycleGraphs.clear();
 }
 
 int validate_shape(int n, int m, int k) {
@@ -477,9 +514,15 @@ void prof_collect(dmrContext *c) {
 // current approximation in w.xa, tracked residual norm rho and ||b|| = beta.  Returns DMR_OK on
 // convergence, DMR_ERR_NOCONV / DMR_ERR_RUNAWAY on the reference's failure exits, or an error.
 template <typename TS, typename Op>
-int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double tol, int matRank,
-               double beta, double &rho, int &grdItr, double histScale) {
+int run_cycles(dmrContext *c, const Op &op0, hipStream_t s0, int startDef, double tol,
+               int matRank, double beta, double &rho, int &grdItr, double histScale) {
   Work<TS> &w = work_of<TS>(c);
+  // A cycle's launches go to s: the caller's stream s0, or, when cycles are captured, the
+  // context's own stream cs (the operator launches there too).
+  const bool useGraph = c->graphs && !c->prof.on && !c->fused && c->opKind == OP_CSR;
+  hipStream_t s = useGraph ? c->cs : s0;
+  Op op = op0;
+  op.s = s;
   const long long n = c->n, ld = c->ld;
   const int m = c->m, k = c->k;
   const dmr::RedWork red = c->red;
@@ -498,6 +541,25 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
     const bool fromResidual =
         c->kryLoopFlag == 0 || period == 0 || (c->kryFlagCount % period) == 0;
     TS *Po = w.P + (long long)o * ld;
+    const bool implicitImg = d > 0 && c->implicitImage && c->imgValid;
+    // ---- genDeflatDMR, first half (gDMR.cu:1583-1704): the SVD of the Hessenberg matrix runs on
+    // the side stream, overlapped with the work that does not need it.  It takes the leading
+    // (cols + 1) x cols block, cols = d + lead: the whole matrix (lead = m - d) in the FP64 path;
+    // in the FP32 and bf16 inner solves the block that is final refreshTail Krylov steps before
+    // the end of the cycle, so that the SVD runs beside those steps (the refreshed subspace then
+    // combines Z and the first lead Krylov vectors).
+    const bool refresh = k > 0;
+    const int lead = (m - d - c->refreshTail >= 1 && d + m - d - c->refreshTail >= k)
+                         ? m - d - c->refreshTail
+                         : m - d;
+    const int cols = d + lead;
+    // rho of this cycle for the solution update, read on the device (the host's value).
+    c->hostStatus[20] = rho;
+    if (hipMemcpyAsync(c->rhoDev, c->hostStatus + 20, sizeof(double), hipMemcpyHostToDevice,
+                       s0) != hipSuccess)
+      return set_err(DMR_ERR_HIP, "rho upload failed");
+    // The cycle's launches, from the start vector to the refreshed subspace.
+    auto enqueue = [&]() -> int {
     if (fromResidual) {
       if (hipMemcpyAsync(Po, w.r, sizeof(TS) * ld, hipMemcpyDeviceToDevice, s) != hipSuccess)
         return set_err(DMR_ERR_HIP, "setup copy failed");
@@ -560,7 +622,6 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
     // Deflation image: A Z = W R by CGS (gDMR.cu:1823-2031), or, in an FP32 inner solve whose Z
     // was refreshed by the previous cycle under the same operator, from that cycle's Arnoldi
     // relation: W = P Q (in Wn) and R diag(1 / ||z~||) (image_qr at the previous cycle's end).
-    const bool implicitImg = d > 0 && c->implicitImage && c->imgValid;
     if (implicitImg) {
       const int rcImg = timed(c, s, DMR_PH_COMBO, vb * 2.0 * k, [&] {
         if (hipMemcpyAsync(w.P, w.Wn, sizeof(TS) * (size_t)ld * k, hipMemcpyDeviceToDevice, s) !=
@@ -585,17 +646,6 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
       dmr::Fin fin{dmr::FIN_NORM, c->scal, nullptr, 0.0, 0, nullptr};
       orthogonalize(Po, Po, nullptr, 0, nullptr, w.P, k, c->cv, 1, fin, nullptr);
     }
-    // ---- genDeflatDMR, first half (gDMR.cu:1583-1704): the SVD of the Hessenberg matrix runs on
-    // the side stream, overlapped with the work that does not need it.  It takes the leading
-    // (cols + 1) x cols block, cols = d + lead: the whole matrix (lead = m - d) in the FP64 path;
-    // in the FP32 and bf16 inner solves.the block final refreshTail Krylov steps before the end
-    // of the cycle, so that the SVD runs beside those steps (the refreshed subspace then combines
-    // Z and the first lead Krylov vectors).
-    const bool refresh = k > 0;
-    const int lead = (m - d - c->refreshTail >= 1 && d + m - d - c->refreshTail >= k)
-                         ? m - d - c->refreshTail
-                         : m - d;
-    const int cols = d + lead;
     bool forked = false;
     auto fork_refresh = [&]() -> int {
       forked = true;
@@ -663,8 +713,8 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
       });
       // x_a <- x_a + rho * ([V] y[d..m-1] + [Z] y[0..d-1]).
       timed(c, s, DMR_PH_COMBO, vb * (m + 2), [&] {
-        dmr::combo<TS>(dmr::COMB_ACC, w.xa, w.xa, Po, m - d, c->y + d, w.Z, d, c->y, ld, n, rho,
-                       0, nullptr, fin_none(), red, nullptr, s);
+        dmr::combo<TS>(dmr::COMB_ACC, w.xa, w.xa, Po, m - d, c->y + d, w.Z, d, c->y, ld, n, 0.0,
+                       0, nullptr, fin_none(), red, nullptr, s, c->rhoDev);
         return 0;
       });
     }
@@ -706,6 +756,53 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
         dmr::image_check_final(c->imgCol, k, c->imgChk, image_tol(), c->imgOk, s);
       }
     }
+    return 0;
+    };
+    if (useGraph) {
+      // Replay of this variant's captured cycle (captured now if it is new).  What the launch
+      // sequence depends on: d, the start vector's source, the implicit image, which buffer holds
+      // Z, and the breakdown tolerance; the operator's pointers are fixed while graphs exist.
+      const int zParity = w.Z < w.Zn ? 0 : 1;
+      const void *val = c->csr.val, *val32 = c->csr.val32;
+      const dmrContext::CycleGraph *g = nullptr;
+      for (const auto &e : c->cycleGraphs)
+        if (e.d == d && e.fromResidual == (int)fromResidual && e.implicitImg == (int)implicitImg &&
+            e.zParity == zParity && e.tol == tol && e.val == val && e.val32 == val32)
+          g = &e;
+      if (!g) {
+        const long long mv0 = c->st.matvecs;
+        if (hipStreamBeginCapture(c->cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+          return set_err(DMR_ERR_HIP, "cycle capture failed");
+        const int rcE = enqueue();
+        hipGraph_t graph = nullptr;
+        const hipError_t ee = hipStreamEndCapture(c->cs, &graph);
+        if (rcE) {
+          if (graph) hipGraphDestroy(graph);
+          return rcE;
+        }
+        if (ee != hipSuccess) return set_err(DMR_ERR_HIP, "cycle capture failed");
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ei != hipSuccess) return set_err(DMR_ERR_HIP, "cycle graph instantiation failed");
+        if ((int)c->cycleGraphs.size() >= kMaxCycleGraphs) {
+          hipGraphExecDestroy(c->cycleGraphs.front().exec);
+          c->cycleGraphs.erase(c->cycleGraphs.begin());
+        }
+        c->cycleGraphs.push_back({d, (int)fromResidual, (int)implicitImg, zParity, tol, val,
+                                  val32, ex, c->st.matvecs - mv0});
+        c->st.matvecs = mv0;
+        g = &c->cycleGraphs.back();
+      }
+      if (hipEventRecord(c->evIn, s0) != hipSuccess ||
+          hipStreamWaitEvent(c->cs, c->evIn, 0) != hipSuccess ||
+          hipGraphLaunch(g->exec, c->cs) != hipSuccess)
+        return set_err(DMR_ERR_HIP, "cycle graph launch failed");
+      c->st.matvecs += g->matvecs;
+    } else {
+      const int rcE = enqueue();
+      if (rcE) return rcE;
+    }
     if (hipMemcpyAsync(c->hostStatus, c->scal2, 4 * sizeof(double), hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
         hipMemcpyAsync(c->hostStatus + 8, c->flag, sizeof(int), hipMemcpyDeviceToHost, s) !=
@@ -718,6 +815,9 @@ int run_cycles(dmrContext *c, const Op &op, hipStream_t s, int startDef, double 
          hipMemcpyAsync(c->hostStatus + 11, c->imgOk, sizeof(int), hipMemcpyDeviceToHost, s) !=
              hipSuccess))
       return set_err(DMR_ERR_HIP, "status readback failed");
+    if (useGraph && (hipEventRecord(c->evOut, c->cs) != hipSuccess ||
+                     hipStreamWaitEvent(s0, c->evOut, 0) != hipSuccess))
+      return set_err(DMR_ERR_HIP, "stream hand-off failed");
     if (hipStreamSynchronize(s) != hipSuccess) return set_err(DMR_ERR_HIP, "stream sync failed");
     if (c->prof.on) prof_collect(c);
     const double q = c->hostStatus[2];
@@ -1000,6 +1100,7 @@ int dmrSetOperatorCsr(dmrHandle h, int nnz, const int *rowPtr, const int *colInd
   if (!h || !rowPtr || !colInd || !val || nnz < 0)
     return set_err(DMR_ERR_ARG, "invalid CSR arguments");
   HIPCHK(hipSetDevice(h->device));
+  drop_cycle_graphs(h);
   const int n = h->n;
   std::vector<int> rp(n + 1);
   HIPCHK(hipMemcpy(rp.data(), rowPtr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));

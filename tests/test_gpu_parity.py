@@ -353,3 +353,36 @@ def test_stagnation_restart_matches_oracle(torch_gpu, n, alpha, m, k, mat_rank):
         assert gdmr.lib.dmrGetStagnationCount(s.h) == ora.stagnation_count()
         assert np.max(np.abs(s.export_z() - ora.get_z())) <= 1e-9
     s.close()
+
+
+@pytest.mark.parametrize("prec", [gdmr.PREC_FP64, gdmr.PREC_FP32], ids=["fp64", "fp32"])
+def test_captured_cycles_match_eager(torch_gpu, monkeypatch, prec):
+    """Restart cycles captured as HIP graphs and replayed (DMR_GRAPHS=1) give the eager launches'
+    bits: a recycled sequence of three systems with new operator values each (so graphs are
+    captured for several variants and replayed over many cycles, with the host's reads and
+    synchronisations between replays); histories, cycle counts and solutions are identical."""
+    torch = torch_gpu
+    mats = [gdmr.conv_diff_3d(16, jitter=0.01, seed=21, stream=t) for t in range(3)]
+    n, m, k = mats[0].n, 24, 8
+    rp, ci = _dev(torch, mats[0].rowptr), _dev(torch, mats[0].col)
+    vals = [_dev(torch, A.val) for A in mats]
+    out = {}
+    for graphs in ("0", "1"):
+        monkeypatch.setenv("DMR_GRAPHS", graphs)
+        s = gdmr.Solver(n, m, k, precision=prec)
+        s.set_csr(rp.data_ptr(), ci.data_ptr(), vals[0].data_ptr(), mats[0].nnz,
+                  keep=(rp, ci, vals[0]))
+        runs = []
+        for t, A in enumerate(mats):
+            s.set_values(vals[t].data_ptr())
+            b = gdmr.rand_normal(21, 50 + t, n)
+            xd = torch.zeros(n, dtype=torch.float64, device="cuda")
+            res, its = s.solve(xd.data_ptr(), _dev(torch, b).data_ptr(), 1e-10, 10000,
+                               mode=1 if t else 0)
+            st, hist = s.stats()
+            assert _true_res(A, b, xd.cpu().numpy()) <= 1e-10
+            runs.append((st.cycles, np.array(hist), xd.cpu().numpy()))
+        s.close()
+        out[graphs] = runs
+    for (c0, h0, x0), (c1, h1, x1) in zip(out["0"], out["1"]):
+        assert c0 == c1 and np.array_equal(h0, h1) and np.array_equal(x0, x1)

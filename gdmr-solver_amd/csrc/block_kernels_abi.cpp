@@ -191,6 +191,20 @@ int dmrKernelCholQr(const double *G, int s, double relTol, float *Sinv32, double
   return launched("dmrKernelCholQr");
 }
 
+int dmrKernelGramChol(const float *part, int G, int s, double relTol, float *Sinv32, double *Sacc,
+                      double *Hdst, int ldh, int *flags, void *work, hipStream_t stream) {
+  dmr::clear_error();
+  static_assert(DMR_KGRAMCHOL_WORK_BYTES == 256 + blk::kGramCholSegs * 256 * sizeof(double),
+                "work layout");
+  if (!part || !Sinv32 || !Sacc || !flags || !work || G < 1 || s < 1 || s > 16 ||
+      !(relTol >= 0.0) || (Hdst && ldh < s))
+    return bad("dmrKernelGramChol: invalid shape");
+  char *w = static_cast<char *>(work);
+  blk::gram_chol(part, G, s, relTol, Sinv32, Sacc, 1, Hdst, ldh, flags,
+                 reinterpret_cast<double *>(w + 256), reinterpret_cast<unsigned *>(w), stream);
+  return launched("dmrKernelGramChol");
+}
+
 int dmrKernelScaleBlock(const float *U, long long ldu, const float *C, void *Out, long long ldo,
                         int outBasis, int s, long long n, hipStream_t stream) {
   dmr::clear_error();

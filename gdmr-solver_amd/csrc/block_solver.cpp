@@ -81,12 +81,15 @@ struct dmrBlockContext {
   double *rho = nullptr, *q = nullptr, *scal = nullptr, *bn = nullptr, *rn = nullptr,
          *f = nullptr, *zq = nullptr, *cpart = nullptr;
   int *flags = nullptr;  // [0] dependent column seen this cycle, [1] Jacobi sweeps
+  double *gcSeg = nullptr;   // gram_chol's segment sums
+  unsigned *gcCtr = nullptr;  // gram_chol's arrival counter (zero between launches)
   double *host = nullptr;  // pinned staging
   bool hasZ = false;
   int krMax = 64;  // cap on the leading block steps of the subspace refresh
   double svdTol = kSvdTol;
   int cholPasses = 0;  // Cholesky QR passes per block normalisation; 0: by storage type
   bool fusedGram = true;  // first Gram matrix from the update's epilogue where possible
+  bool gramChol = true;   // its reduce and the Cholesky step in one launch (gram_chol)
   int spmmForce = 0;      // SpMM variant pinned by DMR_BLOCK_SPMM (0: the tuned one)
   bool foldT = true;      // R's projection in the last block step's pass where possible
   bool svdSide = true;    // the refresh's SVD on the side stream (DMR_BLOCK_SVD_SIDE=0: inline)
@@ -211,7 +214,8 @@ Sizes sizes(long long n, int mb, int k, int sMax, int G, long long ld) {
                               (size_t)dmr::blk::colnorm_blocks(n) * 32) +
             sizeof(float) * ((size_t)rowsMax * 32 * 3 + (size_t)colsMax * 32 + 32 * 32 +
                              (size_t)cRed * 32 + (size_t)G * pPadMax * 32) +
-            sizeof(int) * 64 + 64 * 32;
+            sizeof(int) * 64 + 64 * 32 +
+            sizeof(double) * 256 * dmr::blk::kGramCholSegs + sizeof(unsigned) * 64;
   return z;
 }
 
@@ -298,6 +302,8 @@ int ctx_alloc(dmrBlockContext *c) {
   c->partFloats = (size_t)c->G * pPadMax * 32;
   c->part = m.take<float>(c->partFloats);
   c->flags = m.take<int>(64);
+  c->gcSeg = m.take<double>((size_t)256 * dmr::blk::kGramCholSegs);
+  c->gcCtr = m.take<unsigned>(64);
   return DMR_OK;
 }
 
@@ -560,17 +566,25 @@ void cholqr2(dmrBlockContext *c, float *U, int sc, double *Hdst, int ldh, TO *ds
   // pass, orthogonal to about kappa^2 * 6e-8, is enough there; FP32 storage keeps two.
   const int passes = c->cholPasses ? c->cholPasses : (c->bfBasis ? 1 : 2);
   for (int pass = 0; pass < passes; ++pass) {
-    if (pass == 0 && gramParts > 0) {
-      timed(c, st, DMR_PH_REDUCE, reduce_bytes(gramParts, sc, sc),
-            [&] { dmr::blk::gram_reduce(c->part, gramParts, sc, c->G64, st); });
+    double *hd = pass == passes - 1 ? Hdst : nullptr;
+    if (pass == 0 && gramParts > 0 && c->gramChol) {
+      timed(c, st, DMR_PH_REDUCE, reduce_bytes(gramParts, sc, sc), [&] {
+        dmr::blk::gram_chol(c->part, gramParts, sc, kDepTol, c->Sinv32, c->Sacc, 1, hd, ldh,
+                            c->flags, c->gcSeg, c->gcCtr, st);
+      });
     } else {
-      tn(c, DMR_PH_ORTH, 4.0 * n * sc, 2.0 * n * sc * sc, U, sc, U, sc, c->G64, sc, nullptr, true,
-         st);
+      if (pass == 0 && gramParts > 0) {
+        timed(c, st, DMR_PH_REDUCE, reduce_bytes(gramParts, sc, sc),
+              [&] { dmr::blk::gram_reduce(c->part, gramParts, sc, c->G64, st); });
+      } else {
+        tn(c, DMR_PH_ORTH, 4.0 * n * sc, 2.0 * n * sc * sc, U, sc, U, sc, c->G64, sc, nullptr,
+           true, st);
+      }
+      timed(c, st, DMR_PH_ORTH, 0.0, [&] {
+        dmr::blk::chol_qr(c->G64, sc, kDepTol, c->Sinv32, c->Sacc, pass == 0, hd, ldh, c->flags,
+                          st);
+      });
     }
-    timed(c, st, DMR_PH_ORTH, 0.0, [&] {
-      dmr::blk::chol_qr(c->G64, sc, kDepTol, c->Sinv32, c->Sacc, pass == 0,
-                        pass == passes - 1 ? Hdst : nullptr, ldh, c->flags, st);
-    });
     timedf(c, st, DMR_PH_ORTH, 4.0 * n * sc + (double)sizeof(TO) * n * sc,
            2.0 * n * sc * sc, [&] {
       if (pass < passes - 1) {
@@ -961,6 +975,7 @@ int dmrBlockCreate(dmrBlockHandle *h, int device, int elements, int blockSteps, 
     if (v == 1 || v == 2) c->cholPasses = v;
   }
   if (const char *e = std::getenv("DMR_BLOCK_FUSED_GRAM")) c->fusedGram = e[0] != '0';
+  if (const char *e = std::getenv("DMR_BLOCK_GRAM_CHOL")) c->gramChol = e[0] != '0';
   if (const char *e = std::getenv("DMR_BLOCK_FOLD_T")) c->foldT = e[0] != '0';
   if (const char *e = std::getenv("DMR_BLOCK_SVD_SIDE")) c->svdSide = e[0] != '0';
   if (const char *e = std::getenv("DMR_BLOCK_NN_BASIS")) {

@@ -100,6 +100,14 @@ bool scale_block(const float *U, long long ldu, const float *C, float *Out, long
                  long long n, hipStream_t st);
 bool scale_block(const float *U, long long ldu, const float *C, bf16 *Out, long long ldo, int s,
                  long long n, hipStream_t st);
+// The Gram reduce of G partials (s <= 16, the layout gram_reduce reads) and chol_qr's step on
+// the sum, in one launch of gram_chol_segments(G) workgroups.  seg: 256 doubles per segment;
+// ctr: one counter, zero on entry, left at zero.
+int gram_chol_segments(int G);
+constexpr int kGramCholSegs = 64;  // most segments
+void gram_chol(const float *part, int G, int s, double relTol, float *Sinv32, double *Sacc,
+               int first, double *Hdst, int ldh, int *brk, double *seg, unsigned *ctr,
+               hipStream_t st);
 // Cholesky QR step on the Gram matrix G (s x s, ld s): see k_chol_qr.
 void chol_qr(const double *G, int s, double relTol, float *Sinv32, double *Sacc, int first,
              double *Hdst, int ldh, int *brk, hipStream_t st);

@@ -1368,12 +1368,12 @@ __global__ __launch_bounds__(64) void k_chol_qr(const double *__restrict__ Gin, 
 // The same factorisation with one thread per matrix element (D x D threads, D = 16 or 32 >= s):
 // the operands of every step come from LDS, and the operations and their order per element are
 // those of k_chol_qr, so both give the same bits.
+// Thread t = D i + j of a D x D workgroup brings element (i, j) of the Gram matrix in g (zero
+// outside s x s).
 template <int D>
-__global__ __launch_bounds__(D * D) void k_chol_qr_el(const double *__restrict__ Gin, int s,
-                                                      double relTol, float *__restrict__ Sinv32,
-                                                      double *__restrict__ Sacc, int first,
-                                                      double *__restrict__ Hdst, int ldh,
-                                                      int *__restrict__ brk) {
+__device__ inline void chol_el(double g, int s, double relTol, float *__restrict__ Sinv32,
+                               double *__restrict__ Sacc, int first, double *__restrict__ Hdst,
+                               int ldh, int *__restrict__ brk) {
   constexpr int LD = D + 1;
   __shared__ double A[D * LD];  // G, then S in the upper triangle
   __shared__ double X[D * LD];  // unit columns, then S^-1 (column j of S^-1 in column j)
@@ -1383,7 +1383,7 @@ __global__ __launch_bounds__(D * D) void k_chol_qr_el(const double *__restrict__
   __shared__ int sh_bad;
   const int t = threadIdx.x, i = t / D, j = t % D;
   const bool own = i < s && j < s;
-  A[i * LD + j] = own ? Gin[j * s + i] : 0.0;
+  A[i * LD + j] = own ? g : 0.0;
   X[i * LD + j] = i == j ? 1.0 : 0.0;
   Y[i * LD + j] = (!first && own) ? Sacc[j * s + i] : 0.0;
   if (t == 0) sh_dep = 0u;
@@ -1444,6 +1444,95 @@ __global__ __launch_bounds__(D * D) void k_chol_qr_el(const double *__restrict__
     if (Hdst) Hdst[(long long)j * ldh + i] = a;
   }
   if (t == 0 && dep != 0u) atomicOr(brk, 1);
+}
+
+template <int D>
+__global__ __launch_bounds__(D * D) void k_chol_qr_el(const double *__restrict__ Gin, int s,
+                                                      double relTol, float *__restrict__ Sinv32,
+                                                      double *__restrict__ Sacc, int first,
+                                                      double *__restrict__ Hdst, int ldh,
+                                                      int *__restrict__ brk) {
+  const int i = threadIdx.x / D, j = threadIdx.x % D;
+  chol_el<D>(i < s && j < s ? Gin[j * s + i] : 0.0, s, relTol, Sinv32, Sacc, first, Hdst, ldh,
+             brk);
+}
+
+// The Gram reduce of G partials (rows i < s <= 16 of the layout part[(i G + g) 32 + j]) and the
+// Cholesky factorisation of k_chol_qr_el<16> in one launch.  Workgroup b sums partials
+// g0 .. g0 + L - 1 (L = ceil(G / gridDim.x), g0 = b L) in FP64: thread (h, r, c4) = (t / 64,
+// (t / 4) % 16, t % 4) adds columns 4 c4 .. 4 c4 + 3 of row r over g = g0 + h, g0 + h + 4, ... in
+// order (kGcLoads loads in flight), the four h in order, and writes the 16 x 16 segment sum to
+// seg[b]; the last workgroup to arrive adds the segments in order of b and factorises.  The
+// order is fixed by G and the grid, independent of timing.  ctr is zero on entry and on exit.
+constexpr int kGcSegs = kGramCholSegs;
+constexpr int kGcLoads = 16;
+
+__global__ __launch_bounds__(256) void k_gram_chol(const float *__restrict__ part, int G, int L,
+                                                   int s, double relTol,
+                                                   float *__restrict__ Sinv32,
+                                                   double *__restrict__ Sacc, int first,
+                                                   double *__restrict__ Hdst, int ldh,
+                                                   int *__restrict__ brk,
+                                                   double *__restrict__ seg, unsigned *ctr) {
+  __shared__ double red[4 * 256];
+  __shared__ int s_last;
+  const int t = threadIdx.x, h = t >> 6, r = (t >> 2) & 15, c4 = t & 3;
+  const int g0 = blockIdx.x * L, g1 = g0 + L < G ? g0 + L : G;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (r < s) {
+    const float4 *src = reinterpret_cast<const float4 *>(part + (long long)r * G * 32) + c4;
+    for (int gb = g0 + h; gb < g1; gb += 4 * kGcLoads) {
+      float4 v[kGcLoads];
+#pragma unroll
+      for (int u = 0; u < kGcLoads; ++u) {
+        const int g = gb + 4 * u;
+        v[u] = g < g1 ? src[(long long)g * 8] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < kGcLoads; ++u) {
+        if (gb + 4 * u < g1) {
+          a0 = a0 + (double)v[u].x;
+          a1 = a1 + (double)v[u].y;
+          a2 = a2 + (double)v[u].z;
+          a3 = a3 + (double)v[u].w;
+        }
+      }
+    }
+  }
+  red[h * 256 + r * 16 + 4 * c4 + 0] = a0;
+  red[h * 256 + r * 16 + 4 * c4 + 1] = a1;
+  red[h * 256 + r * 16 + 4 * c4 + 2] = a2;
+  red[h * 256 + r * 16 + 4 * c4 + 3] = a3;
+  __syncthreads();
+  // Thread t = 16 i + j: element (i, j) of this segment.
+  seg[(long long)blockIdx.x * 256 + t] = ((red[t] + red[256 + t]) + red[512 + t]) + red[768 + t];
+  __syncthreads();
+  if (t == 0) {
+    const unsigned total = gridDim.x;
+    const bool last =
+        __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == total - 1u;
+    if (last) __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = last ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  const int i = t >> 4, j = t & 15;
+  double g = 0.0;
+  if (i < s && j < s) {
+    const int nb = gridDim.x;
+    for (int b0 = 0; b0 < nb; b0 += kGcLoads) {
+      double v[kGcLoads];
+#pragma unroll
+      for (int u = 0; u < kGcLoads; ++u)
+        v[u] = b0 + u < nb ? __hip_atomic_load(seg + (long long)(b0 + u) * 256 + t,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           : 0.0;
+#pragma unroll
+      for (int u = 0; u < kGcLoads; ++u)
+        if (b0 + u < nb) g = g + v[u];
+    }
+  }
+  chol_el<16>(g, s, relTol, Sinv32, Sacc, first, Hdst, ldh, brk);
 }
 
 
@@ -2177,6 +2266,20 @@ void gemm_nn(const float *P, long long ldp, int p, const float *C, const float *
              float a, float b, bf16 *Out, long long ldo, int s, long long n, int un,
              hipStream_t st) {
   gemm_nn_t<float, bf16>(P, ldp, p, C, In, ldi, a, b, Out, ldo, s, n, un, st);
+}
+
+int gram_chol_segments(int G) {
+  const int nb = (G + 63) / 64;
+  return nb < kGcSegs ? nb : kGcSegs;
+}
+
+void gram_chol(const float *part, int G, int s, double relTol, float *Sinv32, double *Sacc,
+               int first, double *Hdst, int ldh, int *brk, double *seg, unsigned *ctr,
+               hipStream_t st) {
+  const int nb = gram_chol_segments(G);
+  const int L = (G + nb - 1) / nb;
+  hipLaunchKernelGGL(k_gram_chol, dim3(nb), dim3(256), 0, st, part, G, L, s, relTol, Sinv32,
+                     Sacc, first, Hdst, ldh, brk, seg, ctr);
 }
 
 void chol_qr(const double *G, int s, double relTol, float *Sinv32, double *Sacc, int first,

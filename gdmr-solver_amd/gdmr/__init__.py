@@ -31,6 +31,7 @@ lib = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
 
 # ---- error codes (gDMR.h) ----
 DMR_OK, DMR_ERR_ARG, DMR_ERR_STATE, DMR_ERR_HIP, DMR_ERR_ALLOC = 0, 1, 2, 3, 4
+DMR_KGRAMCHOL_WORK_BYTES = 256 + 64 * 256 * 8  # gDMRKernels.h
 DMR_ERR_OPERATOR, DMR_ERR_NOCONV, DMR_ERR_RUNAWAY, DMR_ERR_NOTBUILT = 5, 6, 7, 8
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 
@@ -143,6 +144,7 @@ _sig("dmrKernelGemmNnPair", C.c_int, [_vp, _ll, _i32, _vp, _vp, _vp, _ll, _i32, 
 _sig("dmrKernelSpmm", C.c_int, [_i32, _i32, _i32, _vp, _vp, _vp, _vp, _ll, _vp, _ll, _i32, _i32,
                                 _vp])
 _sig("dmrKernelCholQr", C.c_int, [_vp, _i32, _f64, _vp, _vp, _i32, _vp, _i32, _vp, _vp])
+_sig("dmrKernelGramChol", C.c_int, [_vp, _i32, _i32, _f64, _vp, _vp, _vp, _i32, _vp, _vp, _vp])
 _sig("dmrKernelScaleBlock", C.c_int, [_vp, _ll, _vp, _vp, _ll, _i32, _i32, _ll, _vp])
 _sig("dmrKernelLsqBlock", C.c_int, [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp,
                                     _vp, _vp])
@@ -187,7 +189,7 @@ HEADER_SYMBOLS = (
     "impCsrLS nnzLS dmrReadMatrixMarket dmrWriteMatrixMarket impMtxLS dmrGenConvDiff2D dmrGenConvDiff3D dmrGenAniso3D dmrRandNormal "
     "dmrRandUniform dmrSplitMix64 "
     "dmrKernelTnWorkspace dmrKernelGemmTn dmrKernelGemmTnPair dmrKernelGemmNn dmrKernelGramBlocks "
-    "dmrKernelGramReduce dmrKernelGemmNnPair dmrKernelSpmm dmrKernelCholQr dmrKernelScaleBlock "
+    "dmrKernelGramReduce dmrKernelGemmNnPair dmrKernelSpmm dmrKernelCholQr dmrKernelGramChol dmrKernelScaleBlock "
     "dmrKernelLsqBlock dmrKernelLsqFits dmrProbeReadBandwidth").split()
 
 

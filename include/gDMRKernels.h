@@ -70,6 +70,15 @@ int dmrKernelSpmm(int xBasis, int n, int nnz, const int *rowPtr, const int *colI
  * when first != 0; upper triangular s x s, ld s), copied to Hdst (ld ldh) when given. */
 int dmrKernelCholQr(const double *G, int s, double relTol, float *Sinv32, double *Sacc, int first,
                     double *Hdst, int ldh, int *flags, hipStream_t stream);
+/* dmrKernelGramReduce of G partials (s <= 16) followed by dmrKernelCholQr on the sum (first = 1),
+ * in one launch: workgroups sum segments of the partials and the last one to finish adds the
+ * segments in a fixed order and factorises.  The sum is the FP64 sum of the same partials in
+ * another fixed order, so it may differ from dmrKernelGramReduce's in the last bits.  work:
+ * DMR_KGRAMCHOL_WORK_BYTES bytes of device memory, zero before the first call; each call leaves
+ * its counter at zero again, and calls that share a work buffer must not overlap. */
+#define DMR_KGRAMCHOL_WORK_BYTES (256 + 64 * 256 * 8)
+int dmrKernelGramChol(const float *part, int G, int s, double relTol, float *Sinv32, double *Sacc,
+                      double *Hdst, int ldh, int *flags, void *work, hipStream_t stream);
 /* Out[:, 0:s] = U[:, 0:s] C (C row-major s x 32, s <= 16, ldu and ldo even) into FP32 or bf16
  * (outBasis). */
 int dmrKernelScaleBlock(const float *U, long long ldu, const float *C, void *Out, long long ldo,

@@ -96,6 +96,14 @@ def test_kernel_entry_points_check_shapes_without_device():
                              None) == gdmr.DMR_ERR_ARG  # bf16 X takes 104-216
     assert lib.dmrKernelCholQr(one, 33, 1e-12, one, one, 1, None, 0, one,
                                None) == gdmr.DMR_ERR_ARG
+    assert lib.dmrKernelGramChol(one, 100, 17, 1e-12, one, one, None, 0, one, one,
+                                 None) == gdmr.DMR_ERR_ARG  # s > 16
+    assert lib.dmrKernelGramChol(one, 100, 10, 1e-12, one, one, None, 0, one, None,
+                                 None) == gdmr.DMR_ERR_ARG  # no work buffer
+    hdr = open(os.path.join(ROOT, "include", "gDMRKernels.h")).read()
+    m = re.search(r"#define DMR_KGRAMCHOL_WORK_BYTES \((\d+) \+ (\d+) \* (\d+) \* (\d+)\)", hdr)
+    a, b, c, e = (int(x) for x in m.groups())
+    assert a + b * c * e == gdmr.DMR_KGRAMCHOL_WORK_BYTES
     assert lib.dmrKernelLsqBlock(one, 30, 20, 8, one, 4, one, one, one, one, one, one,
                                  None) == gdmr.DMR_ERR_ARG  # rows != cols + bw
     assert lib.dmrKernelLsqFits(100, 16, 16) == 1 and lib.dmrKernelLsqFits(100, 40, 16) == 0

@@ -309,6 +309,75 @@ def test_chol_qr(torch_gpu, s):
     assert not np.any(Sacc.cpu().numpy().T[3]) and not np.any(Sinv.cpu().numpy()[:, 3])
 
 
+def _gram_partials(rng, G, s, dep=False):
+    """G partial Gram matrices B_g^T B_g (B_g: 16 x s) in the update epilogue's layout
+    part[(i G + g) 32 + j] (16 rows), and their FP64 sum (s x s, symmetric positive definite;
+    with dep, column 3 repeats column 1 in every B_g, so the sum is singular)."""
+    B = rng.standard_normal((G, 16, s)).astype(np.float32)
+    if dep:
+        B[:, :, 3] = B[:, :, 1]
+    pg = np.einsum("gki,gkj->gij", B.astype(np.float64), B.astype(np.float64)).astype(np.float32)
+    part = np.zeros((16, G, 32), np.float32)
+    part[:s, :, :s] = pg.transpose(1, 0, 2)
+    return part.reshape(-1), pg.astype(np.float64).sum(axis=0)
+
+
+@pytest.mark.parametrize("s,G", [(10, 4096), (16, 98), (10, 1), (16, 20011)])
+def test_gram_chol(torch_gpu, s, G):
+    """The fused Gram reduce + Cholesky step: S from the FP64 sum of the partials against numpy's
+    Cholesky factor, S^-1 to FP32 rounding, and the same factor as the two-launch path
+    (dmrKernelGramReduce + dmrKernelCholQr) to the rounding of the sum order; the work counter
+    is zero after each call, so a second call on the same work buffer (other G) is right too."""
+    torch = torch_gpu
+    rng = _rng(700 + s + G)
+    d = Dev(torch)
+    parth, Gref = _gram_partials(rng, G, s)
+    part = d.put(parth)
+    work = d.zeros(gdmr.DMR_KGRAMCHOL_WORK_BYTES // 4, torch.int32)
+    Sinv = d.zeros((32, 32), torch.float32)
+    Sacc = d.zeros((s, s), torch.float64)
+    H = d.zeros((s, s + 3), torch.float64)
+    flags = d.zeros(2, torch.int32)
+    rc = gdmr.lib.dmrKernelGramChol(part.data_ptr(), G, s, 1e-12, Sinv.data_ptr(),
+                                    Sacc.data_ptr(), H.data_ptr(), s + 3, flags.data_ptr(),
+                                    work.data_ptr(), None)
+    torch.cuda.synchronize()
+    assert rc == gdmr.DMR_OK and flags.cpu().numpy()[0] == 0
+    assert work.cpu().numpy()[0] == 0  # the arrival counter is back at zero
+    S = Sacc.cpu().numpy().T
+    Sref = np.linalg.cholesky(Gref).T
+    err = np.max(np.abs(S - Sref)) / np.max(np.abs(Sref))
+    print(f"gram_chol s {s} G {G}: max |S - S_ref| / max |S_ref| = {err:.2e}", flush=True)
+    assert err <= 1e-12
+    assert np.array_equal(np.tril(S, -1), np.zeros((s, s)))
+    assert np.array_equal(H.cpu().numpy()[:, :s].T, S)
+    Si = Sinv.cpu().numpy()
+    assert not np.any(Si[:, s:]) and not np.any(Si[s:, :])
+    Sinv_ref = np.linalg.inv(Sref)
+    assert np.max(np.abs(Si[:s, :s] - Sinv_ref)) <= 1e-6 * np.max(np.abs(Sinv_ref))
+    # The two-launch path on the same partials.
+    G64 = d.zeros((s, s), torch.float64)
+    Sacc2 = d.zeros((s, s), torch.float64)
+    assert gdmr.lib.dmrKernelGramReduce(part.data_ptr(), G, s, G64.data_ptr(), None) == 0
+    assert gdmr.lib.dmrKernelCholQr(G64.data_ptr(), s, 1e-12, Sinv.data_ptr(), Sacc2.data_ptr(), 1,
+                                    None, 0, flags.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    S2 = Sacc2.cpu().numpy().T
+    assert np.max(np.abs(S - S2)) <= 1e-13 * np.max(np.abs(S2))
+    # A second call on the same work buffer, with a singular sum: flagged, column 3 zeroed.
+    G2 = max(G // 2, 1) + 3
+    parth2, _ = _gram_partials(rng, G2, s, dep=True)
+    part2 = d.put(parth2)
+    flags.zero_()
+    rc = gdmr.lib.dmrKernelGramChol(part2.data_ptr(), G2, s, 1e-12, Sinv.data_ptr(),
+                                    Sacc.data_ptr(), None, 0, flags.data_ptr(), work.data_ptr(),
+                                    None)
+    torch.cuda.synchronize()
+    assert rc == gdmr.DMR_OK and flags.cpu().numpy()[0] == 1
+    assert work.cpu().numpy()[0] == 0
+    assert not np.any(Sacc.cpu().numpy().T[3]) and not np.any(Sinv.cpu().numpy()[:, 3])
+
+
 @pytest.mark.parametrize("outbasis", [0, 1], ids=["fp32", "bf16"])
 def test_scale_block(torch_gpu, outbasis):
     """Out = U C for the Cholesky QR normalisation (C upper triangular), into FP32 or bf16."""

@@ -1474,13 +1474,17 @@ __global__ __launch_bounds__(D * D) void k_chol_qr_el(const double *__restrict__
 
 // The Gram reduce of G partials (rows i < s <= 16 of the layout part[(i G + g) 32 + j]) and the
 // Cholesky factorisation of k_chol_qr_el<16> in one launch.  Workgroup b sums partials
-// g0 .. g0 + L - 1 (L = ceil(G / gridDim.x), g0 = b L) in FP64: thread (h, r, c4) = (t / 64,
-// (t / 4) % 16, t % 4) adds columns 4 c4 .. 4 c4 + 3 of row r over g = g0 + h, g0 + h + 4, ... in
-// order (kGcLoads loads in flight), the four h in order, and writes the 16 x 16 segment sum to
-// seg[b]; the last workgroup to arrive adds the segments in order of b and factorises.  The
-// order is fixed by G and the grid, independent of timing.  ctr is zero on entry and on exit.
+// g0 .. g0 + L - 1 (L = ceil(G / gridDim.x), g0 = b L) in FP64.  Phase 1: the s rows times
+// nc4 = ceil(s / 4) four-column groups are the ng = s nc4 element groups; thread t < H ng
+// (H = min(8, 256 / ng)) takes group t % ng and adds partials g0 + h, g0 + h + H, ... in order
+// (h = t / ng, kGcLoads loads in flight); the H sums of each element are added in order of h and
+// the workgroup's s x 16 segment sum goes to seg[b].  Phase 2, in the last workgroup to arrive:
+// T2 = min(4, 256 / s^2) threads per element of the s x s sum, thread part c adding segments
+// c, c + T2, ... in order, then the T2 parts in order of c; then the factorisation.  The order is
+// fixed by G, s and the grid, independent of timing.  ctr is zero on entry and on exit.
 constexpr int kGcSegs = kGramCholSegs;
 constexpr int kGcLoads = 16;
+constexpr int kGcMaxH = 8;
 
 __global__ __launch_bounds__(256) void k_gram_chol(const float *__restrict__ part, int G, int L,
                                                    int s, double relTol,
@@ -1489,23 +1493,26 @@ __global__ __launch_bounds__(256) void k_gram_chol(const float *__restrict__ par
                                                    double *__restrict__ Hdst, int ldh,
                                                    int *__restrict__ brk,
                                                    double *__restrict__ seg, unsigned *ctr) {
-  __shared__ double red[4 * 256];
+  __shared__ double red[kGcMaxH * 256];
   __shared__ int s_last;
-  const int t = threadIdx.x, h = t >> 6, r = (t >> 2) & 15, c4 = t & 3;
+  const int t = threadIdx.x;
+  const int nc4 = (s + 3) >> 2, ng = s * nc4;
+  const int H = 256 / ng < kGcMaxH ? 256 / ng : kGcMaxH;
   const int g0 = blockIdx.x * L, g1 = g0 + L < G ? g0 + L : G;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  if (r < s) {
+  if (t < H * ng) {
+    const int grp = t % ng, h = t / ng, r = grp / nc4, c4 = grp % nc4;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     const float4 *src = reinterpret_cast<const float4 *>(part + (long long)r * G * 32) + c4;
-    for (int gb = g0 + h; gb < g1; gb += 4 * kGcLoads) {
+    for (int gb = g0 + h; gb < g1; gb += H * kGcLoads) {
       float4 v[kGcLoads];
 #pragma unroll
       for (int u = 0; u < kGcLoads; ++u) {
-        const int g = gb + 4 * u;
+        const int g = gb + H * u;
         v[u] = g < g1 ? src[(long long)g * 8] : make_float4(0.f, 0.f, 0.f, 0.f);
       }
 #pragma unroll
       for (int u = 0; u < kGcLoads; ++u) {
-        if (gb + 4 * u < g1) {
+        if (gb + H * u < g1) {
           a0 = a0 + (double)v[u].x;
           a1 = a1 + (double)v[u].y;
           a2 = a2 + (double)v[u].z;
@@ -1513,14 +1520,19 @@ __global__ __launch_bounds__(256) void k_gram_chol(const float *__restrict__ par
         }
       }
     }
+    double *rd = red + h * 256 + r * 16 + 4 * c4;
+    rd[0] = a0;
+    rd[1] = a1;
+    rd[2] = a2;
+    rd[3] = a3;
   }
-  red[h * 256 + r * 16 + 4 * c4 + 0] = a0;
-  red[h * 256 + r * 16 + 4 * c4 + 1] = a1;
-  red[h * 256 + r * 16 + 4 * c4 + 2] = a2;
-  red[h * 256 + r * 16 + 4 * c4 + 3] = a3;
   __syncthreads();
-  // Thread t = 16 i + j: element (i, j) of this segment.
-  seg[(long long)blockIdx.x * 256 + t] = ((red[t] + red[256 + t]) + red[512 + t]) + red[768 + t];
+  // Thread t = 16 i + j: element (i, j) of this segment (i < s, j < 4 nc4).
+  if ((t >> 4) < s && (t & 15) < 4 * nc4) {
+    double a = red[t];
+    for (int h = 1; h < H; ++h) a = a + red[h * 256 + t];
+    seg[(long long)blockIdx.x * 256 + t] = a;
+  }
   __syncthreads();
   if (t == 0) {
     const unsigned total = gridDim.x;
@@ -1531,22 +1543,36 @@ __global__ __launch_bounds__(256) void k_gram_chol(const float *__restrict__ par
   }
   __syncthreads();
   if (!s_last) return;
+  const int ne = s * s, nb = gridDim.x;
+  const int T2 = 256 / ne < 4 ? 256 / ne : 4;
+  if (t < T2 * ne) {
+    const int e = t % ne, c = t / ne, i = e / s, j = e % s;
+    const double *src = seg + i * 16 + j;
+    double a = 0.0;
+    for (int b0 = c; b0 < nb; b0 += T2 * kGcLoads) {
+      double v[kGcLoads];
+#pragma unroll
+      for (int u = 0; u < kGcLoads; ++u) {
+        const int b = b0 + T2 * u;
+        v[u] = b < nb ? __hip_atomic_load(src + (long long)b * 256, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT)
+                      : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kGcLoads; ++u)
+        if (b0 + T2 * u < nb) a = a + v[u];
+    }
+    red[c * 256 + e] = a;
+  }
+  __syncthreads();
   const int i = t >> 4, j = t & 15;
   double g = 0.0;
   if (i < s && j < s) {
-    const int nb = gridDim.x;
-    for (int b0 = 0; b0 < nb; b0 += kGcLoads) {
-      double v[kGcLoads];
-#pragma unroll
-      for (int u = 0; u < kGcLoads; ++u)
-        v[u] = b0 + u < nb ? __hip_atomic_load(seg + (long long)(b0 + u) * 256 + t,
-                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                           : 0.0;
-#pragma unroll
-      for (int u = 0; u < kGcLoads; ++u)
-        if (b0 + u < nb) g = g + v[u];
-    }
+    const int e = i * s + j;
+    g = red[e];
+    for (int c = 1; c < T2; ++c) g = g + red[c * 256 + e];
   }
+  __syncthreads();  // red is free again before chol_el's own shared arrays are written
   chol_el<16>(g, s, relTol, Sinv32, Sacc, first, Hdst, ldh, brk);
 }
 

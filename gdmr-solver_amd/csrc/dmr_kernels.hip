@@ -942,6 +942,15 @@ __device__ inline float ldv(const bf16 *p, long long i) {
 __device__ inline void stv(double *p, long long i, double v) { p[i] = v; }
 __device__ inline void stv(float *p, long long i, float v) { p[i] = v; }
 __device__ inline void stv(bf16 *p, long long i, float v) { p[i].bits = (uint16_t)bf_bits(v); }
+// stv with the non-temporal hint (NT): the SpMV writes y that way, so its write-back overlaps the
+// SpMV's gathers instead of the bandwidth-bound multi-dot that reads y next.
+template <bool NT> __device__ inline void stv_t(double *p, long long i, double v) {
+  if (NT) __builtin_nontemporal_store(v, p + i); else p[i] = v;
+}
+template <bool NT> __device__ inline void stv_t(float *p, long long i, float v) {
+  if (NT) __builtin_nontemporal_store(v, p + i); else p[i] = v;
+}
+template <bool NT> __device__ inline void stv_t(bf16 *p, long long i, float v) { stv(p, i, v); }
 
 // prod[e] = va[e] * x[ci[e]] for e < ne <= kSpmvNnzCap, entry e on thread e % kBlock: all index
 // and value loads of the thread first, then all of its gathers, then the products.  Slots past
@@ -1002,7 +1011,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
   if (hasRow) {
     TA s = 0;
     for (int e = ra; e < rb; ++e) s = s + prod[e];
-    stv(y, rs + threadIdx.x, s);
+    stv_t<true>(y, rs + threadIdx.x, s);
   }
 }
 

@@ -1282,6 +1282,10 @@ void combo_multi(TS *out, int J, const TS *B1, int p1, const double *C1, const T
                      scal);
 }
 
+// Most units per thread of k_orth: its two register copies of the thread's units hold at most
+// 2 x 64 values (bf16: 8 units; 16 would not fit in registers).
+template <typename TS> constexpr int kOrthMaxUpt = 64 / Unit<TS>::N < 16 ? 64 / Unit<TS>::N : 16;
+
 template <typename TS>
 int orth(const TS *g, TS *dst, const TS *A, int pa, double *ha, const TS *B, int pb, double *hb,
          long long ld, long long n, int red, const Fin &fin, RedWork w, unsigned *bar,
@@ -1307,9 +1311,9 @@ int orth(const TS *g, TS *dst, const TS *A, int pa, double *ha, const TS *B, int
   else if (upt <= 8)
     e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_orth<TS, 8>), dim3(G),
                                    dim3(kBlock), args, 0, s);
-  else if (upt <= 16)
-    e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_orth<TS, 16>), dim3(G),
-                                   dim3(kBlock), args, 0, s);
+  else if (upt <= 16 && kOrthMaxUpt<TS> >= 16)
+    e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_orth<TS, kOrthMaxUpt<TS>>),
+                                   dim3(G), dim3(kBlock), args, 0, s);
   else
     return 0;
   if (e != hipSuccess) {

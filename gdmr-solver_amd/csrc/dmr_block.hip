@@ -22,6 +22,7 @@
 #include <cstdlib>
 
 #include "dmr_block.h"
+#include "dmr_lanes.h"
 
 namespace dmr {
 namespace blk {
@@ -316,8 +317,7 @@ __global__ __launch_bounds__(kThreads) void k_colsumsq(const T *__restrict__ X, 
     const double x = (double)col[r];
     acc = fma(x, x, acc);
   }
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_xor(acc, off, 64);
+  acc = xor_sum<64>(acc);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
   if (threadIdx.x == 0)
@@ -1409,8 +1409,7 @@ __device__ inline void chol_el(double g, int s, double relTol, float *__restrict
   if (t < 64) {
     double d = t < s ? A[t * LD + t] : 0.0;
     d = fmax(d, 0.0);
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) d = fmax(d, __shfl_xor(d, off, 64));
+    d = xor_max64(d);
     if (t == 0) sh_dmax = d;
   }
   __syncthreads();
@@ -1712,8 +1711,7 @@ __global__ __launch_bounds__(BWT == 32 ? 512 : 1024) void k_lsq_block(const doub
         double x = 0.0;
         if (tid >= 1 && tid <= L) x = win[((c + tid) % W) * nc + c];
         double xn2 = x * x;
-  #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) xn2 = xn2 + __shfl_xor(xn2, off, 64);
+        xn2 = xor_sum<64>(xn2);
         const double alpha = rowc[c];
         double tau = 0.0, sc = 0.0, beta = alpha;
         if (xn2 != 0.0) {
@@ -1787,8 +1785,7 @@ __global__ __launch_bounds__(BWT == 32 ? 512 : 1024) void k_lsq_block(const doub
   if (tid < 64) {
     double m = 0.0;
     for (int c = tid; c < cols; c += 64) m = fmax(m, fabs(R[(long long)c * cols + c]));
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    m = xor_max64(m);
     if (tid == 0) s_rmax = m;
   }
   __syncthreads();

@@ -21,14 +21,14 @@
 #include <cmath>
 
 #include "dmr_internal.h"
+#include "dmr_lanes.h"
 
 namespace dmr {
 namespace {
 
 // Xor butterfly over the 32 lanes of a half wavefront.
 __device__ inline double half_sum(double v) {
-#pragma unroll
-  for (int off = 16; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 32);
+  v = xor_sum<32>(v);
   return v;
 }
 // Sum of squares of a[0..len): lane hl adds rows hl, hl+32, ... in order, then half_sum.
@@ -221,8 +221,7 @@ __global__ __launch_bounds__(kSvdThreads) void k_svd(const double *__restrict__ 
 
 // Sum over a 64-lane wavefront (xor butterfly); every lane gets the result.
 __device__ inline double wave_sum64(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+  v = xor_sum<64>(v);
   return v;
 }
 

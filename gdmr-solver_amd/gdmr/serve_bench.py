@@ -125,8 +125,9 @@ class Load:
                 worst = 0.0
                 for i, x in zip(sp["rhs"], xs):
                     xh = x.cpu().numpy()
-                    worst = max(worst, float(np.linalg.norm(f["B"][i] - A.matvec(xh)) /
-                                             np.linalg.norm(f["B"][i])))
+                    r = float(np.linalg.norm(f["B"][i] - A.matvec(xh)) / np.linalg.norm(f["B"][i]))
+                    worst = r if (r != r or r > worst) else worst  # a NaN stays NaN
+                _log(f"request {res.id}: FP64 check on the host, worst relative residual {worst:.2e}")
                 self.checked.append((sp["fam"], worst))
 
     def run_until(self, n_done, check_budget=0, timeout=3600.0):
@@ -222,7 +223,8 @@ def main(argv=None):
                                                     max(1, sum(r.nrhs for r in rs))))
     status_ok = all(r.status == 0 for r in res)
     not_converged = sum(1 for r in res if r.status != 0)
-    worst = max((c[1] for c in L.checked), default=None)
+    ws = [c[1] for c in L.checked]
+    worst = (float("nan") if any(w != w for w in ws) else max(ws)) if ws else None
     line = {
         "metric": "serving requests/s at 1e-10 (FP64-checked sample), p50/p99 latency",
         "value": len(res) / wall,

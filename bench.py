@@ -318,8 +318,8 @@ def summarize(leg, tol):
         "p99_ms": float(np.percentile(lat, 99) * 1e3),
         "mean_ms": float(lat.mean() * 1e3),
         "cycles": leg["cycles"],
-        "max_true_rel_residual": max(leg["true"]),
-        "all_converged": bool(max(leg["true"]) <= tol),
+        "max_true_rel_residual": float(np.max(leg["true"])),  # a NaN propagates
+        "all_converged": bool(np.max(leg["true"]) <= tol),
     }
 
 
@@ -542,7 +542,7 @@ def main():
         for nm in [nm for nm in legs if nm != "fp64"]:
             errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b))
                     for a, b in zip(legs[nm]["x"], legs["fp64"]["x"])]
-            out_legs[nm]["max_rel_err_vs_fp64_solution"] = max(errs)
+            out_legs[nm]["max_rel_err_vs_fp64_solution"] = float(np.max(errs))
     # Headline: the fastest leg whose every solve reached the tolerance (FP64-checked).
     ok = [nm for nm in out_legs if out_legs[nm]["all_converged"]] or list(out_legs)
     head = max(ok, key=lambda nm: out_legs[nm]["solves_per_s"])
@@ -628,8 +628,8 @@ def main():
             "fp64_sample": br["fp64_sequential"]["sample"],
             "speedup_vs_fp64": br["speedup_vs_fp64"],
             "speedup_first_block_vs_fp64": br["speedup_first_block_vs_fp64"],
-            "max_true_rel_residual": max(br["block_first"]["max_true_rel_residual"],
-                                         br["block_recycled"]["max_true_rel_residual"]),
+            "max_true_rel_residual": float(np.max([br["block_first"]["max_true_rel_residual"],
+                                                   br["block_recycled"]["max_true_rel_residual"]])),
             "max_rel_err_vs_fp64_solution": br["max_rel_err_vs_fp64_solution"],
             "tuning": br["tuning"],
         }

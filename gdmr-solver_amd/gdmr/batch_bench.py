@@ -96,7 +96,7 @@ def run(args) -> dict:
         tr = [float(np.linalg.norm(blocks[b][j] - A.matvec(Xh[j])) / np.linalg.norm(blocks[b][j]))
               for j in range(s)]
         legs.append(dict(rc=rc, seconds=dt, solves_per_s=s / dt, cycles=st.cycles,
-                         breakdowns=st.breakdowns, max_true_rel_residual=max(tr), X=Xh))
+                         breakdowns=st.breakdowns, max_true_rel_residual=float(np.max(tr)), X=Xh))
         _log(f"block {b}: rc {rc}, {st.cycles} cycles, {dt:.2f}s -> {s / dt:.2f} solves/s, "
              f"max true residual {max(tr):.2e}")
     tuning = bs.tuning()
@@ -132,7 +132,7 @@ def run(args) -> dict:
     errs = [float(np.linalg.norm(legs[0]["X"][j] - x64[j]) / np.linalg.norm(x64[j]))
             for j in range(args.fp64_sample)]
     _log(f"fp64 single: cycles {cyc64}, {fp64_rate:.2f} solves/s; block vs fp64 solution error "
-         f"max {max(errs):.2e}")
+         f"max {float(np.max(errs)):.2e}")
     out = {
         "metric": "batched solves/s @1e-10 (FP64-checked), block DMR vs sequential FP64 DMR",
         "value": legs[1]["solves_per_s"],
@@ -150,7 +150,7 @@ def run(args) -> dict:
                             "sample": args.fp64_sample, "m": args.m, "k": args.k},
         "speedup_vs_fp64": legs[1]["solves_per_s"] / fp64_rate,
         "speedup_first_block_vs_fp64": legs[0]["solves_per_s"] / fp64_rate,
-        "max_rel_err_vs_fp64_solution": max(errs),
+        "max_rel_err_vs_fp64_solution": float(np.max(errs)),
         "tuning": tuning,
     }
     if prof:

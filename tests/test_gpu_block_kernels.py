@@ -355,7 +355,8 @@ def test_gram_chol(torch_gpu, s, G):
     assert not np.any(Si[:, s:]) and not np.any(Si[s:, :])
     Sinv_ref = np.linalg.inv(Sref)
     assert np.max(np.abs(Si[:s, :s] - Sinv_ref)) <= 1e-6 * np.max(np.abs(Sinv_ref))
-    # The two-launch path on the same partials.
+    # The two-launch path on the same partials (for G = 1 both sum the one partial, so the
+    # factorisations must give the same bits).
     G64 = d.zeros((s, s), torch.float64)
     Sacc2 = d.zeros((s, s), torch.float64)
     assert gdmr.lib.dmrKernelGramReduce(part.data_ptr(), G, s, G64.data_ptr(), None) == 0
@@ -364,6 +365,8 @@ def test_gram_chol(torch_gpu, s, G):
     torch.cuda.synchronize()
     S2 = Sacc2.cpu().numpy().T
     assert np.max(np.abs(S - S2)) <= 1e-13 * np.max(np.abs(S2))
+    if G == 1:
+        assert np.array_equal(S, S2) and np.array_equal(Si, Sinv.cpu().numpy())
     # A second call on the same work buffer, with a singular sum: flagged, column 3 zeroed.
     G2 = max(G // 2, 1) + 3
     parth2, _ = _gram_partials(rng, G2, s, dep=True)

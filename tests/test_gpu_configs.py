@@ -8,6 +8,11 @@
 * C: the B sequence with FP32 and with bf16 basis storage inside FP64 refinement: every system
   meets 1e-10 in FP64, and the error of systems 0 and 1 against the oracle's solution is bounded
   by XTOL_MIXED.
+* The benchmark's sequence path: the B sequence extended to 20 systems as two blocks of 10
+  (block DMR with per-system FP64 operators in the refinement, the mean operator and a bf16
+  basis inside, 20 block steps, k = 20, the subspace recycled from the first block to the
+  second): every system meets 1e-10 against its own operator, and systems 0 and 1 agree with
+  the oracle's solutions to XTOL_MIXED.
 * D: 3-D anisotropic diffusion 96^3 (1, 0.1, 0.01), 32 right-hand sides as one block (block DMR,
   10 block steps, k = 32, bf16 basis inside FP64 refinement): every column meets 1e-10, and two
   columns agree with the oracle's solutions to XTOL_BLOCK.
@@ -247,3 +252,52 @@ def test_mixed_long_cycles_on_the_2d_serving_family(torch_gpu, m):
               f"(true {tr:.2e})", flush=True)
         assert st.status == 0 and np.isfinite(res) and tr <= TOL and st.cycles <= 200
     s.close()
+
+
+def test_sequence_as_two_blocks_128(torch_gpu, seq128):
+    """The benchmark's headline path at its size: systems 0..9 and 10..19 of the 128^3 sequence
+    as two blocks (dmrBlockSetColumnOperators, inner mean operator, bf16 basis, inner tolerance
+    3e-2), one handle recycling the deflation subspace from block to block.  A size-independent
+    check on every system (its FP64 residual against its own operator, and far above the
+    tolerance against a neighbour's), the oracle on the two systems it solved."""
+    torch = torch_gpu
+    mats, rhs, ref = seq128
+    mats = list(mats) + [gdmr.conv_diff_3d(128, nu0=0.1, beta=(1.0, 0.5, 0.25), jitter=0.01,
+                                           seed=2024, stream=t) for t in range(16, 20)]
+    rhs = list(rhs) + [gdmr.rand_normal(2024, 1000 + t, mats[0].n) for t in range(16, 20)]
+    A0 = mats[0]
+    n, nnz = A0.n, A0.nnz
+    vals = _dev(torch, np.stack([A.val for A in mats]))
+    rp, ci = _dev(torch, A0.rowptr), _dev(torch, A0.col)
+    B = _dev(torch, np.stack(rhs))
+    X = torch.zeros(len(mats), n, dtype=torch.float64, device="cuda")
+    h = gdmr.BlockSolver(n, 20, 20, 10)
+    try:
+        h.set_csr(rp.data_ptr(), ci.data_ptr(), vals[0].data_ptr(), nnz, keep=(rp, ci))
+        h.set_refinement(3e-2, 60)
+        h.set_precision(gdmr.PREC_BF16)
+        torch.cuda.synchronize()
+        for i, (g0, g1) in enumerate([(0, 10), (10, 20)]):
+            h.set_column_operators(vals[g0].data_ptr(), nnz, g1 - g0, inner_mean=True,
+                                   keep=(vals,))
+            rc, rel = h.solve(X[g0].data_ptr(), n, B[g0].data_ptr(), n, g1 - g0, TOL,
+                              max_cycles=4000, mode=1 if i else 0)
+            st, _ = h.stats()
+            print(f"systems {g0}..{g1 - 1}: rc {rc}, {st.cycles} cycles, max reported residual "
+                  f"{np.max(rel):.2e}", flush=True)
+            assert rc == 0
+    finally:
+        h.close()
+    Xh = X.cpu().numpy()
+    for t, (A, b) in enumerate(zip(mats, rhs)):
+        tr = _true(A, b, Xh[t])
+        other = _true(mats[(t + 1) % len(mats)], b, Xh[t])
+        line = f"system {t}: true residual {tr:.2e}, against system {(t + 1) % len(mats)}'s " \
+               f"operator {other:.2e}"
+        if t < 2:
+            err = np.linalg.norm(Xh[t] - ref[t]["x"]) / np.linalg.norm(ref[t]["x"])
+            line += f", error vs oracle {err:.2e}"
+            assert err <= XTOL_MIXED, (t, err)
+        print(line, flush=True)
+        assert tr <= TOL * 1.01, (t, tr)
+        assert other > 1e3 * TOL, (t, other)

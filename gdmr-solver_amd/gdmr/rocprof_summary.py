@@ -8,6 +8,10 @@
         counter each), in KiB, averaged over each kernel's dispatches.  The fetch factor is
         calibrated on KERNEL, which reads exactly the bytes it writes (WRITE_SIZE counts written
         bytes one for one): factor = mean WRITE / mean FETCH of that kernel.
+    python -m gdmr.rocprof_summary counters DIR [DIR ...] --counters C1,C2,... [--kernels K,...]
+        the mean per dispatch of each listed counter (looked up in the counter_collection.csv of
+        every DIR, so counters collected in separate runs combine), per kernel, for the kernels
+        listed (default: every kernel that has all the counters).
     --record KERNEL --bytes B --n N --s S --json PATH (pmc only) also writes the calibrated
         traffic per dispatch of KERNEL beside its algorithmic bytes per launch B (from the bench
         line of the same command) into the JSON file the benchmark reads.
@@ -161,6 +165,41 @@ def cmd_pmc(a):
               f"algorithmic per launch ({traffic / a.bytes:.3f}x).")
 
 
+def cmd_counters(a):
+    names = [c for c in a.counters.split(",") if c]
+    per = {}
+    srcs = []
+    for c in names:
+        found = None
+        for d in a.dirs:
+            try:
+                means, path = counter_means(d, c)
+            except SystemExit:
+                continue
+            if means:
+                found = means
+                srcs.append(f"`{c}`: {os.path.relpath(path)}")
+                break
+        if found is None:
+            sys.exit(f"counter {c} not found under {' '.join(a.dirs)}")
+        per[c] = found
+    kernels = [k for k in (a.kernels.split(";") if a.kernels else []) if k] or \
+        sorted(set.intersection(*(set(v) for v in per.values())))
+    print(f"# {a.title}\n")
+    if a.command:
+        print("Commands (from the repository root, one run per group of counters):\n\n```\n" +
+              a.command + "\n```\n")
+    print("Sources: " + "; ".join(srcs) + ".\n")
+    print("Mean per dispatch over the kernel's dispatches in the run.\n")
+    print("| kernel | dispatches | " + " | ".join(f"`{c}`" for c in names) + " |")
+    print("|---|---:|" + "---:|" * len(names))
+    for k in kernels:
+        if not all(k in per[c] for c in names):
+            continue
+        disp = per[names[0]][k][0]
+        print(f"| `{k}` | {disp} | " + " | ".join(f"{per[c][k][1]:.4g}" for c in names) + " |")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     sub = ap.add_subparsers(dest="mode", required=True)
@@ -176,12 +215,16 @@ def main(argv=None):
     p.add_argument("--s", type=int, default=0)
     p.add_argument("--json", default="profiles/roofline_traffic.json")
     p.add_argument("--source", default="")
-    for q in (s, p):
+    c = sub.add_parser("counters")
+    c.add_argument("dirs", nargs="+")
+    c.add_argument("--counters", required=True)
+    c.add_argument("--kernels", default="", help="short kernel names separated by ';'")
+    for q in (s, p, c):
         q.add_argument("--top", type=int, default=40)
         q.add_argument("--title", default="rocprofv3 summary")
         q.add_argument("--command", default="")
     a = ap.parse_args(argv)
-    (cmd_stats if a.mode == "stats" else cmd_pmc)(a)
+    {"stats": cmd_stats, "pmc": cmd_pmc, "counters": cmd_counters}[a.mode](a)
 
 
 if __name__ == "__main__":
